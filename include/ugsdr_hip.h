@@ -1,0 +1,222 @@
+/*
+ * ugsdr_hip.h — C ABI of the MI355X (gfx950) acquisition / correlator library.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or a
+ * negative UGSDR_ERR_* code; ugsdr_hip_last_error() gives the message of the
+ * last failure on the calling thread.  No exception crosses this boundary.
+ * The caller owns host buffers; the library owns device memory and caches.
+ * Calls on one context are serialised internally, so a context may be shared
+ * by several host threads (the reference's par_unseq callers).
+ *
+ * Citations (file:line) name the reference interface each entry replaces,
+ * relative to the ugsdr checkout.
+ */
+#ifndef UGSDR_HIP_H
+#define UGSDR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UGSDR_OK 0
+#define UGSDR_ERR_ARG (-1)         /* bad argument / null pointer                      */
+#define UGSDR_ERR_HIP (-2)         /* HIP runtime failure                              */
+#define UGSDR_ERR_SIZE (-3)        /* size contract violated (reference throws)        */
+#define UGSDR_ERR_UNSUPPORTED (-4) /* shape not supported by the device path           */
+#define UGSDR_ERR_NOMEM (-5)       /* device memory / admission limit                  */
+#define UGSDR_ERR_STATE (-6)       /* unknown key, closed server, ...                  */
+
+/* sample formats (src/signal_parameters.hpp:23-32 FileType, IQ subset) */
+#define UGSDR_FMT_CI8 1  /* interleaved int8 I/Q   (Iq_8_plus_8)   */
+#define UGSDR_FMT_CI16 2 /* interleaved int16 I/Q  (Iq_16_plus_16) */
+#define UGSDR_FMT_CF32 3 /* interleaved float I/Q                  */
+#define UGSDR_FMT_R8 4   /* real int8 (Real_8), imaginary part 0   */
+
+/* arithmetic of the correlation surfaces */
+#define UGSDR_PREC_FP32 0 /* plain: FP32 complex like the reference            */
+#define UGSDR_PREC_FP16 1 /* reduced: FP16 storage of samples and spectra      */
+#define UGSDR_PREC_BF16 2 /* reduced: BF16 storage of samples and spectra      */
+
+typedef struct ugsdr_hip_ctx ugsdr_hip_ctx;
+
+const char* ugsdr_hip_last_error(void);
+/* library version string */
+const char* ugsdr_hip_version(void);
+
+/* ------------------------------------------------------------------ context */
+/* Creates a context on `device`.  Replaces the implicit per-thread state of
+ * the reference's operators (thread_local scratch in src/mixer/ipp_mixer.hpp:15-19,
+ * src/math/ipp_dft.hpp:24-41) with one device-resident owner. */
+int ugsdr_hip_ctx_create(int device, ugsdr_hip_ctx** out);
+int ugsdr_hip_ctx_destroy(ugsdr_hip_ctx* ctx);
+/* the HIP stream (hipStream_t) the context launches on, for event timing */
+int ugsdr_hip_ctx_stream(ugsdr_hip_ctx* ctx, void** stream);
+/* bytes of device memory held by the context (caches + scratch) */
+int64_t ugsdr_hip_ctx_device_bytes(ugsdr_hip_ctx* ctx);
+
+/* ------------------------------------------------------------- PRN codes (host)
+ * Codegen<Impl>::Get (src/prn_codes/codegen.hpp:23-36).  sv is 0-based. */
+int ugsdr_code_gps_l1ca(int sv, int8_t* out1023);  /* GpsL1Ca.hpp:9-21, GoldCodes.hpp:24-41 */
+int ugsdr_code_gps_l5i(int sv, int8_t* out10230);  /* GpsL5I.hpp:9-21, L5.hpp:25-43 */
+int ugsdr_code_gps_l5q(int sv, int8_t* out10230);  /* GpsL5Q.hpp:9-21, L5.hpp:25-43 */
+int ugsdr_code_glonass_of(int8_t* out511);         /* GlonassOf.hpp:24-36 */
+/* BOC(1,1) chip expansion c -> (c,-c): MemoryCodes::DecodeBoc (MemoryCodes.hpp:32-40) */
+int ugsdr_code_boc(const int8_t* chips, int64_t n, int8_t* out2n);
+/* FastSearchEngineBase::AdjustSamplingRate (src/acquisition/fse.hpp:222-240) */
+double ugsdr_adjust_sampling_rate(double fs, double target);
+
+/* ------------------------------------------------------------ replica cache
+ * Device-resident conj(DFT(replica)) of length n_fft.  Replaces
+ * MatchedFilter::PrepareCodeSpectrum (src/matched_filter/matched_filter.hpp:27-30,
+ * ipp_matched_filter.hpp:71-78) applied to
+ * Upsampler::Transform(RepeatCodeNTimes(code, repeats), n_fft)
+ * (src/acquisition/fse.hpp:295-296, src/common.hpp:78-85,
+ * src/resample/upsampler.hpp:31-37): replica[i] = chips[(i*len/n_fft) % len],
+ * len = n_chips*repeats.  `precision` is UGSDR_PREC_*. */
+int ugsdr_hip_replica_put(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chips, int64_t repeats,
+                          int64_t n_fft, int precision, uint64_t* key);
+/* same from an arbitrary real replica of n_fft samples (e.g. the per-ms sign
+ * permutations of src/acquisition/fse.hpp:374-385) */
+int ugsdr_hip_replica_put_samples(ugsdr_hip_ctx* ctx, const float* samples, int64_t n_fft, int precision,
+                                  uint64_t* key);
+int ugsdr_hip_replica_drop(ugsdr_hip_ctx* ctx, uint64_t key);
+/* copies the stored spectrum (n_fft complex float) back, for inspection */
+int ugsdr_hip_replica_get(ugsdr_hip_ctx* ctx, uint64_t key, float* spectrum_c32, int64_t n_fft);
+
+/* ------------------------------------------------------- fused acquisition */
+typedef struct {
+    const void* iq;        /* samples at fs_in in `fmt`                              */
+    int32_t fmt;           /* UGSDR_FMT_*                                            */
+    int32_t iq_on_device;  /* 1: iq is a device pointer, 0: host pointer             */
+    int64_t n_samples;     /* complex samples available at iq                        */
+    double fs_in;          /* input rate                                             */
+    double fs_acq;         /* acquisition rate (Resampler::Transform target)         */
+    double mix_frequency;  /* applied first as Mixer::Translate(x, fs_in, f, phase)  */
+    double mix_phase;      /*   (the -IF wipe-off of fse.hpp:289, and/or a DFE tone)  */
+    double intermediate_frequency; /* added to reported Doppler (fse.hpp:263)       */
+    int64_t n_fft;         /* samples per coherent block at fs_acq                   */
+    int64_t n_blocks;      /* K non-coherent blocks                                  */
+    int64_t surface_len;   /* fold length: spm (reshape) or n_fft (no reshape)       */
+    int32_t reshape;       /* GetOneMsPeak<reshape, coherent> (fse.hpp:201-220)      */
+    int32_t coherent;
+    const double* doppler; /* host array, loop order of fse.hpp:250-252              */
+    int64_t n_doppler;
+    const uint64_t* replica_keys; /* host array, one per PRN, each of length n_fft */
+    int64_t n_prn;
+    double threshold;      /* 3.3 in the reference (fse.hpp:150)                     */
+    int32_t precision;     /* UGSDR_PREC_*                                           */
+    int32_t reserved;
+} ugsdr_acq_request;
+
+typedef struct {
+    double level;        /* max of the best bin's surface (reference scale)  */
+    double sigma;        /* mean + sample sigma (fse.hpp:261)                */
+    double mean;
+    double stddev;
+    double doppler;      /* bin frequency + IF                               */
+    double code_offset;  /* fs_in/fs_acq * index (fse.hpp:262)               */
+    double snr;          /* level / sigma                                    */
+    double runner_up_code;    /* second value of the best row              */
+    double runner_up_doppler; /* best level of the other Doppler bins      */
+    int64_t index;
+    int32_t doppler_bin; /* -1 if no bin beat the default (level 0)          */
+    int32_t detected;    /* snr > threshold, or !reshape (fse.hpp:272)       */
+} ugsdr_acq_result;
+
+/* FastSearchEngineBase::ProcessBpsk for n_prn PRNs at once
+ * (src/acquisition/fse.hpp:242-276, generalised to K blocks): mix, resample,
+ * per (PRN, Doppler, block) circular correlation against the cached replica
+ * spectrum, |.| and fold, sum over blocks, first-index max, mean, n-1 sigma,
+ * strict-< Doppler selection, threshold.  out: n_prn results.  surfaces
+ * (optional host buffer): n_prn x n_doppler x surface_len.  peaks (optional):
+ * n_prn x surface_len, the output_peak of the selected bin. */
+int ugsdr_hip_acquire(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* req, ugsdr_acq_result* out, float* surfaces,
+                      float* peaks);
+
+/* ---------------------------------------- per-operator drop-ins (host data)
+ * Slow path behind the Hip* CRTP classes: one call = one reference operator
+ * call, data copied in and out like the ArrayFire proxy backends. */
+/* Mixer::Translate const overload + IppMixer::Process (src/mixer/mixer.hpp:43-51,
+ * ipp_mixer.hpp:83-88): f<0 folds to fs+f, phase reduced to [0,2pi). */
+int ugsdr_hip_mix(ugsdr_hip_ctx* ctx, const float* in_c32, float* out_c32, int64_t n, double fs, double f,
+                  double phase);
+/* Mixer::Translate in-place overload (mixer.hpp:33-41): no-op when f == 0 */
+int ugsdr_hip_mix_inplace(ugsdr_hip_ctx* ctx, float* io_c32, int64_t n, double fs, double f, double phase);
+/* DiscreteFourierTransform::Transform (src/math/ipp_dft.hpp:106-122): any n,
+ * forward unscaled, inverse scaled 1/n */
+int ugsdr_hip_dft(ugsdr_hip_ctx* ctx, const float* in_c32, float* out_c32, int64_t n, int inverse);
+/* MatchedFilter::PrepareCodeSpectrum (matched_filter.hpp:27-30): conj(DFT(real code)) */
+int ugsdr_hip_mf_prepare(ugsdr_hip_ctx* ctx, const float* code, float* spec_c32, int64_t n);
+/* MatchedFilter::FilterOptimized (matched_filter.hpp:32-42, ipp_matched_filter.hpp:80-90) */
+int ugsdr_hip_mf_filter(ugsdr_hip_ctx* ctx, const float* in_c32, const float* spec_c32, float* out_c32,
+                        int64_t n);
+/* MatchedFilter::Filter (matched_filter.hpp:15-25): prepare + filter */
+int ugsdr_hip_mf_filter_code(ugsdr_hip_ctx* ctx, const float* in_c32, const float* code, float* out_c32,
+                             int64_t n);
+/* Abs::Transform (src/math/abs.hpp:14-19) */
+int ugsdr_hip_abs(ugsdr_hip_ctx* ctx, const float* in_c32, float* out, int64_t n);
+/* ReshapeAndSum::Transform (src/math/reshape_and_sum.hpp:27-31); real (complex=0)
+ * or complex (complex=1); UGSDR_ERR_SIZE when n % block != 0 */
+int ugsdr_hip_reshape_sum(ugsdr_hip_ctx* ctx, const float* in, float* out, int64_t n, int64_t block, int complex_);
+/* MaxIndex::Transform (src/math/max_index.hpp:21-24): first index of the max */
+int ugsdr_hip_max_index(ugsdr_hip_ctx* ctx, const float* in, int64_t n, float* value, int64_t* index);
+/* MeanStdDev::Calculate (src/math/mean_stddev.hpp:21-24): mean, sample sigma */
+int ugsdr_hip_mean_std(ugsdr_hip_ctx* ctx, const float* in, int64_t n, float* mean, float* sigma);
+/* Resampler::Transform (src/resample/resampler.hpp:15-25, ipp_resampler.hpp:131-143):
+ * sample-and-hold to lcm, boxcar down.  out must hold ugsdr_resample_len(). */
+int64_t ugsdr_resample_len(int64_t n, int64_t new_fs, int64_t old_fs);
+int ugsdr_hip_resample(ugsdr_hip_ctx* ctx, const float* in_c32, int64_t n, int64_t new_fs, int64_t old_fs,
+                       float* out_c32);
+/* Upsampler::Transform (src/resample/upsampler.hpp:13-23, 31-37): real */
+int ugsdr_hip_upsample(ugsdr_hip_ctx* ctx, const float* in, int64_t n, float* out, int64_t samples);
+/* Correlator::Correlate (src/correlator/correlator.hpp:16-20, ipp_correlator.hpp:25-38):
+ * sum signal[i]*code[i]; UGSDR_ERR_SIZE on n_signal != n_code */
+int ugsdr_hip_correlate(ugsdr_hip_ctx* ctx, const float* signal_c32, int64_t n_signal, const float* code,
+                        int64_t n_code, float* out_c32);
+
+/* ------------------------------------------------------- correlator bank */
+typedef struct {
+    double carrier_frequency;  /* Hz                                             */
+    double carrier_phase;      /* ref_split: radians; nco: cycles                */
+    double code_rate;          /* nco: chips/s                                   */
+    double code_phase;         /* ref_split: samples; nco: chips at block start  */
+} ugsdr_channel_state;
+
+#define UGSDR_BANK_REF_SPLIT 0 /* Tracker::GetEpl + CorrelateSplit semantics */
+#define UGSDR_BANK_NCO 1       /* FP64 carrier and code NCOs per sample      */
+
+typedef struct {
+    const void* iq;            /* n_blocks * spm samples in fmt               */
+    int32_t fmt;
+    int32_t iq_on_device;
+    int64_t spm;               /* samples per block (1 ms)                    */
+    int64_t n_blocks;
+    double fs;
+    int32_t mode;              /* UGSDR_BANK_*                                 */
+    int32_t n_channels;
+    int32_t n_taps;
+    int32_t reserved;
+    const double* taps_chips;  /* n_taps tap offsets in chips                  */
+    const int8_t* codes;       /* n_channels x code_len chips (+-1), host      */
+    int64_t code_len;          /* chips per code period                        */
+    double samples_per_chip;   /* ref_split: replica table rate (fs/chip rate) */
+    double code_period_ms;     /* ref_split: code period                       */
+} ugsdr_bank_request;
+
+/* Correlator bank over n_blocks consecutive blocks.  ch holds each channel's
+ * state at block 0 and is advanced in place (carrier phase += 2 pi f/1000 per
+ * block in ref_split mode, TrackingParameters::UpdatePhase,
+ * src/tracking/tracking_parameters.hpp:163-165; FP64 NCO advance in nco mode).
+ * ref_split replaces Tracker::GetEpl (src/tracking/tracker.hpp:119-139) with
+ * TrackingParameters::CorrelateSplit (tracking_parameters.hpp:471-493).
+ * out: n_blocks x n_channels x n_taps complex float (host buffer). */
+int ugsdr_hip_correlate_bank(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* req, ugsdr_channel_state* ch,
+                             float* out_c32);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

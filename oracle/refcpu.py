@@ -1,0 +1,332 @@
+"""ctypes bindings for the oracle library (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, never by the product package.  Each
+wrapper mirrors one function of refcpu.h, which cites the reference file:line
+it restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class AcqParams(C.Structure):
+    _fields_ = [
+        ("n_fft", C.c_int64), ("n_blocks", C.c_int64), ("surface_len", C.c_int64),
+        ("reshape", C.c_int32), ("coherent", C.c_int32),
+        ("fs_in", C.c_double), ("fs_acq", C.c_double), ("intermediate_frequency", C.c_double),
+        ("threshold", C.c_double), ("n_doppler", C.c_int64), ("doppler", C.POINTER(C.c_double)),
+        ("n_prn", C.c_int64), ("replicas", C.POINTER(C.c_float)),
+    ]
+
+
+class AcqResult(C.Structure):
+    _fields_ = [
+        ("level", C.c_double), ("sigma", C.c_double), ("mean", C.c_double), ("stddev", C.c_double),
+        ("doppler", C.c_double), ("code_offset", C.c_double), ("snr", C.c_double),
+        ("runner_up_code", C.c_double), ("runner_up_doppler", C.c_double),
+        ("index", C.c_int64), ("doppler_bin", C.c_int32), ("detected", C.c_int32),
+    ]
+
+
+class Channel(C.Structure):
+    _fields_ = [("carrier_frequency", C.c_double), ("carrier_phase", C.c_double),
+                ("code_rate", C.c_double), ("code_phase", C.c_double)]
+
+
+P = np.ctypeslib.ndpointer
+F32 = P(dtype=np.float32, flags="C_CONTIGUOUS")
+F64 = P(dtype=np.float64, flags="C_CONTIGUOUS")
+I8 = P(dtype=np.int8, flags="C_CONTIGUOUS")
+
+
+def _declare(L):
+    for name in ("refcpu_gps_l1ca", "refcpu_gps_l5i", "refcpu_gps_l5q"):
+        getattr(L, name).argtypes = [C.c_int, I8]
+        getattr(L, name).restype = C.c_int
+    L.refcpu_glonass_of.argtypes = [I8]
+    L.refcpu_boc_expand.argtypes = [I8, C.c_size_t, I8]
+    L.refcpu_repeat_upsample.argtypes = [I8, C.c_size_t, C.c_size_t, F32, C.c_size_t]
+    L.refcpu_upsample_f.argtypes = [F32, C.c_size_t, F32, C.c_size_t]
+    L.refcpu_upsample_hold_c.argtypes = [F32, C.c_size_t, F32, C.c_size_t]
+    L.refcpu_accumulate_c.argtypes = [F32, C.c_size_t, C.c_size_t, F32]
+    L.refcpu_accumulate_c.restype = C.c_size_t
+    L.refcpu_resample_len.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t]
+    L.refcpu_resample_len.restype = C.c_size_t
+    L.refcpu_resample_c.argtypes = [F32, C.c_size_t, C.c_size_t, C.c_size_t, F32]
+    L.refcpu_resample_c.restype = C.c_size_t
+    L.refcpu_adjust_sampling_rate.argtypes = [C.c_double, C.c_double]
+    L.refcpu_adjust_sampling_rate.restype = C.c_double
+    L.refcpu_mix_c.argtypes = [F32, C.c_size_t, C.c_double, C.c_double, C.c_double]
+    L.refcpu_mix_inplace_c.argtypes = [F32, C.c_size_t, C.c_double, C.c_double, C.c_double]
+    L.refcpu_mixer_member_translate_c.argtypes = [F32, C.c_size_t, C.c_double, C.c_double,
+                                                  C.POINTER(C.c_double)]
+    L.refcpu_fft_d.argtypes = [F64, C.c_size_t, C.c_int]
+    L.refcpu_fft_f.argtypes = [F32, C.c_size_t, C.c_int]
+    L.refcpu_mf_prepare_f.argtypes = [F32, C.c_size_t, F32]
+    L.refcpu_mf_process_f.argtypes = [F32, F32, C.c_size_t]
+    L.refcpu_abs_c.argtypes = [F32, C.c_size_t, F32]
+    L.refcpu_reshape_sum_f.argtypes = [F32, C.c_size_t, C.c_size_t, F32]
+    L.refcpu_reshape_sum_f.restype = C.c_int
+    L.refcpu_max_index_f.argtypes = [F32, C.c_size_t, C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+    L.refcpu_mean_std_f.argtypes = [F32, C.c_size_t, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.refcpu_acquire.argtypes = [C.POINTER(AcqParams), F32, C.c_int, C.c_int,
+                                 C.POINTER(AcqResult), C.c_void_p, C.c_void_p]
+    L.refcpu_acquire.restype = C.c_int
+    L.refcpu_correlate.argtypes = [F32, F32, C.c_size_t, F64]
+    L.refcpu_correlate_split.argtypes = [F32, C.c_size_t, F32, C.c_double, C.c_double, F64]
+    L.refcpu_epl.argtypes = [F32, C.c_size_t, C.c_double, C.c_double, C.c_double, F32, C.c_double,
+                             C.c_double, C.c_double, F64, C.c_int, F64]
+    L.refcpu_bank_nco.argtypes = [F32, C.c_int64, C.c_int64, C.c_double, C.POINTER(Channel), C.c_int,
+                                  I8, C.c_int64, F64, C.c_int, C.c_int, F64]
+
+
+# ------------------------------------------------------------------ helpers --
+def c2f(x: np.ndarray) -> np.ndarray:
+    """complex array -> interleaved float32"""
+    return np.ascontiguousarray(np.asarray(x, np.complex64)).view(np.float32).copy()
+
+
+def f2c(x: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(x, np.float32).view(np.complex64).copy()
+
+
+def gps_l1ca(sv: int) -> np.ndarray:
+    out = np.zeros(1023, np.int8)
+    if lib().refcpu_gps_l1ca(sv, out):
+        raise RuntimeError("SV number for the GPS C/A code exceed available values")
+    return out
+
+
+def gps_l5i(sv: int) -> np.ndarray:
+    out = np.zeros(10230, np.int8)
+    if lib().refcpu_gps_l5i(sv, out):
+        raise RuntimeError("SV number for the GPS L5I code exceed available values")
+    return out
+
+
+def gps_l5q(sv: int) -> np.ndarray:
+    out = np.zeros(10230, np.int8)
+    if lib().refcpu_gps_l5q(sv, out):
+        raise RuntimeError("SV number for the GPS L5Q code exceed available values")
+    return out
+
+
+def glonass_of() -> np.ndarray:
+    out = np.zeros(511, np.int8)
+    lib().refcpu_glonass_of(out)
+    return out
+
+
+def boc_expand(chips: np.ndarray) -> np.ndarray:
+    chips = np.ascontiguousarray(chips, np.int8)
+    out = np.zeros(2 * len(chips), np.int8)
+    lib().refcpu_boc_expand(chips, len(chips), out)
+    return out
+
+
+def repeat_upsample(code: np.ndarray, repeats: int, samples: int) -> np.ndarray:
+    code = np.ascontiguousarray(code, np.int8)
+    out = np.zeros(samples, np.float32)
+    lib().refcpu_repeat_upsample(code, len(code), repeats, out, samples)
+    return out
+
+
+def upsample(src: np.ndarray, samples: int) -> np.ndarray:
+    src = np.ascontiguousarray(src, np.float32)
+    out = np.zeros(samples, np.float32)
+    lib().refcpu_upsample_f(src, len(src), out, samples)
+    return out
+
+
+def upsample_hold(src: np.ndarray, samples: int) -> np.ndarray:
+    s = c2f(src)
+    out = np.zeros(2 * samples, np.float32)
+    lib().refcpu_upsample_hold_c(s, len(src), out, samples)
+    return f2c(out)
+
+
+def accumulate(src: np.ndarray, ratio: int) -> np.ndarray:
+    s = c2f(src)
+    out = np.zeros(2 * (len(src) // ratio + 1), np.float32)
+    n = lib().refcpu_accumulate_c(s, len(src), ratio, out)
+    return f2c(out[: 2 * n])
+
+
+def resample(src: np.ndarray, new_fs: int, old_fs: int) -> np.ndarray:
+    s = c2f(src)
+    n_out = lib().refcpu_resample_len(len(src), new_fs, old_fs)
+    out = np.zeros(2 * n_out, np.float32)
+    n = lib().refcpu_resample_c(s, len(src), new_fs, old_fs, out)
+    return f2c(out[: 2 * n])
+
+
+def adjust_sampling_rate(fs: float, target: float) -> float:
+    return lib().refcpu_adjust_sampling_rate(fs, target)
+
+
+def mix(x: np.ndarray, fs: float, f: float, phase: float = 0.0) -> np.ndarray:
+    b = c2f(x)
+    lib().refcpu_mix_c(b, len(x), fs, f, phase)
+    return f2c(b)
+
+
+def mix_inplace(x: np.ndarray, fs: float, f: float, phase: float = 0.0) -> np.ndarray:
+    b = c2f(x)
+    lib().refcpu_mix_inplace_c(b, len(x), fs, f, phase)
+    return f2c(b)
+
+
+def mixer_member_translate(x: np.ndarray, fs: float, f: float, phase: float):
+    b = c2f(x)
+    ph = C.c_double(phase)
+    lib().refcpu_mixer_member_translate_c(b, len(x), fs, f, C.byref(ph))
+    return f2c(b), ph.value
+
+
+def fft(x: np.ndarray, inverse: bool = False, precision: int = 64) -> np.ndarray:
+    if precision == 64:
+        b = np.ascontiguousarray(np.asarray(x, np.complex128)).view(np.float64).copy()
+        lib().refcpu_fft_d(b, len(x), int(inverse))
+        return b.view(np.complex128)
+    b = c2f(x)
+    lib().refcpu_fft_f(b, len(x), int(inverse))
+    return f2c(b)
+
+
+def mf_prepare(code: np.ndarray) -> np.ndarray:
+    code = np.ascontiguousarray(code, np.float32)
+    out = np.zeros(2 * len(code), np.float32)
+    lib().refcpu_mf_prepare_f(code, len(code), out)
+    return f2c(out)
+
+
+def mf_process(x: np.ndarray, spec: np.ndarray) -> np.ndarray:
+    b = c2f(x)
+    lib().refcpu_mf_process_f(b, c2f(spec), len(x))
+    return f2c(b)
+
+
+def abs_(x: np.ndarray) -> np.ndarray:
+    out = np.zeros(len(x), np.float32)
+    lib().refcpu_abs_c(c2f(x), len(x), out)
+    return out
+
+
+def reshape_sum(x: np.ndarray, block: int) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(block, np.float32)
+    if lib().refcpu_reshape_sum_f(x, len(x), block, out):
+        raise RuntimeError("Signal size is not divided by block size")
+    return out
+
+
+def max_index(x: np.ndarray):
+    x = np.ascontiguousarray(x, np.float32)
+    v, i = C.c_float(), C.c_int64()
+    lib().refcpu_max_index_f(x, len(x), C.byref(v), C.byref(i))
+    return v.value, i.value
+
+
+def mean_std(x: np.ndarray):
+    x = np.ascontiguousarray(x, np.float32)
+    m, s = C.c_double(), C.c_double()
+    lib().refcpu_mean_std_f(x, len(x), C.byref(m), C.byref(s))
+    return m.value, s.value
+
+
+RESULT_FIELDS = [f for f, _ in AcqResult._fields_]
+
+
+def acquire(signal: np.ndarray, replicas: np.ndarray, *, n_fft: int, n_blocks: int, surface_len: int,
+            fs_in: float, fs_acq: float, doppler, reshape: bool = True, coherent: bool = True,
+            intermediate_frequency: float = 0.0, threshold: float = 3.3, precision: int = 64,
+            threads: int | None = None, want_surfaces: bool = False, want_peaks: bool = False):
+    """FastSearchEngineBase::ProcessBpsk generalised to K blocks (refcpu.h).
+
+    signal: complex, n_blocks*n_fft samples at fs_acq.  replicas: (n_prn, n_fft)
+    real.  Returns (list of dict per PRN, surfaces or None, peaks or None)."""
+    sig = c2f(np.asarray(signal)[: n_blocks * n_fft])
+    reps = np.ascontiguousarray(replicas, np.float32)
+    n_prn = reps.shape[0]
+    dop = np.ascontiguousarray(doppler, np.float64)
+    prm = AcqParams(n_fft, n_blocks, surface_len, int(reshape), int(coherent), fs_in, fs_acq,
+                    intermediate_frequency, threshold, len(dop),
+                    dop.ctypes.data_as(C.POINTER(C.c_double)), n_prn,
+                    reps.ctypes.data_as(C.POINTER(C.c_float)))
+    out = (AcqResult * n_prn)()
+    surf = np.zeros((n_prn, len(dop), surface_len), np.float32) if want_surfaces else None
+    peaks = np.zeros((n_prn, surface_len), np.float32) if want_peaks else None
+    threads = threads or min(os.cpu_count() or 1, 16)
+    rc = lib().refcpu_acquire(C.byref(prm), sig, precision, threads, out,
+                              surf.ctypes.data if surf is not None else None,
+                              peaks.ctypes.data if peaks is not None else None)
+    if rc:
+        raise RuntimeError(f"refcpu_acquire failed ({rc})")
+    res = [{f: getattr(out[i], f) for f in RESULT_FIELDS} for i in range(n_prn)]
+    return res, surf, peaks
+
+
+def correlate(signal: np.ndarray, code: np.ndarray) -> complex:
+    if len(signal) != len(code):
+        raise RuntimeError("Size mismatch")
+    out = np.zeros(2)
+    lib().refcpu_correlate(c2f(signal), np.ascontiguousarray(code, np.float32), len(code), out)
+    return complex(out[0], out[1])
+
+
+def correlate_split(signal: np.ndarray, spm: int, code3: np.ndarray, code_period_ms: float,
+                    code_phase: float) -> complex:
+    out = np.zeros(2)
+    lib().refcpu_correlate_split(c2f(signal), spm, np.ascontiguousarray(code3, np.float32),
+                                 code_period_ms, code_phase, out)
+    return complex(out[0], out[1])
+
+
+def epl(x: np.ndarray, fs: float, carrier_frequency: float, carrier_phase: float, code3: np.ndarray,
+        code_period_ms: float, code_phase: float, samples_per_chip: float, taps) -> np.ndarray:
+    b = c2f(x)
+    taps = np.ascontiguousarray(taps, np.float64)
+    out = np.zeros(2 * len(taps))
+    lib().refcpu_epl(b, len(x), fs, carrier_frequency, carrier_phase,
+                     np.ascontiguousarray(code3, np.float32), code_period_ms, code_phase,
+                     samples_per_chip, taps, len(taps), out)
+    return out.view(np.complex128)
+
+
+def bank_nco(x: np.ndarray, spm: int, n_blocks: int, fs: float, channels, codes: np.ndarray, taps,
+             threads: int | None = None) -> np.ndarray:
+    """channels: list of (carrier_frequency, carrier_phase_cycles, code_rate, code_phase_chips)."""
+    n_ch = len(channels)
+    ch = (Channel * n_ch)(*[Channel(*c) for c in channels])
+    codes = np.ascontiguousarray(codes, np.int8)
+    taps = np.ascontiguousarray(taps, np.float64)
+    out = np.zeros(2 * n_blocks * n_ch * len(taps))
+    xb = np.ascontiguousarray(x, np.float32) if x.dtype == np.float32 else c2f(x)
+    lib().refcpu_bank_nco(xb, spm, n_blocks, fs, ch, n_ch, codes, codes.shape[1], taps, len(taps),
+                          threads or min(os.cpu_count() or 1, 16), out)
+    return out.view(np.complex128).reshape(n_blocks, n_ch, len(taps))

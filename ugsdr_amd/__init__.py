@@ -1,0 +1,411 @@
+"""ugsdr_amd: MI355X-native acquisition hot path of the ugsdr GNSS receiver.
+
+Python bindings over the C ABI of libugsdr_hip.so (include/ugsdr_hip.h).  The
+native library is required: importing works without it, but every GPU entry
+point raises if it cannot be loaded — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB_PATH = os.path.join(HERE, "libugsdr_hip.so")
+
+FMT_CI8, FMT_CI16, FMT_CF32, FMT_R8 = 1, 2, 3, 4
+PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
+BANK_REF_SPLIT, BANK_NCO = 0, 1
+
+ERRORS = {-1: "ARG", -2: "HIP", -3: "SIZE", -4: "UNSUPPORTED", -5: "NOMEM", -6: "STATE"}
+
+
+class UgsdrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class AcqRequest(C.Structure):
+    _fields_ = [
+        ("iq", C.c_void_p), ("fmt", C.c_int32), ("iq_on_device", C.c_int32), ("n_samples", C.c_int64),
+        ("fs_in", C.c_double), ("fs_acq", C.c_double), ("mix_frequency", C.c_double), ("mix_phase", C.c_double),
+        ("intermediate_frequency", C.c_double), ("n_fft", C.c_int64), ("n_blocks", C.c_int64),
+        ("surface_len", C.c_int64), ("reshape", C.c_int32), ("coherent", C.c_int32),
+        ("doppler", C.POINTER(C.c_double)), ("n_doppler", C.c_int64),
+        ("replica_keys", C.POINTER(C.c_uint64)), ("n_prn", C.c_int64), ("threshold", C.c_double),
+        ("precision", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class AcqResult(C.Structure):
+    _fields_ = [
+        ("level", C.c_double), ("sigma", C.c_double), ("mean", C.c_double), ("stddev", C.c_double),
+        ("doppler", C.c_double), ("code_offset", C.c_double), ("snr", C.c_double),
+        ("runner_up_code", C.c_double), ("runner_up_doppler", C.c_double),
+        ("index", C.c_int64), ("doppler_bin", C.c_int32), ("detected", C.c_int32),
+    ]
+
+
+class ChannelState(C.Structure):
+    _fields_ = [("carrier_frequency", C.c_double), ("carrier_phase", C.c_double),
+                ("code_rate", C.c_double), ("code_phase", C.c_double)]
+
+
+class BankRequest(C.Structure):
+    _fields_ = [
+        ("iq", C.c_void_p), ("fmt", C.c_int32), ("iq_on_device", C.c_int32), ("spm", C.c_int64),
+        ("n_blocks", C.c_int64), ("fs", C.c_double), ("mode", C.c_int32), ("n_channels", C.c_int32),
+        ("n_taps", C.c_int32), ("reserved", C.c_int32), ("taps_chips", C.POINTER(C.c_double)),
+        ("codes", C.POINTER(C.c_int8)), ("code_len", C.c_int64), ("samples_per_chip", C.c_double),
+        ("code_period_ms", C.c_double),
+    ]
+
+
+RESULT_FIELDS = [f for f, _ in AcqResult._fields_]
+
+# every exported symbol with its ctypes signature (checked against include/ugsdr_hip.h by the tests)
+VP, I64, I32, D, F = C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_float
+SIGNATURES = {
+    "ugsdr_hip_last_error": ([], C.c_char_p),
+    "ugsdr_hip_version": ([], C.c_char_p),
+    "ugsdr_hip_ctx_create": ([C.c_int, C.POINTER(VP)], C.c_int),
+    "ugsdr_hip_ctx_destroy": ([VP], C.c_int),
+    "ugsdr_hip_ctx_stream": ([VP, C.POINTER(VP)], C.c_int),
+    "ugsdr_hip_ctx_device_bytes": ([VP], I64),
+    "ugsdr_code_gps_l1ca": ([C.c_int, VP], C.c_int),
+    "ugsdr_code_gps_l5i": ([C.c_int, VP], C.c_int),
+    "ugsdr_code_gps_l5q": ([C.c_int, VP], C.c_int),
+    "ugsdr_code_glonass_of": ([VP], C.c_int),
+    "ugsdr_code_boc": ([VP, I64, VP], C.c_int),
+    "ugsdr_adjust_sampling_rate": ([D, D], D),
+    "ugsdr_hip_replica_put": ([VP, VP, I64, I64, I64, C.c_int, C.POINTER(C.c_uint64)], C.c_int),
+    "ugsdr_hip_replica_put_samples": ([VP, VP, I64, C.c_int, C.POINTER(C.c_uint64)], C.c_int),
+    "ugsdr_hip_replica_drop": ([VP, C.c_uint64], C.c_int),
+    "ugsdr_hip_replica_get": ([VP, C.c_uint64, VP, I64], C.c_int),
+    "ugsdr_hip_acquire": ([VP, C.POINTER(AcqRequest), C.POINTER(AcqResult), VP, VP], C.c_int),
+    "ugsdr_hip_mix": ([VP, VP, VP, I64, D, D, D], C.c_int),
+    "ugsdr_hip_mix_inplace": ([VP, VP, I64, D, D, D], C.c_int),
+    "ugsdr_hip_dft": ([VP, VP, VP, I64, C.c_int], C.c_int),
+    "ugsdr_hip_mf_prepare": ([VP, VP, VP, I64], C.c_int),
+    "ugsdr_hip_mf_filter": ([VP, VP, VP, VP, I64], C.c_int),
+    "ugsdr_hip_mf_filter_code": ([VP, VP, VP, VP, I64], C.c_int),
+    "ugsdr_hip_abs": ([VP, VP, VP, I64], C.c_int),
+    "ugsdr_hip_reshape_sum": ([VP, VP, VP, I64, I64, C.c_int], C.c_int),
+    "ugsdr_hip_max_index": ([VP, VP, I64, C.POINTER(F), C.POINTER(I64)], C.c_int),
+    "ugsdr_hip_mean_std": ([VP, VP, I64, C.POINTER(F), C.POINTER(F)], C.c_int),
+    "ugsdr_resample_len": ([I64, I64, I64], I64),
+    "ugsdr_hip_resample": ([VP, VP, I64, I64, I64, VP], C.c_int),
+    "ugsdr_hip_upsample": ([VP, VP, I64, VP, I64], C.c_int),
+    "ugsdr_hip_correlate": ([VP, VP, I64, VP, I64, VP], C.c_int),
+    "ugsdr_hip_correlate_bank": ([VP, C.POINTER(BankRequest), C.POINTER(ChannelState), VP], C.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libugsdr_hip.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(HIP_LIB_PATH):
+            raise RuntimeError(f"{HIP_LIB_PATH} not built: run python -m ugsdr_amd.build")
+        L = C.CDLL(HIP_LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise UgsdrError(rc, lib().ugsdr_hip_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ------------------------------------------------------------------ host codes
+def gps_l1ca(sv: int) -> np.ndarray:
+    out = np.zeros(1023, np.int8)
+    _check(lib().ugsdr_code_gps_l1ca(sv, _ptr(out)))
+    return out
+
+
+def gps_l5i(sv: int) -> np.ndarray:
+    out = np.zeros(10230, np.int8)
+    _check(lib().ugsdr_code_gps_l5i(sv, _ptr(out)))
+    return out
+
+
+def gps_l5q(sv: int) -> np.ndarray:
+    out = np.zeros(10230, np.int8)
+    _check(lib().ugsdr_code_gps_l5q(sv, _ptr(out)))
+    return out
+
+
+def glonass_of() -> np.ndarray:
+    out = np.zeros(511, np.int8)
+    _check(lib().ugsdr_code_glonass_of(_ptr(out)))
+    return out
+
+
+def boc(chips: np.ndarray) -> np.ndarray:
+    chips = np.ascontiguousarray(chips, np.int8)
+    out = np.zeros(2 * len(chips), np.int8)
+    _check(lib().ugsdr_code_boc(_ptr(chips), len(chips), _ptr(out)))
+    return out
+
+
+def adjust_sampling_rate(fs: float, target: float) -> float:
+    return lib().ugsdr_adjust_sampling_rate(fs, target)
+
+
+def resample_len(n: int, new_fs: int, old_fs: int) -> int:
+    return lib().ugsdr_resample_len(n, new_fs, old_fs)
+
+
+def doppler_bins(doppler_range: float, doppler_step: float, center: float = 0.0) -> np.ndarray:
+    """Doppler loop of FastSearchEngineBase::ProcessBpsk (src/acquisition/fse.hpp:250-252):
+    f = -range; f <= range; f += step (accumulated in FP64)."""
+    out = []
+    f = -doppler_range
+    while f <= doppler_range:
+        out.append(f + center)
+        f += doppler_step
+    return np.array(out, np.float64)
+
+
+def _c64(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, np.complex64))
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """One HIP device context (stream + device-resident caches)."""
+
+    def __init__(self, device: int = 0):
+        self._p = C.c_void_p()
+        _check(lib().ugsdr_hip_ctx_create(device, C.byref(self._p)))
+        self.device = device
+
+    def close(self):
+        if self._p:
+            lib().ugsdr_hip_ctx_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._p
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        _check(lib().ugsdr_hip_ctx_stream(self._p, C.byref(s)))
+        return s.value or 0
+
+    def device_bytes(self) -> int:
+        return lib().ugsdr_hip_ctx_device_bytes(self._p)
+
+    # ---- replica cache
+    def replica_put(self, chips: np.ndarray, repeats: int, n_fft: int, precision: int = PREC_FP32) -> int:
+        chips = np.ascontiguousarray(chips, np.int8)
+        key = C.c_uint64()
+        _check(lib().ugsdr_hip_replica_put(self._p, _ptr(chips), len(chips), repeats, n_fft, precision,
+                                           C.byref(key)))
+        return key.value
+
+    def replica_put_samples(self, samples: np.ndarray, precision: int = PREC_FP32) -> int:
+        samples = np.ascontiguousarray(samples, np.float32)
+        key = C.c_uint64()
+        _check(lib().ugsdr_hip_replica_put_samples(self._p, _ptr(samples), len(samples), precision,
+                                                   C.byref(key)))
+        return key.value
+
+    def replica_drop(self, key: int):
+        _check(lib().ugsdr_hip_replica_drop(self._p, key))
+
+    def replica_get(self, key: int, n_fft: int) -> np.ndarray:
+        out = np.zeros(n_fft, np.complex64)
+        _check(lib().ugsdr_hip_replica_get(self._p, key, _ptr(out), n_fft))
+        return out
+
+    # ---- fused acquisition
+    def make_request(self, iq, fmt: int, *, fs_in: float, fs_acq: float, n_fft: int, n_blocks: int,
+                     surface_len: int, doppler, keys, reshape: bool = True, coherent: bool = True,
+                     mix_frequency: float = 0.0, mix_phase: float = 0.0, intermediate_frequency: float = 0.0,
+                     threshold: float = 3.3, precision: int = PREC_FP32, n_samples: int | None = None,
+                     iq_device_ptr: int | None = None):
+        dop = np.ascontiguousarray(doppler, np.float64)
+        keys = np.ascontiguousarray(keys, np.uint64)
+        req = AcqRequest()
+        if iq_device_ptr is not None:
+            req.iq = iq_device_ptr
+            req.iq_on_device = 1
+            req.n_samples = int(n_samples)
+            hold = None
+        else:
+            arr = np.ascontiguousarray(iq)
+            req.iq = arr.ctypes.data
+            req.iq_on_device = 0
+            per = {FMT_CI8: 2, FMT_CI16: 2, FMT_CF32: 2, FMT_R8: 1}[fmt]
+            req.n_samples = int(n_samples if n_samples is not None else arr.size // per
+                                if arr.dtype != np.complex64 else arr.size)
+            hold = arr
+        req.fmt = fmt
+        req.fs_in, req.fs_acq = fs_in, fs_acq
+        req.mix_frequency, req.mix_phase = mix_frequency, mix_phase
+        req.intermediate_frequency = intermediate_frequency
+        req.n_fft, req.n_blocks, req.surface_len = n_fft, n_blocks, surface_len
+        req.reshape, req.coherent = int(reshape), int(coherent)
+        req.doppler = dop.ctypes.data_as(C.POINTER(C.c_double))
+        req.n_doppler = len(dop)
+        req.replica_keys = keys.ctypes.data_as(C.POINTER(C.c_uint64))
+        req.n_prn = len(keys)
+        req.threshold = threshold
+        req.precision = precision
+        return req, (hold, dop, keys)
+
+    def acquire_req(self, req, want_surfaces: bool = False, want_peaks: bool = False):
+        P, D, L = req.n_prn, req.n_doppler, req.surface_len
+        out = (AcqResult * P)()
+        surf = np.zeros((P, D, L), np.float32) if want_surfaces else None
+        peaks = np.zeros((P, L), np.float32) if want_peaks else None
+        _check(lib().ugsdr_hip_acquire(self._p, C.byref(req), out,
+                                       _ptr(surf) if surf is not None else None,
+                                       _ptr(peaks) if peaks is not None else None))
+        res = [{f: getattr(out[i], f) for f in RESULT_FIELDS} for i in range(P)]
+        return res, surf, peaks
+
+    def acquire(self, iq, fmt: int, want_surfaces: bool = False, want_peaks: bool = False, **kw):
+        req, hold = self.make_request(iq, fmt, **kw)
+        return self.acquire_req(req, want_surfaces, want_peaks)
+
+    # ---- per-operator drop-ins
+    def mix(self, x, fs: float, f: float, phase: float = 0.0) -> np.ndarray:
+        x = _c64(x)
+        out = np.empty_like(x)
+        _check(lib().ugsdr_hip_mix(self._p, _ptr(x), _ptr(out), len(x), fs, f, phase))
+        return out
+
+    def mix_inplace(self, x, fs: float, f: float, phase: float = 0.0) -> np.ndarray:
+        x = _c64(x).copy()
+        _check(lib().ugsdr_hip_mix_inplace(self._p, _ptr(x), len(x), fs, f, phase))
+        return x
+
+    def dft(self, x, inverse: bool = False) -> np.ndarray:
+        x = _c64(x)
+        out = np.empty_like(x)
+        _check(lib().ugsdr_hip_dft(self._p, _ptr(x), _ptr(out), len(x), int(inverse)))
+        return out
+
+    def mf_prepare(self, code) -> np.ndarray:
+        code = np.ascontiguousarray(code, np.float32)
+        out = np.zeros(len(code), np.complex64)
+        _check(lib().ugsdr_hip_mf_prepare(self._p, _ptr(code), _ptr(out), len(code)))
+        return out
+
+    def mf_filter(self, x, spec) -> np.ndarray:
+        x, spec = _c64(x), _c64(spec)
+        out = np.empty_like(x)
+        _check(lib().ugsdr_hip_mf_filter(self._p, _ptr(x), _ptr(spec), _ptr(out), len(x)))
+        return out
+
+    def mf_filter_code(self, x, code) -> np.ndarray:
+        x = _c64(x)
+        code = np.ascontiguousarray(code, np.float32)
+        out = np.empty_like(x)
+        _check(lib().ugsdr_hip_mf_filter_code(self._p, _ptr(x), _ptr(code), _ptr(out), len(x)))
+        return out
+
+    def abs(self, x) -> np.ndarray:
+        x = _c64(x)
+        out = np.zeros(len(x), np.float32)
+        _check(lib().ugsdr_hip_abs(self._p, _ptr(x), _ptr(out), len(x)))
+        return out
+
+    def reshape_sum(self, x, block: int) -> np.ndarray:
+        if np.iscomplexobj(x):
+            x = _c64(x)
+            out = np.zeros(block, np.complex64)
+            _check(lib().ugsdr_hip_reshape_sum(self._p, _ptr(x), _ptr(out), len(x), block, 1))
+        else:
+            x = np.ascontiguousarray(x, np.float32)
+            out = np.zeros(block, np.float32)
+            _check(lib().ugsdr_hip_reshape_sum(self._p, _ptr(x), _ptr(out), len(x), block, 0))
+        return out
+
+    def max_index(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        v, i = C.c_float(), C.c_int64()
+        _check(lib().ugsdr_hip_max_index(self._p, _ptr(x), len(x), C.byref(v), C.byref(i)))
+        return v.value, i.value
+
+    def mean_std(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        m, s = C.c_float(), C.c_float()
+        _check(lib().ugsdr_hip_mean_std(self._p, _ptr(x), len(x), C.byref(m), C.byref(s)))
+        return m.value, s.value
+
+    def resample(self, x, new_fs: int, old_fs: int) -> np.ndarray:
+        x = _c64(x)
+        out = np.zeros(resample_len(len(x), new_fs, old_fs), np.complex64)
+        _check(lib().ugsdr_hip_resample(self._p, _ptr(x), len(x), new_fs, old_fs, _ptr(out)))
+        return out
+
+    def upsample(self, x, samples: int) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(samples, np.float32)
+        _check(lib().ugsdr_hip_upsample(self._p, _ptr(x), len(x), _ptr(out), samples))
+        return out
+
+    def correlate(self, signal, code) -> complex:
+        s = _c64(signal)
+        c = np.ascontiguousarray(code, np.float32)
+        out = np.zeros(1, np.complex64)
+        _check(lib().ugsdr_hip_correlate(self._p, _ptr(s), len(s), _ptr(c), len(c), _ptr(out)))
+        return complex(out[0])
+
+    # ---- correlator bank
+    def correlate_bank(self, iq, fmt: int, *, spm: int, n_blocks: int, fs: float, channels, codes, taps_chips,
+                       mode: int = BANK_NCO, samples_per_chip: float = 0.0, code_period_ms: float = 1.0,
+                       iq_device_ptr: int | None = None):
+        """channels: list of (carrier_frequency, carrier_phase, code_rate, code_phase); returns
+        (out[n_blocks, n_ch, n_taps] complex64, advanced channel states)."""
+        codes = np.ascontiguousarray(codes, np.int8)
+        taps = np.ascontiguousarray(taps_chips, np.float64)
+        n_ch = len(channels)
+        ch = (ChannelState * n_ch)(*[ChannelState(*c) for c in channels])
+        req = BankRequest()
+        hold = None
+        if iq_device_ptr is not None:
+            req.iq, req.iq_on_device = iq_device_ptr, 1
+        else:
+            hold = np.ascontiguousarray(iq)
+            req.iq, req.iq_on_device = hold.ctypes.data, 0
+        req.fmt, req.spm, req.n_blocks, req.fs, req.mode = fmt, spm, n_blocks, fs, mode
+        req.n_channels, req.n_taps = n_ch, len(taps)
+        req.taps_chips = taps.ctypes.data_as(C.POINTER(C.c_double))
+        req.codes = codes.ctypes.data_as(C.POINTER(C.c_int8))
+        req.code_len = codes.shape[1]
+        req.samples_per_chip, req.code_period_ms = samples_per_chip, code_period_ms
+        out = np.zeros((n_blocks, n_ch, len(taps)), np.complex64)
+        _check(lib().ugsdr_hip_correlate_bank(self._p, C.byref(req), ch, _ptr(out)))
+        del hold
+        states = [(c.carrier_frequency, c.carrier_phase, c.code_rate, c.code_phase) for c in ch]
+        return out, states

@@ -1,0 +1,318 @@
+// Tracking correlator bank: many channels x taps over consecutive 1 ms blocks.
+//
+// NCO mode: per sample, FP64 carrier phase and code phase, FP32 sincos of the
+// reduced carrier phase, replica chip looked up from the channel's code in
+// LDS, FP32 I/Q accumulators, workgroup reduction per tap.
+// Ref-split mode: Tracker::GetEpl semantics (in-place Mixer::Translate, then
+// TrackingParameters::CorrelateSplit against a 3-period upsampled replica).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "ctx.hpp"
+#include "dev_fft.hpp"
+#include "internal.hpp"
+
+using namespace ugsdr_dev;
+
+namespace ugsdr_hip {
+
+constexpr int kBankT = 256;
+constexpr int kMaxTaps = 8;
+
+struct SplitTap {
+    int first_len;
+    int first_phase;
+    int second_phase;
+    float pad;
+    double rel;  // fmod(phase, spm) / spm
+};
+
+struct BankArgs {
+    const void* iq;
+    int fmt;
+    int64_t spm;
+    int n_blocks;
+    int n_ch;
+    int n_taps;
+    double fs;
+    // nco mode
+    const double* ch;            // n_ch x 4: f, theta0 (cycles), rate, tau0 (chips)
+    const signed char* codes;    // n_ch x code_len
+    int64_t code_len;
+    double taps[kMaxTaps];       // chips
+    // ref_split mode
+    const float* code3;          // n_ch x code3_len
+    int64_t code3_len;
+    const SplitTap* split;       // n_ch x n_taps
+    const double* carrier;       // n_ch x 2: f_carr, phase0 (rad)
+    float2* out;                 // n_blocks x n_ch x n_taps
+};
+
+__device__ __forceinline__ float2 bank_load(const void* p, int fmt, int64_t i) {
+    if (fmt == UGSDR_FMT_CI16) {
+        const short2 v = static_cast<const short2*>(p)[i];
+        return make_float2((float)v.x, (float)v.y);
+    }
+    if (fmt == UGSDR_FMT_CI8) {
+        const char2 v = static_cast<const char2*>(p)[i];
+        return make_float2((float)v.x, (float)v.y);
+    }
+    return static_cast<const float2*>(p)[i];
+}
+
+template <int NT>
+__device__ __forceinline__ void bank_reduce_store(float2 (&acc)[NT], int n_taps, float2* out, float2* red) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        if (t >= n_taps) break;
+        acc[t].x = wave_sum(acc[t].x);
+        acc[t].y = wave_sum(acc[t].y);
+        if (l == 0) red[w * NT + t] = acc[t];
+    }
+    __syncthreads();
+    if (threadIdx.x < n_taps) {
+        float2 s = make_float2(0.f, 0.f);
+        for (int i = 0; i < kBankT / 64; ++i) s = cadd(s, red[i * NT + threadIdx.x]);
+        out[threadIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
+    const int b = blockIdx.x, c = blockIdx.y;
+    extern __shared__ __align__(16) unsigned char smem[];
+    signed char* code = reinterpret_cast<signed char*>(smem);
+    float2* red = reinterpret_cast<float2*>(smem + ((a.code_len + 15) / 16) * 16);
+    for (int64_t i = threadIdx.x; i < a.code_len; i += kBankT) code[i] = a.codes[(int64_t)c * a.code_len + i];
+    const double f = a.ch[4 * c], th0 = a.ch[4 * c + 1], rate = a.ch[4 * c + 2], tau0 = a.ch[4 * c + 3];
+    const double L = (double)a.code_len;
+    const double n0 = (double)b * (double)a.spm;
+    double theta = fma(f, n0 / a.fs, th0);
+    theta -= floor(theta);
+    double tau = fma(rate, n0 / a.fs, tau0);
+    tau = fmod(tau, L);
+    const double df = f / a.fs, dr = rate / a.fs;
+    float2 acc[kMaxTaps];
+#pragma unroll
+    for (int t = 0; t < kMaxTaps; ++t) acc[t] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const void* blk = a.fmt == UGSDR_FMT_CI16 ? (const void*)(static_cast<const short2*>(a.iq) + b * a.spm)
+                      : a.fmt == UGSDR_FMT_CI8 ? (const void*)(static_cast<const char2*>(a.iq) + b * a.spm)
+                                               : (const void*)(static_cast<const float2*>(a.iq) + b * a.spm);
+    for (int64_t n = threadIdx.x; n < a.spm; n += kBankT) {
+        const float2 x = bank_load(blk, a.fmt, n);
+        double ph = fma((double)n, df, theta);
+        ph -= rint(ph);
+        float s, co;
+        sincosf((float)(2.0 * kPi * ph), &s, &co);
+        // x * exp(-j 2 pi ph)
+        const float2 w = make_float2(fmaf(x.x, co, x.y * s), fmaf(x.y, co, -x.x * s));
+        const double u = fma((double)n, dr, tau);
+#pragma unroll
+        for (int t = 0; t < kMaxTaps; ++t) {
+            if (t >= a.n_taps) break;
+            int64_t idx = (int64_t)floor(u + a.taps[t]);
+            idx %= a.code_len;
+            idx += idx < 0 ? a.code_len : 0;
+            const float cv = (float)code[idx];
+            acc[t].x = fmaf(w.x, cv, acc[t].x);
+            acc[t].y = fmaf(w.y, cv, acc[t].y);
+        }
+    }
+    bank_reduce_store<kMaxTaps>(acc, a.n_taps, a.out + ((int64_t)b * a.n_ch + c) * a.n_taps, red);
+}
+
+__global__ __launch_bounds__(kBankT) void bank_split_kernel(BankArgs a) {
+    const int b = blockIdx.x, c = blockIdx.y;
+    __shared__ float2 red[(kBankT / 64) * 2 * kMaxTaps];
+    __shared__ float2 part[2 * kMaxTaps];
+    const double fc = a.carrier[2 * c];
+    // phase after b UpdatePhase() calls (tracking_parameters.hpp:163-165)
+    const double ph_b = a.carrier[2 * c + 1] + (double)b * (2.0 * kPi * fc / 1000.0);
+    // in-place Translate(x, fs, -fc, -ph_b): FixPhase, no-op for f == 0, fold f < 0
+    double phase = fmod(-ph_b, 2.0 * kPi);
+    if (phase < 0) phase += 2.0 * kPi;
+    double f = -fc;
+    const bool mix = fabs(f) != 0.0;
+    if (f < 0) f = a.fs + f;
+    const double df = f / a.fs, ph_turns = phase / (2.0 * kPi);
+    const float* code3 = a.code3 + (int64_t)c * a.code3_len;
+    const SplitTap* taps = a.split + (int64_t)c * a.n_taps;
+    float2 acc[2 * kMaxTaps];
+#pragma unroll
+    for (int t = 0; t < 2 * kMaxTaps; ++t) acc[t] = make_float2(0.f, 0.f);
+    const void* blk = a.fmt == UGSDR_FMT_CI16 ? (const void*)(static_cast<const short2*>(a.iq) + b * a.spm)
+                      : a.fmt == UGSDR_FMT_CI8 ? (const void*)(static_cast<const char2*>(a.iq) + b * a.spm)
+                                               : (const void*)(static_cast<const float2*>(a.iq) + b * a.spm);
+    for (int64_t n = threadIdx.x; n < a.spm; n += kBankT) {
+        float2 x = bank_load(blk, a.fmt, n);
+        if (mix) {
+            double t = fma((double)n, df, ph_turns);
+            t -= rint(t);
+            float s, co;
+            sincosf((float)(2.0 * kPi * t), &s, &co);
+            x = cmul(x, make_float2(co, s));
+        }
+#pragma unroll
+        for (int t = 0; t < kMaxTaps; ++t) {
+            if (t >= a.n_taps) break;
+            const SplitTap tp = taps[t];
+            const bool first = n < tp.first_len;
+            const int64_t ci = first ? tp.first_phase + n : tp.second_phase + (n - tp.first_len);
+            const float cv = code3[ci];
+            float2& ac = acc[2 * t + (first ? 0 : 1)];
+            ac.x = fmaf(x.x, cv, ac.x);
+            ac.y = fmaf(x.y, cv, ac.y);
+        }
+    }
+    bank_reduce_store<2 * kMaxTaps>(acc, 2 * a.n_taps, part, red);
+    __syncthreads();
+    if (threadIdx.x < a.n_taps) {
+        // AddWithPhase (tracking_parameters.hpp:89-98)
+        const SplitTap tp = taps[threadIdx.x];
+        float2 lhs = part[2 * threadIdx.x], rhs = part[2 * threadIdx.x + 1];
+        if (tp.rel < 0.5) {
+            const float2 tmp = lhs;
+            lhs = rhs;
+            rhs = tmp;
+        }
+        const double wl = (double)lhs.x / tp.rel, wr = (double)rhs.x / (1.0 - tp.rel);
+        const float2 r = fabs(wl + wr) > fabs(wl) ? cadd(lhs, rhs) : csub(lhs, rhs);
+        a.out[((int64_t)b * a.n_ch + c) * a.n_taps + threadIdx.x] = r;
+    }
+}
+
+// CorrelateSplit index arithmetic (tracking_parameters.hpp:475-486)
+static SplitTap split_tap(double phase, int64_t spm, double code_period_ms) {
+    const double cps = (double)spm * code_period_ms;
+    while (phase < 0) phase += cps;
+    if (phase > cps) phase = std::fmod(phase, cps);
+    SplitTap t{};
+    t.first_len = (int)((size_t)std::ceil(phase) % (size_t)spm);
+    t.first_phase = (int)(size_t)std::ceil(2 * cps - phase);
+    t.second_phase = t.first_phase + t.first_len;
+    t.rel = std::fmod(phase, (double)spm) / (double)spm;
+    return t;
+}
+
+int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_state* ch, float* out) {
+    UGSDR_CHECK_ARG(r && ch && out, "null argument");
+    UGSDR_CHECK_ARG(r->iq && r->spm > 0 && r->n_blocks > 0 && r->fs > 0, "bad signal");
+    UGSDR_CHECK_ARG(r->n_channels > 0 && r->n_taps > 0 && r->n_taps <= kMaxTaps && r->taps_chips, "bad taps");
+    UGSDR_CHECK_ARG(r->codes && r->code_len > 0, "bad codes");
+    UGSDR_CHECK_ARG(r->fmt == UGSDR_FMT_CI8 || r->fmt == UGSDR_FMT_CI16 || r->fmt == UGSDR_FMT_CF32, "bad format");
+    UGSDR_CHECK_ARG(r->mode == UGSDR_BANK_NCO || r->mode == UGSDR_BANK_REF_SPLIT, "bad mode");
+    const int64_t nsamp = r->spm * r->n_blocks;
+    const size_t bps = r->fmt == UGSDR_FMT_CI8 ? 2 : r->fmt == UGSDR_FMT_CI16 ? 4 : 8;
+    const void* iq = r->iq;
+    if (!r->iq_on_device) {
+        UGSDR_HIP_TRY(ctx->iq.reserve(nsamp * bps));
+        UGSDR_HIP_TRY(hipMemcpyAsync(ctx->iq.ptr, r->iq, nsamp * bps, hipMemcpyHostToDevice, ctx->stream));
+        iq = ctx->iq.ptr;
+    }
+    BankArgs a{};
+    a.iq = iq;
+    a.fmt = r->fmt;
+    a.spm = r->spm;
+    a.n_blocks = (int)r->n_blocks;
+    a.n_ch = r->n_channels;
+    a.n_taps = r->n_taps;
+    a.fs = r->fs;
+    for (int t = 0; t < r->n_taps; ++t) a.taps[t] = r->taps_chips[t];
+    const size_t out_bytes = (size_t)r->n_blocks * r->n_channels * r->n_taps * sizeof(float2);
+    UGSDR_HIP_TRY(ctx->results.reserve(out_bytes));
+    a.out = ctx->results.as<float2>();
+    const dim3 grid((unsigned)r->n_blocks, (unsigned)r->n_channels);
+    if (r->mode == UGSDR_BANK_NCO) {
+        std::vector<double> chs(4 * (size_t)r->n_channels);
+        for (int c = 0; c < r->n_channels; ++c) {
+            chs[4 * c] = ch[c].carrier_frequency;
+            chs[4 * c + 1] = ch[c].carrier_phase;
+            chs[4 * c + 2] = ch[c].code_rate;
+            chs[4 * c + 3] = ch[c].code_phase;
+        }
+        const size_t code_bytes = (size_t)r->n_channels * r->code_len;
+        UGSDR_HIP_TRY(ctx->tables.reserve(chs.size() * 8 + code_bytes));
+        UGSDR_HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, chs.data(), chs.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        UGSDR_HIP_TRY(hipMemcpyAsync(ctx->tables.as<char>() + chs.size() * 8, r->codes, code_bytes,
+                                     hipMemcpyHostToDevice, ctx->stream));
+        a.ch = ctx->tables.as<double>();
+        a.codes = reinterpret_cast<const signed char*>(ctx->tables.as<char>() + chs.size() * 8);
+        a.code_len = r->code_len;
+        const size_t lds = ((r->code_len + 15) / 16) * 16 + (kBankT / 64) * kMaxTaps * sizeof(float2);
+        if (lds > 160 * 1024) {
+            set_error("code too long for the LDS code table");
+            return UGSDR_ERR_UNSUPPORTED;
+        }
+        UGSDR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(bank_nco_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(bank_nco_kernel, grid, dim3(kBankT), lds, ctx->stream, a);
+        UGSDR_HIP_TRY(hipGetLastError());
+    } else {
+        UGSDR_CHECK_ARG(r->samples_per_chip > 0 && r->code_period_ms > 0, "bad replica rate");
+        // 3-period replica table, Codes::FillCodesImpl (src/tracking/tracker.hpp:39-51)
+        const int64_t cps = (int64_t)std::llround((double)r->spm * r->code_period_ms);
+        const int64_t len3 = 3 * cps;
+        const int64_t chips3 = 3 * r->code_len;
+        std::vector<float> code3((size_t)r->n_channels * len3);
+        for (int c = 0; c < r->n_channels; ++c)
+            for (int64_t i = 0; i < len3; ++i)
+                code3[(size_t)c * len3 + i] =
+                    (float)r->codes[(size_t)c * r->code_len + ((i * chips3 / len3) % chips3) % r->code_len];
+        std::vector<SplitTap> split((size_t)r->n_channels * r->n_taps);
+        std::vector<double> carr(2 * (size_t)r->n_channels);
+        for (int c = 0; c < r->n_channels; ++c) {
+            carr[2 * c] = ch[c].carrier_frequency;
+            carr[2 * c + 1] = ch[c].carrier_phase;
+            for (int t = 0; t < r->n_taps; ++t)
+                split[(size_t)c * r->n_taps + t] =
+                    split_tap(ch[c].code_phase + r->samples_per_chip * r->taps_chips[t], r->spm, r->code_period_ms);
+        }
+        const size_t b1 = code3.size() * 4, b2 = split.size() * sizeof(SplitTap), b3 = carr.size() * 8;
+        UGSDR_HIP_TRY(ctx->tables.reserve(b1 + b2 + b3 + 64));
+        char* d = ctx->tables.as<char>();
+        UGSDR_HIP_TRY(hipMemcpyAsync(d, code3.data(), b1, hipMemcpyHostToDevice, ctx->stream));
+        const size_t o2 = (b1 + 15) / 16 * 16, o3 = o2 + (b2 + 15) / 16 * 16;
+        UGSDR_HIP_TRY(hipMemcpyAsync(d + o2, split.data(), b2, hipMemcpyHostToDevice, ctx->stream));
+        UGSDR_HIP_TRY(hipMemcpyAsync(d + o3, carr.data(), b3, hipMemcpyHostToDevice, ctx->stream));
+        a.code3 = reinterpret_cast<const float*>(d);
+        a.code3_len = len3;
+        a.split = reinterpret_cast<const SplitTap*>(d + o2);
+        a.carrier = reinterpret_cast<const double*>(d + o3);
+        hipLaunchKernelGGL(bank_split_kernel, grid, dim3(kBankT), 0, ctx->stream, a);
+        UGSDR_HIP_TRY(hipGetLastError());
+    }
+    UGSDR_HIP_TRY(hipMemcpyAsync(out, a.out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // advance the channel states past the processed blocks
+    for (int c = 0; c < r->n_channels; ++c) {
+        if (r->mode == UGSDR_BANK_NCO) {
+            const double span = (double)nsamp / r->fs;
+            double th = ch[c].carrier_phase + ch[c].carrier_frequency * span;
+            ch[c].carrier_phase = th - std::floor(th);
+            double tau = ch[c].code_phase + ch[c].code_rate * span;
+            tau = std::fmod(tau, (double)r->code_len);
+            if (tau < 0) tau += (double)r->code_len;
+            ch[c].code_phase = tau;
+        } else {
+            for (int64_t b = 0; b < r->n_blocks; ++b)
+                ch[c].carrier_phase += 2 * M_PI * ch[c].carrier_frequency / 1000.0;
+        }
+    }
+    return UGSDR_OK;
+}
+
+}  // namespace ugsdr_hip
+
+extern "C" int ugsdr_hip_correlate_bank(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* req, ugsdr_channel_state* ch,
+                                        float* out) {
+    if (!ctx) {
+        ugsdr_hip::set_error("null context");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    UGSDR_HIP_TRY(hipSetDevice(ctx->device));
+    return ugsdr_hip::bank_impl(ctx, req, ch, out);
+}

@@ -1,0 +1,81 @@
+// Context, error plumbing and device buffer management shared by the
+// library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ugsdr_hip.h"
+
+namespace ugsdr_hip {
+
+void set_error(const std::string& msg);
+
+#define UGSDR_HIP_TRY(expr)                                                                 \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            ::ugsdr_hip::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+            return UGSDR_ERR_HIP;                                                           \
+        }                                                                                   \
+    } while (0)
+
+#define UGSDR_CHECK_ARG(cond, msg)                  \
+    do {                                            \
+        if (!(cond)) {                              \
+            ::ugsdr_hip::set_error(msg);            \
+            return UGSDR_ERR_ARG;                   \
+        }                                           \
+    } while (0)
+
+// grow-only device buffer
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            if (e != hipSuccess) return e;
+            ptr = nullptr;
+            bytes = 0;
+        }
+        size_t want = n < 4096 ? 4096 : n;
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+struct Replica {
+    void* spec = nullptr;  // n_fft complex (float2, or half2/bf16x2 for reduced)
+    int64_t n_fft = 0;
+    int precision = UGSDR_PREC_FP32;
+    float scale = 1.0f;    // reduced precision: stored = spectrum * scale
+};
+
+}  // namespace ugsdr_hip
+
+struct ugsdr_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::map<uint64_t, ugsdr_hip::Replica> replicas;
+    uint64_t next_key = 1;
+    // scratch
+    ugsdr_hip::DevBuf iq, sig, classes, scratch, rows, results, surf, tables, aux, aux2;
+    std::vector<char> pinned_dummy;
+    int64_t device_bytes() const;
+};

@@ -1,0 +1,24 @@
+// Declarations shared between the library's translation units.
+#pragma once
+
+#include "ctx.hpp"
+
+namespace ugsdr_hip {
+
+int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_result* out, float* surfaces,
+                 float* peaks);
+int acquire_reduced(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_result* out, float* surfaces,
+                    float* peaks);
+// conj(DFT(replica)) of length n_fft into out (device), from chips or samples
+int replica_spectrum(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chips, int64_t repeats, const float* samples,
+                     int64_t n_fft, float2* out);
+
+}  // namespace ugsdr_hip
+
+namespace ugsdr_hip {
+// converts an FP32 spectrum in rep.spec to the reduced storage type in place
+// (new allocation), recording the scale
+int replica_reduce(ugsdr_hip_ctx* ctx, Replica& rep, int precision);
+// copies the spectrum back as FP32 complex (undoing the storage scale)
+int replica_read(ugsdr_hip_ctx* ctx, const Replica& rep, float* out);
+}  // namespace ugsdr_hip

@@ -1,0 +1,256 @@
+"""Synthetic workloads of the acquisition / correlator benchmarks.
+
+Every input is synthesised deterministically (libugsdr_siggen.so: counter-based
+xoshiro256** + FP64 Box-Muller), with a truth record per satellite:
+
+  L1CA_1MS   GPS L1 C/A, int8 IQ @ 4.092 MHz, 1 ms x 1, +-5 kHz / 500 Hz (21 bins),
+             32 PRNs, 6 satellites, C/N0 U[42,48] dB-Hz, sigma 8 LSB
+  L1CA_4MS   same signal model, 4 ms coherent x 1 folded to 1 ms (the reference's
+             own ms_to_process = 4 search)
+  L1CA_WIDE  GPS L1 C/A, int16 IQ @ 16.368 MHz, 1 ms x 10 non-coherent,
+             +-5 kHz / 250 Hz (41 bins), 32 PRNs, 8 satellites C/N0 U[38,48],
+             20 ms navigation bits, sigma 512 LSB
+  BOC_4MS    BOC(1,1) seeded 4092-chip codes, int16 IQ @ 20.46 MHz, 4 ms x 5,
+             +-6 kHz / 125 Hz (97 bins), 36 PRNs, 8 satellites
+  BANK_L5    64 channels x 5 taps (+-0.5, +-0.25, 0 chip), seeded 10230-chip codes,
+             int16 IQ @ 40.92 MHz, 1000 x 1 ms blocks
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import FMT_CI8, FMT_CI16, HERE, doppler_bins
+
+GEN_LIB_PATH = os.path.join(HERE, "libugsdr_siggen.so")
+L1_HZ = 1575.42e6
+L5_HZ = 1176.45e6
+
+
+class SigSat(C.Structure):
+    _fields_ = [
+        ("code", C.POINTER(C.c_int8)), ("code_len", C.c_int64), ("chip_rate", C.c_double),
+        ("carrier_hz", C.c_double), ("code_phase", C.c_double), ("doppler", C.c_double),
+        ("carrier_phase", C.c_double), ("cn0_dbhz", C.c_double), ("nav_period_s", C.c_double),
+        ("nav_offset_s", C.c_double), ("nav_seed", C.c_uint64),
+    ]
+
+
+_gen = None
+
+
+def gen():
+    global _gen
+    if _gen is None:
+        if not os.path.exists(GEN_LIB_PATH):
+            raise RuntimeError(f"{GEN_LIB_PATH} not built: run python -m ugsdr_amd.build")
+        g = C.CDLL(GEN_LIB_PATH)
+        g.ugsdr_siggen_generate.argtypes = [C.POINTER(SigSat), C.c_int, C.c_double, C.c_int64, C.c_double,
+                                            C.c_uint64, C.c_int, C.c_void_p, C.c_int]
+        g.ugsdr_siggen_generate.restype = C.c_int
+        g.ugsdr_siggen_random_code.argtypes = [C.c_uint64, C.c_int64, C.c_void_p]
+        _gen = g
+    return _gen
+
+
+def random_code(seed: int, length: int) -> np.ndarray:
+    out = np.zeros(length, np.int8)
+    gen().ugsdr_siggen_random_code(seed, length, out.ctypes.data)
+    return out
+
+
+@dataclass
+class Sat:
+    prn: int                 # index into the searched code list
+    code: np.ndarray         # +-1 chips as transmitted (BOC expanded if BOC)
+    chip_rate: float
+    carrier_hz: float
+    code_phase: float        # chips of delay
+    doppler: float
+    carrier_phase: float     # cycles
+    cn0: float
+    nav_period_s: float = 0.0
+    nav_offset_s: float = 0.0
+    nav_seed: int = 0
+
+
+def synthesize(sats: list[Sat], fs: float, n: int, sigma: float, seed: int, fmt_bits: int,
+               threads: int | None = None) -> np.ndarray:
+    """Interleaved I/Q array (int8 / int16 / float32) of n complex samples."""
+    arr_t = (SigSat * max(1, len(sats)))()
+    keep = []
+    for i, s in enumerate(sats):
+        code = np.ascontiguousarray(s.code, np.int8)
+        keep.append(code)
+        arr_t[i] = SigSat(code.ctypes.data_as(C.POINTER(C.c_int8)), len(code), s.chip_rate, s.carrier_hz,
+                          s.code_phase, s.doppler, s.carrier_phase, s.cn0, s.nav_period_s, s.nav_offset_s,
+                          s.nav_seed)
+    dtype = {8: np.int8, 16: np.int16, 32: np.float32}[fmt_bits]
+    out = np.zeros(2 * n, dtype)
+    rc = gen().ugsdr_siggen_generate(arr_t, len(sats), fs, n, sigma, seed, fmt_bits, out.ctypes.data,
+                                     threads or min(os.cpu_count() or 1, 16))
+    if rc:
+        raise RuntimeError("signal synthesis failed")
+    del keep
+    return out
+
+
+@dataclass
+class AcqWorkload:
+    name: str
+    fs: float
+    fmt: int
+    iq: np.ndarray                  # interleaved I/Q
+    n_fft: int
+    n_blocks: int
+    surface_len: int
+    reshape: bool
+    doppler: np.ndarray
+    codes: list                     # per searched PRN: +-1 chips (replica source)
+    repeats: int                    # RepeatCodeNTimes factor for the replica
+    sats: list = field(default_factory=list)
+    chip_rate: float = 1.023e6      # of `codes`
+    seed: int = 0
+
+    @property
+    def n_samples(self) -> int:
+        return self.iq.size // 2
+
+    @property
+    def cells(self) -> int:
+        return len(self.codes) * len(self.doppler) * self.surface_len
+
+    def truth(self):
+        """(prn, expected surface index, Doppler) of each planted satellite."""
+        spc = self.fs / self.chip_rate
+        out = []
+        for s in self.sats:
+            idx = (s.code_phase * spc) % self.surface_len
+            out.append((s.prn, idx, s.doppler, s.cn0))
+        return out
+
+    def replicas(self) -> np.ndarray:
+        """(n_prn, n_fft) real replica samples (RepeatCodeNTimes + floor upsampler)."""
+        reps = []
+        for c in self.codes:
+            L = len(c) * self.repeats
+            i = np.arange(self.n_fft, dtype=np.int64)
+            reps.append(c[((i * L) // self.n_fft) % L % len(c)].astype(np.float32))
+        return np.stack(reps)
+
+
+def _draw_sats(rng, n_sats, n_prn, code_of, chip_rate, carrier, code_len, cn0_lo, cn0_hi, dop=4000.0,
+               nav=False, code_ms=1):
+    prns = rng.choice(n_prn, n_sats, replace=False)
+    sats = []
+    for p in prns:
+        tau = rng.uniform(0, code_len)
+        s = Sat(int(p), code_of(int(p)), chip_rate, carrier, tau, rng.uniform(-dop, dop), rng.uniform(0, 1),
+                rng.uniform(cn0_lo, cn0_hi))
+        if nav:
+            s.nav_period_s = 0.02
+            s.nav_offset_s = int(rng.integers(0, 20)) * 1e-3 * code_ms + tau / chip_rate
+            s.nav_seed = int(rng.integers(1, 2**62))
+        sats.append(s)
+    return sats
+
+
+def l1ca_codes(n_prn: int = 32):
+    from . import gps_l1ca
+    return [gps_l1ca(i) for i in range(n_prn)]
+
+
+def l1ca_1ms(seed: int = 0xC1, n_prn: int = 32) -> AcqWorkload:
+    rng = np.random.default_rng(seed)
+    codes = l1ca_codes(32)
+    fs = 4.092e6
+    sats = _draw_sats(rng, 6, 32, lambda p: codes[p], 1.023e6, L1_HZ, 1023, 42, 48)
+    iq = synthesize(sats, fs, 4092, 8.0, seed, 8)
+    return AcqWorkload("L1CA_1MS", fs, FMT_CI8, iq, 4092, 1, 4092, True, doppler_bins(5000, 500),
+                       codes[:n_prn], 1, sats, seed=seed)
+
+
+def l1ca_4ms(seed: int = 0xC0, n_prn: int = 32) -> AcqWorkload:
+    rng = np.random.default_rng(seed)
+    codes = l1ca_codes(32)
+    fs = 4.092e6
+    sats = _draw_sats(rng, 6, 32, lambda p: codes[p], 1.023e6, L1_HZ, 1023, 42, 48)
+    iq = synthesize(sats, fs, 16368, 8.0, seed, 8)
+    return AcqWorkload("L1CA_4MS", fs, FMT_CI8, iq, 16368, 1, 4092, True, doppler_bins(5000, 500),
+                       codes[:n_prn], 4, sats, seed=seed)
+
+
+def l1ca_wide(seed: int = 0xC2, n_prn: int = 32, n_blocks: int = 10, doppler_range: float = 5000,
+              doppler_step: float = 250) -> AcqWorkload:
+    rng = np.random.default_rng(seed)
+    codes = l1ca_codes(32)
+    fs = 16.368e6
+    sats = _draw_sats(rng, 8, 32, lambda p: codes[p], 1.023e6, L1_HZ, 1023, 38, 48, nav=True)
+    iq = synthesize(sats, fs, 16368 * n_blocks, 512.0, seed, 16)
+    return AcqWorkload("L1CA_WIDE", fs, FMT_CI16, iq, 16368, n_blocks, 16368, True,
+                       doppler_bins(doppler_range, doppler_step), codes[:n_prn], 1, sats, seed=seed)
+
+
+def boc_codes(seed: int = 0xC3, n_prn: int = 36):
+    return [random_code(seed * 1000 + p, 4092) for p in range(n_prn)]
+
+
+def boc_4ms(seed: int = 0xC3, n_prn: int = 36, n_blocks: int = 5, doppler_range: float = 6000,
+            doppler_step: float = 125) -> AcqWorkload:
+    from . import boc
+    rng = np.random.default_rng(seed)
+    prim = boc_codes(seed, 36)
+    codes = [boc(c) for c in prim]       # 8184 half-chips per 4 ms
+    fs = 20.46e6
+    sats = _draw_sats(rng, 8, 36, lambda p: codes[p], 2.046e6, L1_HZ, 8184, 40, 48, dop=5000)
+    iq = synthesize(sats, fs, 81840 * n_blocks, 512.0, seed, 16)
+    return AcqWorkload("BOC_4MS", fs, FMT_CI16, iq, 81840, n_blocks, 81840, False,
+                       doppler_bins(doppler_range, doppler_step), codes[:n_prn], 1, sats,
+                       chip_rate=2.046e6, seed=seed)
+
+
+@dataclass
+class BankWorkload:
+    name: str
+    fs: float
+    spm: int
+    n_blocks: int
+    iq: np.ndarray
+    codes: np.ndarray          # (n_ch, code_len)
+    channels: list             # (f, theta_cycles, code_rate, code_phase_chips)
+    taps: np.ndarray
+    sats: list
+
+    @property
+    def channel_samples(self) -> int:
+        return self.spm * self.n_blocks * len(self.channels)
+
+
+def bank_l5(seed: int = 0xC4, n_ch: int = 64, n_blocks: int = 1000, fs: float = 40.92e6) -> BankWorkload:
+    rng = np.random.default_rng(seed)
+    codes = np.stack([random_code(seed * 7919 + c, 10230) for c in range(n_ch)])
+    spm = int(round(fs / 1000))
+    sats, chans = [], []
+    for c in range(n_ch):
+        f = rng.uniform(-5000, 5000)
+        tau = rng.uniform(0, 10230)
+        th = rng.uniform(0, 1)
+        s = Sat(c, codes[c], 10.23e6, L5_HZ, tau, f, th, rng.uniform(40, 50))
+        sats.append(s)
+        rate = 10.23e6 * (1 + f / L5_HZ)
+        # code phase at t = 0 in the NCO convention: chip(t) = floor(tau_nco + rate t)
+        chans.append((f, th, rate, (-tau) % 10230))
+    iq = synthesize(sats, fs, spm * n_blocks, 512.0, seed, 16)
+    taps = np.array([-0.5, -0.25, 0.0, 0.25, 0.5])
+    return BankWorkload("BANK_L5", fs, spm, n_blocks, iq, codes, chans, taps, sats)
+
+
+CONFIGS = {
+    "L1CA_1MS": l1ca_1ms,
+    "L1CA_4MS": l1ca_4ms,
+    "L1CA_WIDE": l1ca_wide,
+    "BOC_4MS": boc_4ms,
+}
