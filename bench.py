@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused acquisition hot path on one MI355X.
+
+Workload (one step = one full search, IQ resident in HBM before timing):
+  GPS L1 C/A wide search, plain FP32: complex int16 IQ at 16.368 MHz,
+  1 ms coherent x 10 non-coherent blocks, Doppler +-5 kHz step 250 Hz
+  (41 bins), 32 PRNs -> 21,474,816 cells (PRN x Doppler x code phase),
+  8 seeded satellites at C/N0 38-48 dB-Hz with navigation-bit flips.
+
+Prints one JSON line: whole-search throughput in cells/s, per-search p50/p99
+latency, the correlation kernel's HBM roofline figure (algorithmic 8 bytes per
+cell-block over the kernel's HIP-event duration) and the CPU baseline (the
+oracle's FP32 restatement of the reference's per-PRN/per-Doppler loop, timed on
+this host's cores on the same input).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "acq cells/s (PRN×Doppler×phase) vs FP32 plain + p50/p99 request latency"
+HBM_PEAK_GBS = 8000.0
+B_CB = 8  # algorithmic bytes per cell-block, FP32 plain
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", default="L1CA_WIDE", choices=["L1CA_WIDE", "L1CA_1MS", "L1CA_4MS", "BOC_4MS"])
+    return ap.parse_args()
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    k = (len(xs) - 1) * q
+    f = int(k)
+    c = min(f + 1, len(xs) - 1)
+    return xs[f] + (xs[c] - xs[f]) * (k - f)
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per launch of the correlation kernel from committed rocprofv3
+    --pmc summaries (FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated)."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("corr_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(w, threads: int):
+    """oracle FP32 path (reference structure: per PRN x Doppler x block: mix,
+    DFT, conj-spectrum multiply, inverse DFT/N, |.|, fold, max, mean/sigma),
+    PRN-parallel over `threads` host threads; the whole search, median of 3."""
+    import numpy as np
+
+    from oracle import refcpu
+
+    x = (w.iq[0::2].astype(np.float32) + 1j * w.iq[1::2].astype(np.float32)).astype(np.complex64)
+    reps = w.replicas()
+    kw = dict(n_fft=w.n_fft, n_blocks=w.n_blocks, surface_len=w.surface_len, fs_in=w.fs, fs_acq=w.fs,
+              doppler=w.doppler, reshape=w.reshape, precision=32, threads=threads)
+    # bounded sample: as many PRNs as fit ~10-20 s of work
+    t0 = time.perf_counter()
+    refcpu.acquire(x, reps[:threads], **kw)
+    per_round = time.perf_counter() - t0
+    n_prn = len(reps)
+    rounds = max(1, min(int(15.0 / max(per_round, 1e-3)), (n_prn + threads - 1) // threads))
+    sample = reps[: min(n_prn, rounds * threads)]
+    times = []
+    for _ in range(3 if per_round * rounds < 5 else 1):
+        t0 = time.perf_counter()
+        res, _, _ = refcpu.acquire(x, sample, **kw)
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    cells = len(sample) * len(w.doppler) * w.surface_len
+    # FFT calibration against numpy's pocketfft (FP64) at this length
+    z = np.random.default_rng(0).normal(size=w.n_fft) + 1j * np.random.default_rng(1).normal(size=w.n_fft)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        refcpu.fft(z)
+    t_or = (time.perf_counter() - t0) / 20
+    t0 = time.perf_counter()
+    for _ in range(20):
+        np.fft.fft(z)
+    t_np = (time.perf_counter() - t0) / 20
+    return {
+        "value": cells / t,
+        "unit": "cells/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(sample)} of {n_prn} PRNs x {len(w.doppler)} bins x {w.n_blocks} blocks of the same "
+                  f"input, FP32, {threads} threads, median of {len(times)} ({t:.2f} s)",
+        "fft_time_vs_numpy_pocketfft": t_or / t_np,
+        "results": res,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the acquisition hot path is measured on a single MI355X "
+              "(see the sharding notes in DESIGN.md); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import ugsdr_amd as U
+    from ugsdr_amd import workloads as W
+
+    w = W.CONFIGS[args.workload]()
+    ctx = U.Context(0)
+    keys = [ctx.replica_put(c, w.repeats, w.n_fft) for c in w.codes]
+    iq_dev = torch.from_numpy(w.iq).to("cuda:0")
+    torch.cuda.synchronize()
+    req, hold = ctx.make_request(None, w.fmt, fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=w.n_blocks,
+                                 surface_len=w.surface_len, doppler=w.doppler, keys=keys, reshape=w.reshape,
+                                 n_samples=w.n_samples, iq_device_ptr=iq_dev.data_ptr())
+    for _ in range(args.warmup):
+        res, _, _ = ctx.acquire_req(req)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s = time.perf_counter()
+        res, _, _ = ctx.acquire_req(req)
+        lat.append(time.perf_counter() - s)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = w.cells / (elapsed / args.steps)
+
+    # kernel-level timing with HIP events on the context's stream
+    ctx.set_profiling(True)
+    stage = {"front": [], "classes": [], "corr": [], "final": []}
+    for _ in range(min(args.steps, 20)):
+        ctx.acquire_req(req)
+        for k, v in ctx.last_times().items():
+            stage[k].append(v)
+    ctx.set_profiling(False)
+    corr_ms = statistics.mean(stage["corr"])
+    alg_bytes = B_CB * w.cells * w.n_blocks
+    achieved = alg_bytes / (corr_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles(args.workload)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": {"L1CA_WIDE": "GPS L1 C/A wide search: int16 IQ @ 16.368 MHz, 1 ms x 10 non-coherent, "
+                                      "+-5 kHz / 250 Hz (41 bins), 32 PRNs",
+                         "L1CA_1MS": "GPS L1 C/A: int8 IQ @ 4.092 MHz, 1 ms x 1, +-5 kHz / 500 Hz, 32 PRNs",
+                         "L1CA_4MS": "GPS L1 C/A: int8 IQ @ 4.092 MHz, 4 ms coherent folded to 1 ms, 32 PRNs",
+                         "BOC_4MS": "BOC(1,1) 4092 chips, int16 IQ @ 20.46 MHz, 4 ms x 5, +-6 kHz / 125 Hz, "
+                                    "36 PRNs"}[args.workload],
+            "prns": len(w.codes),
+            "doppler_bins": len(w.doppler),
+            "n_fft": w.n_fft,
+            "noncoherent_blocks": w.n_blocks,
+            "surface_len": w.surface_len,
+            "cells_per_search": w.cells,
+            "parallelism": "single GPU",
+        },
+        "latency_ms": {"p50": 1e3 * percentile(lat, 0.5), "p99": 1e3 * percentile(lat, 0.99)},
+        "stage_ms": {k: statistics.mean(v) for k, v in stage.items()},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "corr_small<512,16384,PlanCT<31,16,11,3>>" if w.n_fft == 16368 else "corr",
+            "algorithmic_bytes_per_launch": alg_bytes,
+        },
+        "detected_prns": [i for i, r in enumerate(res) if r["detected"]],
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(w, threads)
+        cpu_res = cpu.pop("results")
+        out["cpu_baseline"] = cpu
+        out["speedup_vs_cpu"] = value / cpu["value"]
+        agree = all(cpu_res[i]["detected"] == res[i]["detected"] and
+                    (not res[i]["detected"] or (cpu_res[i]["index"] == res[i]["index"] and
+                                                cpu_res[i]["doppler_bin"] == res[i]["doppler_bin"]))
+                    for i in range(len(cpu_res)))
+        out["cpu_gpu_same_detections"] = agree
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -56,6 +56,13 @@ int ugsdr_hip_ctx_destroy(ugsdr_hip_ctx* ctx);
 int ugsdr_hip_ctx_stream(ugsdr_hip_ctx* ctx, void** stream);
 /* bytes of device memory held by the context (caches + scratch) */
 int64_t ugsdr_hip_ctx_device_bytes(ugsdr_hip_ctx* ctx);
+/* Stage timing with HIP events on the context's stream (off by default).  No
+ * reference counterpart: the reference times nothing at run time; this is the
+ * harness hook that replaces its disabled google-benchmark section
+ * (benchmark/benchmark.cpp:482-539).  ms_out[4] = front end (unpack, mix,
+ * resample), forward class FFTs, correlation, finalize of the last acquire. */
+int ugsdr_hip_ctx_set_profiling(ugsdr_hip_ctx* ctx, int enable);
+int ugsdr_hip_ctx_last_times(ugsdr_hip_ctx* ctx, float* ms_out4);
 
 /* ------------------------------------------------------------- PRN codes (host)
  * Codegen<Impl>::Get (src/prn_codes/codegen.hpp:23-36).  sv is 0-based. */

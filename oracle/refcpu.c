@@ -325,13 +325,21 @@ typedef struct {
             for (size_t j = 0; j < L; ++j) surf[j] = 0;                                            \
             for (int64_t k = 0; k < p->n_blocks; ++k) {                                            \
                 const float* x = job->signal + 2 * (size_t)k * N;                                  \
+                /* tone exp(j 2 pi f n / fs): FP64 rotator re-seeded exactly every 256 */      \
+                const double wr = cos(2 * M_PI * f / p->fs_acq), wi = sin(2 * M_PI * f / p->fs_acq); \
+                double c = 1.0, s = 0.0;                                                           \
                 for (size_t n = 0; n < N; ++n) {                                                   \
-                    double t = fmod((double)n * f / p->fs_acq, 1.0);                               \
-                    double c = cos(2 * M_PI * t), s = sin(2 * M_PI * t);                           \
+                    if ((n & 255) == 0) {                                                          \
+                        double t = fmod((double)n * f / p->fs_acq, 1.0);                           \
+                        c = cos(2 * M_PI * t); s = sin(2 * M_PI * t);                              \
+                    }                                                                              \
                     REAL cr = (REAL)c, ci = (REAL)s;                                               \
                     REAL xr = (REAL)x[2 * n], xi = (REAL)x[2 * n + 1];                             \
                     buf[2 * n] = xr * cr - xi * ci;                                                \
                     buf[2 * n + 1] = xr * ci + xi * cr;                                            \
+                    const double c2 = c * wr - s * wi;                                             \
+                    s = c * wi + s * wr;                                                           \
+                    c = c2;                                                                        \
                 }                                                                                  \
                 rc_exec##SUF(plan, buf, work, 0);                                          \
                 for (size_t n = 0; n < N; ++n) {                                                   \
@@ -549,7 +557,7 @@ void refcpu_bank_nco(const float* x, int64_t spm, int64_t n_blocks, double fs, c
                      const int8_t* codes, int64_t code_len, const double* taps_chips, int n_taps, int threads,
                      double* out) {
     if (n_taps > 64) return;
-    rc_bank_job j = {x, spm, n_blocks, fs, ch, n_ch, codes, code_len, taps_chips, n_taps, out};
+    rc_bank_job j = {x, spm, n_blocks, fs, ch, n_ch, codes, code_len, taps_chips, n_taps, out, 0};
     atomic_init(&j.next, 0);
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;

@@ -1,0 +1,148 @@
+// C++20 host surface test: the reference's unit KATs through the Hip* CRTP
+// backends, and the reference's real_8 acquisition KAT through both the
+// per-operator FastSearchEngine<HipFseConfig> loop and the fused
+// HipFastSearchEngine.  argv[1]: path of tests/golden/real_8_first4ms.bin.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <utility>
+#include <numeric>
+#include <vector>
+
+#include "ugsdr_hip/fse.hpp"
+#include "ugsdr_hip/tracking.hpp"
+
+#define EXPECT(cond)                                                                \
+    do {                                                                            \
+        if (!(cond)) {                                                              \
+            std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+            return 1;                                                               \
+        }                                                                           \
+    } while (0)
+
+using cvec = std::vector<std::complex<float>>;
+
+static std::vector<int8_t> l1ca(int sv) {
+    std::vector<int8_t> c(1023);
+    ugsdr_hip::check(ugsdr_code_gps_l1ca(sv, c.data()));
+    return c;
+}
+
+static int unit_kats() {
+    // mixer, test/basic_tests.cpp:321-350
+    ugsdr::HipMixer mixer(1e6, 1e6 / 2, 0);
+    cvec ones(1000, 1.0f);
+    mixer.Translate(ones);
+    for (std::size_t i = 0; i < ones.size(); ++i) {
+        EXPECT(ones[i].real() == 1.0f - 2.0f * (float)(i & 1));
+        EXPECT(std::abs(ones[i].imag()) < 1e-6f);
+    }
+    // matched filter, :368-379 (GLONASS m-sequence)
+    std::vector<int8_t> g(511);
+    ugsdr_hip::check(ugsdr_code_glonass_of(g.data()));
+    std::vector<float> code(g.begin(), g.end());
+    const cvec sig(code.begin(), code.end());
+    auto mf = ugsdr::HipMatchedFilter::Filter(sig, code);
+    EXPECT(std::abs(mf[0].real() - 511.0f) < 5e-3f);
+    for (std::size_t i = 1; i < mf.size(); ++i) EXPECT(std::abs(mf[i].real() + 1.0f) < 5e-3f);
+    // correlator self-dot, :72-86
+    auto c0 = l1ca(0);
+    std::vector<float> cf(c0.begin(), c0.end());
+    cvec cs(cf.begin(), cf.end());
+    auto dot = ugsdr::HipCorrelator::Correlate(std::span(cs), std::span(cf));
+    EXPECT(dot == std::complex<float>(1023.0f, 0.0f));
+    bool threw = false;
+    try {
+        (void)ugsdr::HipCorrelator::Correlate(std::span(cs).subspan(0, 10), std::span(cf));
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    // abs, max index, mean/std, reshape, :408-433, :528-553, :607-630
+    auto a = ugsdr::HipAbs::Transform(cvec(1000, {1.0f, -1.0f}));
+    for (float v : a) EXPECT(std::abs(v - std::sqrt(2.0f)) < 1e-2f);
+    std::vector<float> iota(1000);
+    std::iota(iota.begin(), iota.end(), 0.0f);
+    auto mx = ugsdr::HipMaxIndex::Transform(iota);
+    EXPECT(mx.value == 999.0f && mx.index == 999);
+    auto ms = ugsdr::HipMeanStdDev::Calculate(std::vector<float>{1, 2, 3, 4});
+    EXPECT(std::abs(ms.mean - 2.5f) < 1e-6f && std::abs(ms.sigma - 1.2909944f) < 1e-5f);
+    auto rs = ugsdr::HipReshapeAndSum::Transform(std::vector<float>(10000, 1.0f), 1000);
+    EXPECT(rs.size() == 1000 && rs[17] == 10.0f);
+    threw = false;
+    try {
+        (void)ugsdr::HipReshapeAndSum::Transform(std::vector<float>(10, 1.0f), 3);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    // upsampler x10, :800-817
+    auto up = ugsdr::HipUpsampler::Transform(std::as_const(iota), 10000);
+    EXPECT(up.size() == 10000 && up[12] == 1.0f && up[9999] == 999.0f);
+    std::puts("unit KATs ok");
+    return 0;
+}
+
+static int real8_kat(const char* path) {
+    std::ifstream f(path, std::ios::binary);
+    std::vector<int8_t> raw(4 * 38192);
+    f.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)raw.size());
+    EXPECT(f.gcount() == (std::streamsize)raw.size());
+    const double fs = 38.192e6;
+    // DFE channel: real samples, member Translate by +9.55 MHz (src/dfe/dfe.hpp:199)
+    cvec x(raw.begin(), raw.end());
+    ugsdr::HipMixer dfe_mixer(fs, 9.55e6, 0);
+    dfe_mixer.Translate(x);
+    std::vector<ugsdr::Sv> svs;
+    std::vector<std::vector<float>> codes_f;
+    std::vector<std::vector<int8_t>> codes;
+    for (int s = 0; s < 32; ++s) {
+        svs.push_back({s, ugsdr::System::Gps, ugsdr::Signal::GpsCoarseAcquisition_L1});
+        codes.push_back(l1ca(s));
+        codes_f.emplace_back(codes.back().begin(), codes.back().end());
+    }
+    ugsdr::SearchEpoch epoch{x.data(), x.size(), fs, 0.0};
+    ugsdr::FastSearchEngine<ugsdr::HipFseConfig> per_op(6e3, 200);
+    auto r1 = per_op.Process(epoch, svs, codes_f, 4);
+    // fused: raw int8 with the DFE tone as the request's mix
+    ugsdr::HipFastSearchEngine::Options opt;
+    ugsdr::HipFastSearchEngine fused(6e3, 200, opt);
+    auto r2 = fused.Process(x.data(), UGSDR_FMT_CF32, x.size(), fs, 0.0, svs, codes);
+    std::printf("per-operator %zu detections, fused %zu detections\n", r1.size(), r2.size());
+    EXPECT(r1.size() == r2.size());
+    for (std::size_t i = 0; i < r1.size(); ++i) {
+        EXPECT(r1[i].sv_number.id == r2[i].sv_number.id);
+        EXPECT(r1[i].doppler == r2[i].doppler);
+        EXPECT(std::abs(r1[i].code_offset - r2[i].code_offset) <= 4.0);
+        EXPECT(std::abs(r1[i].GetSnr() - r2[i].GetSnr()) <= 1e-3 * r1[i].GetSnr());
+        EXPECT(r2[i].output_peak.size() == 9548);
+    }
+    // test/integration_tests.cpp:253-263 (SV ids 0-based), >= 4 within 400 Hz (:314)
+    const std::map<int, double> kat = {{2, 0}, {5, 6000}, {8, -1000}, {14, 0}, {17, 2000}, {20, 3000}, {21, 0}, {25, 5000}};
+    int match = 0;
+    for (auto& r : r2) {
+        auto it = kat.find(r.sv_number.id);
+        if (it != kat.end() && std::abs(r.doppler - it->second) <= 400) ++match;
+    }
+    std::printf("real_8 KAT: %d/8 matched\n", match);
+    EXPECT(match >= 4);
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s real_8_first4ms.bin\n", argv[0]);
+        return 2;
+    }
+    try {
+        if (int rc = unit_kats()) return rc;
+        if (int rc = real8_kat(argv[1])) return rc;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "exception: %s\n", e.what());
+        return 1;
+    }
+    std::puts("host API ok");
+    return 0;
+}

@@ -1,0 +1,236 @@
+"""Fused acquisition (ugsdr_hip_acquire) vs the FP64 oracle, through the C ABI.
+
+Parity bar (tests/parity.py): same detected PRNs, bit-exact surface index and
+Doppler bin outside near-ties, level / metric within 1e-3 relative; full
+surfaces within 2e-5 of the oracle's peak.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import ugsdr_amd as U
+from ugsdr_amd import workloads as W
+
+from parity import compare, iq_to_complex
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def put_replicas(ctx, w, precision=U.PREC_FP32):
+    return [ctx.replica_put(c, w.repeats, w.n_fft, precision) for c in w.codes]
+
+
+def oracle_run(oracle, w, prns=None, dop=None, reshape=None, coherent=True, surfaces=False):
+    x = iq_to_complex(w.iq)
+    reps = w.replicas()
+    if prns is not None:
+        reps = reps[prns]
+    return oracle.acquire(x, reps, n_fft=w.n_fft, n_blocks=w.n_blocks, surface_len=w.surface_len, fs_in=w.fs,
+                          fs_acq=w.fs, doppler=w.doppler if dop is None else dop,
+                          reshape=w.reshape if reshape is None else reshape, coherent=coherent, precision=64,
+                          want_surfaces=surfaces)
+
+
+def gpu_run(ctx, w, keys, dop=None, reshape=None, coherent=True, surfaces=False, peaks=False):
+    return ctx.acquire(w.iq, w.fmt, want_surfaces=surfaces, want_peaks=peaks, fs_in=w.fs, fs_acq=w.fs,
+                       n_fft=w.n_fft, n_blocks=w.n_blocks, surface_len=w.surface_len,
+                       doppler=w.doppler if dop is None else dop, keys=keys,
+                       reshape=w.reshape if reshape is None else reshape, coherent=coherent)
+
+
+def check_truth(w, res, tol_idx=1.5):
+    """every planted satellite the oracle detects sits at the planted delay / Doppler"""
+    step = w.doppler[1] - w.doppler[0]
+    for prn, idx, dop, cn0 in w.truth():
+        r = res[prn]
+        if not r["detected"]:
+            continue
+        d = abs(r["index"] - idx)
+        d = min(d, w.surface_len - d)
+        assert d <= tol_idx, (prn, r["index"], idx)
+        assert abs(r["doppler"] - dop) <= step, (prn, r["doppler"], dop)
+
+
+def assert_parity(gpu, ref, tag):
+    fails, notes = compare(gpu, ref)
+    say(f"{tag}: {len(notes)} notes {notes}")
+    assert not fails, fails
+
+
+def test_l1ca_1ms_full(ctx, oracle):
+    w = W.l1ca_1ms()
+    say("case L1CA_1MS 32 PRN x 21 bins, N=4092")
+    keys = put_replicas(ctx, w)
+    gpu, surf, peaks = gpu_run(ctx, w, keys, surfaces=True, peaks=True)
+    ref, rsurf, _ = oracle_run(oracle, w, surfaces=True)
+    assert_parity(gpu, ref, "L1CA_1MS")
+    scale = rsurf.max()
+    assert np.max(np.abs(surf - rsurf)) < 2e-5 * scale
+    for p, g in enumerate(gpu):
+        if g["doppler_bin"] >= 0:
+            np.testing.assert_array_equal(peaks[p], surf[p, g["doppler_bin"]])
+    check_truth(w, ref)
+    assert sum(r["detected"] for r in ref) >= 4
+    for k in keys:
+        ctx.replica_drop(k)
+
+
+def test_l1ca_4ms_fold(ctx, oracle):
+    w = W.l1ca_4ms()
+    say("case L1CA_4MS fold 16368 -> 4092")
+    keys = put_replicas(ctx, w)
+    gpu, surf, _ = gpu_run(ctx, w, keys, surfaces=True)
+    ref, rsurf, _ = oracle_run(oracle, w, surfaces=True)
+    assert_parity(gpu, ref, "L1CA_4MS")
+    assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
+    check_truth(w, ref)
+    for k in keys:
+        ctx.replica_drop(k)
+
+
+def test_l1ca_4ms_noncoherent_fold(ctx, oracle):
+    """GetOneMsPeak<reshape=true, coherent=false>: fold complex, then |.|"""
+    w = W.l1ca_4ms()
+    say("case L1CA_4MS coherent=false")
+    keys = put_replicas(ctx, w)
+    gpu, _, _ = gpu_run(ctx, w, keys, coherent=False)
+    ref, _, _ = oracle_run(oracle, w, coherent=False)
+    assert_parity(gpu, ref, "L1CA_4MS/nc")
+    for k in keys:
+        ctx.replica_drop(k)
+
+
+def test_l1ca_wide_10ms(ctx, oracle):
+    w = W.l1ca_wide()
+    say("case L1CA_WIDE 32 PRN x 41 bins x 10 blocks, N=16368")
+    keys = put_replicas(ctx, w)
+    gpu, _, _ = gpu_run(ctx, w, keys)
+    ref, _, _ = oracle_run(oracle, w)
+    assert_parity(gpu, ref, "L1CA_WIDE")
+    check_truth(w, ref)
+    assert sum(r["detected"] for r in ref) >= 5
+    for k in keys:
+        ctx.replica_drop(k)
+
+
+def test_boc_4ms_subset(ctx, oracle):
+    """81840-point four-step path; 8 planted PRNs + 4 absent ones, all 97 bins."""
+    w = W.boc_4ms()
+    planted = sorted(s.prn for s in w.sats)
+    absent = [p for p in range(36) if p not in planted][:4]
+    prns = planted + absent
+    say(f"case BOC_4MS PRNs {prns} x 97 bins x 5 blocks, N=81840")
+    keys = [ctx.replica_put(w.codes[p], 1, w.n_fft) for p in prns]
+    gpu, _, _ = gpu_run(ctx, w, keys)
+    ref, _, _ = oracle_run(oracle, w, prns=prns)
+    assert_parity(gpu, ref, "BOC_4MS")
+    for i, p in enumerate(planted):
+        s = next(s for s in w.sats if s.prn == p)
+        r = ref[i]
+        if r["snr"] > 3.3:
+            idx = (s.code_phase * 10) % w.surface_len
+            assert min(abs(r["index"] - idx), w.surface_len - abs(r["index"] - idx)) <= 1.5
+    for k in keys:
+        ctx.replica_drop(k)
+
+
+def test_real8_kat(ctx, oracle):
+    """The reference's real_8 acquisition KAT (test/integration_tests.cpp:253-263,
+    :314, :395-398): real int8 @ 38.192 MHz, IF 9.55 MHz, FseConfig rate 8.192 MHz
+    (-> 9.548 MHz), +-6 kHz / 200 Hz, 4 ms; >= 4 of 8 satellites within 400 Hz."""
+    raw = np.fromfile(os.path.join(GOLDEN, "real_8_first4ms.bin"), np.int8)
+    fs = 38.192e6
+    fs_acq = U.adjust_sampling_rate(fs, 8.192e6)
+    assert fs_acq == 9.548e6
+    spm = int(fs_acq / 1e3)
+    N = 4 * spm
+    dop = U.doppler_bins(6000, 200)
+    say("case real_8 KAT N=38192 fold 4, 32 PRN x 61 bins, R8 + mix + boxcar /4")
+    codes = [U.gps_l1ca(s) for s in range(32)]
+    keys = [ctx.replica_put(c, 4, N) for c in codes]
+    gpu, _, _ = ctx.acquire(raw, U.FMT_R8, n_samples=raw.size, fs_in=fs, fs_acq=fs_acq, mix_frequency=9.55e6,
+                            n_fft=N, n_blocks=1, surface_len=spm, doppler=dop, keys=keys)
+    x = raw.astype(np.float32).astype(np.complex64)
+    x = oracle.mix_inplace(x, fs, 9.55e6, 0.0)
+    y = oracle.resample(x, int(fs_acq), int(fs))
+    reps = np.stack([oracle.repeat_upsample(c, 4, N) for c in codes])
+    ref, _, _ = oracle.acquire(y, reps, n_fft=N, n_blocks=1, surface_len=spm, fs_in=fs, fs_acq=fs_acq,
+                               doppler=dop, precision=64)
+    assert_parity(gpu, ref, "real_8")
+    kat = {2: (0, 34213), 5: (6000, 28203), 8: (-1000, 4695), 14: (0, 36321), 17: (2000, 20725),
+           20: (3000, 13404), 21: (0, 6289), 25: (5000, 26828)}
+    match = sum(1 for sv, (d, c) in kat.items() if gpu[sv]["detected"] and abs(gpu[sv]["doppler"] - d) <= 400)
+    say(f"real_8: {match}/8 KAT satellites matched")
+    assert match >= 4
+    for k in keys:
+        ctx.replica_drop(k)
+
+
+@pytest.mark.parametrize("n_fft", [1000, 2310, 20000])
+def test_runtime_plan_lengths(ctx, oracle, n_fft):
+    """lengths without a compile-time plan (runtime radix plan / four-step)"""
+    rng = np.random.default_rng(n_fft)
+    fs = n_fft * 1000.0
+    code = np.where(rng.random(n_fft // 2) < 0.5, -1, 1).astype(np.int8)
+    x = (rng.normal(0, 20, n_fft) + 1j * rng.normal(0, 20, n_fft)).astype(np.complex64)
+    rep = oracle.repeat_upsample(code, 1, n_fft)
+    x += (3.0 * np.roll(rep, 123) * np.exp(2j * np.pi * 1500 * np.arange(n_fft) / fs)).astype(np.complex64)
+    dop = U.doppler_bins(3000, 500)
+    say(f"case runtime plan N={n_fft}")
+    key = ctx.replica_put(code, 1, n_fft)
+    iq = np.ascontiguousarray(x).view(np.float32)
+    gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft, n_blocks=1,
+                               surface_len=n_fft, doppler=dop, keys=[key])
+    ref, rsurf, _ = oracle.acquire(x, rep[None], n_fft=n_fft, n_blocks=1, surface_len=n_fft, fs_in=fs, fs_acq=fs,
+                                   doppler=dop, precision=64, want_surfaces=True)
+    assert_parity(gpu, ref, f"N={n_fft}")
+    assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
+    assert gpu[0]["index"] == 123 and gpu[0]["doppler"] == 1500.0
+    ctx.replica_drop(key)
+
+
+def test_zero_signal_keeps_default_result(ctx):
+    """all-zero input: no bin beats level 0, so the AcquisitionResult defaults stay
+    (src/acquisition/acquisition_result.hpp:13-18, fse.hpp:265)"""
+    say("case zero signal")
+    key = ctx.replica_put(U.gps_l1ca(0), 1, 4092)
+    iq = np.zeros(2 * 4092, np.int8)
+    gpu, _, _ = ctx.acquire(iq, U.FMT_CI8, fs_in=4.092e6, fs_acq=4.092e6, n_fft=4092, n_blocks=1,
+                            surface_len=4092, doppler=U.doppler_bins(1000, 500), keys=[key])
+    r = gpu[0]
+    assert r["level"] == 0 and r["sigma"] == 1.0 and r["doppler_bin"] == -1 and r["detected"] == 0
+    ctx.replica_drop(key)
+
+
+def test_errors(ctx):
+    say("case error paths")
+    key = ctx.replica_put(U.gps_l1ca(0), 1, 4092)
+    iq = np.zeros(2 * 4092, np.int8)
+    kw = dict(fs_in=4.092e6, fs_acq=4.092e6, n_fft=4092, n_blocks=1, doppler=[0.0])
+    with pytest.raises(U.UgsdrError, match="SIZE"):   # reshape_and_sum.hpp:16-17
+        ctx.acquire(iq, U.FMT_CI8, surface_len=1000, keys=[key], **kw)
+    with pytest.raises(U.UgsdrError, match="STATE"):
+        ctx.acquire(iq, U.FMT_CI8, surface_len=4092, keys=[key + 999], **kw)
+    with pytest.raises(U.UgsdrError, match="SIZE"):
+        ctx.acquire(iq[:1000], U.FMT_CI8, surface_len=4092, keys=[key], **kw)
+    with pytest.raises(U.UgsdrError, match="ARG"):
+        ctx.acquire(iq, U.FMT_CI8, surface_len=4092, keys=[key], **{**kw, "n_fft": 8184})
+    ctx.replica_drop(key)
+    with pytest.raises(U.UgsdrError, match="STATE"):
+        ctx.replica_drop(key)
+
+
+def test_replica_spectrum_matches_oracle(ctx, oracle):
+    say("case replica spectrum")
+    code = U.gps_l1ca(3)
+    key = ctx.replica_put(code, 4, 16368)
+    spec = ctx.replica_get(key, 16368)
+    ref = np.conj(np.fft.fft(oracle.repeat_upsample(code, 4, 16368).astype(np.float64)))
+    assert np.max(np.abs(spec - ref)) < 1e-5 * np.max(np.abs(ref))
+    ctx.replica_drop(key)

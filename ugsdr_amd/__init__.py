@@ -74,6 +74,8 @@ SIGNATURES = {
     "ugsdr_hip_ctx_destroy": ([VP], C.c_int),
     "ugsdr_hip_ctx_stream": ([VP, C.POINTER(VP)], C.c_int),
     "ugsdr_hip_ctx_device_bytes": ([VP], I64),
+    "ugsdr_hip_ctx_set_profiling": ([VP, C.c_int], C.c_int),
+    "ugsdr_hip_ctx_last_times": ([VP, VP], C.c_int),
     "ugsdr_code_gps_l1ca": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5i": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5q": ([C.c_int, VP], C.c_int),
@@ -221,6 +223,15 @@ class Context:
 
     def device_bytes(self) -> int:
         return lib().ugsdr_hip_ctx_device_bytes(self._p)
+
+    def set_profiling(self, enable: bool = True):
+        _check(lib().ugsdr_hip_ctx_set_profiling(self._p, int(enable)))
+
+    def last_times(self) -> dict:
+        """HIP-event stage times (ms) of the last acquire: front, classes, corr, final"""
+        ms = np.zeros(4, np.float32)
+        _check(lib().ugsdr_hip_ctx_last_times(self._p, _ptr(ms)))
+        return dict(zip(("front", "classes", "corr", "final"), map(float, ms)))
 
     # ---- replica cache
     def replica_put(self, chips: np.ndarray, repeats: int, n_fft: int, precision: int = PREC_FP32) -> int:

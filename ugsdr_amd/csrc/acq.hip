@@ -26,6 +26,7 @@
 #include "dev_fft.hpp"
 #include "plan.hpp"
 #include "internal.hpp"
+#include "acq.hpp"
 
 using namespace ugsdr_dev;
 
@@ -127,8 +128,9 @@ __device__ __forceinline__ float2 fwd_source(const FwdArgs& a, int blk, int64_t 
     return make_float2(a.samples[n], 0.f);
 }
 
-// waves per SIMD the LDS footprint allows: 4 for N <= 4096, else 2
-constexpr int waves_for(int CAP) { return CAP <= 4096 ? 4 : 2; }
+// waves per SIMD to budget registers for: 3 for N <= 4096 (three 256-thread
+// workgroups per CU), else 2 (LDS holds one or two transforms per CU)
+constexpr int waves_for(int CAP) { return CAP <= 4096 ? 3 : 2; }
 
 template <int T, int CAP, class P>
 __global__ __launch_bounds__(T, waves_for(CAP)) void fwd_small(FwdArgs a) {
@@ -235,21 +237,39 @@ __global__ __launch_bounds__(T, 2) void fwd_large(FwdArgs a) {
 }
 
 // ------------------------------------------------------------------ correlation
+// One search request inside a correlation batch.  Every request of a batch
+// shares N, the surface length and the fold mode (one kernel instantiation);
+// K, the Doppler bins and the PRN set are per request.
+struct ReqDesc {
+    const float2* X;             // [K][F][N] class spectra
+    const float2* const* specs;  // [P] device pointers to conj replica spectra
+    const int* cls;              // [D] class of each Doppler bin
+    const int* shift;            // [D] integer bin rotation of each Doppler bin
+    RowStat* rowstats;           // [P*D]
+    float* surfaces;             // optional [P*D][L]
+    int K, F, D, P;
+    int row_start;               // first batch row of this request
+    int pad;
+};
+
 struct CorrArgs {
     RadixPlan plan;
     FourStep fs;
-    int N, K, F, L, D, P;
+    int N, L;
     int fold_mode;          // 0: |.| then fold, 1: fold complex then |.|
-    int rows;
-    const float2* X;        // [K][F][N]
-    const float2* const* specs;  // [P] device pointers to conj spectra
-    const int* cls;         // [D]
-    const int* shift;       // [D]
-    RowStat* rowstats;      // [P*D]
-    float* surfaces;        // optional [P*D][L]
+    int rows;               // total rows (sum of P*D)
+    int n_req;
+    const ReqDesc* reqs;    // [n_req], ascending row_start
     float2* scratch;        // large path: per workgroup 2N complex + L floats
     float inv_n;
 };
+
+// the request owning a batch row (uniform per workgroup: kept in scalar registers)
+__device__ __forceinline__ int find_req(const CorrArgs& a, int row) {
+    int r = 0;
+    while (r + 1 < a.n_req && a.reqs[r + 1].row_start <= row) ++r;
+    return __builtin_amdgcn_readfirstlane(r);
+}
 
 // block-wide reductions over T threads (T/64 waves)
 template <int T>
@@ -384,15 +404,17 @@ __global__ __launch_bounds__(T, waves_for(CAP)) void corr_small(CorrArgs a) {
     tw_build(lo, hi, a.N, tid, T);
     const int N = a.N, L = a.L, R = a.N / a.L;
     for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
-        const int p = row / a.D, di = row - (row / a.D) * a.D;
-        const int cls = a.cls[di], sh = a.shift[di];
-        const float2* __restrict__ C = a.specs[p];
+        const ReqDesc q = a.reqs[find_req(a, row)];
+        const int local = row - q.row_start;
+        const int p = local / q.D, di = local - p * q.D;
+        const int cls = q.cls[di], sh = q.shift[di];
+        const float2* __restrict__ C = q.specs[p];
         float S[SR];
 #pragma unroll
         for (int i = 0; i < SR; ++i) S[i] = 0.f;
-        for (int k = 0; k < a.K; ++k) {
+        for (int k = 0; k < q.K; ++k) {
             const int tq = opaque(tid);
-            const float2* __restrict__ Xk = a.X + ((int64_t)k * a.F + cls) * N;
+            const float2* __restrict__ Xk = q.X + ((int64_t)k * q.F + cls) * N;
             for (int i = tq; i < N; i += T) {
                 int src = i + sh;
                 src -= src >= N ? N : 0;
@@ -439,9 +461,9 @@ __global__ __launch_bounds__(T, waves_for(CAP)) void corr_small(CorrArgs a) {
             }
             __syncthreads();
         }
-        row_stats_reg<T, SR>(S, L, red, tid, a.rowstats + row, a.inv_n);
-        if (a.surfaces) {
-            float* o = a.surfaces + (int64_t)row * L;
+        row_stats_reg<T, SR>(S, L, red, tid, q.rowstats + local, a.inv_n);
+        if (q.surfaces) {
+            float* o = q.surfaces + (int64_t)local * L;
 #pragma unroll
             for (int i = 0; i < SR; ++i) {
                 const int j = tid + i * T;
@@ -465,12 +487,14 @@ __global__ __launch_bounds__(T, 2) void corr_large(CorrArgs a) {
     float* A = reinterpret_cast<float*>(B);
     float* S = reinterpret_cast<float*>(B + N);
     for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
-        const int p = row / a.D, di = row - (row / a.D) * a.D;
-        const int cls = a.cls[di], sh = a.shift[di];
-        const float2* __restrict__ C = a.specs[p];
-        for (int k = 0; k < a.K; ++k) {
+        const ReqDesc q = a.reqs[find_req(a, row)];
+        const int local = row - q.row_start;
+        const int p = local / q.D, di = local - p * q.D;
+        const int cls = q.cls[di], sh = q.shift[di];
+        const float2* __restrict__ C = q.specs[p];
+        for (int k = 0; k < q.K; ++k) {
             const int tq = opaque(tid);
-            const float2* __restrict__ Xk = a.X + ((int64_t)k * a.F + cls) * N;
+            const float2* __restrict__ Xk = q.X + ((int64_t)k * q.F + cls) * N;
             const int fm = a.fold_mode;
             fourstep<T, true, P1, P2>(
                 Ld.buf, a.fs, Ld.lo1, Ld.hi1, Ld.lo2, Ld.hi2, Ld.loN, Ld.hiN, G,
@@ -498,9 +522,9 @@ __global__ __launch_bounds__(T, 2) void corr_large(CorrArgs a) {
             }
             __syncthreads();
         }
-        row_stats_glob<T>(S, L, Ld.red, tid, a.rowstats + row, a.inv_n);
-        if (a.surfaces) {
-            float* o = a.surfaces + (int64_t)row * L;
+        row_stats_glob<T>(S, L, Ld.red, tid, q.rowstats + local, a.inv_n);
+        if (q.surfaces) {
+            float* o = q.surfaces + (int64_t)local * L;
             for (int j = tid; j < L; j += T) o[j] = S[j] * a.inv_n;
         }
         __syncthreads();
@@ -582,18 +606,18 @@ static hipError_t set_lds(K kernel, size_t bytes) {
                                (int)bytes);
 }
 
-static int g_cu_count = 0;
-
-static int cu_count(int device) {
-    if (!g_cu_count) {
+int device_cu_count(int device) {
+    static int cached[64] = {0};
+    if (device < 0 || device >= 64) return 256;
+    if (!cached[device]) {
         hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, device) == hipSuccess) g_cu_count = prop.multiProcessorCount;
-        if (g_cu_count <= 0) g_cu_count = 256;
+        int n = 0;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) n = prop.multiProcessorCount;
+        cached[device] = n > 0 ? n : 256;
     }
-    return g_cu_count;
+    return cached[device];
 }
 
-// forward transform(s): `blocks` transforms of length N written to out
 // Compile-time plans for the lengths of the acquisition workloads; any other
 // length runs the runtime-plan kernels.
 using P4092 = PlanCT<31, 11, 4, 3>;
@@ -608,19 +632,19 @@ using P180 = PlanCT<5, 4, 3, 3>;
 using P240 = PlanCT<16, 5, 3>;
 
 template <int T, int CAP, class P>
-static int launch_fwd_small(ugsdr_hip_ctx* ctx, FwdArgs& a, int blocks) {
+static int launch_fwd_small(FwdArgs& a, int blocks, hipStream_t st) {
     const size_t lds = small_lds_bytes(a.N, T);
     UGSDR_HIP_TRY(set_lds(fwd_small<T, CAP, P>, lds));
-    hipLaunchKernelGGL((fwd_small<T, CAP, P>), dim3(blocks), dim3(T), lds, ctx->stream, a);
+    hipLaunchKernelGGL((fwd_small<T, CAP, P>), dim3(blocks), dim3(T), lds, st, a);
     return UGSDR_OK;
 }
 
 template <class P1, class P2>
-static int launch_fwd_large(ugsdr_hip_ctx* ctx, FwdArgs& a, int blocks) {
+static int launch_fwd_large(FwdArgs& a, int blocks, hipStream_t st) {
     constexpr int T = 512;
     const size_t lds = large_lds_bytes(a.fs);
     UGSDR_HIP_TRY(set_lds(fwd_large<T, P1, P2>, lds));
-    hipLaunchKernelGGL((fwd_large<T, P1, P2>), dim3(blocks), dim3(T), lds, ctx->stream, a);
+    hipLaunchKernelGGL((fwd_large<T, P1, P2>), dim3(blocks), dim3(T), lds, st, a);
     return UGSDR_OK;
 }
 
@@ -637,61 +661,64 @@ static bool fourstep_341(int N, FourStep& f) {
     return true;
 }
 
-// forward transform(s): `blocks` transforms of length N written to out
-int launch_forward(ugsdr_hip_ctx* ctx, FwdArgs& a, int blocks) {
+static bool plan_transform(int N, bool& small, RadixPlan& plan, FourStep& fs) {
+    small = N <= kLdsCap;
+    if (small) return make_plan(N, plan);
+    return fourstep_341(N, fs) || make_fourstep(N, fs);
+}
+
+// forward transform(s): `blocks` transforms of length a.N written to a.out
+static int launch_forward(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratch) {
     int rc = UGSDR_OK;
-    if (a.N <= kLdsCap) {
-        if (!make_plan(a.N, a.plan)) {
-            set_error("unsupported transform length " + std::to_string(a.N));
-            return UGSDR_ERR_UNSUPPORTED;
-        }
-        switch (a.N) {
-            case 4092: rc = launch_fwd_small<256, 4096, P4092>(ctx, a, blocks); break;
-            case 8184: rc = launch_fwd_small<256, 8192, P8184>(ctx, a, blocks); break;
-            case 12276: rc = launch_fwd_small<512, kLdsCap, P12276>(ctx, a, blocks); break;
-            case 16368: rc = launch_fwd_small<512, kLdsCap, P16368>(ctx, a, blocks); break;
-            default:
-                if (a.N <= 4096) rc = launch_fwd_small<256, 4096, PlanRT>(ctx, a, blocks);
-                else if (a.N <= 8192) rc = launch_fwd_small<256, 8192, PlanRT>(ctx, a, blocks);
-                else rc = launch_fwd_small<512, kLdsCap, PlanRT>(ctx, a, blocks);
-        }
-        if (rc) return rc;
-    } else {
-        const bool ct = fourstep_341(a.N, a.fs);
-        if (!ct && !make_fourstep(a.N, a.fs)) {
-            set_error("unsupported transform length " + std::to_string(a.N));
-            return UGSDR_ERR_UNSUPPORTED;
-        }
-        UGSDR_HIP_TRY(ctx->aux2.reserve((size_t)blocks * a.N * sizeof(float2)));
-        a.scratch = ctx->aux2.as<float2>();
-        if (!ct) rc = launch_fwd_large<PlanRT, PlanRT>(ctx, a, blocks);
-        else switch (a.fs.n1) {
-            case 60: rc = launch_fwd_large<P60, P341>(ctx, a, blocks); break;
-            case 112: rc = launch_fwd_large<P112, P341>(ctx, a, blocks); break;
-            case 120: rc = launch_fwd_large<P120, P341>(ctx, a, blocks); break;
-            case 180: rc = launch_fwd_large<P180, P341>(ctx, a, blocks); break;
-            default: rc = launch_fwd_large<P240, P341>(ctx, a, blocks); break;
-        }
-        if (rc) return rc;
+    bool small;
+    if (!plan_transform(a.N, small, a.plan, a.fs)) {
+        set_error("unsupported transform length " + std::to_string(a.N));
+        return UGSDR_ERR_UNSUPPORTED;
     }
+    if (small) {
+        switch (a.N) {
+            case 4092: rc = launch_fwd_small<256, 4096, P4092>(a, blocks, st); break;
+            case 8184: rc = launch_fwd_small<256, 8192, P8184>(a, blocks, st); break;
+            case 12276: rc = launch_fwd_small<512, kLdsCap, P12276>(a, blocks, st); break;
+            case 16368: rc = launch_fwd_small<512, kLdsCap, P16368>(a, blocks, st); break;
+            default:
+                if (a.N <= 4096) rc = launch_fwd_small<256, 4096, PlanRT>(a, blocks, st);
+                else if (a.N <= 8192) rc = launch_fwd_small<256, 8192, PlanRT>(a, blocks, st);
+                else rc = launch_fwd_small<512, kLdsCap, PlanRT>(a, blocks, st);
+        }
+    } else {
+        UGSDR_HIP_TRY(scratch.reserve((size_t)blocks * a.N * sizeof(float2)));
+        a.scratch = scratch.as<float2>();
+        FourStep probe;
+        const bool ct = fourstep_341(a.N, probe);
+        if (!ct) rc = launch_fwd_large<PlanRT, PlanRT>(a, blocks, st);
+        else switch (a.fs.n1) {
+            case 60: rc = launch_fwd_large<P60, P341>(a, blocks, st); break;
+            case 112: rc = launch_fwd_large<P112, P341>(a, blocks, st); break;
+            case 120: rc = launch_fwd_large<P120, P341>(a, blocks, st); break;
+            case 180: rc = launch_fwd_large<P180, P341>(a, blocks, st); break;
+            default: rc = launch_fwd_large<P240, P341>(a, blocks, st); break;
+        }
+    }
+    if (rc) return rc;
     UGSDR_HIP_TRY(hipGetLastError());
     return UGSDR_OK;
 }
 
 template <int T, int CAP, class P>
-static int launch_corr_small(ugsdr_hip_ctx* ctx, const CorrArgs& c, int grid) {
+static int launch_corr_small(const CorrArgs& c, int grid, hipStream_t st) {
     const size_t lds = small_lds_bytes(c.N, T);
     UGSDR_HIP_TRY(set_lds(corr_small<T, CAP, P>, lds));
-    hipLaunchKernelGGL((corr_small<T, CAP, P>), dim3(grid), dim3(T), lds, ctx->stream, c);
+    hipLaunchKernelGGL((corr_small<T, CAP, P>), dim3(grid), dim3(T), lds, st, c);
     return UGSDR_OK;
 }
 
 template <class P1, class P2>
-static int launch_corr_large(ugsdr_hip_ctx* ctx, const CorrArgs& c, int grid) {
+static int launch_corr_large(const CorrArgs& c, int grid, hipStream_t st) {
     constexpr int T = 512;
     const size_t lds = large_lds_bytes(c.fs);
     UGSDR_HIP_TRY(set_lds(corr_large<T, P1, P2>, lds));
-    hipLaunchKernelGGL((corr_large<T, P1, P2>), dim3(grid), dim3(T), lds, ctx->stream, c);
+    hipLaunchKernelGGL((corr_large<T, P1, P2>), dim3(grid), dim3(T), lds, st, c);
     return UGSDR_OK;
 }
 
@@ -715,13 +742,13 @@ int replica_spectrum(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chips, i
         a.mode = 2;
         a.samples = ctx->aux.as<float>();
     }
-    int rc = launch_forward(ctx, a, 1);
+    int rc = launch_forward(a, 1, ctx->stream, ctx->aux2);
     if (rc) return rc;
     UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return UGSDR_OK;
 }
 
-// Doppler bins -> (class, integer shift)
+// Doppler bins -> (class, integer shift): d N / fs = m + phi
 static void doppler_classes(const double* dop, int64_t D, int64_t N, double fs, std::vector<double>& cls_frac,
                             std::vector<int>& cls, std::vector<int>& shift) {
     cls_frac.clear();
@@ -746,9 +773,16 @@ static void doppler_classes(const double* dop, int64_t D, int64_t N, double fs, 
 
 static int64_t gcd64(int64_t a, int64_t b) { while (b) { int64_t t = a % b; a = b; b = t; } return a; }
 
-int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_result* out, float* surfaces,
-                 float* peaks) {
-    UGSDR_CHECK_ARG(r && out, "null request/result");
+int64_t AcqPlan::footprint(bool surfaces) const {
+    int64_t b = n_in_used * (int64_t)bps + need * 8 + K * F * N * 8 + (int64_t)table_bytes + P * D * 32 +
+                P * (int64_t)sizeof(ugsdr_acq_result) + P * L * 4;
+    if (!small) b += K * F * N * 8;
+    if (surfaces) b += P * D * L * 4;
+    return b;
+}
+
+int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
+    UGSDR_CHECK_ARG(r, "null request");
     UGSDR_CHECK_ARG(r->iq && r->n_samples > 0, "empty signal");
     UGSDR_CHECK_ARG(r->fmt >= UGSDR_FMT_CI8 && r->fmt <= UGSDR_FMT_R8, "unknown sample format");
     UGSDR_CHECK_ARG(r->n_fft > 0 && r->n_blocks > 0 && r->surface_len > 0, "bad sizes");
@@ -763,189 +797,246 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
         set_error("surface_len must equal n_fft without reshape");
         return UGSDR_ERR_ARG;
     }
-    if (r->precision != UGSDR_PREC_FP32) return acquire_reduced(ctx, r, out, surfaces, peaks);
-    const int64_t N = r->n_fft, K = r->n_blocks, L = r->surface_len, D = r->n_doppler, P = r->n_prn;
-    if (N > (1 << 22) || D * P > (1 << 30)) {
+    pl.N = r->n_fft;
+    pl.K = r->n_blocks;
+    pl.L = r->surface_len;
+    pl.D = r->n_doppler;
+    pl.P = r->n_prn;
+    if (pl.N > (1 << 22) || pl.D * pl.P > (1 << 30) || pl.K > (1 << 20)) {
         set_error("search too large");
         return UGSDR_ERR_UNSUPPORTED;
     }
-    // ---- replicas
-    std::vector<const float2*> specs(P);
-    for (int64_t p = 0; p < P; ++p) {
+    pl.reshape = r->reshape;
+    pl.fold_mode = (r->reshape && !r->coherent) ? 1 : 0;
+    pl.specs.resize(pl.P);
+    for (int64_t p = 0; p < pl.P; ++p) {
         auto it = ctx->replicas.find(r->replica_keys[p]);
         if (it == ctx->replicas.end()) {
             set_error("unknown replica key");
             return UGSDR_ERR_STATE;
         }
-        if (it->second.n_fft != N || it->second.precision != UGSDR_PREC_FP32) {
+        if (it->second.n_fft != pl.N || it->second.precision != r->precision) {
             set_error("replica length/precision does not match the request");
             return UGSDR_ERR_ARG;
         }
-        specs[p] = static_cast<const float2*>(it->second.spec);
+        pl.specs[p] = static_cast<const float2*>(it->second.spec);
     }
-    // ---- front end: unpack + mix + resample
+    if (!plan_transform((int)pl.N, pl.small, pl.plan, pl.fs)) {
+        set_error("unsupported transform length " + std::to_string(pl.N));
+        return UGSDR_ERR_UNSUPPORTED;
+    }
+    // front end: unpack + mix + resample
     const int64_t fs_in = (int64_t)std::llround(r->fs_in), fs_acq = (int64_t)std::llround(r->fs_acq);
-    int64_t up = 1, down = 1;
+    pl.up = pl.down = 1;
     if (fs_in != fs_acq) {
         const int64_t g = gcd64(fs_acq, fs_in);
         const int64_t lcm = fs_acq / g * fs_in;
-        up = lcm / fs_in;
-        down = lcm / fs_acq;
+        pl.up = lcm / fs_in;
+        pl.down = lcm / fs_acq;
     }
-    const int64_t need = N * K;
-    const int64_t avail_out = (up == 1 && down == 1) ? r->n_samples : (r->n_samples * up + down - 1) / down;
-    if (avail_out < need) {
+    pl.need = pl.N * pl.K;
+    const int64_t avail = (pl.up == 1 && pl.down == 1) ? r->n_samples : (r->n_samples * pl.up + pl.down - 1) / pl.down;
+    if (avail < pl.need) {
         set_error("signal shorter than n_blocks * n_fft at fs_acq");
         return UGSDR_ERR_SIZE;
     }
+    pl.fmt = r->fmt;
+    pl.n_in_used = std::min<int64_t>(r->n_samples, (pl.need * pl.down + pl.up - 1) / pl.up + 1);
+    pl.bps = r->fmt == UGSDR_FMT_CI8 ? 2 : r->fmt == UGSDR_FMT_CI16 ? 4 : r->fmt == UGSDR_FMT_CF32 ? 8 : 1;
+    double f = r->mix_frequency;
+    if (f < 0) f = r->fs_in + f;  // Mixer::Translate folding (mixer.hpp:46-47)
+    double ph = std::fmod(r->mix_phase, 2 * M_PI);
+    if (ph < 0) ph += 2 * M_PI;
+    pl.apply_tone = (f != 0.0 || ph != 0.0) ? 1 : 0;
+    pl.f_turns = f / r->fs_in;
+    pl.ph_turns = ph / (2 * M_PI);
+    pl.ratio = r->fs_in / r->fs_acq;
+    pl.intermediate_frequency = r->intermediate_frequency;
+    pl.threshold = r->threshold;
+    // Doppler classes
+    pl.doppler.assign(r->doppler, r->doppler + pl.D);
+    doppler_classes(r->doppler, pl.D, pl.N, r->fs_acq, pl.cls_frac, pl.cls, pl.shift);
+    pl.F = (int)pl.cls_frac.size();
+    pl.off_frac = 0;
+    pl.off_dop = pl.off_frac + pl.cls_frac.size() * 8;
+    pl.off_cls = pl.off_dop + pl.D * 8;
+    pl.off_shift = pl.off_cls + pl.D * 4;
+    pl.off_specs = (pl.off_shift + pl.D * 4 + 7) / 8 * 8;
+    pl.table_bytes = pl.off_specs + pl.P * 8;
+    return UGSDR_OK;
+}
+
+int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st, bool keep_surf) {
+    // tables
+    s.host_tables.resize(pl.table_bytes);
+    char* hp = s.host_tables.data();
+    memcpy(hp + pl.off_frac, pl.cls_frac.data(), pl.cls_frac.size() * 8);
+    memcpy(hp + pl.off_dop, pl.doppler.data(), pl.D * 8);
+    memcpy(hp + pl.off_cls, pl.cls.data(), pl.D * 4);
+    memcpy(hp + pl.off_shift, pl.shift.data(), pl.D * 4);
+    memcpy(hp + pl.off_specs, pl.specs.data(), pl.P * 8);
+    UGSDR_HIP_TRY(s.tables.reserve(pl.table_bytes));
+    UGSDR_HIP_TRY(hipMemcpyAsync(s.tables.ptr, hp, pl.table_bytes, hipMemcpyHostToDevice, st));
+    // input
     const void* iq = r->iq;
-    const int64_t n_in_used = std::min<int64_t>(r->n_samples, (need * down + up - 1) / up + 1);
     if (!r->iq_on_device) {
-        const size_t bps = r->fmt == UGSDR_FMT_CI8 ? 2 : r->fmt == UGSDR_FMT_CI16 ? 4 : r->fmt == UGSDR_FMT_CF32 ? 8 : 1;
-        UGSDR_HIP_TRY(ctx->iq.reserve(n_in_used * bps));
-        UGSDR_HIP_TRY(hipMemcpyAsync(ctx->iq.ptr, r->iq, n_in_used * bps, hipMemcpyHostToDevice, ctx->stream));
-        iq = ctx->iq.ptr;
+        UGSDR_HIP_TRY(s.iq.reserve(pl.n_in_used * pl.bps));
+        UGSDR_HIP_TRY(hipMemcpyAsync(s.iq.ptr, r->iq, pl.n_in_used * pl.bps, hipMemcpyHostToDevice, st));
+        iq = s.iq.ptr;
     }
-    UGSDR_HIP_TRY(ctx->sig.reserve((size_t)need * sizeof(float2)));
-    {
-        FrontArgs fa{};
-        fa.iq = iq;
-        fa.fmt = r->fmt;
-        fa.n_in = n_in_used;
-        fa.n_out = need;
-        double f = r->mix_frequency;
-        if (f < 0) f = r->fs_in + f;  // Mixer::Translate folding (mixer.hpp:46-47)
-        double ph = std::fmod(r->mix_phase, 2 * M_PI);
-        if (ph < 0) ph += 2 * M_PI;
-        fa.apply_tone = (f != 0.0 || ph != 0.0) ? 1 : 0;
-        fa.f_turns = f / r->fs_in;
-        fa.ph_turns = ph / (2 * M_PI);
-        fa.up = up;
-        fa.down = down;
-        fa.out = ctx->sig.as<float2>();
-        const int blocks = (int)std::min<int64_t>((need + 255) / 256, 4096);
-        hipLaunchKernelGGL(front_kernel, dim3(blocks), dim3(256), 0, ctx->stream, fa);
-        UGSDR_HIP_TRY(hipGetLastError());
+    UGSDR_HIP_TRY(s.sig.reserve((size_t)pl.need * sizeof(float2)));
+    UGSDR_HIP_TRY(s.rows.reserve((size_t)pl.P * pl.D * sizeof(RowStat)));
+    if (keep_surf) UGSDR_HIP_TRY(s.surf.reserve((size_t)pl.P * pl.D * pl.L * sizeof(float)));
+    FrontArgs fa{};
+    fa.iq = iq;
+    fa.fmt = pl.fmt;
+    fa.n_in = pl.n_in_used;
+    fa.n_out = pl.need;
+    fa.apply_tone = pl.apply_tone;
+    fa.f_turns = pl.f_turns;
+    fa.ph_turns = pl.ph_turns;
+    fa.up = pl.up;
+    fa.down = pl.down;
+    fa.out = s.sig.as<float2>();
+    const int blocks = (int)std::min<int64_t>((pl.need + 255) / 256, 4096);
+    hipLaunchKernelGGL(front_kernel, dim3(blocks), dim3(256), 0, st, fa);
+    UGSDR_HIP_TRY(hipGetLastError());
+    return UGSDR_OK;
+}
+
+int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
+    UGSDR_HIP_TRY(s.classes.reserve((size_t)pl.K * pl.F * pl.N * sizeof(float2)));
+    FwdArgs a{};
+    a.N = (int)pl.N;
+    a.F = pl.F;
+    a.sig = s.sig.as<float2>();
+    a.class_frac = reinterpret_cast<const double*>(s.tables.as<char>() + pl.off_frac);
+    a.out = s.classes.as<float2>();
+    a.mode = 0;
+    return launch_forward(a, (int)(pl.K * pl.F), st, s.fwd_scratch);
+}
+
+int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots, hipStream_t st,
+                 DevBuf& desc, DevBuf& scratch, int cus) {
+    const AcqPlan& p0 = *plans[0];
+    std::vector<ReqDesc> reqs(plans.size());
+    int rows = 0;
+    for (size_t i = 0; i < plans.size(); ++i) {
+        const AcqPlan& pl = *plans[i];
+        AcqSlot& s = *slots[i];
+        if (pl.bucket() != p0.bucket() || pl.L != p0.L) {
+            set_error("batch mixes transform shapes");
+            return UGSDR_ERR_ARG;
+        }
+        char* dp = s.tables.as<char>();
+        ReqDesc& q = reqs[i];
+        q.X = s.classes.as<float2>();
+        q.specs = reinterpret_cast<const float2* const*>(dp + pl.off_specs);
+        q.cls = reinterpret_cast<const int*>(dp + pl.off_cls);
+        q.shift = reinterpret_cast<const int*>(dp + pl.off_shift);
+        q.rowstats = s.rows.as<RowStat>();
+        q.surfaces = s.surf.ptr ? s.surf.as<float>() : nullptr;
+        q.K = (int)pl.K;
+        q.F = pl.F;
+        q.D = (int)pl.D;
+        q.P = (int)pl.P;
+        q.row_start = rows;
+        rows += (int)(pl.P * pl.D);
     }
-    // ---- Doppler classes and tables
-    std::vector<double> cls_frac;
-    std::vector<int> cls, shift;
-    doppler_classes(r->doppler, D, N, r->fs_acq, cls_frac, cls, shift);
-    const int F = (int)cls_frac.size();
-    const size_t tab_bytes = cls_frac.size() * 8 + (size_t)D * 8 + (size_t)D * 4 * 2 + (size_t)P * 8 + 64;
-    UGSDR_HIP_TRY(ctx->tables.reserve(tab_bytes));
-    std::vector<char> host_tab(tab_bytes);
-    char* hp = host_tab.data();
-    size_t off_frac = 0, off_dop = off_frac + cls_frac.size() * 8, off_cls = off_dop + D * 8,
-           off_shift = off_cls + D * 4, off_specs = (off_shift + D * 4 + 7) / 8 * 8;
-    memcpy(hp + off_frac, cls_frac.data(), cls_frac.size() * 8);
-    memcpy(hp + off_dop, r->doppler, D * 8);
-    memcpy(hp + off_cls, cls.data(), D * 4);
-    memcpy(hp + off_shift, shift.data(), D * 4);
-    memcpy(hp + off_specs, specs.data(), P * 8);
-    UGSDR_HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, hp, off_specs + P * 8, hipMemcpyHostToDevice, ctx->stream));
-    char* dp = ctx->tables.as<char>();
-    // ---- forward transforms of each block for each class
-    UGSDR_HIP_TRY(ctx->classes.reserve((size_t)K * F * N * sizeof(float2)));
-    {
-        FwdArgs a{};
-        a.N = (int)N;
-        a.F = F;
-        a.sig = ctx->sig.as<float2>();
-        a.class_frac = reinterpret_cast<const double*>(dp + off_frac);
-        a.out = ctx->classes.as<float2>();
-        a.mode = 0;
-        int rc = launch_forward(ctx, a, (int)(K * F));
-        if (rc) return rc;
-    }
-    // ---- correlation rows
-    const int rows = (int)(P * D);
-    UGSDR_HIP_TRY(ctx->rows.reserve((size_t)rows * sizeof(RowStat)));
-    const bool keep_surf = surfaces || peaks;
-    if (keep_surf) UGSDR_HIP_TRY(ctx->surf.reserve((size_t)rows * L * sizeof(float)));
+    UGSDR_HIP_TRY(desc.reserve(reqs.size() * sizeof(ReqDesc)));
+    UGSDR_HIP_TRY(hipMemcpyAsync(desc.ptr, reqs.data(), reqs.size() * sizeof(ReqDesc), hipMemcpyHostToDevice, st));
     CorrArgs c{};
-    c.N = (int)N;
-    c.K = (int)K;
-    c.F = F;
-    c.L = (int)L;
-    c.D = (int)D;
-    c.P = (int)P;
-    c.fold_mode = (r->reshape && !r->coherent) ? 1 : 0;
+    c.plan = p0.plan;
+    c.fs = p0.fs;
+    c.N = (int)p0.N;
+    c.L = (int)p0.L;
+    c.fold_mode = p0.fold_mode;
     c.rows = rows;
-    c.X = ctx->classes.as<float2>();
-    c.specs = reinterpret_cast<const float2* const*>(dp + off_specs);
-    c.cls = reinterpret_cast<const int*>(dp + off_cls);
-    c.shift = reinterpret_cast<const int*>(dp + off_shift);
-    c.rowstats = ctx->rows.as<RowStat>();
-    c.surfaces = keep_surf ? ctx->surf.as<float>() : nullptr;
-    c.inv_n = (float)(1.0 / (double)N);
-    const int cus = cu_count(ctx->device);
-    if (N <= kLdsCap) {
-        if (!make_plan((int)N, c.plan)) {
-            set_error("unsupported transform length " + std::to_string(N));
-            return UGSDR_ERR_UNSUPPORTED;
-        }
+    c.n_req = (int)reqs.size();
+    c.reqs = desc.as<ReqDesc>();
+    c.inv_n = (float)(1.0 / (double)p0.N);
+    int rc = UGSDR_OK;
+    if (p0.small) {
         const int grid = rows;
-        int rc;
-        switch (N) {
-            case 4092: rc = launch_corr_small<256, 4096, P4092>(ctx, c, grid); break;
-            case 8184: rc = launch_corr_small<256, 8192, P8184>(ctx, c, grid); break;
-            case 12276: rc = launch_corr_small<512, kLdsCap, P12276>(ctx, c, grid); break;
-            case 16368: rc = launch_corr_small<512, kLdsCap, P16368>(ctx, c, grid); break;
+        switch (c.N) {
+            case 4092: rc = launch_corr_small<256, 4096, P4092>(c, grid, st); break;
+            case 8184: rc = launch_corr_small<256, 8192, P8184>(c, grid, st); break;
+            case 12276: rc = launch_corr_small<512, kLdsCap, P12276>(c, grid, st); break;
+            case 16368: rc = launch_corr_small<512, kLdsCap, P16368>(c, grid, st); break;
             default:
-                if (N <= 4096) rc = launch_corr_small<256, 4096, PlanRT>(ctx, c, grid);
-                else if (N <= 8192) rc = launch_corr_small<256, 8192, PlanRT>(ctx, c, grid);
-                else rc = launch_corr_small<512, kLdsCap, PlanRT>(ctx, c, grid);
+                if (c.N <= 4096) rc = launch_corr_small<256, 4096, PlanRT>(c, grid, st);
+                else if (c.N <= 8192) rc = launch_corr_small<256, 8192, PlanRT>(c, grid, st);
+                else rc = launch_corr_small<512, kLdsCap, PlanRT>(c, grid, st);
         }
-        if (rc) return rc;
     } else {
-        const bool ct = fourstep_341((int)N, c.fs);
-        if (!ct && !make_fourstep((int)N, c.fs)) {
-            set_error("unsupported transform length " + std::to_string(N));
-            return UGSDR_ERR_UNSUPPORTED;
-        }
         const int grid = std::min(rows, cus);
-        const size_t per_wg = (2 * (size_t)N + (L + 1) / 2) * sizeof(float2);
-        UGSDR_HIP_TRY(ctx->scratch.reserve(per_wg * grid));
-        c.scratch = ctx->scratch.as<float2>();
-        int rc;
-        if (!ct) rc = launch_corr_large<PlanRT, PlanRT>(ctx, c, grid);
+        const size_t per_wg = (2 * (size_t)c.N + (c.L + 1) / 2) * sizeof(float2);
+        UGSDR_HIP_TRY(scratch.reserve(per_wg * grid));
+        c.scratch = scratch.as<float2>();
+        FourStep probe;
+        if (!fourstep_341(c.N, probe)) rc = launch_corr_large<PlanRT, PlanRT>(c, grid, st);
         else switch (c.fs.n1) {
-            case 60: rc = launch_corr_large<P60, P341>(ctx, c, grid); break;
-            case 112: rc = launch_corr_large<P112, P341>(ctx, c, grid); break;
-            case 120: rc = launch_corr_large<P120, P341>(ctx, c, grid); break;
-            case 180: rc = launch_corr_large<P180, P341>(ctx, c, grid); break;
-            default: rc = launch_corr_large<P240, P341>(ctx, c, grid); break;
+            case 60: rc = launch_corr_large<P60, P341>(c, grid, st); break;
+            case 112: rc = launch_corr_large<P112, P341>(c, grid, st); break;
+            case 120: rc = launch_corr_large<P120, P341>(c, grid, st); break;
+            case 180: rc = launch_corr_large<P180, P341>(c, grid, st); break;
+            default: rc = launch_corr_large<P240, P341>(c, grid, st); break;
         }
-        if (rc) return rc;
     }
+    if (rc) return rc;
     UGSDR_HIP_TRY(hipGetLastError());
-    // ---- finalize
-    UGSDR_HIP_TRY(ctx->results.reserve((size_t)P * sizeof(ugsdr_acq_result) + (peaks ? (size_t)P * L * 4 : 0)));
+    return UGSDR_OK;
+}
+
+int enqueue_final(const AcqPlan& pl, AcqSlot& s, hipStream_t st, bool peaks) {
+    UGSDR_HIP_TRY(s.results.reserve((size_t)pl.P * sizeof(ugsdr_acq_result) + (peaks ? (size_t)pl.P * pl.L * 4 : 0)));
     FinalArgs fa{};
-    fa.rowstats = ctx->rows.as<RowStat>();
-    fa.doppler = reinterpret_cast<const double*>(dp + off_dop);
-    fa.D = (int)D;
-    fa.P = (int)P;
-    fa.ratio = r->fs_in / r->fs_acq;
-    fa.intermediate_frequency = r->intermediate_frequency;
-    fa.threshold = r->threshold;
-    fa.reshape = r->reshape;
-    fa.out = ctx->results.as<ugsdr_acq_result>();
-    fa.surfaces = keep_surf ? ctx->surf.as<float>() : nullptr;
-    fa.peaks = peaks ? reinterpret_cast<float*>(ctx->results.as<char>() + (size_t)P * sizeof(ugsdr_acq_result))
+    fa.rowstats = s.rows.as<RowStat>();
+    fa.doppler = reinterpret_cast<const double*>(s.tables.as<char>() + pl.off_dop);
+    fa.D = (int)pl.D;
+    fa.P = (int)pl.P;
+    fa.ratio = pl.ratio;
+    fa.intermediate_frequency = pl.intermediate_frequency;
+    fa.threshold = pl.threshold;
+    fa.reshape = pl.reshape;
+    fa.out = s.results.as<ugsdr_acq_result>();
+    fa.surfaces = s.surf.ptr ? s.surf.as<float>() : nullptr;
+    fa.peaks = peaks ? reinterpret_cast<float*>(s.results.as<char>() + (size_t)pl.P * sizeof(ugsdr_acq_result))
                      : nullptr;
-    fa.L = (int)L;
-    hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)P), dim3(256), 0, ctx->stream, fa);
+    fa.L = (int)pl.L;
+    hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)pl.P), dim3(256), 0, st, fa);
     UGSDR_HIP_TRY(hipGetLastError());
-    UGSDR_HIP_TRY(hipMemcpyAsync(out, fa.out, (size_t)P * sizeof(ugsdr_acq_result), hipMemcpyDeviceToHost,
-                                 ctx->stream));
+    return UGSDR_OK;
+}
+
+int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_result* out, float* surfaces,
+                 float* peaks) {
+    UGSDR_CHECK_ARG(out, "null result");
+    if (r && r->precision != UGSDR_PREC_FP32) return acquire_reduced(ctx, r, out, surfaces, peaks);
+    AcqPlan pl;
+    if (int rc = make_acq_plan(ctx, r, pl)) return rc;
+    AcqSlot& s = ctx->slot;
+    const bool keep_surf = surfaces || peaks;
+    if (!keep_surf) s.surf.release();
+    hipStream_t st = ctx->stream;
+    ctx->mark(0);
+    if (int rc = enqueue_front(pl, r, s, st, keep_surf)) return rc;
+    ctx->mark(1);
+    if (int rc = enqueue_classes(pl, s, st)) return rc;
+    ctx->mark(2);
+    if (int rc = enqueue_corr({&pl}, {&s}, st, ctx->desc, ctx->scratch, device_cu_count(ctx->device))) return rc;
+    ctx->mark(3);
+    if (int rc = enqueue_final(pl, s, st, peaks != nullptr)) return rc;
+    ctx->mark(4);
+    UGSDR_HIP_TRY(hipMemcpyAsync(out, s.results.ptr, (size_t)pl.P * sizeof(ugsdr_acq_result), hipMemcpyDeviceToHost, st));
     if (peaks)
-        UGSDR_HIP_TRY(hipMemcpyAsync(peaks, fa.peaks, (size_t)P * L * 4, hipMemcpyDeviceToHost, ctx->stream));
+        UGSDR_HIP_TRY(hipMemcpyAsync(peaks, s.results.as<char>() + (size_t)pl.P * sizeof(ugsdr_acq_result),
+                                     (size_t)pl.P * pl.L * 4, hipMemcpyDeviceToHost, st));
     if (surfaces)
-        UGSDR_HIP_TRY(hipMemcpyAsync(surfaces, ctx->surf.ptr, (size_t)rows * L * 4, hipMemcpyDeviceToHost,
-                                     ctx->stream));
-    UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        UGSDR_HIP_TRY(hipMemcpyAsync(surfaces, s.surf.ptr, (size_t)pl.P * pl.D * pl.L * 4, hipMemcpyDeviceToHost, st));
+    UGSDR_HIP_TRY(hipStreamSynchronize(st));
+    if (ctx->profile)
+        for (int i = 0; i < 4; ++i) UGSDR_HIP_TRY(hipEventElapsedTime(&ctx->last_ms[i], ctx->ev[i], ctx->ev[i + 1]));
     return UGSDR_OK;
 }
 

@@ -1,0 +1,58 @@
+// Host-side staging of one acquisition request: the resolved plan, the device
+// buffers a request holds while in flight, and the enqueue steps shared by the
+// synchronous entry point and the request server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "ctx.hpp"
+#include "plan.hpp"
+
+namespace ugsdr_hip {
+
+struct AcqPlan {
+    int64_t N = 0, K = 0, L = 0, D = 0, P = 0;
+    int F = 0;
+    int fold_mode = 0;        // 0: |.| then fold, 1: fold complex then |.|
+    int reshape = 1;
+    std::vector<double> cls_frac;
+    std::vector<int> cls, shift;
+    std::vector<double> doppler;
+    std::vector<const float2*> specs;
+    // front end
+    int fmt = 0;
+    int64_t up = 1, down = 1, need = 0, n_in_used = 0;
+    size_t bps = 0;
+    int apply_tone = 0;
+    double f_turns = 0, ph_turns = 0;
+    // result scaling
+    double ratio = 1, intermediate_frequency = 0, threshold = 3.3;
+    // transform plan
+    bool small = true;
+    RadixPlan plan{};
+    FourStep fs{};
+    // table layout (bytes) in the request's table buffer
+    size_t off_frac = 0, off_dop = 0, off_cls = 0, off_shift = 0, off_specs = 0, table_bytes = 0;
+    // device bytes the request holds while in flight (admission)
+    int64_t footprint(bool surfaces) const;
+    // batch key: requests sharing it run in one correlation launch
+    int64_t bucket() const { return (N << 4) | (fold_mode << 1) | (L == N ? 1 : 0); }
+};
+
+// Validates the request, resolves replica keys (caller holds ctx->mu).
+int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl);
+// Uploads tables and IQ and runs the front end (unpack, mix, resample).
+int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st, bool keep_surf);
+// Forward FFT of every block for every fractional Doppler class.
+int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
+// One correlation launch over every request of the batch (same bucket).
+int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots, hipStream_t st,
+                 DevBuf& desc, DevBuf& scratch, int cu_count);
+// Per-PRN selection into s.results ([P] ugsdr_acq_result then [P][L] peaks).
+int enqueue_final(const AcqPlan& pl, AcqSlot& s, hipStream_t st, bool peaks);
+int device_cu_count(int device);
+
+}  // namespace ugsdr_hip

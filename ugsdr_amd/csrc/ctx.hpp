@@ -66,6 +66,21 @@ struct Replica {
     float scale = 1.0f;    // reduced precision: stored = spectrum * scale
 };
 
+// device buffers one acquisition request holds while in flight
+struct AcqSlot {
+    DevBuf iq, sig, classes, tables, rows, results, surf, fwd_scratch;
+    std::vector<char> host_tables;   // kept alive until the copies complete
+    void release_all() {
+        for (DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch}) d->release();
+    }
+    int64_t bytes() const {
+        int64_t b = 0;
+        for (const DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch})
+            b += (int64_t)d->bytes;
+        return b;
+    }
+};
+
 }  // namespace ugsdr_hip
 
 struct ugsdr_hip_ctx {
@@ -75,7 +90,14 @@ struct ugsdr_hip_ctx {
     std::map<uint64_t, ugsdr_hip::Replica> replicas;
     uint64_t next_key = 1;
     // scratch
-    ugsdr_hip::DevBuf iq, sig, classes, scratch, rows, results, surf, tables, aux, aux2;
-    std::vector<char> pinned_dummy;
+    ugsdr_hip::DevBuf iq, sig, classes, scratch, rows, results, surf, tables, aux, aux2, desc;
+    ugsdr_hip::AcqSlot slot;  // buffers of the synchronous acquire path
+    // stage timing (ugsdr_hip_ctx_set_profiling)
+    bool profile = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float last_ms[4] = {0, 0, 0, 0};
+    void mark(int i) {
+        if (profile && ev[i]) (void)hipEventRecord(ev[i], stream);
+    }
     int64_t device_bytes() const;
 };

@@ -22,8 +22,9 @@ using namespace ugsdr_hip;
 
 int64_t ugsdr_hip_ctx::device_bytes() const {
     int64_t b = 0;
-    for (const DevBuf* d : {&iq, &sig, &classes, &scratch, &rows, &results, &surf, &tables, &aux, &aux2})
+    for (const DevBuf* d : {&iq, &sig, &classes, &scratch, &rows, &results, &surf, &tables, &aux, &aux2, &desc})
         b += (int64_t)d->bytes;
+    b += slot.bytes();
     for (const auto& kv : replicas)
         b += kv.second.n_fft * (kv.second.precision == UGSDR_PREC_FP32 ? 8 : 4);
     return b;
@@ -68,10 +69,13 @@ int ugsdr_hip_ctx_destroy(ugsdr_hip_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         for (auto& kv : ctx->replicas)
             if (kv.second.spec) (void)hipFree(kv.second.spec);
+        for (auto& e : ctx->ev)
+            if (e) (void)hipEventDestroy(e);
         ctx->replicas.clear();
         for (DevBuf* d : {&ctx->iq, &ctx->sig, &ctx->classes, &ctx->scratch, &ctx->rows, &ctx->results, &ctx->surf,
-                          &ctx->tables, &ctx->aux, &ctx->aux2})
+                          &ctx->tables, &ctx->aux, &ctx->aux2, &ctx->desc})
             d->release();
+        ctx->slot.release_all();
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;
@@ -84,6 +88,29 @@ int ugsdr_hip_ctx_stream(ugsdr_hip_ctx* ctx, void** stream) {
         return UGSDR_ERR_ARG;
     }
     *stream = ctx->stream;
+    return UGSDR_OK;
+}
+
+int ugsdr_hip_ctx_set_profiling(ugsdr_hip_ctx* ctx, int enable) {
+    if (!ctx) {
+        set_error("null context");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    UGSDR_HIP_TRY(hipSetDevice(ctx->device));
+    if (enable && !ctx->ev[0])
+        for (auto& e : ctx->ev) UGSDR_HIP_TRY(hipEventCreate(&e));
+    ctx->profile = enable != 0;
+    return UGSDR_OK;
+}
+
+int ugsdr_hip_ctx_last_times(ugsdr_hip_ctx* ctx, float* ms) {
+    if (!ctx || !ms) {
+        set_error("null argument");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (int i = 0; i < 4; ++i) ms[i] = ctx->last_ms[i];
     return UGSDR_OK;
 }
 

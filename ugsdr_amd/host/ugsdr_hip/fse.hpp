@@ -1,0 +1,256 @@
+// Acquisition surface: FseConfig type list, AcquisitionResult, a generic
+// per-operator search engine that runs the reference's ProcessBpsk loop over
+// any FseConfig (e.g. the Hip* backends), and the fused HipFastSearchEngine
+// that runs the whole PRN x Doppler search in one C-ABI call.
+#pragma once
+
+#include <algorithm>
+#include <complex>
+#include <compare>
+#include <cstdint>
+#include <mutex>
+#include <numeric>
+#include <type_traits>
+#include <vector>
+
+#include "backends.hpp"
+#include "context.hpp"
+
+namespace ugsdr {
+
+#ifndef UGSDR_HIP_REFERENCE_BASES
+enum class System : std::uint32_t { Gps = 0, Glonass, Galileo, BeiDou, NavIC, Sbas, Qzss };
+enum class Signal : std::uint32_t {
+    GpsCoarseAcquisition_L1, Gps_L2CM, Gps_L5I, Gps_L5Q, GlonassCivilFdma_L1, GlonassCivilFdma_L2, Galileo_E1b,
+    Galileo_E1c, Galileo_E5aI, Galileo_E5aQ, Galileo_E5bI, Galileo_E5bQ, Galileo_E6b, Galileo_E6c, BeiDou_B1I,
+    BeiDou_B1C, NavIC_L5, SbasCoarseAcquisition_L1, Sbas_L5I, Sbas_L5Q, QzssCoarseAcquisition_L1, Qzss_L1S,
+    Qzss_L2CM, Qzss_L5I, Qzss_L5Q,
+};
+// satellite id (src/common.hpp:147-154)
+struct Sv {
+    std::int32_t id = 0;
+    System system = System::Gps;
+    Signal signal = Signal::GpsCoarseAcquisition_L1;
+};
+
+// src/acquisition/acquisition_result.hpp:12-31
+template <typename T>
+struct AcquisitionResult final {
+    Sv sv_number;
+    double doppler = 0;
+    double code_offset = 0;
+    double level = 0;
+    double sigma = 1.0;
+    double intermediate_frequency = 0;
+    std::vector<T> output_peak;
+    auto operator<=>(const AcquisitionResult& rhs) const { return level <=> rhs.level; }
+    auto GetSnr() const { return level / sigma; }
+    auto GetAcquiredSignalType() const { return sv_number.signal; }
+};
+
+// backend type list with is-base-of gates (src/acquisition/fse.hpp:36-67)
+template <auto target_sampling_rate, typename MixerT, typename UpsamplerT, typename MatchedFilterT, typename AbsT,
+          typename ReshapeAndSumT, typename MaxIndexT, typename MeanStdDevT, typename ResamplerT>
+struct FseConfig {
+    constexpr static double acquisition_sampling_rate = target_sampling_rate;
+    using MixerType = MixerT;
+    using UpsamplerType = UpsamplerT;
+    using MatchedFilterType = MatchedFilterT;
+    using AbsType = AbsT;
+    using ReshapeAndSumType = ReshapeAndSumT;
+    using MaxIndexType = MaxIndexT;
+    using MeanStdDevType = MeanStdDevT;
+    using ResamplerType = ResamplerT;
+    static_assert(std::is_base_of_v<Mixer<MixerType>, MixerType>, "Incorrect mixer provided, expected ugsdr::Mixer<T>");
+    static_assert(std::is_base_of_v<Upsampler<UpsamplerType>, UpsamplerType>, "Incorrect upsampler provided");
+    static_assert(std::is_base_of_v<MatchedFilter<MatchedFilterType>, MatchedFilterType>,
+                  "Incorrect matched filter provided");
+    static_assert(std::is_base_of_v<Abs<AbsType>, AbsType>, "Incorrect abs provided");
+    static_assert(std::is_base_of_v<ReshapeAndSum<ReshapeAndSumType>, ReshapeAndSumType>,
+                  "Incorrect reshape_and_sum provided");
+    static_assert(std::is_base_of_v<MaxIndex<MaxIndexType>, MaxIndexType>, "Incorrect max_index provided");
+    static_assert(std::is_base_of_v<MeanStdDev<MeanStdDevType>, MeanStdDevType>, "Incorrect mean_std_dev provided");
+    static_assert(std::is_base_of_v<Resampler<ResamplerType>, ResamplerType>, "Incorrect resampler provided");
+};
+#endif
+
+// the Hip backend selectable beside ParametricIppFseConfig / ParametricAfFseConfig
+template <auto acquisition_sampling_rate>
+using ParametricHipFseConfig = FseConfig<acquisition_sampling_rate, HipMixer, HipUpsampler, HipMatchedFilter, HipAbs,
+                                         HipReshapeAndSum, HipMaxIndex, HipMeanStdDev, HipResampler>;
+using HipFseConfig = ParametricHipFseConfig<8192000>;
+
+// AdjustSamplingRate (src/acquisition/fse.hpp:222-240)
+inline double AdjustSamplingRate(double fs, double target) { return ugsdr_adjust_sampling_rate(fs, target); }
+
+// Search parameters of one epoch: the reference's FastSearchEngineBase
+// constructor arguments (range, step) plus what its DFE epoch supplies.
+struct SearchEpoch {
+    const std::complex<float>* samples = nullptr;  // complex float at fs
+    std::size_t n_samples = 0;
+    double sampling_rate = 0;
+    double intermediate_frequency = 0;   // -(central_frequency - 1575.42e6)
+};
+
+// ProcessBpsk loop of src/acquisition/fse.hpp:242-276 over any FseConfig:
+// every reference operator call goes through Config's backends.
+template <typename Config, typename T = float>
+class FastSearchEngine {
+public:
+    FastSearchEngine(double range, double step) : doppler_range(range), doppler_step(step) {}
+
+    // codes: per satellite the +-1 chips of one code period; ms_to_process:
+    // coherent length (the reference uses 4)
+    std::vector<AcquisitionResult<T>> Process(const SearchEpoch& epoch, const std::vector<Sv>& svs,
+                                              const std::vector<std::vector<T>>& codes,
+                                              std::size_t ms_to_process = 4) {
+        const double fs = epoch.sampling_rate;
+        const std::vector<std::complex<T>> signal(epoch.samples, epoch.samples + epoch.n_samples);
+        const auto translated = Config::MixerType::Translate(signal, fs, -epoch.intermediate_frequency);
+        const double new_fs = AdjustSamplingRate(fs, Config::acquisition_sampling_rate);
+        const auto down = Config::ResamplerType::Transform(translated, (std::size_t)new_fs, (std::size_t)fs);
+        std::vector<AcquisitionResult<T>> dst;
+        for (std::size_t i = 0; i < svs.size(); ++i) {
+            const auto repeated = RepeatCodeNTimes(codes[i], ms_to_process);
+            const auto code = Config::UpsamplerType::Transform(repeated, (std::size_t)(ms_to_process * new_fs / 1e3));
+            const std::vector<std::complex<T>> block(down.begin(), down.begin() + code.size());
+            ProcessBpsk(block, code, svs[i], fs, new_fs, epoch.intermediate_frequency, dst);
+        }
+        std::sort(dst.begin(), dst.end(), [](auto& a, auto& b) { return a.sv_number.id < b.sv_number.id; });
+        return dst;
+    }
+
+private:
+    double doppler_range;
+    double doppler_step;
+    static constexpr double peak_threshold = 3.3;
+
+    void ProcessBpsk(const std::vector<std::complex<T>>& signal, const std::vector<T>& code, Sv sv, double fs,
+                     double new_fs, double intermediate_frequency, std::vector<AcquisitionResult<T>>& dst) {
+        AcquisitionResult<T> tmp, max_result;
+        const double ratio = fs / new_fs;
+        const auto spectrum = Config::MatchedFilterType::PrepareCodeSpectrum(code);
+        const auto spm = (std::size_t)(new_fs / 1e3);
+        for (double d = -doppler_range; d <= doppler_range; d += doppler_step) {
+            const auto translated = Config::MixerType::Translate(signal, new_fs, -d);
+            auto matched = Config::MatchedFilterType::FilterOptimized(translated, spectrum);
+            const auto abs_value = Config::AbsType::Transform(matched);
+            auto peak = Config::ReshapeAndSumType::Transform(abs_value, spm);
+            const auto mx = Config::MaxIndexType::Transform(peak);
+            const auto ms = Config::MeanStdDevType::Calculate(peak);
+            tmp.level = mx.value;
+            tmp.sigma = ms.sigma + ms.mean;
+            tmp.code_offset = ratio * (double)mx.index;
+            tmp.doppler = d + intermediate_frequency;
+            if (max_result < tmp) {
+                max_result = tmp;
+                max_result.output_peak = std::move(peak);
+            }
+        }
+        max_result.intermediate_frequency = intermediate_frequency;
+        max_result.sv_number = sv;
+        if (max_result.GetSnr() > peak_threshold) dst.push_back(std::move(max_result));
+    }
+};
+
+struct HipFseOptions {
+    double target_sampling_rate = 8192000;  // FseConfig rate
+    std::size_t coherent_ms = 4;            // ms_to_process of the reference
+    std::size_t noncoherent_blocks = 1;     // K blocks summed after |.|
+    bool reshape = true;                    // GetOneMsPeak<reshape, coherent>
+    bool coherent = true;
+    double threshold = 3.3;
+    int precision = UGSDR_PREC_FP32;
+    bool output_peak = true;
+};
+
+// Fused engine: the whole PRN x Doppler x block search of an epoch in one
+// ugsdr_hip_acquire call, replica spectra cached on the device per code.
+class HipFastSearchEngine {
+public:
+    using Options = HipFseOptions;
+
+    HipFastSearchEngine(double range, double step, Options opt = Options(), ugsdr_hip::HipContext* ctx = nullptr)
+        : doppler_range(range), doppler_step(step), options(opt),
+          context(ctx ? ctx : &ugsdr_hip::HipContext::Default()) {}
+
+    ~HipFastSearchEngine() {
+        for (auto& kv : keys) ugsdr_hip_replica_drop(context->get(), kv.second);
+    }
+
+    // iq: interleaved samples in `fmt` (UGSDR_FMT_*) at fs
+    std::vector<AcquisitionResult<float>> Process(const void* iq, int fmt, std::size_t n_samples, double fs,
+                                                  double intermediate_frequency, const std::vector<Sv>& svs,
+                                                  const std::vector<std::vector<int8_t>>& codes,
+                                                  std::size_t code_ms = 1) {
+        const double new_fs = AdjustSamplingRate(fs, options.target_sampling_rate);
+        const auto spm = (std::size_t)(new_fs / 1e3);
+        const std::size_t n_fft = options.coherent_ms * spm;
+        std::vector<double> doppler;
+        for (double d = -doppler_range; d <= doppler_range; d += doppler_step) doppler.push_back(d);
+        std::vector<uint64_t> prn_keys;
+        for (std::size_t i = 0; i < svs.size(); ++i) prn_keys.push_back(Key(codes[i], options.coherent_ms / code_ms,
+                                                                            n_fft));
+        ugsdr_acq_request r{};
+        r.iq = iq;
+        r.fmt = fmt;
+        r.iq_on_device = 0;
+        r.n_samples = (int64_t)n_samples;
+        r.fs_in = fs;
+        r.fs_acq = new_fs;
+        r.mix_frequency = -intermediate_frequency;  // Translate(signal, fs, -IF) (fse.hpp:289)
+        r.intermediate_frequency = intermediate_frequency;
+        r.n_fft = (int64_t)n_fft;
+        r.n_blocks = (int64_t)options.noncoherent_blocks;
+        r.surface_len = (int64_t)(options.reshape ? spm : n_fft);
+        r.reshape = options.reshape;
+        r.coherent = options.coherent;
+        r.doppler = doppler.data();
+        r.n_doppler = (int64_t)doppler.size();
+        r.replica_keys = prn_keys.data();
+        r.n_prn = (int64_t)prn_keys.size();
+        r.threshold = options.threshold;
+        r.precision = options.precision;
+        std::vector<ugsdr_acq_result> out(svs.size());
+        std::vector<float> peaks(options.output_peak ? svs.size() * (std::size_t)r.surface_len : 0);
+        ugsdr_hip::check(ugsdr_hip_acquire(context->get(), &r, out.data(), nullptr,
+                                           options.output_peak ? peaks.data() : nullptr));
+        std::vector<AcquisitionResult<float>> dst;
+        for (std::size_t i = 0; i < svs.size(); ++i) {
+            if (!out[i].detected) continue;
+            AcquisitionResult<float> a;
+            a.sv_number = svs[i];
+            a.doppler = out[i].doppler;
+            a.code_offset = out[i].code_offset;
+            a.level = out[i].level;
+            a.sigma = out[i].sigma;
+            a.intermediate_frequency = intermediate_frequency;
+            if (options.output_peak)
+                a.output_peak.assign(peaks.begin() + i * r.surface_len, peaks.begin() + (i + 1) * r.surface_len);
+            dst.push_back(std::move(a));
+        }
+        std::sort(dst.begin(), dst.end(), [](auto& a, auto& b) { return a.sv_number.id < b.sv_number.id; });
+        return dst;
+    }
+
+private:
+    double doppler_range;
+    double doppler_step;
+    Options options;
+    ugsdr_hip::HipContext* context;
+    std::vector<std::pair<std::vector<int64_t>, uint64_t>> keys;
+
+    uint64_t Key(const std::vector<int8_t>& code, std::size_t repeats, std::size_t n_fft) {
+        std::vector<int64_t> id{(int64_t)repeats, (int64_t)n_fft, options.precision};
+        id.insert(id.end(), code.begin(), code.end());
+        for (auto& kv : keys)
+            if (kv.first == id) return kv.second;
+        uint64_t k = 0;
+        ugsdr_hip::check(ugsdr_hip_replica_put(context->get(), code.data(), (int64_t)code.size(), (int64_t)repeats,
+                                               (int64_t)n_fft, options.precision, &k));
+        keys.emplace_back(std::move(id), k);
+        return k;
+    }
+};
+
+}  // namespace ugsdr
