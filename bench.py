@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="L1CA_WIDE", choices=["L1CA_WIDE", "L1CA_1MS", "L1CA_4MS", "BOC_4MS"])
+    ap.add_argument("--serve-requests", type=int, default=2000, help="serving leg: requests (0 skips)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving leg: mean Poisson arrival rate")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (rocprofv3 counter calibration)")
     return ap.parse_args()
 
 
@@ -110,6 +115,101 @@ def cpu_baseline(w, threads: int):
     }
 
 
+def time_search(ctx, w, steps: int, warmup: int):
+    """seconds per search of workload w with its IQ resident in HBM"""
+    import torch
+
+    keys = [ctx.replica_put(c, w.repeats, w.n_fft) for c in w.codes]
+    iq_dev = torch.from_numpy(w.iq).to("cuda:0")
+    req, hold = ctx.make_request(None, w.fmt, fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=w.n_blocks,
+                                 surface_len=w.surface_len, doppler=w.doppler, keys=keys, reshape=w.reshape,
+                                 n_samples=w.n_samples, iq_device_ptr=iq_dev.data_ptr())
+    for _ in range(warmup):
+        ctx.acquire_req(req)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res, _, _ = ctx.acquire_req(req)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.set_profiling(True)
+    ctx.acquire_req(req)
+    stages = ctx.last_times()
+    ctx.set_profiling(False)
+    for k in keys:
+        ctx.replica_drop(k)
+    return dt, stages, res
+
+
+def extra_workloads(ctx):
+    """cells/s of the other acquisition workloads and the correlator bank rate"""
+    import numpy as np
+    import torch
+
+    import ugsdr_amd as U
+    from ugsdr_amd import workloads as W
+
+    out = {}
+    for name, steps in (("L1CA_1MS", 50), ("L1CA_4MS", 30), ("BOC_4MS", 3)):
+        w = W.CONFIGS[name]()
+        dt, stages, res = time_search(ctx, w, steps, 2)
+        out[name] = {"cells_per_s": w.cells / dt, "ms_per_search": 1e3 * dt, "cells": w.cells,
+                     "cell_blocks": w.cells * w.n_blocks, "stage_ms": stages,
+                     "detected": [i for i, r in enumerate(res) if r["detected"]]}
+    bw = W.bank_l5()
+    iq_dev = torch.from_numpy(bw.iq).to("cuda:0")
+    ctx.correlate_bank(None, U.FMT_CI16, spm=bw.spm, n_blocks=2, fs=bw.fs, channels=bw.channels, codes=bw.codes,
+                       taps_chips=bw.taps, mode=U.BANK_NCO, iq_device_ptr=iq_dev.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    o, _ = ctx.correlate_bank(None, U.FMT_CI16, spm=bw.spm, n_blocks=bw.n_blocks, fs=bw.fs, channels=bw.channels,
+                              codes=bw.codes, taps_chips=bw.taps, mode=U.BANK_NCO, iq_device_ptr=iq_dev.data_ptr())
+    dt = time.perf_counter() - t0
+    p = np.abs(o[:, :, 2])
+    out["BANK_L5"] = {"channel_samples_per_s": bw.channel_samples / dt, "ms_per_call": 1e3 * dt,
+                      "blocks": bw.n_blocks, "channels": len(bw.channels), "taps": len(bw.taps),
+                      "prompt_dominates": bool(np.all(p > np.abs(o[:, :, 0])))}
+    return out
+
+
+def serving(ctx, n_requests: int, rate: float):
+    """Poisson arrivals of the mixed request stream into the continuous-batching server"""
+    import numpy as np
+
+    import ugsdr_amd as U
+    from ugsdr_amd import workloads as W
+
+    mix = W.ServeMix(ctx, seed=0xC5)
+    warm = [mix.request(mix.draw()) for _ in range(64)]
+    reqs = mix.stream(n_requests, rate)
+    built = [mix.request(r) for r in reqs]
+    with U.Server(ctx, n_streams=4, max_batch=32) as srv:
+        for t in [srv.submit(q) for q, _ in warm]:
+            srv.poll(t, -1)
+        t0 = time.perf_counter()
+        tickets = []
+        for (q, _), r in zip(built, reqs):
+            now = time.perf_counter() - t0
+            if r.arrival_s > now:
+                time.sleep(r.arrival_s - now)
+            tickets.append(srv.submit(q))
+        submit_end = time.perf_counter() - t0
+        lat = [srv.poll(t, -1)[1] for t in tickets]
+        elapsed = time.perf_counter() - t0
+        st = srv.stats()
+    cells = sum(r.cells for r in reqs)
+    kinds = {k: sum(1 for r in reqs if r.kind == k) for k in ("L1", "L5", "BOC")}
+    lat_by = {k: [l for l, r in zip(lat, reqs) if r.kind == k] for k in kinds}
+    return {
+        "requests": n_requests, "offered_rate_hz": n_requests / submit_end, "mean_rate_hz": rate,
+        "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99), "max_ms": max(lat),
+        "mean_ms": statistics.mean(lat),
+        "p99_ms_by_kind": {k: percentile(v, 0.99) for k, v in lat_by.items() if v},
+        "cells_per_s": cells / elapsed, "mix": kinds, "rounds": st["rounds"],
+        "max_batch_seen": st["max_batch_seen"], "failed": st["failed"],
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -121,6 +221,19 @@ def main():
 
     import ugsdr_amd as U
     from ugsdr_amd import workloads as W
+
+    if args.copy_calibration_mib:
+        # a copy kernel of known size: FETCH_SIZE / WRITE_SIZE per dispatch of it
+        # should read the buffer size; the ratio scales the counters of the other kernels
+        n = args.copy_calibration_mib << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": n, "dispatches": 5}))
+        return
 
     w = W.CONFIGS[args.workload]()
     ctx = U.Context(0)
@@ -199,6 +312,16 @@ def main():
         },
         "detected_prns": [i for i, r in enumerate(res) if r["detected"]],
     }
+    if not args.no_extra:
+        try:
+            out["other_workloads"] = extra_workloads(ctx)
+        except Exception as e:  # the headline line must still print
+            out["other_workloads"] = {"error": repr(e)}
+    if args.serve_requests > 0:
+        try:
+            out["serving"] = serving(ctx, args.serve_requests, args.serve_rate)
+        except Exception as e:
+            out["serving"] = {"error": repr(e)}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(w, threads)

@@ -64,6 +64,14 @@ int64_t ugsdr_hip_ctx_device_bytes(ugsdr_hip_ctx* ctx);
 int ugsdr_hip_ctx_set_profiling(ugsdr_hip_ctx* ctx, int enable);
 int ugsdr_hip_ctx_last_times(ugsdr_hip_ctx* ctx, float* ms_out4);
 
+/* Autotuner (no reference counterpart): times the correlation kernel's
+ * workgroup-size variants for a transform length on a synthetic 32 PRN x 32
+ * bin batch and keeps the fastest for this context.  Runs by itself on the
+ * first acquisition of each length unless disabled.  threads: chosen size;
+ * ms_out[n_ms]: each candidate's time per launch. */
+int ugsdr_hip_autotune(ugsdr_hip_ctx* ctx, int64_t n_fft, int32_t* threads, float* ms_out, int32_t n_ms);
+int ugsdr_hip_set_autotune(ugsdr_hip_ctx* ctx, int enable);
+
 /* ------------------------------------------------------------- PRN codes (host)
  * Codegen<Impl>::Get (src/prn_codes/codegen.hpp:23-36).  sv is 0-based. */
 int ugsdr_code_gps_l1ca(int sv, int8_t* out1023);  /* GpsL1Ca.hpp:9-21, GoldCodes.hpp:24-41 */
@@ -222,6 +230,45 @@ typedef struct {
  * out: n_blocks x n_channels x n_taps complex float (host buffer). */
 int ugsdr_hip_correlate_bank(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* req, ugsdr_channel_state* ch,
                              float* out_c32);
+
+/* ---------------------------------------------------------- request server
+ * Continuous-batching acquisition server (no reference counterpart: the
+ * reference runs one FastSearchEngineBase::Process per epoch,
+ * src/acquisition/fse.hpp:457-507).  Requests of uneven shape are submitted as
+ * they arrive; a scheduler thread admits them while request slots and the
+ * device-memory budget allow, runs every admitted request whose transform
+ * length matches in one fused correlation launch, and rotates successive
+ * rounds over several HIP streams so their work overlaps on the device.
+ * Results are per request identical to ugsdr_hip_acquire on the same request. */
+#define UGSDR_PENDING 1  /* ugsdr_hip_server_poll: not finished yet */
+
+typedef struct ugsdr_hip_server ugsdr_hip_server;
+
+typedef struct {
+    int64_t max_device_bytes;  /* admission budget of in-flight requests (0: 8 GiB)   */
+    int32_t max_inflight;      /* request slots (0: 64)                               */
+    int32_t n_streams;         /* streams the rounds rotate over (0: 4)               */
+    int32_t max_batch;         /* requests per correlation launch (0: 32)             */
+    int32_t reserved;
+} ugsdr_server_config;
+
+typedef struct {
+    int64_t submitted, completed, failed, inflight, inflight_bytes, rounds, batches, max_batch_seen;
+} ugsdr_server_stats;
+
+int ugsdr_hip_server_create(ugsdr_hip_ctx* ctx, const ugsdr_server_config* cfg, ugsdr_hip_server** out);
+/* Validates and copies the request (Doppler list, keys and host IQ), so the
+ * caller may reuse its buffers at once; a device IQ pointer must stay valid
+ * until the ticket completes.  Replica keys must not be dropped meanwhile. */
+int ugsdr_hip_server_submit(ugsdr_hip_server* srv, const ugsdr_acq_request* req, uint64_t* ticket);
+/* UGSDR_OK: finished, out[n_prn] filled, latency (submit to completion) in ms,
+ * ticket released; UGSDR_PENDING: still running after wait_ms; < 0: the
+ * request failed (message in ugsdr_hip_last_error), ticket released. */
+int ugsdr_hip_server_poll(ugsdr_hip_server* srv, uint64_t ticket, int wait_ms, ugsdr_acq_result* out,
+                          double* latency_ms);
+int ugsdr_hip_server_stats(ugsdr_hip_server* srv, ugsdr_server_stats* stats);
+/* drains in-flight work, then frees the server */
+int ugsdr_hip_server_destroy(ugsdr_hip_server* srv);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,17 @@ class BankRequest(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_device_bytes", C.c_int64), ("max_inflight", C.c_int32), ("n_streams", C.c_int32),
+                ("max_batch", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("submitted", "completed", "failed", "inflight", "inflight_bytes",
+                                          "rounds", "batches", "max_batch_seen")]
+
+
+PENDING = 1
 RESULT_FIELDS = [f for f, _ in AcqResult._fields_]
 
 # every exported symbol with its ctypes signature (checked against include/ugsdr_hip.h by the tests)
@@ -76,6 +87,8 @@ SIGNATURES = {
     "ugsdr_hip_ctx_device_bytes": ([VP], I64),
     "ugsdr_hip_ctx_set_profiling": ([VP, C.c_int], C.c_int),
     "ugsdr_hip_ctx_last_times": ([VP, VP], C.c_int),
+    "ugsdr_hip_autotune": ([VP, I64, C.POINTER(I32), VP, I32], C.c_int),
+    "ugsdr_hip_set_autotune": ([VP, C.c_int], C.c_int),
     "ugsdr_code_gps_l1ca": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5i": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5q": ([C.c_int, VP], C.c_int),
@@ -102,6 +115,11 @@ SIGNATURES = {
     "ugsdr_hip_upsample": ([VP, VP, I64, VP, I64], C.c_int),
     "ugsdr_hip_correlate": ([VP, VP, I64, VP, I64, VP], C.c_int),
     "ugsdr_hip_correlate_bank": ([VP, C.POINTER(BankRequest), C.POINTER(ChannelState), VP], C.c_int),
+    "ugsdr_hip_server_create": ([VP, C.POINTER(ServerConfig), C.POINTER(VP)], C.c_int),
+    "ugsdr_hip_server_submit": ([VP, C.POINTER(AcqRequest), C.POINTER(C.c_uint64)], C.c_int),
+    "ugsdr_hip_server_poll": ([VP, C.c_uint64, C.c_int, C.POINTER(AcqResult), C.POINTER(C.c_double)], C.c_int),
+    "ugsdr_hip_server_stats": ([VP, C.POINTER(ServerStats)], C.c_int),
+    "ugsdr_hip_server_destroy": ([VP], C.c_int),
 }
 
 _lib = None
@@ -223,6 +241,16 @@ class Context:
 
     def device_bytes(self) -> int:
         return lib().ugsdr_hip_ctx_device_bytes(self._p)
+
+    def autotune(self, n_fft: int):
+        """(chosen workgroup size, per-candidate ms) for a transform length"""
+        t = C.c_int32()
+        ms = np.zeros(4, np.float32)
+        _check(lib().ugsdr_hip_autotune(self._p, n_fft, C.byref(t), _ptr(ms), 4))
+        return t.value, [float(v) for v in ms if v > 0]
+
+    def set_autotune(self, enable: bool):
+        _check(lib().ugsdr_hip_set_autotune(self._p, int(enable)))
 
     def set_profiling(self, enable: bool = True):
         _check(lib().ugsdr_hip_ctx_set_profiling(self._p, int(enable)))
@@ -420,3 +448,55 @@ class Context:
         del hold
         states = [(c.carrier_frequency, c.carrier_phase, c.code_rate, c.code_phase) for c in ch]
         return out, states
+
+
+# ------------------------------------------------------------------ server
+class Server:
+    """Continuous-batching acquisition server on a context (ugsdr_hip_server_*)."""
+
+    def __init__(self, ctx: Context, max_device_bytes: int = 0, max_inflight: int = 0, n_streams: int = 0,
+                 max_batch: int = 0):
+        self._p = C.c_void_p()
+        cfg = ServerConfig(max_device_bytes, max_inflight, n_streams, max_batch, 0)
+        _check(lib().ugsdr_hip_server_create(ctx.handle, C.byref(cfg), C.byref(self._p)))
+        self._n_prn = {}
+
+    def submit(self, req: AcqRequest) -> int:
+        t = C.c_uint64()
+        _check(lib().ugsdr_hip_server_submit(self._p, C.byref(req), C.byref(t)))
+        self._n_prn[t.value] = req.n_prn
+        return t.value
+
+    def poll(self, ticket: int, wait_ms: int = -1):
+        """(results, latency_ms) when finished, None while pending"""
+        n = self._n_prn.get(ticket, 0)
+        out = (AcqResult * max(n, 1))()
+        lat = C.c_double()
+        rc = lib().ugsdr_hip_server_poll(self._p, ticket, wait_ms, out, C.byref(lat))
+        if rc == PENDING:
+            return None
+        self._n_prn.pop(ticket, None)
+        _check(rc)
+        return [{f: getattr(out[i], f) for f in RESULT_FIELDS} for i in range(n)], lat.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().ugsdr_hip_server_stats(self._p, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._p:
+            lib().ugsdr_hip_server_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -131,6 +131,8 @@ __device__ __forceinline__ float2 fwd_source(const FwdArgs& a, int blk, int64_t 
 // waves per SIMD to budget registers for: 3 for N <= 4096 (three 256-thread
 // workgroups per CU), else 2 (LDS holds one or two transforms per CU)
 constexpr int waves_for(int CAP) { return CAP <= 4096 ? 3 : 2; }
+// a workgroup of T threads puts T/256 waves on each SIMD
+constexpr int waves_for_t(int T, int CAP) { return T / 256 > waves_for(CAP) ? T / 256 : waves_for(CAP); }
 
 template <int T, int CAP, class P>
 __global__ __launch_bounds__(T, waves_for(CAP)) void fwd_small(FwdArgs a) {
@@ -393,7 +395,7 @@ __device__ __forceinline__ void row_stats_glob(const float* S, int L, double* re
 }
 
 template <int T, int CAP, class P>
-__global__ __launch_bounds__(T, waves_for(CAP)) void corr_small(CorrArgs a) {
+__global__ __launch_bounds__(T, waves_for_t(T, CAP)) void corr_small(CorrArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int SR = CAP / T;
     float2* buf = reinterpret_cast<float2*>(smem);
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(T, waves_for(CAP)) void corr_small(CorrArgs a) {
 }
 
 template <int T, class P1, class P2>
-__global__ __launch_bounds__(T, 2) void corr_large(CorrArgs a) {
+__global__ __launch_bounds__(T, (T / 256 > 2 ? T / 256 : 2)) void corr_large(CorrArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const LargeLds Ld = large_lds(smem, a.fs);
     const int tid = threadIdx.x;
@@ -713,9 +715,8 @@ static int launch_corr_small(const CorrArgs& c, int grid, hipStream_t st) {
     return UGSDR_OK;
 }
 
-template <class P1, class P2>
+template <int T, class P1, class P2>
 static int launch_corr_large(const CorrArgs& c, int grid, hipStream_t st) {
-    constexpr int T = 512;
     const size_t lds = large_lds_bytes(c.fs);
     UGSDR_HIP_TRY(set_lds(corr_large<T, P1, P2>, lds));
     hipLaunchKernelGGL((corr_large<T, P1, P2>), dim3(grid), dim3(T), lds, st, c);
@@ -916,8 +917,27 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     return launch_forward(a, (int)(pl.K * pl.F), st, s.fwd_scratch);
 }
 
+int default_threads(int64_t N) {
+    if (N <= 8192) return 256;
+    return 512;
+}
+
+std::vector<int> thread_candidates(int64_t N) {
+    switch (N) {
+        case 4092:
+        case 8184: return {256, 512};
+        case 12276:
+        case 16368: return {512, 1024};
+        default: {
+            FourStep probe;
+            if (N > kLdsCap && fourstep_341((int)N, probe)) return {512, 1024};
+            return {default_threads(N)};
+        }
+    }
+}
+
 int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots, hipStream_t st,
-                 DevBuf& desc, DevBuf& scratch, int cus) {
+                 DevBuf& desc, DevBuf& scratch, int cus, int threads) {
     const AcqPlan& p0 = *plans[0];
     std::vector<ReqDesc> reqs(plans.size());
     int rows = 0;
@@ -956,13 +976,26 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
     c.reqs = desc.as<ReqDesc>();
     c.inv_n = (float)(1.0 / (double)p0.N);
     int rc = UGSDR_OK;
+    const int T = threads > 0 ? threads : default_threads(c.N);
     if (p0.small) {
         const int grid = rows;
         switch (c.N) {
-            case 4092: rc = launch_corr_small<256, 4096, P4092>(c, grid, st); break;
-            case 8184: rc = launch_corr_small<256, 8192, P8184>(c, grid, st); break;
-            case 12276: rc = launch_corr_small<512, kLdsCap, P12276>(c, grid, st); break;
-            case 16368: rc = launch_corr_small<512, kLdsCap, P16368>(c, grid, st); break;
+            case 4092:
+                rc = T == 512 ? launch_corr_small<512, 4096, P4092>(c, grid, st)
+                              : launch_corr_small<256, 4096, P4092>(c, grid, st);
+                break;
+            case 8184:
+                rc = T == 512 ? launch_corr_small<512, 8192, P8184>(c, grid, st)
+                              : launch_corr_small<256, 8192, P8184>(c, grid, st);
+                break;
+            case 12276:
+                rc = T == 1024 ? launch_corr_small<1024, kLdsCap, P12276>(c, grid, st)
+                               : launch_corr_small<512, kLdsCap, P12276>(c, grid, st);
+                break;
+            case 16368:
+                rc = T == 1024 ? launch_corr_small<1024, kLdsCap, P16368>(c, grid, st)
+                               : launch_corr_small<512, kLdsCap, P16368>(c, grid, st);
+                break;
             default:
                 if (c.N <= 4096) rc = launch_corr_small<256, 4096, PlanRT>(c, grid, st);
                 else if (c.N <= 8192) rc = launch_corr_small<256, 8192, PlanRT>(c, grid, st);
@@ -974,13 +1007,19 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
         UGSDR_HIP_TRY(scratch.reserve(per_wg * grid));
         c.scratch = scratch.as<float2>();
         FourStep probe;
-        if (!fourstep_341(c.N, probe)) rc = launch_corr_large<PlanRT, PlanRT>(c, grid, st);
+        const bool big = T == 1024;
+        if (!fourstep_341(c.N, probe)) rc = launch_corr_large<512, PlanRT, PlanRT>(c, grid, st);
         else switch (c.fs.n1) {
-            case 60: rc = launch_corr_large<P60, P341>(c, grid, st); break;
-            case 112: rc = launch_corr_large<P112, P341>(c, grid, st); break;
-            case 120: rc = launch_corr_large<P120, P341>(c, grid, st); break;
-            case 180: rc = launch_corr_large<P180, P341>(c, grid, st); break;
-            default: rc = launch_corr_large<P240, P341>(c, grid, st); break;
+            case 60: rc = big ? launch_corr_large<1024, P60, P341>(c, grid, st)
+                              : launch_corr_large<512, P60, P341>(c, grid, st); break;
+            case 112: rc = big ? launch_corr_large<1024, P112, P341>(c, grid, st)
+                               : launch_corr_large<512, P112, P341>(c, grid, st); break;
+            case 120: rc = big ? launch_corr_large<1024, P120, P341>(c, grid, st)
+                               : launch_corr_large<512, P120, P341>(c, grid, st); break;
+            case 180: rc = big ? launch_corr_large<1024, P180, P341>(c, grid, st)
+                               : launch_corr_large<512, P180, P341>(c, grid, st); break;
+            default: rc = big ? launch_corr_large<1024, P240, P341>(c, grid, st)
+                              : launch_corr_large<512, P240, P341>(c, grid, st); break;
         }
     }
     if (rc) return rc;
@@ -1009,6 +1048,123 @@ int enqueue_final(const AcqPlan& pl, AcqSlot& s, hipStream_t st, bool peaks) {
     return UGSDR_OK;
 }
 
+// ------------------------------------------------------------------ autotuner
+int autotune_corr(ugsdr_hip_ctx* ctx, int64_t N, int* best, std::vector<float>* ms_out) {
+    const std::vector<int> cands = thread_candidates(N);
+    if (ms_out) ms_out->assign(cands.size(), 0.f);
+    if (cands.size() == 1) {
+        ctx->tuned[N] = cands[0];
+        if (best) *best = cands[0];
+        return UGSDR_OK;
+    }
+    // synthetic batch that fills the device: 32 PRNs x 32 Doppler bins, 1 block
+    const int P = 32, D = 32;
+    uint64_t x = 0x9E3779B97F4A7C15ull ^ (uint64_t)N;
+    auto next = [&x]() {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        return x;
+    };
+    std::vector<float> iq(2 * (size_t)N);
+    for (auto& v : iq) v = (float)((int)(next() % 255) - 127);
+    std::vector<uint64_t> keys;
+    std::vector<int8_t> chips(1023);
+    int rc = UGSDR_OK;
+    for (int p = 0; p < P && !rc; ++p) {
+        for (auto& c : chips) c = (next() >> 63) ? 1 : -1;
+        Replica rep;
+        rep.n_fft = N;
+        if (hipMalloc(&rep.spec, (size_t)N * sizeof(float2)) != hipSuccess) {
+            set_error("autotune: out of device memory");
+            rc = UGSDR_ERR_NOMEM;
+            break;
+        }
+        rc = replica_spectrum(ctx, chips.data(), 1023, 1, nullptr, N, static_cast<float2*>(rep.spec));
+        const uint64_t key = ~(uint64_t)p;  // keys from the top of the space, removed below
+        ctx->replicas[key] = rep;
+        keys.push_back(key);
+    }
+    std::vector<double> dop(D);
+    const double fs = (double)N * 1000.0;
+    for (int d = 0; d < D; ++d) dop[d] = -4000.0 + 250.0 * d;
+    ugsdr_acq_request r{};
+    r.iq = iq.data();
+    r.fmt = UGSDR_FMT_CF32;
+    r.n_samples = N;
+    r.fs_in = r.fs_acq = fs;
+    r.n_fft = N;
+    r.n_blocks = 1;
+    r.surface_len = N;
+    r.reshape = 1;
+    r.coherent = 1;
+    r.doppler = dop.data();
+    r.n_doppler = D;
+    r.replica_keys = keys.data();
+    r.n_prn = P;
+    r.threshold = 3.3;
+    AcqPlan pl;
+    AcqSlot slot;
+    DevBuf desc, scratch;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) rc = make_acq_plan(ctx, &r, pl);
+    if (!rc) rc = enqueue_front(pl, &r, slot, ctx->stream, false);
+    if (!rc) rc = enqueue_classes(pl, slot, ctx->stream);
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+        set_error("autotune: event creation failed");
+        rc = UGSDR_ERR_HIP;
+    }
+    int chosen = cands[0];
+    float best_ms = 1e30f;
+    const int cus = device_cu_count(ctx->device);
+    for (size_t i = 0; i < cands.size() && !rc; ++i) {
+        rc = enqueue_corr({&pl}, {&slot}, ctx->stream, desc, scratch, cus, cands[i]);  // warm-up
+        if (rc) break;
+        (void)hipEventRecord(e0, ctx->stream);
+        for (int it = 0; it < 3 && !rc; ++it) rc = enqueue_corr({&pl}, {&slot}, ctx->stream, desc, scratch, cus, cands[i]);
+        (void)hipEventRecord(e1, ctx->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            set_error("autotune: kernel failed");
+            rc = UGSDR_ERR_HIP;
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= 3.f;
+        if (ms_out) (*ms_out)[i] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            chosen = cands[i];
+        }
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (uint64_t k : keys) {
+        (void)hipFree(ctx->replicas[k].spec);
+        ctx->replicas.erase(k);
+    }
+    slot.release_all();
+    desc.release();
+    scratch.release();
+    if (rc) return rc;
+    ctx->tuned[N] = chosen;
+    if (best) *best = chosen;
+    return UGSDR_OK;
+}
+
+int tuned_threads(ugsdr_hip_ctx* ctx, int64_t N) {
+    auto it = ctx->tuned.find(N);
+    if (it != ctx->tuned.end()) return it->second;
+    if (!ctx->autotune || thread_candidates(N).size() == 1) return default_threads(N);
+    int best = 0;
+    if (autotune_corr(ctx, N, &best, nullptr) != UGSDR_OK) {
+        ctx->tuned[N] = default_threads(N);  // keep serving with the default shape
+        return default_threads(N);
+    }
+    return best;
+}
+
 int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_result* out, float* surfaces,
                  float* peaks) {
     UGSDR_CHECK_ARG(out, "null result");
@@ -1024,7 +1180,9 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
     ctx->mark(1);
     if (int rc = enqueue_classes(pl, s, st)) return rc;
     ctx->mark(2);
-    if (int rc = enqueue_corr({&pl}, {&s}, st, ctx->desc, ctx->scratch, device_cu_count(ctx->device))) return rc;
+    if (int rc = enqueue_corr({&pl}, {&s}, st, ctx->desc, ctx->scratch, device_cu_count(ctx->device),
+                              tuned_threads(ctx, pl.N)))
+        return rc;
     ctx->mark(3);
     if (int rc = enqueue_final(pl, s, st, peaks != nullptr)) return rc;
     ctx->mark(4);
@@ -1051,4 +1209,29 @@ extern "C" int ugsdr_hip_acquire(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* re
     std::lock_guard<std::mutex> lk(ctx->mu);
     UGSDR_HIP_TRY(hipSetDevice(ctx->device));
     return ugsdr_hip::acquire_impl(ctx, req, out, surfaces, peaks);
+}
+
+extern "C" int ugsdr_hip_autotune(ugsdr_hip_ctx* ctx, int64_t n_fft, int32_t* threads, float* ms_out, int32_t n_ms) {
+    if (!ctx || n_fft <= 0) {
+        ugsdr_hip::set_error("bad arguments");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    UGSDR_HIP_TRY(hipSetDevice(ctx->device));
+    int best = 0;
+    std::vector<float> ms;
+    if (int rc = ugsdr_hip::autotune_corr(ctx, n_fft, &best, &ms)) return rc;
+    if (threads) *threads = best;
+    for (int i = 0; ms_out && i < n_ms && i < (int)ms.size(); ++i) ms_out[i] = ms[i];
+    return UGSDR_OK;
+}
+
+extern "C" int ugsdr_hip_set_autotune(ugsdr_hip_ctx* ctx, int enable) {
+    if (!ctx) {
+        ugsdr_hip::set_error("null context");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->autotune = enable != 0;
+    return UGSDR_OK;
 }

@@ -32,6 +32,7 @@ struct AcqPlan {
     double ratio = 1, intermediate_frequency = 0, threshold = 3.3;
     // transform plan
     bool small = true;
+    int threads = 0;          // correlation workgroup size (0: default)
     RadixPlan plan{};
     FourStep fs{};
     // table layout (bytes) in the request's table buffer
@@ -48,9 +49,18 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl);
 int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st, bool keep_surf);
 // Forward FFT of every block for every fractional Doppler class.
 int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
-// One correlation launch over every request of the batch (same bucket).
+// One correlation launch over every request of the batch (same bucket) with
+// `threads` per workgroup (0: default for the length).
 int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots, hipStream_t st,
-                 DevBuf& desc, DevBuf& scratch, int cu_count);
+                 DevBuf& desc, DevBuf& scratch, int cu_count, int threads = 0);
+// workgroup sizes the correlation kernel is built with for a length
+std::vector<int> thread_candidates(int64_t N);
+int default_threads(int64_t N);
+// times every candidate on a synthetic batch of this length and records the
+// fastest in ctx->tuned (caller holds ctx->mu); ms_out gets each candidate's time
+int autotune_corr(ugsdr_hip_ctx* ctx, int64_t N, int* best, std::vector<float>* ms_out);
+// tuned workgroup size for a length (autotunes on first use when enabled)
+int tuned_threads(ugsdr_hip_ctx* ctx, int64_t N);
 // Per-PRN selection into s.results ([P] ugsdr_acq_result then [P][L] peaks).
 int enqueue_final(const AcqPlan& pl, AcqSlot& s, hipStream_t st, bool peaks);
 int device_cu_count(int device);

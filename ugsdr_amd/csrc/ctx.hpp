@@ -92,6 +92,9 @@ struct ugsdr_hip_ctx {
     // scratch
     ugsdr_hip::DevBuf iq, sig, classes, scratch, rows, results, surf, tables, aux, aux2, desc;
     ugsdr_hip::AcqSlot slot;  // buffers of the synchronous acquire path
+    // autotuner: transform length -> correlation workgroup size
+    std::map<int64_t, int> tuned;
+    bool autotune = true;
     // stage timing (ugsdr_hip_ctx_set_profiling)
     bool profile = false;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

@@ -1,0 +1,432 @@
+// Continuous-batching acquisition server.
+//
+//   submit()   -> validates + plans the request (ctx lock), copies it, queues it
+//   scheduler  -> admits queued requests in arrival order while a slot is free
+//                 and the in-flight device bytes stay under budget; groups the
+//                 admitted ones by transform bucket (N, fold, surface length);
+//                 per group: front end + class FFTs per request, ONE correlation
+//                 launch for the whole group, per-request finalize and result
+//                 copy; each round goes to the next stream of a small pool so
+//                 rounds overlap on the device
+//   completer  -> polls the rounds' events, stamps latencies, frees slots
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "acq.hpp"
+#include "ctx.hpp"
+#include "internal.hpp"
+
+namespace ugsdr_hip {
+
+using Clock = std::chrono::steady_clock;
+
+struct PinnedBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipHostMalloc(&ptr, n, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    ~PinnedBuf() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+};
+
+struct ServerSlot {
+    AcqSlot dev;
+    PinnedBuf iq_stage, results;
+};
+
+struct Job {
+    uint64_t ticket = 0;
+    ugsdr_acq_request req{};
+    std::vector<double> doppler;
+    std::vector<uint64_t> keys;
+    std::vector<char> iq;           // host IQ copy (empty for device IQ)
+    AcqPlan plan;
+    int64_t bytes = 0;
+    int slot = -1;
+    Clock::time_point t_submit, t_done;
+    int state = 0;                  // 0 queued, 1 running, 2 done, 3 failed
+    int error = 0;
+    std::string message;
+    std::vector<ugsdr_acq_result> results;
+};
+
+struct Round {
+    std::vector<std::shared_ptr<Job>> jobs;
+    hipEvent_t done = nullptr;
+    int stream = 0;
+};
+
+struct Server {
+    ugsdr_hip_ctx* ctx = nullptr;
+    ugsdr_server_config cfg{};
+    std::vector<hipStream_t> streams;
+    std::vector<DevBuf> desc, scratch;  // per stream
+    std::vector<ServerSlot> slots;
+    std::vector<int> free_slots;
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_done;
+    std::deque<std::shared_ptr<Job>> queue;
+    std::unordered_map<uint64_t, std::shared_ptr<Job>> jobs;
+    std::deque<Round> inflight;
+    std::vector<hipEvent_t> event_pool;
+    uint64_t next_ticket = 1;
+    int64_t inflight_bytes = 0;
+    int next_stream = 0;
+    bool stop = false;
+    bool sched_done = false;
+    ugsdr_server_stats st{};
+    std::thread sched, completer;
+    int cus = 256;
+};
+
+static hipEvent_t take_event(Server& s) {
+    if (!s.event_pool.empty()) {
+        hipEvent_t e = s.event_pool.back();
+        s.event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
+}
+
+static void fail_job(Job& j, int rc) {
+    j.state = 3;
+    j.error = rc;
+    j.message = ugsdr_hip_last_error();
+    j.t_done = Clock::now();
+}
+
+// launches one round (jobs of one bucket) on stream index si; returns rc
+static int launch_round(Server& s, Round& rd) {
+    hipStream_t st = s.streams[rd.stream];
+    std::vector<const AcqPlan*> plans;
+    std::vector<AcqSlot*> devs;
+    for (auto& jp : rd.jobs) {
+        Job& j = *jp;
+        ServerSlot& sl = s.slots[j.slot];
+        ugsdr_acq_request r = j.req;
+        if (!r.iq_on_device) {
+            const size_t bytes = (size_t)j.plan.n_in_used * j.plan.bps;
+            UGSDR_HIP_TRY(sl.iq_stage.reserve(bytes));
+            std::memcpy(sl.iq_stage.ptr, j.iq.data(), bytes);
+            r.iq = sl.iq_stage.ptr;  // pinned: the async copy does not block this thread
+        }
+        if (int rc = enqueue_front(j.plan, &r, sl.dev, st, false)) return rc;
+        if (int rc = enqueue_classes(j.plan, sl.dev, st)) return rc;
+        plans.push_back(&j.plan);
+        devs.push_back(&sl.dev);
+    }
+    // the per-stream batch buffers may only grow once the stream's earlier rounds are done
+    const AcqPlan& p0 = *plans[0];
+    const size_t need_desc = plans.size() * 64 * 2;
+    const size_t need_scratch =
+        p0.small ? 0 : (2 * (size_t)p0.N + (p0.L + 1) / 2) * sizeof(float2) * (size_t)s.cus;
+    if (s.desc[rd.stream].bytes < need_desc || s.scratch[rd.stream].bytes < need_scratch)
+        UGSDR_HIP_TRY(hipStreamSynchronize(st));
+    if (int rc = enqueue_corr(plans, devs, st, s.desc[rd.stream], s.scratch[rd.stream], s.cus, p0.threads)) return rc;
+    for (auto& jp : rd.jobs) {
+        Job& j = *jp;
+        ServerSlot& sl = s.slots[j.slot];
+        if (int rc = enqueue_final(j.plan, sl.dev, st, false)) return rc;
+        const size_t rb = (size_t)j.plan.P * sizeof(ugsdr_acq_result);
+        UGSDR_HIP_TRY(sl.results.reserve(rb));
+        UGSDR_HIP_TRY(hipMemcpyAsync(sl.results.ptr, sl.dev.results.ptr, rb, hipMemcpyDeviceToHost, st));
+    }
+    UGSDR_HIP_TRY(hipEventRecord(rd.done, st));
+    return UGSDR_OK;
+}
+
+static void scheduler_loop(Server* sp) {
+    Server& s = *sp;
+    (void)hipSetDevice(s.ctx->device);
+    std::unique_lock<std::mutex> lk(s.mu);
+    while (true) {
+        s.cv_sched.wait(lk, [&] { return s.stop || (!s.queue.empty() && !s.free_slots.empty()); });
+        if (s.stop && s.queue.empty()) {
+            s.sched_done = true;
+            break;
+        }
+        // admission in arrival order
+        std::vector<std::shared_ptr<Job>> admitted;
+        while (!s.queue.empty() && !s.free_slots.empty()) {
+            auto& j = s.queue.front();
+            if (s.inflight_bytes + j->bytes > s.cfg.max_device_bytes && s.inflight_bytes > 0) break;
+            j->slot = s.free_slots.back();
+            s.free_slots.pop_back();
+            s.inflight_bytes += j->bytes;
+            j->state = 1;
+            admitted.push_back(j);
+            s.queue.pop_front();
+        }
+        if (admitted.empty()) {
+            // budget exhausted: wait for a completion to free memory
+            s.cv_sched.wait_for(lk, std::chrono::microseconds(200));
+            continue;
+        }
+        // group by bucket, chunk by max_batch
+        std::map<std::pair<int64_t, int64_t>, std::vector<std::shared_ptr<Job>>> groups;
+        for (auto& j : admitted) groups[{j->plan.bucket(), j->plan.L}].push_back(j);
+        std::vector<Round> rounds;
+        for (auto& g : groups)
+            for (size_t i = 0; i < g.second.size(); i += (size_t)s.cfg.max_batch) {
+                Round rd;
+                rd.jobs.assign(g.second.begin() + i,
+                               g.second.begin() + std::min(g.second.size(), i + (size_t)s.cfg.max_batch));
+                rd.done = take_event(s);
+                rd.stream = s.next_stream;
+                s.next_stream = (s.next_stream + 1) % (int)s.streams.size();
+                rounds.push_back(std::move(rd));
+            }
+        lk.unlock();
+        for (auto& rd : rounds) {
+            int rc = rd.done ? launch_round(s, rd) : UGSDR_ERR_HIP;
+            std::lock_guard<std::mutex> g(s.mu);
+            if (rc) {
+                // the round's work may be partly queued: wait for the stream, then fail its jobs
+                (void)hipStreamSynchronize(s.streams[rd.stream]);
+                for (auto& j : rd.jobs) {
+                    fail_job(*j, rc);
+                    s.free_slots.push_back(j->slot);
+                    s.inflight_bytes -= j->bytes;
+                    ++s.st.failed;
+                }
+                if (rd.done) s.event_pool.push_back(rd.done);
+                s.cv_done.notify_all();
+                continue;
+            }
+            ++s.st.rounds;
+            ++s.st.batches;
+            s.st.max_batch_seen = std::max<int64_t>(s.st.max_batch_seen, (int64_t)rd.jobs.size());
+            s.inflight.push_back(std::move(rd));
+        }
+        lk.lock();
+    }
+}
+
+static void completer_loop(Server* sp) {
+    Server& s = *sp;
+    (void)hipSetDevice(s.ctx->device);
+    while (true) {
+        bool any = false;
+        {
+            std::unique_lock<std::mutex> lk(s.mu);
+            if (s.sched_done && s.inflight.empty()) break;
+            for (auto it = s.inflight.begin(); it != s.inflight.end();) {
+                const hipError_t q = hipEventQuery(it->done);
+                if (q == hipErrorNotReady) {
+                    ++it;
+                    continue;
+                }
+                const auto now = Clock::now();
+                for (auto& jp : it->jobs) {
+                    Job& j = *jp;
+                    if (q == hipSuccess) {
+                        j.results.resize(j.plan.P);
+                        std::memcpy(j.results.data(), s.slots[j.slot].results.ptr,
+                                    (size_t)j.plan.P * sizeof(ugsdr_acq_result));
+                        j.state = 2;
+                        j.t_done = now;
+                        ++s.st.completed;
+                    } else {
+                        set_error(std::string("round failed: ") + hipGetErrorString(q));
+                        fail_job(j, UGSDR_ERR_HIP);
+                        ++s.st.failed;
+                    }
+                    s.free_slots.push_back(j.slot);
+                    s.inflight_bytes -= j.bytes;
+                }
+                s.event_pool.push_back(it->done);
+                it = s.inflight.erase(it);
+                any = true;
+            }
+            if (any) {
+                s.cv_done.notify_all();
+                s.cv_sched.notify_all();
+            }
+        }
+        if (!any) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+}  // namespace ugsdr_hip
+
+using namespace ugsdr_hip;
+
+struct ugsdr_hip_server {
+    Server s;
+};
+
+extern "C" {
+
+int ugsdr_hip_server_create(ugsdr_hip_ctx* ctx, const ugsdr_server_config* cfg, ugsdr_hip_server** out) {
+    if (!ctx || !out) {
+        set_error("null argument");
+        return UGSDR_ERR_ARG;
+    }
+    *out = nullptr;
+    UGSDR_HIP_TRY(hipSetDevice(ctx->device));
+    auto* h = new ugsdr_hip_server();
+    Server& s = h->s;
+    s.ctx = ctx;
+    if (cfg) s.cfg = *cfg;
+    if (s.cfg.max_device_bytes <= 0) s.cfg.max_device_bytes = 8ll << 30;
+    if (s.cfg.max_inflight <= 0) s.cfg.max_inflight = 64;
+    if (s.cfg.n_streams <= 0) s.cfg.n_streams = 4;
+    if (s.cfg.max_batch <= 0) s.cfg.max_batch = 32;
+    s.cus = device_cu_count(ctx->device);
+    s.streams.resize(s.cfg.n_streams);
+    for (auto& st : s.streams) {
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);  // default priority
+        if (e != hipSuccess) {
+            set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+            for (auto& x : s.streams)
+                if (x) (void)hipStreamDestroy(x);
+            delete h;
+            return UGSDR_ERR_HIP;
+        }
+    }
+    s.desc.resize(s.cfg.n_streams);
+    s.scratch.resize(s.cfg.n_streams);
+    s.slots.resize(s.cfg.max_inflight);
+    for (int i = s.cfg.max_inflight - 1; i >= 0; --i) s.free_slots.push_back(i);
+    s.sched = std::thread(scheduler_loop, &s);
+    s.completer = std::thread(completer_loop, &s);
+    *out = h;
+    return UGSDR_OK;
+}
+
+int ugsdr_hip_server_submit(ugsdr_hip_server* h, const ugsdr_acq_request* req, uint64_t* ticket) {
+    if (!h || !req || !ticket) {
+        set_error("null argument");
+        return UGSDR_ERR_ARG;
+    }
+    Server& s = h->s;
+    auto j = std::make_shared<Job>();
+    j->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s.ctx->mu);
+        if (req->precision != UGSDR_PREC_FP32) {
+            set_error("the server runs the FP32 plain form");
+            return UGSDR_ERR_UNSUPPORTED;
+        }
+        if (int rc = make_acq_plan(s.ctx, req, j->plan)) return rc;
+        j->plan.threads = tuned_threads(s.ctx, j->plan.N);
+    }
+    j->req = *req;
+    j->doppler.assign(req->doppler, req->doppler + req->n_doppler);
+    j->keys.assign(req->replica_keys, req->replica_keys + req->n_prn);
+    j->req.doppler = j->doppler.data();
+    j->req.replica_keys = j->keys.data();
+    if (!req->iq_on_device) {
+        const size_t bytes = (size_t)j->plan.n_in_used * j->plan.bps;
+        j->iq.assign(static_cast<const char*>(req->iq), static_cast<const char*>(req->iq) + bytes);
+        j->req.iq = j->iq.data();
+    }
+    j->bytes = j->plan.footprint(false);
+    {
+        std::lock_guard<std::mutex> lk(s.mu);
+        if (s.stop) {
+            set_error("server is shutting down");
+            return UGSDR_ERR_STATE;
+        }
+        if (j->bytes > s.cfg.max_device_bytes) {
+            set_error("request exceeds the server's device-memory budget");
+            return UGSDR_ERR_NOMEM;
+        }
+        j->ticket = s.next_ticket++;
+        s.jobs[j->ticket] = j;
+        s.queue.push_back(j);
+        ++s.st.submitted;
+    }
+    s.cv_sched.notify_one();
+    *ticket = j->ticket;
+    return UGSDR_OK;
+}
+
+int ugsdr_hip_server_poll(ugsdr_hip_server* h, uint64_t ticket, int wait_ms, ugsdr_acq_result* out,
+                          double* latency_ms) {
+    if (!h) {
+        set_error("null server");
+        return UGSDR_ERR_ARG;
+    }
+    Server& s = h->s;
+    std::unique_lock<std::mutex> lk(s.mu);
+    auto it = s.jobs.find(ticket);
+    if (it == s.jobs.end()) {
+        set_error("unknown ticket");
+        return UGSDR_ERR_STATE;
+    }
+    auto j = it->second;
+    if (j->state < 2 && wait_ms != 0) {
+        auto pred = [&] { return j->state >= 2; };
+        if (wait_ms < 0) s.cv_done.wait(lk, pred);
+        else s.cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), pred);
+    }
+    if (j->state < 2) return UGSDR_PENDING;
+    s.jobs.erase(ticket);
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(j->t_done - j->t_submit).count();
+    if (j->state == 3) {
+        set_error(j->message);
+        return j->error;
+    }
+    if (out) std::memcpy(out, j->results.data(), j->results.size() * sizeof(ugsdr_acq_result));
+    return UGSDR_OK;
+}
+
+int ugsdr_hip_server_stats(ugsdr_hip_server* h, ugsdr_server_stats* out) {
+    if (!h || !out) {
+        set_error("null argument");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(h->s.mu);
+    *out = h->s.st;
+    out->inflight = (int64_t)(h->s.cfg.max_inflight - (int)h->s.free_slots.size());
+    out->inflight_bytes = h->s.inflight_bytes;
+    return UGSDR_OK;
+}
+
+int ugsdr_hip_server_destroy(ugsdr_hip_server* h) {
+    if (!h) return UGSDR_OK;
+    Server& s = h->s;
+    {
+        std::lock_guard<std::mutex> lk(s.mu);
+        s.stop = true;
+    }
+    s.cv_sched.notify_all();
+    if (s.sched.joinable()) s.sched.join();
+    if (s.completer.joinable()) s.completer.join();
+    (void)hipSetDevice(s.ctx->device);
+    for (auto& st : s.streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    for (auto e : s.event_pool) (void)hipEventDestroy(e);
+    for (auto& sl : s.slots) sl.dev.release_all();
+    for (auto& d : s.desc) d.release();
+    for (auto& d : s.scratch) d.release();
+    delete h;
+    return UGSDR_OK;
+}
+
+}  // extern "C"

@@ -254,3 +254,91 @@ CONFIGS = {
     "L1CA_WIDE": l1ca_wide,
     "BOC_4MS": boc_4ms,
 }
+
+
+# ------------------------------------------------------------------ serving
+@dataclass
+class ServeRequest:
+    kind: str
+    iq: np.ndarray
+    fmt: int
+    fs: float
+    n_fft: int
+    n_blocks: int
+    surface_len: int
+    reshape: bool
+    doppler: np.ndarray
+    keys: list
+    arrival_s: float = 0.0
+
+    @property
+    def cells(self) -> int:
+        return len(self.keys) * len(self.doppler) * self.surface_len
+
+
+class ServeMix:
+    """Mixed acquisition request stream with Poisson arrivals:
+    50 % L1 C/A-like (int8 @ 4.092 MHz, 1-10 ms as 1 ms x K blocks),
+    30 % L5-like (seeded 10230-chip codes, int16 @ 20.46 MHz, 1-4 ms as 1 ms x K),
+    20 % BOC(1,1) (seeded 4092-chip codes, int16 @ 20.46 MHz, 4 ms coherent);
+    each with 1-16 PRNs and a Doppler span of +-2..+-6 kHz, Doppler step
+    1/(2 T_coh).  Signals come from a small pool of pre-synthesised records
+    (4 planted satellites each, C/N0 U[44,48]); replica spectra are put in the
+    context's cache once per (code, length)."""
+
+    def __init__(self, ctx, seed: int = 0xC5, pool: int = 4):
+        from . import gps_l1ca, boc
+        self.ctx = ctx
+        self.rng = np.random.default_rng(seed)
+        self.l1_codes = [gps_l1ca(i) for i in range(32)]
+        self.l5_codes = [random_code(seed * 31 + i, 10230) for i in range(32)]
+        self.boc_codes = [boc(random_code(seed * 37 + i, 4092)) for i in range(36)]
+        self.keys = {"L1": [ctx.replica_put(c, 1, 4092) for c in self.l1_codes],
+                     "L5": [ctx.replica_put(c, 1, 20460) for c in self.l5_codes],
+                     "BOC": [ctx.replica_put(c, 1, 81840) for c in self.boc_codes]}
+        self.pool = {"L1": [], "L5": [], "BOC": []}
+        for i in range(pool):
+            r = np.random.default_rng(seed * 101 + i)
+            sats = _draw_sats(r, 4, 32, lambda p: self.l1_codes[p], 1.023e6, L1_HZ, 1023, 44, 48)
+            self.pool["L1"].append(synthesize(sats, 4.092e6, 4092 * 10, 8.0, seed + i, 8))
+            sats = _draw_sats(r, 4, 32, lambda p: self.l5_codes[p], 10.23e6, L5_HZ, 10230, 44, 48)
+            self.pool["L5"].append(synthesize(sats, 20.46e6, 20460 * 4, 512.0, seed + 100 + i, 16))
+            sats = _draw_sats(r, 4, 36, lambda p: self.boc_codes[p], 2.046e6, L1_HZ, 8184, 44, 48)
+            self.pool["BOC"].append(synthesize(sats, 20.46e6, 81840, 512.0, seed + 200 + i, 16))
+
+    def draw(self) -> ServeRequest:
+        u = self.rng.random()
+        n_prn = int(self.rng.integers(1, 17))
+        span = float(self.rng.integers(2, 7)) * 1000.0
+        sig = self.pool_pick
+        if u < 0.5:
+            T = int(self.rng.integers(1, 11))
+            prns = self.rng.choice(32, n_prn, replace=False)
+            return ServeRequest("L1", sig("L1")[: 2 * 4092 * T], FMT_CI8, 4.092e6, 4092, T, 4092, True,
+                                doppler_bins(span, 500.0), [self.keys["L1"][p] for p in prns])
+        if u < 0.8:
+            T = int(self.rng.integers(1, 5))
+            prns = self.rng.choice(32, n_prn, replace=False)
+            return ServeRequest("L5", sig("L5")[: 2 * 20460 * T], FMT_CI16, 20.46e6, 20460, T, 20460, True,
+                                doppler_bins(span, 500.0), [self.keys["L5"][p] for p in prns])
+        prns = self.rng.choice(36, n_prn, replace=False)
+        return ServeRequest("BOC", sig("BOC"), FMT_CI16, 20.46e6, 81840, 1, 81840, False,
+                            doppler_bins(span, 125.0), [self.keys["BOC"][p] for p in prns])
+
+    def pool_pick(self, kind):
+        return self.pool[kind][int(self.rng.integers(0, len(self.pool[kind])))]
+
+    def stream(self, n: int, rate_hz: float):
+        t = 0.0
+        out = []
+        for _ in range(n):
+            t += float(self.rng.exponential(1.0 / rate_hz))
+            r = self.draw()
+            r.arrival_s = t
+            out.append(r)
+        return out
+
+    def request(self, r: ServeRequest):
+        """C request struct + the arrays it points to (keep both alive)"""
+        return self.ctx.make_request(r.iq, r.fmt, fs_in=r.fs, fs_acq=r.fs, n_fft=r.n_fft, n_blocks=r.n_blocks,
+                                     surface_len=r.surface_len, doppler=r.doppler, keys=r.keys, reshape=r.reshape)
