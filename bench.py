@@ -46,6 +46,11 @@ def parse():
     return ap.parse_args()
 
 
+def log(msg: str):
+    """progress line on stderr (the JSON result goes to stdout)"""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def percentile(xs, q):
     xs = sorted(xs)
     k = (len(xs) - 1) * q
@@ -235,6 +240,7 @@ def main():
         print(json.dumps({"copy_bytes_per_dispatch": n, "dispatches": 5}))
         return
 
+    log(f"workload {args.workload}")
     w = W.CONFIGS[args.workload]()
     ctx = U.Context(0)
     keys = [ctx.replica_put(c, w.repeats, w.n_fft) for c in w.codes]
@@ -246,6 +252,7 @@ def main():
     for _ in range(args.warmup):
         res, _, _ = ctx.acquire_req(req)
     torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
     lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -312,18 +319,22 @@ def main():
         },
         "detected_prns": [i for i, r in enumerate(res) if r["detected"]],
     }
+    log(f"headline {value:.4g} cells/s, {ms_per_step:.3f} ms/search, corr kernel {corr_ms:.3f} ms")
     if not args.no_extra:
+        log("other workloads")
         try:
             out["other_workloads"] = extra_workloads(ctx)
         except Exception as e:  # the headline line must still print
             out["other_workloads"] = {"error": repr(e)}
     if args.serve_requests > 0:
+        log(f"serving {args.serve_requests} requests at {args.serve_rate} req/s")
         try:
             out["serving"] = serving(ctx, args.serve_requests, args.serve_rate)
         except Exception as e:
             out["serving"] = {"error": repr(e)}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
         cpu = cpu_baseline(w, threads)
         cpu_res = cpu.pop("results")
         out["cpu_baseline"] = cpu
