@@ -120,15 +120,15 @@ def cpu_baseline(w, threads: int):
     }
 
 
-def time_search(ctx, w, steps: int, warmup: int):
+def time_search(ctx, w, steps: int, warmup: int, precision: int = 0):
     """seconds per search of workload w with its IQ resident in HBM"""
     import torch
 
-    keys = [ctx.replica_put(c, w.repeats, w.n_fft) for c in w.codes]
+    keys = [ctx.replica_put(c, w.repeats, w.n_fft, precision) for c in w.codes]
     iq_dev = torch.from_numpy(w.iq).to("cuda:0")
     req, hold = ctx.make_request(None, w.fmt, fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=w.n_blocks,
                                  surface_len=w.surface_len, doppler=w.doppler, keys=keys, reshape=w.reshape,
-                                 n_samples=w.n_samples, iq_device_ptr=iq_dev.data_ptr())
+                                 n_samples=w.n_samples, iq_device_ptr=iq_dev.data_ptr(), precision=precision)
     for _ in range(warmup):
         ctx.acquire_req(req)
     torch.cuda.synchronize()
@@ -155,12 +155,33 @@ def extra_workloads(ctx):
     from ugsdr_amd import workloads as W
 
     out = {}
+    boc32 = None
     for name, steps in (("L1CA_1MS", 50), ("L1CA_4MS", 30), ("BOC_4MS", 3)):
         w = W.CONFIGS[name]()
         dt, stages, res = time_search(ctx, w, steps, 2)
         out[name] = {"cells_per_s": w.cells / dt, "ms_per_search": 1e3 * dt, "cells": w.cells,
                      "cell_blocks": w.cells * w.n_blocks, "stage_ms": stages,
                      "detected": [i for i, r in enumerate(res) if r["detected"]]}
+        if name == "BOC_4MS":
+            boc32 = (w, res)
+    # the BOC search in the reduced form: 16-bit storage, matrix-core DFT stages
+    w, res32 = boc32
+    planted = sorted(s.prn for s in w.sats)
+    for tag, prec in (("BOC_4MS_FP16", U.PREC_FP16), ("BOC_4MS_BF16", U.PREC_BF16)):
+        dt, stages, res = time_search(ctx, w, 5, 2, prec)
+        d_db = [20 * np.log10(res[p]["snr"] / res32[p]["snr"]) for p in planted]
+        same = all(res[p]["index"] == res32[p]["index"] and res[p]["doppler_bin"] == res32[p]["doppler_bin"]
+                   for p in planted)
+        alg = 4 * w.cells * w.n_blocks
+        out[tag] = {"cells_per_s": w.cells / dt, "ms_per_search": 1e3 * dt, "cells": w.cells,
+                    "cell_blocks": w.cells * w.n_blocks, "stage_ms": stages,
+                    "roofline": {"bound": "hbm", "achieved": alg / (stages["corr"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": alg / (stages["corr"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                 "bytes_per_cell_block": 4},
+                    "pnr_delta_db_vs_fp32_planted": {"max_abs": float(np.max(np.abs(d_db))),
+                                                     "values": [float(v) for v in d_db]},
+                    "planted_index_and_bin_equal_fp32": same,
+                    "speedup_vs_fp32": out["BOC_4MS"]["ms_per_search"] / (1e3 * dt)}
     bw = W.bank_l5()
     iq_dev = torch.from_numpy(bw.iq).to("cuda:0")
     ctx.correlate_bank(None, U.FMT_CI16, spm=bw.spm, n_blocks=2, fs=bw.fs, channels=bw.channels, codes=bw.codes,
@@ -173,7 +194,7 @@ def extra_workloads(ctx):
     p = np.abs(o[:, :, 2])
     out["BANK_L5"] = {"channel_samples_per_s": bw.channel_samples / dt, "ms_per_call": 1e3 * dt,
                       "blocks": bw.n_blocks, "channels": len(bw.channels), "taps": len(bw.taps),
-                      "prompt_dominates": bool(np.all(p > np.abs(o[:, :, 0])))}
+                      "prompt_over_very_early_fraction": float(np.mean(p > np.abs(o[:, :, 0])))}
     return out
 
 
@@ -215,6 +236,15 @@ def serving(ctx, n_requests: int, rate: float):
     }
 
 
+def corr_kernel_name(n_fft: int, surface_len: int, threads: int) -> str:
+    """the correlation kernel a (length, variant) pair runs"""
+    if n_fft % 1023 == 0 and threads == -(-(33 * (n_fft // 1023)) // 64) * 64:
+        return f"corr1023<{n_fft // 1023},{surface_len // 1023}> (31 x 33 x {n_fft // 1023}, {threads} threads)"
+    if n_fft <= 16384:
+        return f"corr_small<{threads},...> (N={n_fft})"
+    return f"corr_large<{threads},...> (N={n_fft})"
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -246,6 +276,9 @@ def main():
     keys = [ctx.replica_put(c, w.repeats, w.n_fft) for c in w.codes]
     iq_dev = torch.from_numpy(w.iq).to("cuda:0")
     torch.cuda.synchronize()
+    variants = U.corr_variants(w.n_fft)
+    chosen, tune_ms = ctx.autotune(w.n_fft)
+    log(f"autotune N={w.n_fft}: variants {variants} ms {tune_ms} -> {chosen}")
     req, hold = ctx.make_request(None, w.fmt, fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=w.n_blocks,
                                  surface_len=w.surface_len, doppler=w.doppler, keys=keys, reshape=w.reshape,
                                  n_samples=w.n_samples, iq_device_ptr=iq_dev.data_ptr())
@@ -314,10 +347,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "corr_small<512,16384,PlanCT<31,16,11,3>>" if w.n_fft == 16368 else "corr",
+            "kernel": corr_kernel_name(w.n_fft, w.surface_len, chosen),
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "detected_prns": [i for i, r in enumerate(res) if r["detected"]],
+        "autotune": {"variants": variants, "ms_per_launch": tune_ms, "chosen": chosen},
     }
     log(f"headline {value:.4g} cells/s, {ms_per_step:.3f} ms/search, corr kernel {corr_ms:.3f} ms")
     if not args.no_extra:

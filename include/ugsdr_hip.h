@@ -71,6 +71,14 @@ int ugsdr_hip_ctx_last_times(ugsdr_hip_ctx* ctx, float* ms_out4);
  * ms_out[n_ms]: each candidate's time per launch. */
 int ugsdr_hip_autotune(ugsdr_hip_ctx* ctx, int64_t n_fft, int32_t* threads, float* ms_out, int32_t n_ms);
 int ugsdr_hip_set_autotune(ugsdr_hip_ctx* ctx, int enable);
+/* The correlation variants built for a transform length, each named by its
+ * workgroup size (a 1023*M length also has the three-stage kernel, named by
+ * its own size: 192, 320, 448 or 576).  Writes up to n of them to threads_out
+ * and returns how many exist (no reference counterpart). */
+int ugsdr_hip_corr_variants(int64_t n_fft, int32_t* threads_out, int32_t n);
+/* Pins the correlation variant for a length on this context (threads from
+ * ugsdr_hip_corr_variants; 0 unpins, returning to the autotuner/default). */
+int ugsdr_hip_set_threads(ugsdr_hip_ctx* ctx, int64_t n_fft, int32_t threads);
 
 /* ------------------------------------------------------------- PRN codes (host)
  * Codegen<Impl>::Get (src/prn_codes/codegen.hpp:23-36).  sv is 0-based. */
@@ -163,6 +171,13 @@ int ugsdr_hip_mix_inplace(ugsdr_hip_ctx* ctx, float* io_c32, int64_t n, double f
 /* DiscreteFourierTransform::Transform (src/math/ipp_dft.hpp:106-122): any n,
  * forward unscaled, inverse scaled 1/n */
 int ugsdr_hip_dft(ugsdr_hip_ctx* ctx, const float* in_c32, float* out_c32, int64_t n, int inverse);
+/* The same transform in reduced precision over a batch of sequences: 16-bit
+ * (FP16 or BF16) operands on the matrix cores with FP32 accumulation, for the
+ * two-stage lengths of the reduced acquisition path (60, 112, 120, 180, 240,
+ * 341).  Forward unscaled, inverse 1/n, as IppDft (ipp_dft.hpp:106-122); the
+ * 16-bit form itself has no reference counterpart. */
+int ugsdr_hip_dft_reduced(ugsdr_hip_ctx* ctx, int precision, int inverse, const float* in_c32, float* out_c32,
+                          int64_t n, int64_t batch);
 /* MatchedFilter::PrepareCodeSpectrum (matched_filter.hpp:27-30): conj(DFT(real code)) */
 int ugsdr_hip_mf_prepare(ugsdr_hip_ctx* ctx, const float* code, float* spec_c32, int64_t n);
 /* MatchedFilter::FilterOptimized (matched_filter.hpp:32-42, ipp_matched_filter.hpp:80-90) */

@@ -195,6 +195,76 @@ def test_runtime_plan_lengths(ctx, oracle, n_fft):
     ctx.replica_drop(key)
 
 
+@pytest.mark.parametrize("n_fft,surface_len,coherent", [
+    (4092, 4092, True), (8184, 8184, True), (8184, 4092, False), (12276, 12276, True), (12276, 4092, True),
+    (16368, 16368, True), (16368, 4092, True), (16368, 4092, False)])
+def test_corr_variants_match_oracle(ctx, oracle, n_fft, surface_len, coherent):
+    """every correlation variant of a C/A length (the three-stage 1023*M
+    kernel and the generic LDS kernels) against the oracle on one input"""
+    rng = np.random.default_rng(n_fft + surface_len)
+    fs = n_fft * 1000.0 * surface_len / n_fft
+    P, dop = 3, U.doppler_bins(2000, 500)
+    reps, keys = [], []
+    for _ in range(P):
+        code = np.where(rng.random(1023) < 0.5, -1, 1).astype(np.int8)
+        keys.append(ctx.replica_put(code, n_fft // surface_len, n_fft))
+        reps.append(oracle.repeat_upsample(code, n_fft // surface_len, n_fft))
+    x = (rng.normal(0, 20, n_fft) + 1j * rng.normal(0, 20, n_fft)).astype(np.complex64)
+    x += (2.0 * np.roll(reps[1], 777) * np.exp(2j * np.pi * 1000 * np.arange(n_fft) / fs)).astype(np.complex64)
+    iq = np.ascontiguousarray(x).view(np.float32)
+    ref, rsurf, _ = oracle.acquire(x, np.stack(reps), n_fft=n_fft, n_blocks=1, surface_len=surface_len, fs_in=fs,
+                                   fs_acq=fs, doppler=dop, coherent=coherent, precision=64, want_surfaces=True)
+    variants = U.corr_variants(n_fft)
+    assert len(variants) >= 2
+    for t in variants:
+        say(f"case corr variant N={n_fft} L={surface_len} coherent={coherent} threads={t}")
+        ctx.set_threads(n_fft, t)
+        gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft,
+                                   n_blocks=1, surface_len=surface_len, doppler=dop, keys=keys, coherent=coherent)
+        assert_parity(gpu, ref, f"N={n_fft}/L={surface_len}/T={t}")
+        assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
+        assert gpu[1]["index"] == 777 % surface_len and gpu[1]["doppler"] == 1000.0
+    ctx.set_threads(n_fft, 0)
+    with pytest.raises(U.UgsdrError):
+        ctx.set_threads(n_fft, 333)
+    for k in keys:
+        ctx.replica_drop(k)
+
+
+@pytest.mark.parametrize("n_prn,n_bins,n_blocks", [(3, 9, 3), (4, 70, 4)])
+def test_corr1023_tail_rows_split_over_workgroups(ctx, oracle, n_prn, n_bins, n_blocks):
+    """N = 16368 with several blocks per row: whole rounds of rows run one row
+    per workgroup, the rest are split block-wise over all workgroups and
+    merged (3 x 9 rows: every row split; 4 x 70 rows: one round + a tail)"""
+    n_fft = 16368
+    rng = np.random.default_rng(n_prn * 100 + n_bins)
+    fs = 16.368e6
+    reps, keys = [], []
+    for _ in range(n_prn):
+        code = np.where(rng.random(1023) < 0.5, -1, 1).astype(np.int8)
+        keys.append(ctx.replica_put(code, 1, n_fft))
+        reps.append(oracle.repeat_upsample(code, 1, n_fft))
+    n = n_fft * n_blocks
+    x = (rng.normal(0, 20, n) + 1j * rng.normal(0, 20, n)).astype(np.complex64)
+    x += (1.0 * np.tile(np.roll(reps[1], 4321), n_blocks) *
+          np.exp(2j * np.pi * 1000 * np.arange(n) / fs)).astype(np.complex64)
+    iq = np.ascontiguousarray(x).view(np.float32)
+    dop = np.arange(n_bins, dtype=np.float64) * 500.0 - 500.0 * (n_bins // 2)
+    ref, rsurf, _ = oracle.acquire(x, np.stack(reps), n_fft=n_fft, n_blocks=n_blocks, surface_len=n_fft, fs_in=fs,
+                                   fs_acq=fs, doppler=dop, precision=64, want_surfaces=True)
+    t1023 = U.corr_variants(n_fft)[0]
+    ctx.set_threads(n_fft, t1023)
+    say(f"case corr1023 tail split P={n_prn} D={n_bins} K={n_blocks}")
+    gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft,
+                               n_blocks=n_blocks, surface_len=n_fft, doppler=dop, keys=keys)
+    ctx.set_threads(n_fft, 0)
+    assert_parity(gpu, ref, f"tail P={n_prn} D={n_bins} K={n_blocks}")
+    assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
+    assert gpu[1]["index"] == 4321 and gpu[1]["doppler"] == 1000.0
+    for k in keys:
+        ctx.replica_drop(k)
+
+
 def test_zero_signal_keeps_default_result(ctx):
     """all-zero input: no bin beats level 0, so the AcquisitionResult defaults stay
     (src/acquisition/acquisition_result.hpp:13-18, fse.hpp:265)"""

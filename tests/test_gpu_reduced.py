@@ -1,0 +1,106 @@
+"""Reduced-precision path (16-bit storage, matrix-core DFT stages) through the
+C ABI, against numpy's FP64 FFT, the FP64 oracle and the FP32 GPU path.
+
+Stated error bounds (written into the asserts below):
+  DFT operator, relative RMS error vs FP64:  FP16 <= 1e-3, BF16 <= 8e-3
+  acquisition surface, ||S - S64|| / ||S64||: FP16 <= 1e-3, BF16 <= 8e-3
+  peak-to-noise (level / (mean + sigma)) vs FP32, planted satellites: <= 0.1 dB
+  code-phase index and Doppler bin of planted satellites: equal to FP32's
+"""
+import numpy as np
+import pytest
+
+import ugsdr_amd as U
+from ugsdr_amd import workloads as W
+
+from parity import iq_to_complex
+
+pytestmark = pytest.mark.gpu
+
+DFT_TOL = {U.PREC_FP16: 1e-3, U.PREC_BF16: 8e-3}
+SURF_TOL = {U.PREC_FP16: 1e-3, U.PREC_BF16: 8e-3}
+PNR_DB = 0.1
+NAME = {U.PREC_FP16: "fp16", U.PREC_BF16: "bf16"}
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("prec", [U.PREC_FP16, U.PREC_BF16])
+@pytest.mark.parametrize("n", [60, 112, 120, 180, 240, 341])
+def test_dft_reduced_vs_fp64(ctx, n, prec):
+    """every two-stage matrix-core plan, both directions, random input"""
+    rng = np.random.default_rng(n)
+    x = (rng.normal(size=(37, n)) + 1j * rng.normal(size=(37, n))).astype(np.complex64)
+    say(f"case dft_reduced n={n} {NAME[prec]}")
+    for inverse in (False, True):
+        got = ctx.dft_reduced(x, prec, inverse)
+        ref = np.fft.ifft(x.astype(np.complex128), axis=1) if inverse else np.fft.fft(x.astype(np.complex128), axis=1)
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        say(f"  inverse={inverse} rel rms err {err:.2e}")
+        assert err <= DFT_TOL[prec], (n, prec, inverse, err)
+    # a tone lands in one bin
+    t = np.exp(2j * np.pi * 7 * np.arange(n) / n).astype(np.complex64)
+    spec = ctx.dft_reduced(t, prec)[0]
+    assert int(np.argmax(np.abs(spec))) == 7
+
+
+def test_dft_reduced_errors(ctx):
+    with pytest.raises(U.UgsdrError):
+        ctx.dft_reduced(np.zeros(100, np.complex64), U.PREC_FP16)
+    with pytest.raises(U.UgsdrError):
+        ctx.dft_reduced(np.zeros(240, np.complex64), U.PREC_FP32)
+
+
+def _boc_subset():
+    w = W.boc_4ms()
+    planted = sorted(s.prn for s in w.sats)
+    absent = [p for p in range(len(w.codes)) if p not in planted][:2]
+    return w, planted[:4] + absent, planted[:4]
+
+
+@pytest.mark.parametrize("prec", [U.PREC_FP16, U.PREC_BF16])
+def test_reduced_boc_vs_fp64_and_fp32(ctx, oracle, prec):
+    """BOC(1,1) 4 ms x 5 blocks, N = 81840, all 97 bins, 4 planted + 2 absent PRNs"""
+    w, prns, planted = _boc_subset()
+    say(f"case reduced BOC_4MS {NAME[prec]} PRNs {prns}")
+    k32 = [ctx.replica_put(w.codes[p], 1, w.n_fft) for p in prns]
+    kr = [ctx.replica_put(w.codes[p], 1, w.n_fft, prec) for p in prns]
+    kw = dict(fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=w.n_blocks, surface_len=w.surface_len,
+              doppler=w.doppler, reshape=w.reshape)
+    g32, s32, _ = ctx.acquire(w.iq, w.fmt, want_surfaces=True, keys=k32, **kw)
+    gr, sr, _ = ctx.acquire(w.iq, w.fmt, want_surfaces=True, keys=kr, precision=prec, **kw)
+    x = iq_to_complex(w.iq)
+    reps = w.replicas()[prns]
+    ref, s64, _ = oracle.acquire(x, reps, n_fft=w.n_fft, n_blocks=w.n_blocks, surface_len=w.surface_len,
+                                 fs_in=w.fs, fs_acq=w.fs, doppler=w.doppler, reshape=w.reshape, precision=64,
+                                 want_surfaces=True)
+    err_r = np.linalg.norm(sr - s64) / np.linalg.norm(s64)
+    err_32 = np.linalg.norm(s32 - s64) / np.linalg.norm(s64)
+    say(f"  surface rel err vs FP64: reduced {err_r:.2e}, FP32 {err_32:.2e}")
+    assert err_r <= SURF_TOL[prec]
+    for i, p in enumerate(prns):
+        d_db = 20 * np.log10(gr[i]["snr"] / g32[i]["snr"])
+        d64 = 20 * np.log10(gr[i]["snr"] / ref[i]["snr"])
+        say(f"  PRN {p}: snr fp32 {g32[i]['snr']:.4f} reduced {gr[i]['snr']:.4f} fp64 {ref[i]['snr']:.4f} "
+            f"delta {d_db:+.4f} dB (vs fp64 {d64:+.4f} dB) idx {gr[i]['index']}/{g32[i]['index']} "
+            f"bin {gr[i]['doppler_bin']}/{g32[i]['doppler_bin']}")
+        if p in planted:
+            assert abs(d_db) <= PNR_DB, (p, d_db)
+            assert abs(d64) <= PNR_DB, (p, d64)
+            assert gr[i]["index"] == g32[i]["index"] == ref[i]["index"]
+            assert gr[i]["doppler_bin"] == g32[i]["doppler_bin"] == ref[i]["doppler_bin"]
+    for k in k32 + kr:
+        ctx.replica_drop(k)
+
+
+def test_reduced_rejects_unsupported(ctx):
+    """lengths without a matrix-core plan and folded surfaces report UNSUPPORTED"""
+    w = W.l1ca_1ms()
+    key = ctx.replica_put(w.codes[0], w.repeats, w.n_fft, U.PREC_FP16)
+    with pytest.raises(U.UgsdrError) as e:
+        ctx.acquire(w.iq, w.fmt, fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=1, surface_len=w.surface_len,
+                    doppler=w.doppler, keys=[key], precision=U.PREC_FP16)
+    assert e.value.code == -4
+    ctx.replica_drop(key)

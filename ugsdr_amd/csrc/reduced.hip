@@ -1,14 +1,30 @@
-// Reduced-precision storage of replica spectra (FP16 / BF16) and the
-// reduced-precision acquisition entry.
+// Reduced-precision acquisition (FP16 / BF16 storage, matrix-core DFTs).
+//
+//   replicas : conj(DFT) stored as 16-bit pairs scaled by 1/sqrt(N);
+//   classes  : the FP32 class spectra X[k][c] of each search are packed to
+//              16-bit pairs with a power-of-two scale per block (k, c), so
+//              every block uses the full 16-bit range;
+//   corr     : N = N1 * 341 four-step inverse DFT per row and block, every
+//              stage (N1 = Ra * Rb, 341 = 11 * 31) a matrix-core product with
+//              FP32 accumulation; the DFT matrices carry 1/R, so values stay
+//              O(1) and the result is already 1/N scaled; |.| accumulates in
+//              FP32 with the block scales undone, into a workgroup-private
+//              surface in global memory, then the FP32 row statistics.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
+#include "acq.hpp"
+#include "corr_common.hpp"
 #include "ctx.hpp"
+#include "dev_mfma.hpp"
 #include "internal.hpp"
+
+using namespace ugsdr_dev;
 
 namespace ugsdr_hip {
 
@@ -77,9 +93,377 @@ int replica_read(ugsdr_hip_ctx* ctx, const Replica& rep, float* out) {
     return UGSDR_OK;
 }
 
-int acquire_reduced(ugsdr_hip_ctx*, const ugsdr_acq_request*, ugsdr_acq_result*, float*, float*) {
-    set_error("reduced-precision acquisition is not built yet");
-    return UGSDR_ERR_UNSUPPORTED;
+// ------------------------------------------------------------------ class blocks
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// max |X| of block blockIdx.y (magnitudes are non-negative, so their bit
+// patterns order like unsigned integers)
+__global__ void block_absmax_kernel(const float2* X, int64_t N, unsigned* bmax) {
+    const float2* x = X + (int64_t)blockIdx.y * N;
+    float m = 0.f;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, sqrtf(fmaf(x[i].x, x[i].x, x[i].y * x[i].y)));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(bmax + blockIdx.y, __float_as_uint(m));
+}
+
+// 2^-e with max |X| = f 2^e, f in [0.5, 1): the block lands in [0.5, 1)
+__device__ __forceinline__ int block_exponent(unsigned bits) {
+    const float m = __uint_as_float(bits);
+    int e = 0;
+    if (m > 0.f) frexpf(m, &e);
+    return e;
+}
+
+template <class H>
+__global__ void block_pack_kernel(const float2* X, int64_t N, const unsigned* bmax, uint32_t* Xr, float* inv) {
+    const int b = blockIdx.y;
+    const int e = block_exponent(bmax[b]);
+    const float s = ldexpf(1.f, -e);
+    const float2* x = X + (int64_t)b * N;
+    uint32_t* o = Xr + (int64_t)b * N;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        o[i] = pack16<H>(make_float2(x[i].x * s, x[i].y * s));
+    if (blockIdx.x == 0 && threadIdx.x == 0) inv[b] = ldexpf(1.f, e);
+}
+
+// ------------------------------------------------------------------ correlation
+constexpr int kRedCap = 16384;   // 32-bit pairs per LDS buffer (two buffers)
+constexpr int kRedN2 = 341;      // second four-step factor, 11 * 31
+
+template <int R1a, int R1b>
+struct RedShape {
+    static constexpr int N1 = R1a * R1b;
+    static size_t lds_bytes(int N) {
+        return (size_t)2 * kRedCap * 4 +
+               (size_t)(MfmaShape<R1a>::ENTRIES + MfmaShape<R1b>::ENTRIES + MfmaShape<11>::ENTRIES +
+                        MfmaShape<31>::ENTRIES) * 16 +
+               (size_t)(N1 + kRedN2 + kTwLo + (N + kTwLo - 1) / kTwLo) * 8 + 128 * 8;
+    }
+};
+
+template <class H, int R1a, int R1b>
+__global__ __launch_bounds__(512, 2) void corr_mfma(CorrArgs a) {
+    constexpr int T = 512, NW = T / 64, N1 = R1a * R1b, N2 = kRedN2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* bufA = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* bufB = bufA + kRedCap;
+    uint4* bt1a = reinterpret_cast<uint4*>(bufB + kRedCap);
+    uint4* bt1b = bt1a + MfmaShape<R1a>::ENTRIES;
+    uint4* bt2a = bt1b + MfmaShape<R1b>::ENTRIES;
+    uint4* bt2b = bt2a + MfmaShape<11>::ENTRIES;
+    float2* tw1 = reinterpret_cast<float2*>(bt2b + MfmaShape<31>::ENTRIES);
+    float2* tw2 = tw1 + N1;
+    float2* twlo = tw2 + N2;
+    float2* twhi = twlo + kTwLo;
+    const int N = a.N;
+    double* red = reinterpret_cast<double*>(twhi + (N + kTwLo - 1) / kTwLo);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    mfma_btab_build<R1a, H>(bt1a, true, 1.f / R1a, tid, T);
+    mfma_btab_build<R1b, H>(bt1b, true, 1.f / R1b, tid, T);
+    mfma_btab_build<11, H>(bt2a, true, 1.f / 11, tid, T);
+    mfma_btab_build<31, H>(bt2b, true, 1.f / 31, tid, T);
+    tw_table_build(tw1, N1, true, tid, T);
+    tw_table_build(tw2, N2, true, tid, T);
+    tw_build(twlo, twhi, N, tid, T);
+    __syncthreads();
+    // column batches with a padded stride (conflict-free strided LDS access),
+    // row batches contiguous; both balanced over their batch count
+    const int ld1 = N1 + 1;
+    const int nbat1 = (N2 + kRedCap / ld1 - 1) / (kRedCap / ld1), cb = (N2 + nbat1 - 1) / nbat1;
+    const int nbat2 = (N1 + kRedCap / N2 - 1) / (kRedCap / N2), rb = (N1 + nbat2 - 1) / nbat2;
+    uint32_t* G = reinterpret_cast<uint32_t*>(a.scratch) + (int64_t)blockIdx.x * 2 * N;
+    float* S = reinterpret_cast<float*>(G + N);
+    for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
+        const ReqDesc q = a.reqs[find_req(a, row)];
+        const int local = row - q.row_start;
+        const int p = local / q.D, di = local - p * q.D;
+        const int cls = q.cls[di], sh = q.shift[di];
+        const gptr<uint32_t> C = as_global(reinterpret_cast<const uint32_t*>(q.specs[p]));
+        for (int k = 0; k < q.K; ++k) {
+            const int t0 = opaque(tid);
+            const gptr<uint32_t> X =
+                as_global(reinterpret_cast<const uint32_t*>(q.X) + ((int64_t)k * q.F + cls) * N);
+            const float sc = q.block_inv[k * q.F + cls] * q.spec_inv;
+            for (int c0 = 0; c0 < N2; c0 += cb) {
+                const int nb = min(cb, N2 - c0);
+                for (int idx = t0; idx < nb * N1; idx += T) {
+                    const int t = idx % nb, n1 = idx / nb;
+                    const int n = c0 + t + N2 * n1;
+                    int src = n + sh;
+                    src -= src >= N ? N : 0;
+                    bufA[t * ld1 + n1] = pack16<H>(cmul(unpack16<H>(X[src]), unpack16<H>(C[n])));
+                }
+                __syncthreads();
+                mfma_stage<R1a, H, true, NW>(bufA, ld1, bufB, N1, N1, nb, nullptr, bt1a, lane, wave);
+                __syncthreads();
+                mfma_stage<R1b, H, false, NW>(bufB, N1, bufA, ld1, N1, nb, tw1, bt1b, lane, wave);
+                __syncthreads();
+                for (int idx = t0; idx < nb * N1; idx += T) {
+                    const int t = idx % nb, k1 = idx / nb;
+                    const int n2 = c0 + t;
+                    G[k1 * N2 + n2] =
+                        pack16<H>(cmul(unpack16<H>(bufA[t * ld1 + k1]), tw_get<true>(twlo, twhi, n2 * k1)));
+                }
+                __syncthreads();
+            }
+            for (int r0 = 0; r0 < N1; r0 += rb) {
+                const int nb = min(rb, N1 - r0);
+                for (int idx = t0; idx < nb * N2; idx += T) bufA[idx] = G[r0 * N2 + idx];
+                __syncthreads();
+                mfma_stage<11, H, true, NW>(bufA, N2, bufB, N2, N2, nb, nullptr, bt2a, lane, wave);
+                __syncthreads();
+                mfma_stage<31, H, false, NW>(bufB, N2, bufA, N2, N2, nb, tw2, bt2b, lane, wave);
+                __syncthreads();
+                for (int idx = t0; idx < nb * N2; idx += T) {
+                    const int t = idx % nb, k2 = idx / nb;
+                    const int m = r0 + t + N1 * k2;
+                    const float mag = cabs_hw(unpack16<H>(bufA[t * N2 + k2])) * sc;
+                    S[m] = k == 0 ? mag : S[m] + mag;
+                }
+                __syncthreads();
+            }
+        }
+        row_stats_glob<T>(S, N, red, tid, q.rowstats + local, 1.0f);
+        if (q.surfaces) {
+            float* o = q.surfaces + (int64_t)local * N;
+            for (int j = tid; j < N; j += T) o[j] = S[j];
+        }
+        __syncthreads();
+    }
+}
+
+// lengths N1 * 341 with a two-stage matrix-core plan for N1
+static bool red_plan(int64_t N, int& ra, int& rb_) {
+    if (N <= 0 || N % kRedN2) return false;
+    switch (N / kRedN2) {
+        case 60: ra = 4; rb_ = 15; return true;
+        case 112: ra = 16; rb_ = 7; return true;
+        case 120: ra = 8; rb_ = 15; return true;
+        case 180: ra = 12; rb_ = 15; return true;
+        case 240: ra = 16; rb_ = 15; return true;
+        default: return false;
+    }
+}
+
+template <class H, int R1a, int R1b>
+static int launch_corr_mfma(const CorrArgs& c, int grid, hipStream_t st) {
+    const size_t lds = RedShape<R1a, R1b>::lds_bytes(c.N);
+    UGSDR_HIP_TRY(set_lds_limit(reinterpret_cast<const void*>(corr_mfma<H, R1a, R1b>), lds));
+    hipLaunchKernelGGL((corr_mfma<H, R1a, R1b>), dim3(grid), dim3(512), lds, st, c);
+    return UGSDR_OK;
+}
+
+template <class H>
+static int launch_corr_mfma_n(const CorrArgs& c, int grid, hipStream_t st) {
+    switch (c.N / kRedN2) {
+        case 60: return launch_corr_mfma<H, 4, 15>(c, grid, st);
+        case 112: return launch_corr_mfma<H, 16, 7>(c, grid, st);
+        case 120: return launch_corr_mfma<H, 8, 15>(c, grid, st);
+        case 180: return launch_corr_mfma<H, 12, 15>(c, grid, st);
+        default: return launch_corr_mfma<H, 16, 15>(c, grid, st);
+    }
+}
+
+static int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st, int precision) {
+    const int nblk = (int)(pl.K * pl.F);
+    UGSDR_HIP_TRY(s.classes_r.reserve((size_t)nblk * pl.N * 4));
+    UGSDR_HIP_TRY(s.block_scale.reserve((size_t)nblk * 8));
+    unsigned* bmax = s.block_scale.as<unsigned>();
+    float* inv = reinterpret_cast<float*>(bmax + nblk);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, st, bmax, nblk);
+    const dim3 grid((unsigned)std::min<int64_t>((pl.N + 1023) / 1024, 64), (unsigned)nblk);
+    hipLaunchKernelGGL(block_absmax_kernel, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.N, bmax);
+    if (precision == UGSDR_PREC_FP16)
+        hipLaunchKernelGGL(block_pack_kernel<Fp16Ops>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.N, bmax,
+                           s.classes_r.as<uint32_t>(), inv);
+    else
+        hipLaunchKernelGGL(block_pack_kernel<Bf16Ops>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.N, bmax,
+                           s.classes_r.as<uint32_t>(), inv);
+    UGSDR_HIP_TRY(hipGetLastError());
+    return UGSDR_OK;
+}
+
+static int enqueue_corr_reduced(const AcqPlan& pl, AcqSlot& s, hipStream_t st, DevBuf& scratch, int cus,
+                                int precision, float spec_scale) {
+    char* dp = s.tables.as<char>();
+    ReqDesc q{};
+    q.X = reinterpret_cast<const float2*>(s.classes_r.ptr);
+    q.specs = reinterpret_cast<const float2* const*>(dp + pl.off_specs);
+    q.cls = reinterpret_cast<const int*>(dp + pl.off_cls);
+    q.shift = reinterpret_cast<const int*>(dp + pl.off_shift);
+    q.rowstats = s.rows.as<RowStat>();
+    q.surfaces = s.surf.ptr ? s.surf.as<float>() : nullptr;
+    q.K = (int)pl.K;
+    q.F = pl.F;
+    q.D = (int)pl.D;
+    q.P = (int)pl.P;
+    q.row_start = 0;
+    q.block_inv = reinterpret_cast<const float*>(s.block_scale.as<unsigned>() + pl.K * pl.F);
+    q.spec_inv = 1.0f / spec_scale;
+    CorrArgs c{};
+    c.reqs[0] = q;
+    c.N = (int)pl.N;
+    c.L = (int)pl.L;
+    c.fold_mode = pl.fold_mode;
+    c.rows = (int)(pl.P * pl.D);
+    c.n_req = 1;
+    c.inv_n = 1.0f;
+    const int grid = std::min(c.rows, cus);
+    UGSDR_HIP_TRY(scratch.reserve((size_t)grid * 2 * pl.N * 4));
+    c.scratch = scratch.as<float2>();
+    int rc = precision == UGSDR_PREC_FP16 ? launch_corr_mfma_n<Fp16Ops>(c, grid, st)
+                                          : launch_corr_mfma_n<Bf16Ops>(c, grid, st);
+    if (rc) return rc;
+    UGSDR_HIP_TRY(hipGetLastError());
+    return UGSDR_OK;
+}
+
+bool reduced_supported(int64_t N, int64_t L) {
+    int ra, rb_;
+    return red_plan(N, ra, rb_) && L == N;
+}
+
+int acquire_reduced(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_result* out, float* surfaces,
+                    float* peaks) {
+    UGSDR_CHECK_ARG(r->precision == UGSDR_PREC_FP16 || r->precision == UGSDR_PREC_BF16, "unknown precision");
+    AcqPlan pl;
+    if (int rc = make_acq_plan(ctx, r, pl)) return rc;
+    if (!reduced_supported(pl.N, pl.L)) {
+        set_error("reduced precision runs the four-step lengths N1 * 341 (N1 in 60, 112, 120, 180, 240) with "
+                  "surface_len == n_fft; got n_fft " + std::to_string(pl.N) + ", surface_len " +
+                  std::to_string(pl.L));
+        return UGSDR_ERR_UNSUPPORTED;
+    }
+    const float spec_scale = ctx->replicas.at(r->replica_keys[0]).scale;
+    AcqSlot& s = ctx->slot;
+    const bool keep_surf = surfaces || peaks;
+    if (!keep_surf) s.surf.release();
+    hipStream_t st = ctx->stream;
+    ctx->mark(0);
+    if (int rc = enqueue_front(pl, r, s, st, keep_surf)) return rc;
+    ctx->mark(1);
+    if (int rc = enqueue_classes(pl, s, st)) return rc;
+    if (int rc = enqueue_reduce_classes(pl, s, st, r->precision)) return rc;
+    ctx->mark(2);
+    if (int rc = enqueue_corr_reduced(pl, s, st, ctx->scratch, device_cu_count(ctx->device), r->precision,
+                                      spec_scale))
+        return rc;
+    ctx->mark(3);
+    if (int rc = enqueue_final(pl, s, st, peaks != nullptr)) return rc;
+    ctx->mark(4);
+    UGSDR_HIP_TRY(hipMemcpyAsync(out, s.results.ptr, (size_t)pl.P * sizeof(ugsdr_acq_result), hipMemcpyDeviceToHost, st));
+    if (peaks)
+        UGSDR_HIP_TRY(hipMemcpyAsync(peaks, s.results.as<char>() + (size_t)pl.P * sizeof(ugsdr_acq_result),
+                                     (size_t)pl.P * pl.L * 4, hipMemcpyDeviceToHost, st));
+    if (surfaces)
+        UGSDR_HIP_TRY(hipMemcpyAsync(surfaces, s.surf.ptr, (size_t)pl.P * pl.D * pl.L * 4, hipMemcpyDeviceToHost, st));
+    UGSDR_HIP_TRY(hipStreamSynchronize(st));
+    if (ctx->profile)
+        for (int i = 0; i < 4; ++i) UGSDR_HIP_TRY(hipEventElapsedTime(&ctx->last_ms[i], ctx->ev[i], ctx->ev[i + 1]));
+    return UGSDR_OK;
+}
+
+// ------------------------------------------------------------------ DFT operator
+// Batched two-stage matrix-core DFT of n = Ra * Rb points (16-bit operands,
+// FP32 accumulation): the reduced-precision DiscreteFourierTransform.
+template <class H, int Ra, int Rb>
+__global__ __launch_bounds__(256) void dft2_mfma_kernel(const float2* in, float2* out, int batch, int inverse,
+                                                         float in_scale, float out_scale) {
+    constexpr int n = Ra * Rb, T = 256, NW = T / 64;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* bufA = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* bufB = bufA + kRedCap / 2;
+    uint4* bta = reinterpret_cast<uint4*>(bufB + kRedCap / 2);
+    uint4* btb = bta + MfmaShape<Ra>::ENTRIES;
+    float2* tw = reinterpret_cast<float2*>(btb + MfmaShape<Rb>::ENTRIES);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    mfma_btab_build<Ra, H>(bta, inverse != 0, 1.f, tid, T);
+    mfma_btab_build<Rb, H>(btb, inverse != 0, 1.f, tid, T);
+    tw_table_build(tw, n, inverse != 0, tid, T);
+    const int per_wg = (kRedCap / 2) / n;
+    const int s0 = blockIdx.x * per_wg, nb = min(per_wg, batch - s0);
+    for (int i = tid; i < nb * n; i += T) {
+        const float2 v = in[(int64_t)s0 * n + i];
+        bufA[i] = pack16<H>(make_float2(v.x * in_scale, v.y * in_scale));
+    }
+    __syncthreads();
+    mfma_stage<Ra, H, true, NW>(bufA, n, bufB, n, n, nb, nullptr, bta, lane, wave);
+    __syncthreads();
+    mfma_stage<Rb, H, false, NW>(bufB, n, bufA, n, n, nb, tw, btb, lane, wave);
+    __syncthreads();
+    for (int i = tid; i < nb * n; i += T) {
+        const float2 v = unpack16<H>(bufA[i]);
+        out[(int64_t)s0 * n + i] = make_float2(v.x * out_scale, v.y * out_scale);
+    }
+}
+
+template <class H, int Ra, int Rb>
+static int launch_dft2(const float2* in, float2* out, int batch, int inverse, float in_scale, float out_scale,
+                       hipStream_t st) {
+    constexpr int n = Ra * Rb;
+    const int per_wg = (kRedCap / 2) / n;
+    const size_t lds = (size_t)kRedCap * 4 + (size_t)(MfmaShape<Ra>::ENTRIES + MfmaShape<Rb>::ENTRIES) * 16 +
+                       (size_t)n * 8;
+    UGSDR_HIP_TRY(set_lds_limit(reinterpret_cast<const void*>(dft2_mfma_kernel<H, Ra, Rb>), lds));
+    hipLaunchKernelGGL((dft2_mfma_kernel<H, Ra, Rb>), dim3((batch + per_wg - 1) / per_wg), dim3(256), lds, st, in,
+                       out, batch, inverse, in_scale, out_scale);
+    return UGSDR_OK;
+}
+
+template <class H>
+static int launch_dft2_n(int n, const float2* in, float2* out, int batch, int inverse, float is, float os,
+                         hipStream_t st) {
+    switch (n) {
+        case 60: return launch_dft2<H, 4, 15>(in, out, batch, inverse, is, os, st);
+        case 112: return launch_dft2<H, 16, 7>(in, out, batch, inverse, is, os, st);
+        case 120: return launch_dft2<H, 8, 15>(in, out, batch, inverse, is, os, st);
+        case 180: return launch_dft2<H, 12, 15>(in, out, batch, inverse, is, os, st);
+        case 240: return launch_dft2<H, 16, 15>(in, out, batch, inverse, is, os, st);
+        case 341: return launch_dft2<H, 11, 31>(in, out, batch, inverse, is, os, st);
+        default:
+            set_error("reduced DFT lengths are 60, 112, 120, 180, 240 and 341");
+            return UGSDR_ERR_UNSUPPORTED;
+    }
 }
 
 }  // namespace ugsdr_hip
+
+extern "C" int ugsdr_hip_dft_reduced(ugsdr_hip_ctx* ctx, int precision, int inverse, const float* in, float* out,
+                                     int64_t n, int64_t batch) {
+    using namespace ugsdr_hip;
+    if (!ctx || !in || !out || n <= 0 || batch <= 0 || batch > (1 << 20)) {
+        set_error("bad arguments");
+        return UGSDR_ERR_ARG;
+    }
+    if (precision != UGSDR_PREC_FP16 && precision != UGSDR_PREC_BF16) {
+        set_error("reduced DFT precision is FP16 or BF16");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    UGSDR_HIP_TRY(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)n * batch * 8;
+    // power-of-two input scale to [0.5, 1); forward unscaled, inverse 1/n
+    float mx = 0.f;
+    for (int64_t i = 0; i < 2 * n * batch; ++i) mx = std::max(mx, std::fabs(in[i]));
+    int e = 0;
+    if (mx > 0.f) std::frexp(mx, &e);
+    const float is = std::ldexp(1.f, -e);
+    const float os = std::ldexp(1.f, e) / (inverse ? (float)n : 1.f);
+    UGSDR_HIP_TRY(ctx->aux.reserve(bytes));
+    UGSDR_HIP_TRY(ctx->aux2.reserve(bytes));
+    UGSDR_HIP_TRY(hipMemcpyAsync(ctx->aux.ptr, in, bytes, hipMemcpyHostToDevice, ctx->stream));
+    int rc = precision == UGSDR_PREC_FP16
+                 ? launch_dft2_n<ugsdr_dev::Fp16Ops>((int)n, ctx->aux.as<float2>(), ctx->aux2.as<float2>(),
+                                                     (int)batch, inverse, is, os, ctx->stream)
+                 : launch_dft2_n<ugsdr_dev::Bf16Ops>((int)n, ctx->aux.as<float2>(), ctx->aux2.as<float2>(),
+                                                     (int)batch, inverse, is, os, ctx->stream);
+    if (rc) return rc;
+    UGSDR_HIP_TRY(hipGetLastError());
+    UGSDR_HIP_TRY(hipMemcpyAsync(out, ctx->aux2.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return UGSDR_OK;
+}

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "acq.hpp"
+#include "corr_common.hpp"
 #include "ctx.hpp"
 #include "internal.hpp"
 
@@ -80,7 +81,7 @@ struct Server {
     ugsdr_hip_ctx* ctx = nullptr;
     ugsdr_server_config cfg{};
     std::vector<hipStream_t> streams;
-    std::vector<DevBuf> desc, scratch;  // per stream
+    std::vector<DevBuf> scratch;  // per stream
     std::vector<ServerSlot> slots;
     std::vector<int> free_slots;
     std::mutex mu;
@@ -139,12 +140,12 @@ static int launch_round(Server& s, Round& rd) {
     }
     // the per-stream batch buffers may only grow once the stream's earlier rounds are done
     const AcqPlan& p0 = *plans[0];
-    const size_t need_desc = plans.size() * 64 * 2;
-    const size_t need_scratch =
-        p0.small ? 0 : (2 * (size_t)p0.N + (p0.L + 1) / 2) * sizeof(float2) * (size_t)s.cus;
-    if (s.desc[rd.stream].bytes < need_desc || s.scratch[rd.stream].bytes < need_scratch)
+    const size_t need_scratch = corr_scratch_bytes(p0, s.cus);
+    if (s.scratch[rd.stream].bytes < need_scratch) {
         UGSDR_HIP_TRY(hipStreamSynchronize(st));
-    if (int rc = enqueue_corr(plans, devs, st, s.desc[rd.stream], s.scratch[rd.stream], s.cus, p0.threads)) return rc;
+        UGSDR_HIP_TRY(s.scratch[rd.stream].reserve(need_scratch));
+    }
+    if (int rc = enqueue_corr(plans, devs, st, s.scratch[rd.stream], s.cus, p0.threads)) return rc;
     for (auto& jp : rd.jobs) {
         Job& j = *jp;
         ServerSlot& sl = s.slots[j.slot];
@@ -293,7 +294,7 @@ int ugsdr_hip_server_create(ugsdr_hip_ctx* ctx, const ugsdr_server_config* cfg, 
     if (s.cfg.max_device_bytes <= 0) s.cfg.max_device_bytes = 8ll << 30;
     if (s.cfg.max_inflight <= 0) s.cfg.max_inflight = 64;
     if (s.cfg.n_streams <= 0) s.cfg.n_streams = 4;
-    if (s.cfg.max_batch <= 0) s.cfg.max_batch = 32;
+    if (s.cfg.max_batch <= 0 || s.cfg.max_batch > kMaxBatch) s.cfg.max_batch = kMaxBatch;
     s.cus = device_cu_count(ctx->device);
     s.streams.resize(s.cfg.n_streams);
     for (auto& st : s.streams) {
@@ -306,7 +307,6 @@ int ugsdr_hip_server_create(ugsdr_hip_ctx* ctx, const ugsdr_server_config* cfg, 
             return UGSDR_ERR_HIP;
         }
     }
-    s.desc.resize(s.cfg.n_streams);
     s.scratch.resize(s.cfg.n_streams);
     s.slots.resize(s.cfg.max_inflight);
     for (int i = s.cfg.max_inflight - 1; i >= 0; --i) s.free_slots.push_back(i);
@@ -423,7 +423,6 @@ int ugsdr_hip_server_destroy(ugsdr_hip_server* h) {
     }
     for (auto e : s.event_pool) (void)hipEventDestroy(e);
     for (auto& sl : s.slots) sl.dev.release_all();
-    for (auto& d : s.desc) d.release();
     for (auto& d : s.scratch) d.release();
     delete h;
     return UGSDR_OK;
