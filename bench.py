@@ -198,18 +198,18 @@ def extra_workloads(ctx):
     return out
 
 
-def serving(ctx, n_requests: int, rate: float):
+def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0):
     """Poisson arrivals of the mixed request stream into the continuous-batching server"""
     import numpy as np
 
     import ugsdr_amd as U
     from ugsdr_amd import workloads as W
 
-    mix = W.ServeMix(ctx, seed=0xC5)
+    mix = W.ServeMix(ctx, seed=0xC5, boc_precision=boc_precision)
     warm = [mix.request(mix.draw()) for _ in range(64)]
     reqs = mix.stream(n_requests, rate)
     built = [mix.request(r) for r in reqs]
-    with U.Server(ctx, n_streams=4, max_batch=32) as srv:
+    with U.Server(ctx, n_streams=4, max_batch=16) as srv:
         for t in [srv.submit(q) for q, _ in warm]:
             srv.poll(t, -1)
         t0 = time.perf_counter()
@@ -233,6 +233,7 @@ def serving(ctx, n_requests: int, rate: float):
         "p99_ms_by_kind": {k: percentile(v, 0.99) for k, v in lat_by.items() if v},
         "cells_per_s": cells / elapsed, "mix": kinds, "rounds": st["rounds"],
         "max_batch_seen": st["max_batch_seen"], "failed": st["failed"],
+        "boc_precision": {0: "fp32", 1: "fp16", 2: "bf16"}[boc_precision],
     }
 
 
@@ -366,6 +367,11 @@ def main():
             out["serving"] = serving(ctx, args.serve_requests, args.serve_rate)
         except Exception as e:
             out["serving"] = {"error": repr(e)}
+        log("serving again with the BOC requests in the reduced form (FP16)")
+        try:
+            out["serving_boc_fp16"] = serving(ctx, args.serve_requests, args.serve_rate, 1)
+        except Exception as e:
+            out["serving_boc_fp16"] = {"error": repr(e)}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")

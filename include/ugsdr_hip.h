@@ -71,6 +71,14 @@ int ugsdr_hip_ctx_last_times(ugsdr_hip_ctx* ctx, float* ms_out4);
  * ms_out[n_ms]: each candidate's time per launch. */
 int ugsdr_hip_autotune(ugsdr_hip_ctx* ctx, int64_t n_fft, int32_t* threads, float* ms_out, int32_t n_ms);
 int ugsdr_hip_set_autotune(ugsdr_hip_ctx* ctx, int enable);
+/* hipGraph replay of repeated searches (no reference counterpart; on by
+ * default): when ugsdr_hip_acquire sees the same request shape (everything
+ * but the host IQ pointer and its contents) twice in a row, it captures the
+ * search's kernels into a graph and replays it on later calls; the host IQ
+ * upload and the result copies stay outside the graph.  Results are those
+ * of the eager path.  graph_replays counts the replays of this context. */
+int ugsdr_hip_set_graphs(ugsdr_hip_ctx* ctx, int enable);
+int64_t ugsdr_hip_graph_replays(ugsdr_hip_ctx* ctx);
 /* The correlation variants built for a transform length, each named by its
  * workgroup size (a 1023*M length also has the three-stage kernel, named by
  * its own size: 192, 320, 448 or 576).  Writes up to n of them to threads_out

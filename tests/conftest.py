@@ -27,3 +27,29 @@ def oracle():
 
     refcpu.lib()
     return refcpu
+
+
+class DeviceArray:
+    """Device memory from the HIP runtime the library itself uses (for tests
+    of the device-pointer entry points)."""
+
+    def __init__(self, host):
+        import ctypes as C
+
+        import numpy as np
+
+        import ugsdr_amd
+
+        ugsdr_amd.lib()   # the library's HIP runtime is loaded first
+        self._hip = C.CDLL("libamdhip64.so.7")
+        self.host = np.ascontiguousarray(host)
+        self.ptr = C.c_void_p()
+        assert self._hip.hipMalloc(C.byref(self.ptr), C.c_size_t(self.host.nbytes)) == 0
+        assert self._hip.hipMemcpy(self.ptr, self.host.ctypes.data_as(C.c_void_p), C.c_size_t(self.host.nbytes),
+                                   1) == 0   # hipMemcpyHostToDevice
+        assert self._hip.hipDeviceSynchronize() == 0
+
+    def close(self):
+        if self.ptr:
+            self._hip.hipFree(self.ptr)
+            self.ptr = None

@@ -304,3 +304,44 @@ def test_replica_spectrum_matches_oracle(ctx, oracle):
     ref = np.conj(np.fft.fft(oracle.repeat_upsample(code, 4, 16368).astype(np.float64)))
     assert np.max(np.abs(spec - ref)) < 1e-5 * np.max(np.abs(ref))
     ctx.replica_drop(key)
+
+
+def test_graph_replay_matches_eager(ctx):
+    """repeated request shapes: eager, captured, then replayed from a hipGraph;
+    every call (with other eager work between them) equals the graphs-off
+    result bit for bit, for host IQ and for device IQ"""
+    from conftest import DeviceArray
+
+    w = W.l1ca_1ms()
+    keys = put_replicas(ctx, w)
+    rng = np.random.default_rng(7)
+    iqs = [w.iq] + [np.ascontiguousarray(np.roll(w.iq, int(rng.integers(1, 4000)) * 2)) for _ in range(3)]
+    kw = dict(fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=w.n_blocks, surface_len=w.surface_len,
+              doppler=w.doppler, keys=keys)
+    ctx.set_graphs(False)
+    ref = [ctx.acquire(iq, w.fmt, want_peaks=True, **kw) for iq in iqs]
+    ctx.set_graphs(True)
+    r0 = ctx.graph_replays()
+    say("case graph replay, host IQ")
+    for rep in range(2):
+        for i, iq in enumerate(iqs):
+            res, _, peaks = ctx.acquire(iq, w.fmt, want_peaks=True, **kw)
+            assert res == ref[i][0], (rep, i)
+            np.testing.assert_array_equal(peaks, ref[i][2])
+            ctx.dft(np.ones(64, np.complex64))   # unrelated eager work between replays
+    assert ctx.graph_replays() >= r0 + 5
+    say("case graph replay, device IQ")
+    dev = DeviceArray(iqs[2])
+    r1 = ctx.graph_replays()
+    for _ in range(4):
+        req, hold = ctx.make_request(None, w.fmt, n_samples=len(iqs[2]) // 2, iq_device_ptr=dev.ptr.value, **kw)
+        res, _, _ = ctx.acquire_req(req)
+        assert res == ref[2][0]
+    assert ctx.graph_replays() >= r1 + 2
+    dev.close()
+    # a different shape in between drops the graph; results stay right
+    res, _, _ = ctx.acquire(iqs[0], w.fmt, **{**kw, "doppler": w.doppler[:5]})
+    res2, _, _ = ctx.acquire(iqs[1], w.fmt, want_peaks=True, **kw)
+    assert res2 == ref[1][0]
+    for k in keys:
+        ctx.replica_drop(k)

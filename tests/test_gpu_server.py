@@ -27,9 +27,10 @@ def assert_same(res, ref):
             assert a[k] == pytest.approx(b[k], rel=1e-5, abs=1e-12), (k, a, b)
 
 
-def test_server_results_equal_solo(ctx):
-    print("case server isolation: 24 mixed requests", flush=True)
-    mix = W.ServeMix(ctx, seed=0xC5, pool=2)
+@pytest.mark.parametrize("boc_precision", [U.PREC_FP32, U.PREC_FP16])
+def test_server_results_equal_solo(ctx, boc_precision):
+    print(f"case server isolation: 24 mixed requests, BOC precision {boc_precision}", flush=True)
+    mix = W.ServeMix(ctx, seed=0xC5, pool=2, boc_precision=boc_precision)
     reqs = mix.stream(24, 2000.0)
     solo = []
     for r in reqs:

@@ -91,6 +91,8 @@ SIGNATURES = {
     "ugsdr_hip_autotune": ([VP, I64, C.POINTER(I32), VP, I32], C.c_int),
     "ugsdr_hip_set_autotune": ([VP, C.c_int], C.c_int),
     "ugsdr_hip_corr_variants": ([I64, VP, I32], C.c_int),
+    "ugsdr_hip_set_graphs": ([VP, C.c_int], C.c_int),
+    "ugsdr_hip_graph_replays": ([VP], I64),
     "ugsdr_hip_set_threads": ([VP, I64, I32], C.c_int),
     "ugsdr_code_gps_l1ca": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5i": ([C.c_int, VP], C.c_int),
@@ -264,6 +266,13 @@ class Context:
 
     def set_autotune(self, enable: bool):
         _check(lib().ugsdr_hip_set_autotune(self._p, int(enable)))
+
+    def set_graphs(self, enable: bool):
+        """hipGraph replay of repeated request shapes (on by default)"""
+        _check(lib().ugsdr_hip_set_graphs(self._p, int(enable)))
+
+    def graph_replays(self) -> int:
+        return int(lib().ugsdr_hip_graph_replays(self._p))
 
     def set_threads(self, n_fft: int, threads: int):
         """pin the correlation variant for a length (0 unpins)"""

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "ctx.hpp"
@@ -156,43 +157,55 @@ __global__ __launch_bounds__(T, waves_for(CAP)) void fwd_small(FwdArgs a) {
 }
 
 // four-step: X[k1 + N1 k2] = sum_n2 W_N2^{n2 k2} W_N^{n2 k1} sum_n1 x[N2 n1 + n2] W_N1^{n1 k1}
-template <int T, bool INV, class P1, class P2, class Src, class Sink>
+// Sequences sit in LDS at an odd stride (conflict-free strided access of
+// 8-byte elements); batches are balanced over their count; every global
+// loop keeps several loads in flight per thread.  sink_pre(k) is read for
+// output k before sink(k, value, pre) is called (so a read-modify-write of
+// the output batches its reads).
+template <int T, bool INV, class P1, class P2, class Src, class SinkPre, class Sink>
 __device__ void fourstep(float2* lds, const FourStep& f, const float2* lo1, const float2* hi1, const float2* lo2,
-                         const float2* hi2, const float2* loN, const float2* hiN, float2* G, Src src, Sink sink,
-                         int tid0) {
+                         const float2* hi2, const float2* loN, const float2* hiN, float2* G, Src src,
+                         SinkPre sink_pre, Sink sink, int tid0) {
     const int N1 = P1::kCompileTime ? P1::N : f.n1;
     const int N2 = P2::kCompileTime ? P2::N : f.n2;
-    const int cb = max(1, kLdsCap / N1);
+    const int ld1 = N1 | 1, ld2 = N2 | 1;
+    const int nbat1 = (N2 + max(1, kLdsCap / ld1) - 1) / max(1, kLdsCap / ld1);
+    const int cb = (N2 + nbat1 - 1) / nbat1;
     for (int c0 = 0; c0 < N2; c0 += cb) {
         const int tid = opaque(tid0);
         const int nb = min(cb, N2 - c0);
-        for (int idx = tid; idx < nb * N1; idx += T) {
-            const int t = idx % nb, n1 = idx / nb;
-            lds[t * N1 + n1] = src((int64_t)N2 * n1 + c0 + t);
-        }
+        walk_cols<8>(
+            tid, T, nb, N1, [&](int t, int n1) { return src((int64_t)N2 * n1 + c0 + t); },
+            [&](int t, int n1, float2 v) { lds[t * ld1 + n1] = v; });
         __syncthreads();
-        Fft<T, kLdsCap, INV, P1>::run(lds, f.p1, nb, lo1, hi1, tid);
-        for (int idx = tid; idx < nb * N1; idx += T) {
-            const int t = idx % nb, k1 = idx / nb;
-            const int n2 = c0 + t;
-            G[(int64_t)k1 * N2 + n2] = cmul(lds[t * N1 + k1], tw_get<INV>(loN, hiN, n2 * k1));
-        }
+        Fft<T, kLdsCap, INV, P1>::run(lds, f.p1, nb, lo1, hi1, tid, ld1);
+        walk_cols<4>(
+            tid, T, nb, N1, [&](int t, int k1) { return lds[t * ld1 + k1]; },
+            [&](int t, int k1, float2 v) {
+                const int n2 = c0 + t;
+                G[(int64_t)k1 * N2 + n2] = cmul(v, tw_get<INV>(loN, hiN, n2 * k1));
+            });
         __syncthreads();
     }
-    const int rb = max(1, kLdsCap / N2);
+    const int nbat2 = (N1 + max(1, kLdsCap / ld2) - 1) / max(1, kLdsCap / ld2);
+    const int rb = (N1 + nbat2 - 1) / nbat2;
     for (int r0 = 0; r0 < N1; r0 += rb) {
         const int tid = opaque(tid0);
         const int nb = min(rb, N1 - r0);
-        for (int idx = tid; idx < nb * N2; idx += T) {
-            const int t = idx / N2, n2 = idx - t * N2;
-            lds[idx] = G[(int64_t)(r0 + t) * N2 + n2];
-        }
+        walk_cols<8>(
+            tid, T, N2, nb, [&](int n2, int t) { return G[(int64_t)(r0 + t) * N2 + n2]; },
+            [&](int n2, int t, float2 v) { lds[t * ld2 + n2] = v; });
         __syncthreads();
-        Fft<T, kLdsCap, INV, P2>::run(lds, f.p2, nb, lo2, hi2, tid);
-        for (int idx = tid; idx < nb * N2; idx += T) {
-            const int t = idx % nb, k2 = idx / nb;
-            sink((int64_t)(r0 + t) + (int64_t)N1 * k2, lds[t * N2 + k2]);
-        }
+        Fft<T, kLdsCap, INV, P2>::run(lds, f.p2, nb, lo2, hi2, tid, ld2);
+        walk_cols<8>(
+            tid, T, nb, N2,
+            [&](int t, int k2) {
+                const int64_t k = (int64_t)(r0 + t) + (int64_t)N1 * k2;
+                return make_float3(lds[t * ld2 + k2].x, lds[t * ld2 + k2].y, sink_pre(k));
+            },
+            [&](int t, int k2, float3 v) {
+                sink((int64_t)(r0 + t) + (int64_t)N1 * k2, make_float2(v.x, v.y), v.z);
+            });
         __syncthreads();
     }
 }
@@ -231,8 +244,8 @@ __global__ __launch_bounds__(T, 2) void fwd_large(FwdArgs a) {
     const bool conj_out = a.mode != 0;
     fourstep<T, false, P1, P2>(
         L.buf, a.fs, L.lo1, L.hi1, L.lo2, L.hi2, L.loN, L.hiN, G,
-        [&](int64_t n) { return fwd_source(a, blk, n); },
-        [&](int64_t k, float2 v) {
+        [&](int64_t n) { return fwd_source(a, blk, n); }, [](int64_t) { return 0.f; },
+        [&](int64_t k, float2 v, float) {
             if (conj_out) v.y = -v.y;
             o[k] = v;
         },
@@ -352,10 +365,11 @@ __global__ __launch_bounds__(T, (T / 256 > 2 ? T / 256 : 2)) void corr_large(Cor
                     src -= src >= N ? N : 0;
                     return cmul(Xk[src], C[n]);
                 },
-                [&](int64_t kk, float2 v) {
+                [&](int64_t kk) { return R == 1 && !first ? S[kk] : 0.f; },
+                [&](int64_t kk, float2 v, float pre) {
                     // unfolded surface: |.| straight into the block sum
                     // (each output index is written by one thread)
-                    if (R == 1) S[kk] = first ? cabs_hw(v) : S[kk] + cabs_hw(v);
+                    if (R == 1) S[kk] = pre + cabs_hw(v);
                     else if (fm == 0) A[kk] = cabs_hw(v);
                     else B[kk] = v;
                 },
@@ -636,6 +650,7 @@ int64_t AcqPlan::footprint(bool surfaces) const {
 }
 
 int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
+    pl.precision = r ? r->precision : UGSDR_PREC_FP32;
     UGSDR_CHECK_ARG(r, "null request");
     UGSDR_CHECK_ARG(r->iq && r->n_samples > 0, "empty signal");
     UGSDR_CHECK_ARG(r->fmt >= UGSDR_FMT_CI8 && r->fmt <= UGSDR_FMT_R8, "unknown sample format");
@@ -674,6 +689,7 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
             return UGSDR_ERR_ARG;
         }
         pl.specs[p] = static_cast<const float2*>(it->second.spec);
+        pl.spec_scale = it->second.scale;
     }
     if (!plan_transform((int)pl.N, pl.small, pl.plan, pl.fs)) {
         set_error("unsupported transform length " + std::to_string(pl.N));
@@ -720,7 +736,9 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
     return UGSDR_OK;
 }
 
-int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st, bool keep_surf) {
+int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st, bool keep_surf,
+                  FrontStep step) {
+    if (step == FrontStep::Launch) return launch_front(pl, r, s, st);
     // tables
     s.host_tables.resize(pl.table_bytes);
     char* hp = s.host_tables.data();
@@ -741,8 +759,15 @@ int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hip
     UGSDR_HIP_TRY(s.sig.reserve((size_t)pl.need * sizeof(float2)));
     UGSDR_HIP_TRY(s.rows.reserve((size_t)pl.P * pl.D * sizeof(RowStat)));
     if (keep_surf) UGSDR_HIP_TRY(s.surf.reserve((size_t)pl.P * pl.D * pl.L * sizeof(float)));
+    (void)iq;
+    if (step == FrontStep::Upload) return UGSDR_OK;
+    return launch_front(pl, r, s, st);
+}
+
+// the front-end kernel alone (its tables and host IQ already uploaded)
+int launch_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st) {
     FrontArgs fa{};
-    fa.iq = iq;
+    fa.iq = r->iq_on_device ? r->iq : s.iq.ptr;
     fa.fmt = pl.fmt;
     fa.n_in = pl.n_in_used;
     fa.n_out = pl.need;
@@ -794,6 +819,7 @@ std::vector<int> thread_candidates(int64_t N) {
 }
 
 size_t corr_scratch_bytes(const AcqPlan& p0, int cus) {
+    if (p0.precision != UGSDR_PREC_FP32) return (size_t)2 * p0.N * 4 * (size_t)cus;
     if (!p0.small) return (2 * (size_t)p0.N + (p0.L + 1) / 2) * sizeof(float2) * (size_t)cus;
     if (corr1023_supports(p0.N, p0.L)) return (size_t)3 * corr1023_grid(p0.N, cus) * p0.L * 4;
     return 0;
@@ -1050,6 +1076,37 @@ int tuned_threads(ugsdr_hip_ctx* ctx, int64_t N) {
     return best;
 }
 
+// Key of a request's device work: everything but the host IQ pointer (host
+// IQ is uploaded on every call, outside any graph).
+static std::string graph_key(const ugsdr_acq_request* r, bool surfaces, bool peaks, int threads) {
+    ugsdr_acq_request c = *r;
+    if (!c.iq_on_device) c.iq = nullptr;
+    c.doppler = nullptr;
+    c.replica_keys = nullptr;
+    c.reserved = 0;
+    std::string k(reinterpret_cast<const char*>(&c), sizeof c);
+    k.append(reinterpret_cast<const char*>(r->doppler), (size_t)r->n_doppler * sizeof(double));
+    k.append(reinterpret_cast<const char*>(r->replica_keys), (size_t)r->n_prn * sizeof(uint64_t));
+    const int32_t tail[3] = {surfaces ? 1 : 0, peaks ? 1 : 0, threads};
+    k.append(reinterpret_cast<const char*>(tail), sizeof tail);
+    return k;
+}
+
+// front end, class FFTs, correlation and selection of one request on st
+static int enqueue_search(ugsdr_hip_ctx* ctx, const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s,
+                          hipStream_t st, bool keep_surf, bool peaks, int threads, FrontStep front) {
+    ctx->mark(0);
+    if (int rc = enqueue_front(pl, r, s, st, keep_surf, front)) return rc;
+    ctx->mark(1);
+    if (int rc = enqueue_classes(pl, s, st)) return rc;
+    ctx->mark(2);
+    if (int rc = enqueue_corr({&pl}, {&s}, st, ctx->scratch, device_cu_count(ctx->device), threads)) return rc;
+    ctx->mark(3);
+    if (int rc = enqueue_final(pl, s, st, peaks)) return rc;
+    ctx->mark(4);
+    return UGSDR_OK;
+}
+
 int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_result* out, float* surfaces,
                  float* peaks) {
     UGSDR_CHECK_ARG(out, "null result");
@@ -1058,19 +1115,51 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
     if (int rc = make_acq_plan(ctx, r, pl)) return rc;
     AcqSlot& s = ctx->slot;
     const bool keep_surf = surfaces || peaks;
-    if (!keep_surf) s.surf.release();
     hipStream_t st = ctx->stream;
-    ctx->mark(0);
-    if (int rc = enqueue_front(pl, r, s, st, keep_surf)) return rc;
-    ctx->mark(1);
-    if (int rc = enqueue_classes(pl, s, st)) return rc;
-    ctx->mark(2);
-    if (int rc = enqueue_corr({&pl}, {&s}, st, ctx->scratch, device_cu_count(ctx->device),
-                              tuned_threads(ctx, pl.N)))
-        return rc;
-    ctx->mark(3);
-    if (int rc = enqueue_final(pl, s, st, peaks != nullptr)) return rc;
-    ctx->mark(4);
+    const int threads = tuned_threads(ctx, pl.N);
+    // A request shape seen twice in a row is captured into a hipGraph on the
+    // second call and replayed afterwards: the kernels and their arguments
+    // (device buffers included) are the same, only the host IQ upload and
+    // the result copies run outside the graph.  Any other call drops the
+    // graph, since it may move the buffers the graph refers to.
+    auto& gc = ctx->graph;
+    const bool graphs = gc.enabled && !ctx->profile;
+    const std::string key = graphs ? graph_key(r, surfaces != nullptr, peaks != nullptr, threads) : std::string();
+    if (graphs && gc.exec && key == gc.key) {
+        if (!r->iq_on_device)
+            UGSDR_HIP_TRY(hipMemcpyAsync(s.iq.ptr, r->iq, pl.n_in_used * pl.bps, hipMemcpyHostToDevice, st));
+        UGSDR_HIP_TRY(hipGraphLaunch(gc.exec, st));
+        ++gc.replays;
+    } else if (graphs && key == gc.pending) {
+        gc.drop();
+        if (int rc = enqueue_front(pl, r, s, st, keep_surf, FrontStep::Upload)) return rc;
+        UGSDR_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_search(ctx, pl, r, s, st, keep_surf, peaks != nullptr, threads, FrontStep::Launch);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        hipGraphExec_t exec = nullptr;
+        if (!rc && ec == hipSuccess && g && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess) {
+            gc.exec = exec;
+            gc.key = key;
+            gc.pending.clear();
+        } else {
+            gc.enabled = false;  // this context runs eagerly from now on
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        if (gc.exec) {
+            UGSDR_HIP_TRY(hipGraphLaunch(gc.exec, st));
+        } else if (int rc2 = enqueue_search(ctx, pl, r, s, st, keep_surf, peaks != nullptr, threads,
+                                            FrontStep::Launch)) {
+            return rc2;
+        }
+    } else {
+        gc.drop();
+        if (!keep_surf) s.surf.release();
+        if (int rc = enqueue_search(ctx, pl, r, s, st, keep_surf, peaks != nullptr, threads, FrontStep::All))
+            return rc;
+        if (graphs) gc.pending = key;
+    }
     UGSDR_HIP_TRY(hipMemcpyAsync(out, s.results.ptr, (size_t)pl.P * sizeof(ugsdr_acq_result), hipMemcpyDeviceToHost, st));
     if (peaks)
         UGSDR_HIP_TRY(hipMemcpyAsync(peaks, s.results.as<char>() + (size_t)pl.P * sizeof(ugsdr_acq_result),
@@ -1140,6 +1229,20 @@ extern "C" int ugsdr_hip_set_threads(ugsdr_hip_ctx* ctx, int64_t n_fft, int32_t 
     ctx->tuned[n_fft] = threads;
     return UGSDR_OK;
 }
+
+extern "C" int ugsdr_hip_set_graphs(ugsdr_hip_ctx* ctx, int enable) {
+    if (!ctx) {
+        ugsdr_hip::set_error("null context");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->graph.drop();
+    ctx->graph.pending.clear();
+    ctx->graph.enabled = enable != 0;
+    return UGSDR_OK;
+}
+
+extern "C" int64_t ugsdr_hip_graph_replays(ugsdr_hip_ctx* ctx) { return ctx ? ctx->graph.replays : -1; }
 
 extern "C" int ugsdr_hip_set_autotune(ugsdr_hip_ctx* ctx, int enable) {
     if (!ctx) {

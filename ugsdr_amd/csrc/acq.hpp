@@ -40,13 +40,21 @@ struct AcqPlan {
     // device bytes the request holds while in flight (admission)
     int64_t footprint(bool surfaces) const;
     // batch key: requests sharing it run in one correlation launch
-    int64_t bucket() const { return (N << 4) | (fold_mode << 1) | (L == N ? 1 : 0); }
+    int64_t bucket() const { return (N << 6) | ((int64_t)precision << 3) | (fold_mode << 1) | (L == N ? 1 : 0); }
+    // reduced precision: the request's storage type and its replicas' scale
+    int precision = UGSDR_PREC_FP32;
+    float spec_scale = 1.0f;
 };
 
 // Validates the request, resolves replica keys (caller holds ctx->mu).
 int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl);
 // Uploads tables and IQ and runs the front end (unpack, mix, resample).
-int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st, bool keep_surf);
+// Upload only queues the copies and sizes the buffers; Launch only runs the
+// kernel (for graph capture, where the copies stay outside the graph).
+enum class FrontStep { All, Upload, Launch };
+int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st, bool keep_surf,
+                  FrontStep step = FrontStep::All);
+int launch_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st);
 // Forward FFT of every block for every fractional Doppler class.
 int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
 // One correlation launch over every request of the batch (same bucket) with
@@ -76,6 +84,13 @@ bool corr1023_supports(int64_t N, int64_t L);
 struct CorrArgs;
 int corr1023_grid(int64_t N, int cus);
 int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st);
+// Reduced precision (16-bit storage, matrix-core stages): whether a length
+// pair has a plan, the 16-bit packing of the class spectra, and one
+// correlation launch over a batch of requests of one bucket.
+bool reduced_supported(int64_t N, int64_t L);
+int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
+int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots,
+                         hipStream_t st, DevBuf& scratch, int cu_count);
 int enqueue_final(const AcqPlan& pl, AcqSlot& s, hipStream_t st, bool peaks);
 int device_cu_count(int device);
 

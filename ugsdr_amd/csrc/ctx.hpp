@@ -100,6 +100,19 @@ struct ugsdr_hip_ctx {
     // scratch
     ugsdr_hip::DevBuf iq, sig, classes, scratch, rows, results, surf, tables, aux, aux2;
     ugsdr_hip::AcqSlot slot;  // buffers of the synchronous acquire path
+    // hipGraph of the last repeated request shape (ugsdr_hip_acquire): the
+    // shape is captured on its second consecutive call and replayed after
+    struct GraphCache {
+        bool enabled = true;
+        std::string key, pending;
+        hipGraphExec_t exec = nullptr;
+        int64_t replays = 0;
+        void drop() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+            key.clear();
+        }
+    } graph;
     // autotuner: transform length -> correlation workgroup size
     std::map<int64_t, int> tuned;
     bool autotune = true;

@@ -343,9 +343,10 @@ constexpr int rounds_for(int R, int T, int CAP) { return (CAP / R + T - 1) / T; 
 // outputs are formed and stored after it.
 template <int R, int T, int CAP, bool INV, bool FIRST, int NC = 0, int NSC = 0>
 __device__ __forceinline__ void fft_stage(float2* buf, int n_rt, int nb, int Ns_rt, const float2* lo,
-                                          const float2* hi, int tid) {
+                                          const float2* hi, int tid, int ld = 0) {
     constexpr int RM = rounds_for(R, T, CAP);
     const int n = NC > 0 ? NC : n_rt;
+    const int lds = ld > 0 ? ld : n;  // sequence stride in buf
     const int Ns = NSC > 0 ? NSC : Ns_rt;
     const int per = n / R;
     const int B = nb * per;
@@ -360,7 +361,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n_rt, int nb, int Ns_
             const int seq = b / per;
             const int j = b - seq * per;
             const int k = FIRST ? 0 : j % Ns;
-            const float2* src = buf + seq * n + j;
+            const float2* src = buf + seq * lds + j;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[rr][r] = src[r * per];
             if (!FIRST && k) {
@@ -369,7 +370,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n_rt, int nb, int Ns_
                 for (int r = 1; r < R; ++r) v[rr][r] = cmul(v[rr][r], tw_get<INV>(lo, hi, r * step));
             }
             Stage<R, INV>::prep(v[rr]);
-            obase[rr] = seq * n + (j - k) * R + k;
+            obase[rr] = seq * lds + (j - k) * R + k;
         }
     }
     __syncthreads();
@@ -390,18 +391,18 @@ struct RadixPlan {
 // after filling buf; the result is in natural order and synchronised.
 template <int T, int CAP, bool INV, bool FIRST>
 __device__ __forceinline__ void fft_stage_any(int R, float2* buf, int n, int nb, int Ns, const float2* lo,
-                                              const float2* hi, int tid) {
+                                              const float2* hi, int tid, int ld) {
     switch (R) {
-        case 2: fft_stage<2, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
-        case 3: fft_stage<3, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
-        case 4: fft_stage<4, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
-        case 5: fft_stage<5, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
-        case 7: fft_stage<7, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
-        case 8: fft_stage<8, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
-        case 11: fft_stage<11, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
-        case 13: fft_stage<13, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
-        case 16: fft_stage<16, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
-        case 31: fft_stage<31, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid); break;
+        case 2: fft_stage<2, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 3: fft_stage<3, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 4: fft_stage<4, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 5: fft_stage<5, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 7: fft_stage<7, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 8: fft_stage<8, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 11: fft_stage<11, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 13: fft_stage<13, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 16: fft_stage<16, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 31: fft_stage<31, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
         default: break;
     }
 }
@@ -410,13 +411,14 @@ __device__ __forceinline__ void fft_stage_any(int R, float2* buf, int n, int nb,
 // caller has synchronised after filling buf; the result is in natural order
 // and synchronised.  The first stage needs no twiddles.
 template <int T, int CAP, bool INV>
-__device__ void fft_lds(float2* buf, const RadixPlan& plan, int nb, const float2* lo, const float2* hi, int tid) {
+__device__ void fft_lds(float2* buf, const RadixPlan& plan, int nb, const float2* lo, const float2* hi, int tid,
+                        int ld = 0) {
     if (plan.nst == 0) return;
-    fft_stage_any<T, CAP, INV, true>(plan.radix[0], buf, plan.n, nb, 1, lo, hi, tid);
+    fft_stage_any<T, CAP, INV, true>(plan.radix[0], buf, plan.n, nb, 1, lo, hi, tid, ld);
     int Ns = plan.radix[0];
     for (int s = 1; s < plan.nst; ++s) {
         const int R = plan.radix[s];
-        fft_stage_any<T, CAP, INV, false>(R, buf, plan.n, nb, Ns, lo, hi, tid);
+        fft_stage_any<T, CAP, INV, false>(R, buf, plan.n, nb, Ns, lo, hi, tid, ld);
         Ns *= R;
     }
 }
@@ -435,9 +437,10 @@ struct PlanRT {
 };
 
 template <int T, int CAP, bool INV, int N, int NS, int R, int... Rest>
-__device__ __forceinline__ void fft_ct_stages(float2* buf, int nb, const float2* lo, const float2* hi, int tid) {
-    fft_stage<R, T, CAP, INV, NS == 1, N, NS>(buf, N, nb, NS, lo, hi, tid);
-    if constexpr (sizeof...(Rest) > 0) fft_ct_stages<T, CAP, INV, N, NS * R, Rest...>(buf, nb, lo, hi, tid);
+__device__ __forceinline__ void fft_ct_stages(float2* buf, int nb, const float2* lo, const float2* hi, int tid,
+                                              int ld) {
+    fft_stage<R, T, CAP, INV, NS == 1, N, NS>(buf, N, nb, NS, lo, hi, tid, ld);
+    if constexpr (sizeof...(Rest) > 0) fft_ct_stages<T, CAP, INV, N, NS * R, Rest...>(buf, nb, lo, hi, tid, ld);
 }
 
 template <int T, int CAP, bool INV, class P>
@@ -446,15 +449,15 @@ struct Fft;
 template <int T, int CAP, bool INV, int... Rs>
 struct Fft<T, CAP, INV, PlanCT<Rs...>> {
     __device__ __forceinline__ static void run(float2* buf, const RadixPlan&, int nb, const float2* lo,
-                                               const float2* hi, int tid) {
-        fft_ct_stages<T, CAP, INV, PlanCT<Rs...>::N, 1, Rs...>(buf, nb, lo, hi, tid);
+                                               const float2* hi, int tid, int ld = 0) {
+        fft_ct_stages<T, CAP, INV, PlanCT<Rs...>::N, 1, Rs...>(buf, nb, lo, hi, tid, ld);
     }
 };
 template <int T, int CAP, bool INV>
 struct Fft<T, CAP, INV, PlanRT> {
     __device__ __forceinline__ static void run(float2* buf, const RadixPlan& plan, int nb, const float2* lo,
-                                               const float2* hi, int tid) {
-        fft_lds<T, CAP, INV>(buf, plan, nb, lo, hi, tid);
+                                               const float2* hi, int tid, int ld = 0) {
+        fft_lds<T, CAP, INV>(buf, plan, nb, lo, hi, tid, ld);
     }
 };
 
@@ -503,6 +506,35 @@ __device__ __forceinline__ void order_after(float& v) { asm volatile("" : "+v"(v
 __device__ __forceinline__ int after(int idx, float& v) {
     asm volatile("" : "+v"(idx), "+v"(v));
     return idx;
+}
+
+// Walks idx = tid, tid + T, ... < nb * rows as (t, r) = (idx % nb, idx / nb)
+// with one division at the start.  U elements per step: every load of a
+// step is issued before the first store, so U loads per thread are in
+// flight instead of one.
+template <int U, class Load, class Store>
+__device__ __forceinline__ void walk_cols(int tid, int T, int nb, int rows, Load load, Store store) {
+    int t = tid % nb, r = tid / nb;
+    const int dt = T % nb, dr = T / nb;
+    while (r < rows) {
+        decltype(load(0, 0)) v[U];
+        int ts[U], rs[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ts[u] = t;
+            rs[u] = r;
+            if (r < rows) v[u] = load(t, r);
+            t += dt;
+            r += dr;
+            if (t >= nb) {
+                t -= nb;
+                ++r;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (rs[u] < rows) store(ts[u], rs[u], v[u]);
+    }
 }
 
 // ------------------------------------------------------------ reductions

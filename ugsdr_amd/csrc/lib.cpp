@@ -92,6 +92,7 @@ int ugsdr_hip_ctx_destroy(ugsdr_hip_ctx* ctx) {
         for (DevBuf* d : {&ctx->iq, &ctx->sig, &ctx->classes, &ctx->scratch, &ctx->rows, &ctx->results, &ctx->surf,
                           &ctx->tables, &ctx->aux, &ctx->aux2})
             d->release();
+        ctx->graph.drop();
         ctx->slot.release_all();
         (void)hipStreamDestroy(ctx->stream);
     }

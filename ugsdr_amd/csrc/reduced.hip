@@ -190,40 +190,58 @@ __global__ __launch_bounds__(512, 2) void corr_mfma(CorrArgs a) {
             const float sc = q.block_inv[k * q.F + cls] * q.spec_inv;
             for (int c0 = 0; c0 < N2; c0 += cb) {
                 const int nb = min(cb, N2 - c0);
-                for (int idx = t0; idx < nb * N1; idx += T) {
-                    const int t = idx % nb, n1 = idx / nb;
-                    const int n = c0 + t + N2 * n1;
-                    int src = n + sh;
-                    src -= src >= N ? N : 0;
-                    bufA[t * ld1 + n1] = pack16<H>(cmul(unpack16<H>(X[src]), unpack16<H>(C[n])));
-                }
+                walk_cols<8>(
+                    t0, T, nb, N1,
+                    [&](int t, int n1) {
+                        const int n = c0 + t + N2 * n1;
+                        int src = n + sh;
+                        src -= src >= N ? N : 0;
+                        return make_uint2(X[src], C[n]);
+                    },
+                    [&](int t, int n1, uint2 v) {
+                        bufA[t * ld1 + n1] = pack16<H>(cmul(unpack16<H>(v.x), unpack16<H>(v.y)));
+                    });
                 __syncthreads();
                 mfma_stage<R1a, H, true, NW>(bufA, ld1, bufB, N1, N1, nb, nullptr, bt1a, lane, wave);
                 __syncthreads();
                 mfma_stage<R1b, H, false, NW>(bufB, N1, bufA, ld1, N1, nb, tw1, bt1b, lane, wave);
                 __syncthreads();
-                for (int idx = t0; idx < nb * N1; idx += T) {
-                    const int t = idx % nb, k1 = idx / nb;
-                    const int n2 = c0 + t;
-                    G[k1 * N2 + n2] =
-                        pack16<H>(cmul(unpack16<H>(bufA[t * ld1 + k1]), tw_get<true>(twlo, twhi, n2 * k1)));
-                }
+                walk_cols<4>(
+                    t0, T, nb, N1, [&](int t, int k1) { return bufA[t * ld1 + k1]; },
+                    [&](int t, int k1, uint32_t v) {
+                        const int n2 = c0 + t;
+                        G[k1 * N2 + n2] = pack16<H>(cmul(unpack16<H>(v), tw_get<true>(twlo, twhi, n2 * k1)));
+                    });
                 __syncthreads();
             }
             for (int r0 = 0; r0 < N1; r0 += rb) {
                 const int nb = min(rb, N1 - r0);
-                for (int idx = t0; idx < nb * N2; idx += T) bufA[idx] = G[r0 * N2 + idx];
+                {
+                    const gptr<uint32_t> g = as_global(G + r0 * N2);
+                    const int tot = nb * N2;
+                    for (int base = t0; base < tot; base += 8 * T) {
+                        uint32_t v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (base + u * T < tot) v[u] = g[base + u * T];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (base + u * T < tot) bufA[base + u * T] = v[u];
+                    }
+                }
                 __syncthreads();
                 mfma_stage<11, H, true, NW>(bufA, N2, bufB, N2, N2, nb, nullptr, bt2a, lane, wave);
                 __syncthreads();
                 mfma_stage<31, H, false, NW>(bufB, N2, bufA, N2, N2, nb, tw2, bt2b, lane, wave);
                 __syncthreads();
-                for (int idx = t0; idx < nb * N2; idx += T) {
-                    const int t = idx % nb, k2 = idx / nb;
-                    const int m = r0 + t + N1 * k2;
-                    const float mag = cabs_hw(unpack16<H>(bufA[t * N2 + k2])) * sc;
-                    S[m] = k == 0 ? mag : S[m] + mag;
-                }
+                const bool first = k == 0;
+                walk_cols<8>(
+                    t0, T, nb, N2,
+                    [&](int t, int k2) {
+                        const float mag = cabs_hw(unpack16<H>(bufA[t * N2 + k2])) * sc;
+                        return first ? mag : mag + S[r0 + t + N1 * k2];
+                    },
+                    [&](int t, int k2, float v) { S[r0 + t + N1 * k2] = v; });
                 __syncthreads();
             }
         }
@@ -268,7 +286,7 @@ static int launch_corr_mfma_n(const CorrArgs& c, int grid, hipStream_t st) {
     }
 }
 
-static int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st, int precision) {
+int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     const int nblk = (int)(pl.K * pl.F);
     UGSDR_HIP_TRY(s.classes_r.reserve((size_t)nblk * pl.N * 4));
     UGSDR_HIP_TRY(s.block_scale.reserve((size_t)nblk * 8));
@@ -277,7 +295,7 @@ static int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st,
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, st, bmax, nblk);
     const dim3 grid((unsigned)std::min<int64_t>((pl.N + 1023) / 1024, 64), (unsigned)nblk);
     hipLaunchKernelGGL(block_absmax_kernel, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.N, bmax);
-    if (precision == UGSDR_PREC_FP16)
+    if (pl.precision == UGSDR_PREC_FP16)
         hipLaunchKernelGGL(block_pack_kernel<Fp16Ops>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.N, bmax,
                            s.classes_r.as<uint32_t>(), inv);
     else
@@ -287,36 +305,50 @@ static int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st,
     return UGSDR_OK;
 }
 
-static int enqueue_corr_reduced(const AcqPlan& pl, AcqSlot& s, hipStream_t st, DevBuf& scratch, int cus,
-                                int precision, float spec_scale) {
-    char* dp = s.tables.as<char>();
-    ReqDesc q{};
-    q.X = reinterpret_cast<const float2*>(s.classes_r.ptr);
-    q.specs = reinterpret_cast<const float2* const*>(dp + pl.off_specs);
-    q.cls = reinterpret_cast<const int*>(dp + pl.off_cls);
-    q.shift = reinterpret_cast<const int*>(dp + pl.off_shift);
-    q.rowstats = s.rows.as<RowStat>();
-    q.surfaces = s.surf.ptr ? s.surf.as<float>() : nullptr;
-    q.K = (int)pl.K;
-    q.F = pl.F;
-    q.D = (int)pl.D;
-    q.P = (int)pl.P;
-    q.row_start = 0;
-    q.block_inv = reinterpret_cast<const float*>(s.block_scale.as<unsigned>() + pl.K * pl.F);
-    q.spec_inv = 1.0f / spec_scale;
+int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots,
+                         hipStream_t st, DevBuf& scratch, int cus) {
+    if (plans.empty() || plans.size() > (size_t)kMaxBatch) {
+        set_error("a correlation batch holds 1 to " + std::to_string(kMaxBatch) + " requests");
+        return UGSDR_ERR_ARG;
+    }
+    const AcqPlan& p0 = *plans[0];
     CorrArgs c{};
-    c.reqs[0] = q;
-    c.N = (int)pl.N;
-    c.L = (int)pl.L;
-    c.fold_mode = pl.fold_mode;
-    c.rows = (int)(pl.P * pl.D);
-    c.n_req = 1;
+    int rows = 0;
+    for (size_t i = 0; i < plans.size(); ++i) {
+        const AcqPlan& pl = *plans[i];
+        AcqSlot& s = *slots[i];
+        if (pl.bucket() != p0.bucket() || pl.L != p0.L || pl.precision != p0.precision) {
+            set_error("batch mixes transform shapes");
+            return UGSDR_ERR_ARG;
+        }
+        char* dp = s.tables.as<char>();
+        ReqDesc& q = c.reqs[i];
+        q.X = reinterpret_cast<const float2*>(s.classes_r.ptr);
+        q.specs = reinterpret_cast<const float2* const*>(dp + pl.off_specs);
+        q.cls = reinterpret_cast<const int*>(dp + pl.off_cls);
+        q.shift = reinterpret_cast<const int*>(dp + pl.off_shift);
+        q.rowstats = s.rows.as<RowStat>();
+        q.surfaces = s.surf.ptr ? s.surf.as<float>() : nullptr;
+        q.K = (int)pl.K;
+        q.F = pl.F;
+        q.D = (int)pl.D;
+        q.P = (int)pl.P;
+        q.row_start = rows;
+        q.block_inv = reinterpret_cast<const float*>(s.block_scale.as<unsigned>() + pl.K * pl.F);
+        q.spec_inv = 1.0f / pl.spec_scale;
+        rows += (int)(pl.P * pl.D);
+    }
+    c.N = (int)p0.N;
+    c.L = (int)p0.L;
+    c.fold_mode = p0.fold_mode;
+    c.rows = rows;
+    c.n_req = (int)plans.size();
     c.inv_n = 1.0f;
     const int grid = std::min(c.rows, cus);
-    UGSDR_HIP_TRY(scratch.reserve((size_t)grid * 2 * pl.N * 4));
+    UGSDR_HIP_TRY(scratch.reserve((size_t)grid * 2 * p0.N * 4));
     c.scratch = scratch.as<float2>();
-    int rc = precision == UGSDR_PREC_FP16 ? launch_corr_mfma_n<Fp16Ops>(c, grid, st)
-                                          : launch_corr_mfma_n<Bf16Ops>(c, grid, st);
+    int rc = p0.precision == UGSDR_PREC_FP16 ? launch_corr_mfma_n<Fp16Ops>(c, grid, st)
+                                             : launch_corr_mfma_n<Bf16Ops>(c, grid, st);
     if (rc) return rc;
     UGSDR_HIP_TRY(hipGetLastError());
     return UGSDR_OK;
@@ -338,7 +370,8 @@ int acquire_reduced(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_re
                   std::to_string(pl.L));
         return UGSDR_ERR_UNSUPPORTED;
     }
-    const float spec_scale = ctx->replicas.at(r->replica_keys[0]).scale;
+    ctx->graph.drop();   // this call may move buffers a cached graph refers to
+    ctx->graph.pending.clear();
     AcqSlot& s = ctx->slot;
     const bool keep_surf = surfaces || peaks;
     if (!keep_surf) s.surf.release();
@@ -347,11 +380,9 @@ int acquire_reduced(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_re
     if (int rc = enqueue_front(pl, r, s, st, keep_surf)) return rc;
     ctx->mark(1);
     if (int rc = enqueue_classes(pl, s, st)) return rc;
-    if (int rc = enqueue_reduce_classes(pl, s, st, r->precision)) return rc;
+    if (int rc = enqueue_reduce_classes(pl, s, st)) return rc;
     ctx->mark(2);
-    if (int rc = enqueue_corr_reduced(pl, s, st, ctx->scratch, device_cu_count(ctx->device), r->precision,
-                                      spec_scale))
-        return rc;
+    if (int rc = enqueue_corr_reduced({&pl}, {&s}, st, ctx->scratch, device_cu_count(ctx->device))) return rc;
     ctx->mark(3);
     if (int rc = enqueue_final(pl, s, st, peaks != nullptr)) return rc;
     ctx->mark(4);

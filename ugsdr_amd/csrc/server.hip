@@ -135,6 +135,8 @@ static int launch_round(Server& s, Round& rd) {
         }
         if (int rc = enqueue_front(j.plan, &r, sl.dev, st, false)) return rc;
         if (int rc = enqueue_classes(j.plan, sl.dev, st)) return rc;
+        if (j.plan.precision != UGSDR_PREC_FP32)
+            if (int rc = enqueue_reduce_classes(j.plan, sl.dev, st)) return rc;
         plans.push_back(&j.plan);
         devs.push_back(&sl.dev);
     }
@@ -145,7 +147,10 @@ static int launch_round(Server& s, Round& rd) {
         UGSDR_HIP_TRY(hipStreamSynchronize(st));
         UGSDR_HIP_TRY(s.scratch[rd.stream].reserve(need_scratch));
     }
-    if (int rc = enqueue_corr(plans, devs, st, s.scratch[rd.stream], s.cus, p0.threads)) return rc;
+    if (int rc = p0.precision == UGSDR_PREC_FP32
+                     ? enqueue_corr(plans, devs, st, s.scratch[rd.stream], s.cus, p0.threads)
+                     : enqueue_corr_reduced(plans, devs, st, s.scratch[rd.stream], s.cus))
+        return rc;
     for (auto& jp : rd.jobs) {
         Job& j = *jp;
         ServerSlot& sl = s.slots[j.slot];
@@ -326,11 +331,12 @@ int ugsdr_hip_server_submit(ugsdr_hip_server* h, const ugsdr_acq_request* req, u
     j->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s.ctx->mu);
-        if (req->precision != UGSDR_PREC_FP32) {
-            set_error("the server runs the FP32 plain form");
+        if (int rc = make_acq_plan(s.ctx, req, j->plan)) return rc;
+        if (req->precision != UGSDR_PREC_FP32 && !reduced_supported(j->plan.N, j->plan.L)) {
+            set_error("no reduced-precision plan for n_fft " + std::to_string(j->plan.N) + ", surface_len " +
+                      std::to_string(j->plan.L));
             return UGSDR_ERR_UNSUPPORTED;
         }
-        if (int rc = make_acq_plan(s.ctx, req, j->plan)) return rc;
         j->plan.threads = tuned_threads(s.ctx, j->plan.N);
     }
     j->req = *req;

@@ -270,6 +270,7 @@ class ServeRequest:
     doppler: np.ndarray
     keys: list
     arrival_s: float = 0.0
+    precision: int = 0       # PREC_FP32; BOC requests may run the reduced form
 
     @property
     def cells(self) -> int:
@@ -284,9 +285,11 @@ class ServeMix:
     each with 1-16 PRNs and a Doppler span of +-2..+-6 kHz, Doppler step
     1/(2 T_coh).  Signals come from a small pool of pre-synthesised records
     (4 planted satellites each, C/N0 U[44,48]); replica spectra are put in the
-    context's cache once per (code, length)."""
+    context's cache once per (code, length).  boc_precision selects the
+    storage of the BOC requests (FP32 plain, or FP16/BF16: the reduced form
+    with matrix-core DFT stages)."""
 
-    def __init__(self, ctx, seed: int = 0xC5, pool: int = 4):
+    def __init__(self, ctx, seed: int = 0xC5, pool: int = 4, boc_precision: int = 0):
         from . import gps_l1ca, boc
         self.ctx = ctx
         self.rng = np.random.default_rng(seed)
@@ -295,7 +298,8 @@ class ServeMix:
         self.boc_codes = [boc(random_code(seed * 37 + i, 4092)) for i in range(36)]
         self.keys = {"L1": [ctx.replica_put(c, 1, 4092) for c in self.l1_codes],
                      "L5": [ctx.replica_put(c, 1, 20460) for c in self.l5_codes],
-                     "BOC": [ctx.replica_put(c, 1, 81840) for c in self.boc_codes]}
+                     "BOC": [ctx.replica_put(c, 1, 81840, boc_precision) for c in self.boc_codes]}
+        self.boc_precision = boc_precision
         self.pool = {"L1": [], "L5": [], "BOC": []}
         for i in range(pool):
             r = np.random.default_rng(seed * 101 + i)
@@ -323,7 +327,8 @@ class ServeMix:
                                 doppler_bins(span, 500.0), [self.keys["L5"][p] for p in prns])
         prns = self.rng.choice(36, n_prn, replace=False)
         return ServeRequest("BOC", sig("BOC"), FMT_CI16, 20.46e6, 81840, 1, 81840, False,
-                            doppler_bins(span, 125.0), [self.keys["BOC"][p] for p in prns])
+                            doppler_bins(span, 125.0), [self.keys["BOC"][p] for p in prns],
+                            precision=self.boc_precision)
 
     def pool_pick(self, kind):
         return self.pool[kind][int(self.rng.integers(0, len(self.pool[kind])))]
@@ -341,4 +346,5 @@ class ServeMix:
     def request(self, r: ServeRequest):
         """C request struct + the arrays it points to (keep both alive)"""
         return self.ctx.make_request(r.iq, r.fmt, fs_in=r.fs, fs_acq=r.fs, n_fft=r.n_fft, n_blocks=r.n_blocks,
-                                     surface_len=r.surface_len, doppler=r.doppler, keys=r.keys, reshape=r.reshape)
+                                     surface_len=r.surface_len, doppler=r.doppler, keys=r.keys, reshape=r.reshape,
+                                     precision=r.precision)
