@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=2000, help="serving leg: requests (0 skips)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving leg: mean Poisson arrival rate")
     ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"],
+                    help="storage of the measured search (fp16/bf16: the reduced form)")
+    ap.add_argument("--threads", type=int, default=0,
+                    help="pin the correlation variant (workgroup size) instead of autotuning")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (rocprofv3 counter calibration)")
     return ap.parse_args()
@@ -124,19 +128,22 @@ def time_search(ctx, w, steps: int, warmup: int, precision: int = 0):
     """seconds per search of workload w with its IQ resident in HBM"""
     import torch
 
+    import ugsdr_amd as U
+
     keys = [ctx.replica_put(c, w.repeats, w.n_fft, precision) for c in w.codes]
     iq_dev = torch.from_numpy(w.iq).to("cuda:0")
     req, hold = ctx.make_request(None, w.fmt, fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=w.n_blocks,
                                  surface_len=w.surface_len, doppler=w.doppler, keys=keys, reshape=w.reshape,
                                  n_samples=w.n_samples, iq_device_ptr=iq_dev.data_ptr(), precision=precision)
     for _ in range(warmup):
-        ctx.acquire_req(req)
+        ctx.acquire_req(req, raw=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        res, _, _ = ctx.acquire_req(req)
+        res, _, _ = ctx.acquire_req(req, raw=True)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    res = U.results_to_dicts(res)
     ctx.set_profiling(True)
     ctx.acquire_req(req)
     stages = ctx.last_times()
@@ -274,27 +281,35 @@ def main():
     log(f"workload {args.workload}")
     w = W.CONFIGS[args.workload]()
     ctx = U.Context(0)
-    keys = [ctx.replica_put(c, w.repeats, w.n_fft) for c in w.codes]
+    prec = {"fp32": U.PREC_FP32, "fp16": U.PREC_FP16, "bf16": U.PREC_BF16}[args.precision]
+    keys = [ctx.replica_put(c, w.repeats, w.n_fft, prec) for c in w.codes]
     iq_dev = torch.from_numpy(w.iq).to("cuda:0")
     torch.cuda.synchronize()
     variants = U.corr_variants(w.n_fft)
-    chosen, tune_ms = ctx.autotune(w.n_fft)
+    if args.threads:
+        ctx.set_threads(w.n_fft, args.threads)
+        chosen, tune_ms = args.threads, []
+    else:
+        chosen, tune_ms = ctx.autotune(w.n_fft) if prec == U.PREC_FP32 else (0, [])
     log(f"autotune N={w.n_fft}: variants {variants} ms {tune_ms} -> {chosen}")
     req, hold = ctx.make_request(None, w.fmt, fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=w.n_blocks,
                                  surface_len=w.surface_len, doppler=w.doppler, keys=keys, reshape=w.reshape,
-                                 n_samples=w.n_samples, iq_device_ptr=iq_dev.data_ptr())
+                                 n_samples=w.n_samples, iq_device_ptr=iq_dev.data_ptr(), precision=prec)
     for _ in range(args.warmup):
-        res, _, _ = ctx.acquire_req(req)
+        res, _, _ = ctx.acquire_req(req, raw=True)
     torch.cuda.synchronize()
     log(f"timing {args.steps} steps")
     lat = []
+    replays0 = ctx.graph_replays()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s = time.perf_counter()
-        res, _, _ = ctx.acquire_req(req)
+        res, _, _ = ctx.acquire_req(req, raw=True)
         lat.append(time.perf_counter() - s)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    replays = ctx.graph_replays() - replays0
+    res = U.results_to_dicts(res)
     ms_per_step = 1e3 * elapsed / args.steps
     value = w.cells / (elapsed / args.steps)
 
@@ -307,7 +322,7 @@ def main():
             stage[k].append(v)
     ctx.set_profiling(False)
     corr_ms = statistics.mean(stage["corr"])
-    alg_bytes = B_CB * w.cells * w.n_blocks
+    alg_bytes = (B_CB if prec == U.PREC_FP32 else 4) * w.cells * w.n_blocks
     achieved = alg_bytes / (corr_ms * 1e-3) / 1e9
     traffic = traffic_from_profiles(args.workload)
 
@@ -322,7 +337,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": args.precision,
         "data": "synthetic",
         "config": {
             "workload": {"L1CA_WIDE": "GPS L1 C/A wide search: int16 IQ @ 16.368 MHz, 1 ms x 10 non-coherent, "
@@ -348,11 +363,13 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": corr_kernel_name(w.n_fft, w.surface_len, chosen),
+            "kernel": corr_kernel_name(w.n_fft, w.surface_len, chosen) if prec == U.PREC_FP32
+            else f"corr_mfma<{args.precision},{chosen or 1024}> (N={w.n_fft})",
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "detected_prns": [i for i, r in enumerate(res) if r["detected"]],
         "autotune": {"variants": variants, "ms_per_launch": tune_ms, "chosen": chosen},
+        "graph_replays_in_timed_steps": replays,
     }
     log(f"headline {value:.4g} cells/s, {ms_per_step:.3f} ms/search, corr kernel {corr_ms:.3f} ms")
     if not args.no_extra:

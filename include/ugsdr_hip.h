@@ -250,7 +250,8 @@ typedef struct {
  * src/tracking/tracking_parameters.hpp:163-165; FP64 NCO advance in nco mode).
  * ref_split replaces Tracker::GetEpl (src/tracking/tracker.hpp:119-139) with
  * TrackingParameters::CorrelateSplit (tracking_parameters.hpp:471-493).
- * out: n_blocks x n_channels x n_taps complex float (host buffer). */
+ * out: n_blocks x n_channels x n_taps complex float (host buffer).  At most
+ * 65535 blocks per call (UGSDR_ERR_SIZE beyond). */
 int ugsdr_hip_correlate_bank(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* req, ugsdr_channel_state* ch,
                              float* out_c32);
 

@@ -25,7 +25,6 @@ class UgsdrError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
         self.code = code
-        self.code = code
 
 
 class AcqRequest(C.Structure):
@@ -195,6 +194,11 @@ def resample_len(n: int, new_fs: int, old_fs: int) -> int:
     return lib().ugsdr_resample_len(n, new_fs, old_fs)
 
 
+def results_to_dicts(out) -> list:
+    """ugsdr_acq_result array -> one dict per PRN"""
+    return [{f: getattr(r, f) for f in RESULT_FIELDS} for r in out]
+
+
 def corr_variants(n_fft: int) -> list:
     """workgroup sizes naming the correlation variants built for a length"""
     out = np.zeros(8, np.int32)
@@ -346,7 +350,8 @@ class Context:
         req.precision = precision
         return req, (hold, dop, keys)
 
-    def acquire_req(self, req, want_surfaces: bool = False, want_peaks: bool = False):
+    def acquire_req(self, req, want_surfaces: bool = False, want_peaks: bool = False, raw: bool = False):
+        """results as dicts per PRN (raw: the ugsdr_acq_result ctypes array, no conversion)"""
         P, D, L = req.n_prn, req.n_doppler, req.surface_len
         out = (AcqResult * P)()
         surf = np.zeros((P, D, L), np.float32) if want_surfaces else None
@@ -354,7 +359,7 @@ class Context:
         _check(lib().ugsdr_hip_acquire(self._p, C.byref(req), out,
                                        _ptr(surf) if surf is not None else None,
                                        _ptr(peaks) if peaks is not None else None))
-        res = [{f: getattr(out[i], f) for f in RESULT_FIELDS} for i in range(P)]
+        res = out if raw else results_to_dicts(out)
         return res, surf, peaks
 
     def acquire(self, iq, fmt: int, want_surfaces: bool = False, want_peaks: bool = False, **kw):

@@ -819,7 +819,7 @@ std::vector<int> thread_candidates(int64_t N) {
 }
 
 size_t corr_scratch_bytes(const AcqPlan& p0, int cus) {
-    if (p0.precision != UGSDR_PREC_FP32) return (size_t)2 * p0.N * 4 * (size_t)cus;
+    if (p0.precision != UGSDR_PREC_FP32) return (size_t)(p0.K + 1) * p0.N * 4 * (size_t)cus;
     if (!p0.small) return (2 * (size_t)p0.N + (p0.L + 1) / 2) * sizeof(float2) * (size_t)cus;
     if (corr1023_supports(p0.N, p0.L)) return (size_t)3 * corr1023_grid(p0.N, cus) * p0.L * 4;
     return 0;

@@ -90,7 +90,7 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
 bool reduced_supported(int64_t N, int64_t L);
 int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
 int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots,
-                         hipStream_t st, DevBuf& scratch, int cu_count);
+                         hipStream_t st, DevBuf& scratch, int cu_count, int threads);
 int enqueue_final(const AcqPlan& pl, AcqSlot& s, hipStream_t st, bool peaks);
 int device_cu_count(int device);
 

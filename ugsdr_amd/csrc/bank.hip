@@ -81,7 +81,7 @@ __device__ __forceinline__ void bank_reduce_store(float2 (&acc)[NT], int n_taps,
 }
 
 __global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
-    const int b = blockIdx.x, c = blockIdx.y;
+    const int c = blockIdx.x, b = blockIdx.y;  // the channels of one block run together (IQ block reused from L2)
     extern __shared__ __align__(16) unsigned char smem[];
     signed char* code = reinterpret_cast<signed char*>(smem);
     float2* red = reinterpret_cast<float2*>(smem + ((a.code_len + 15) / 16) * 16);
@@ -93,7 +93,8 @@ __global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
     theta -= floor(theta);
     double tau = fma(rate, n0 / a.fs, tau0);
     tau = fmod(tau, L);
-    const double df = f / a.fs, dr = rate / a.fs;
+    const double df = f / a.fs, dr = rate / a.fs, invL = 1.0 / L;
+    const int Li = (int)a.code_len;
     float2 acc[kMaxTaps];
 #pragma unroll
     for (int t = 0; t < kMaxTaps; ++t) acc[t] = make_float2(0.f, 0.f);
@@ -103,19 +104,23 @@ __global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
                                                : (const void*)(static_cast<const float2*>(a.iq) + b * a.spm);
     for (int64_t n = threadIdx.x; n < a.spm; n += kBankT) {
         const float2 x = bank_load(blk, a.fmt, n);
+        // carrier phase in cycles (FP64), reduced to [-1/2, 1/2]; the hardware
+        // sin/cos take revolutions
         double ph = fma((double)n, df, theta);
         ph -= rint(ph);
-        float s, co;
-        sincosf((float)(2.0 * kPi * ph), &s, &co);
+        const float s = __builtin_amdgcn_sinf((float)ph), co = __builtin_amdgcn_cosf((float)ph);
         // x * exp(-j 2 pi ph)
         const float2 w = make_float2(fmaf(x.x, co, x.y * s), fmaf(x.y, co, -x.x * s));
         const double u = fma((double)n, dr, tau);
 #pragma unroll
         for (int t = 0; t < kMaxTaps; ++t) {
             if (t >= a.n_taps) break;
-            int64_t idx = (int64_t)floor(u + a.taps[t]);
-            idx %= a.code_len;
-            idx += idx < 0 ? a.code_len : 0;
+            // chip index floor(u + tap) mod L, reduced in FP64 (no integer division)
+            double r = floor(u + a.taps[t]);
+            r = fma(-L, floor(r * invL), r);
+            int idx = (int)r;
+            idx -= idx >= Li ? Li : 0;
+            idx += idx < 0 ? Li : 0;
             const float cv = (float)code[idx];
             acc[t].x = fmaf(w.x, cv, acc[t].x);
             acc[t].y = fmaf(w.y, cv, acc[t].y);
@@ -125,7 +130,7 @@ __global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
 }
 
 __global__ __launch_bounds__(kBankT) void bank_split_kernel(BankArgs a) {
-    const int b = blockIdx.x, c = blockIdx.y;
+    const int c = blockIdx.x, b = blockIdx.y;  // the channels of one block run together (IQ block reused from L2)
     __shared__ float2 red[(kBankT / 64) * 2 * kMaxTaps];
     __shared__ float2 part[2 * kMaxTaps];
     const double fc = a.carrier[2 * c];
@@ -200,6 +205,10 @@ static SplitTap split_tap(double phase, int64_t spm, double code_period_ms) {
 int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_state* ch, float* out) {
     UGSDR_CHECK_ARG(r && ch && out, "null argument");
     UGSDR_CHECK_ARG(r->iq && r->spm > 0 && r->n_blocks > 0 && r->fs > 0, "bad signal");
+    if (r->n_blocks > 65535) {
+        set_error("at most 65535 blocks per call");
+        return UGSDR_ERR_SIZE;
+    }
     UGSDR_CHECK_ARG(r->n_channels > 0 && r->n_taps > 0 && r->n_taps <= kMaxTaps && r->taps_chips, "bad taps");
     UGSDR_CHECK_ARG(r->codes && r->code_len > 0, "bad codes");
     UGSDR_CHECK_ARG(r->fmt == UGSDR_FMT_CI8 || r->fmt == UGSDR_FMT_CI16 || r->fmt == UGSDR_FMT_CF32, "bad format");
@@ -224,7 +233,7 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
     const size_t out_bytes = (size_t)r->n_blocks * r->n_channels * r->n_taps * sizeof(float2);
     UGSDR_HIP_TRY(ctx->results.reserve(out_bytes));
     a.out = ctx->results.as<float2>();
-    const dim3 grid((unsigned)r->n_blocks, (unsigned)r->n_channels);
+    const dim3 grid((unsigned)r->n_channels, (unsigned)r->n_blocks);
     if (r->mode == UGSDR_BANK_NCO) {
         std::vector<double> chs(4 * (size_t)r->n_channels);
         for (int c = 0; c < r->n_channels; ++c) {

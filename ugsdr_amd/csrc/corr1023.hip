@@ -235,7 +235,52 @@ __device__ __forceinline__ void corr1023_stats(const float (&S)[C1023Shape<M>::N
     }
 }
 
-// One row per workgroup (row = blockIdx.x): every block, then the statistics.
+// Tail row `tail` from the stream-K shares' partial surfaces, summed in
+// share order (a fixed order, so results do not depend on timing), then its
+// statistics.  G is the stream-K kernel's grid.  S has room for L floats.
+template <int T>
+__device__ __forceinline__ void corr1023_merge_row(const CorrArgs& a, int tail, int G, float* S, double* red,
+                                                   int tid) {
+    const int L = a.L;
+    const int row = a.full_rows + tail;
+    const ReqDesc q = a.reqs[find_req(a, row)];
+    const int local = row - q.row_start;
+    const int ur0 = q.tail_unit_start + (local - q.tail_row0) * q.K;
+    const int wa = sk_owner(a.units, G, ur0), wb = sk_owner(a.units, G, ur0 + q.K - 1);
+    // parts in share order; L is a multiple of 4 (1023 * QL, QL in 4, 8, 12, 16)
+    const float* part = reinterpret_cast<const float*>(a.scratch);
+    float4* S4 = reinterpret_cast<float4*>(S);
+    const int L4 = L / 4;
+    for (int w = wa; w <= wb; ++w) {
+        const int64_t base = wa == wb ? ((int64_t)2 * G + tail) * L
+                                      : ((int64_t)w * 2 + (sk_begin(a.units, G, w) >= ur0 ? 0 : 1)) * L;
+        const float4* src = reinterpret_cast<const float4*>(part + base);
+        for (int j = tid; j < L4; j += T) {
+            const float4 v = src[j];
+            if (w == wa) {
+                S4[j] = v;
+            } else {
+                float4 o = S4[j];
+                o.x += v.x;
+                o.y += v.y;
+                o.z += v.z;
+                o.w += v.w;
+                S4[j] = o;
+            }
+        }
+    }
+    __syncthreads();
+    row_stats_glob<T>(S, L, red, tid, q.rowstats + local, a.inv_n);
+    if (q.surfaces) {
+        float* o = q.surfaces + (int64_t)local * L;
+        for (int j = tid; j < L; j += T) o[j] = S[j] * a.inv_n;
+    }
+}
+
+// One row per workgroup: every block, then the statistics.  The first
+// a.n_merge workgroups instead finish the tail rows from the stream-K
+// partial surfaces (launched before this kernel), so that short work runs
+// beside the first round of rows rather than after the last.
 template <int M, int QL>
 __global__ __launch_bounds__(C1023Shape<M>::T, 3) void corr1023(CorrArgs a) {
     using S_ = C1023Shape<M>;
@@ -243,8 +288,14 @@ __global__ __launch_bounds__(C1023Shape<M>::T, 3) void corr1023(CorrArgs a) {
     static_assert(M % QL == 0, "surface length must divide the transform");
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
+    if ((int)blockIdx.x < a.n_merge) {
+        float* S = reinterpret_cast<float*>(smem);
+        double* red = reinterpret_cast<double*>(S + (a.L + 1) / 2 * 2);
+        corr1023_merge_row<S_::T>(a, blockIdx.x, a.sk_grid, S, red, tid);
+        return;
+    }
     const C1023Lds<M> Ld(smem, tid);
-    const int row = blockIdx.x;
+    const int row = blockIdx.x - a.n_merge;
     const ReqDesc q = a.reqs[find_req(a, row)];
     const int local = row - q.row_start;
     const int p = local / q.D, di = local - p * q.D;
@@ -301,66 +352,22 @@ __global__ __launch_bounds__(C1023Shape<M>::T, 3) void corr1023_sk(CorrArgs a) {
     }
 }
 
-// Tail rows: the shares' partial surfaces summed in share order (a fixed
-// order, so results do not depend on timing), then the row statistics.  One
-// workgroup per tail row; G is the tail kernel's grid.
-template <int T>
-__global__ __launch_bounds__(T) void corr1023_merge(CorrArgs a, int G) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int L = a.L;
-    float* S = reinterpret_cast<float*>(smem);
-    double* red = reinterpret_cast<double*>(S + (L + 1) / 2 * 2);
-    const int row = a.full_rows + blockIdx.x;
-    const ReqDesc q = a.reqs[find_req(a, row)];
-    const int local = row - q.row_start;
-    const int ur0 = q.tail_unit_start + (local - q.tail_row0) * q.K;
-    const int wa = sk_owner(a.units, G, ur0), wb = sk_owner(a.units, G, ur0 + q.K - 1);
-    // parts in share order; L is a multiple of 4 (1023 * QL, QL in 4, 8, 12, 16)
-    const float* part = reinterpret_cast<const float*>(a.scratch);
-    float4* S4 = reinterpret_cast<float4*>(S);
-    const int L4 = L / 4;
-    for (int w = wa; w <= wb; ++w) {
-        const int64_t base = wa == wb ? ((int64_t)2 * G + blockIdx.x) * L
-                                      : ((int64_t)w * 2 + (sk_begin(a.units, G, w) >= ur0 ? 0 : 1)) * L;
-        const float4* src = reinterpret_cast<const float4*>(part + base);
-        for (int j = threadIdx.x; j < L4; j += T) {
-            const float4 v = src[j];
-            if (w == wa) {
-                S4[j] = v;
-            } else {
-                float4 o = S4[j];
-                o.x += v.x;
-                o.y += v.y;
-                o.z += v.z;
-                o.w += v.w;
-                S4[j] = o;
-            }
-        }
-    }
-    __syncthreads();
-    row_stats_glob<T>(S, L, red, threadIdx.x, q.rowstats + local, a.inv_n);
-    if (q.surfaces) {
-        float* o = q.surfaces + (int64_t)local * L;
-        for (int j = threadIdx.x; j < L; j += T) o[j] = S[j] * a.inv_n;
-    }
-}
-
 template <class K>
 static hipError_t lds_attr(K kernel, size_t bytes) {
     return set_lds_limit(reinterpret_cast<const void*>(kernel), bytes);
 }
 
 template <int M, int QL>
-static int launch_1023(const CorrArgs& c, int grid_sk, hipStream_t st) {
+static int launch_1023(const CorrArgs& c, hipStream_t st) {
     const size_t lds = C1023Shape<M>::lds_bytes;
     constexpr int T = C1023Shape<M>::T;
-    if (c.full_rows > 0) {
-        UGSDR_HIP_TRY(lds_attr(corr1023<M, QL>, lds));
-        hipLaunchKernelGGL((corr1023<M, QL>), dim3(c.full_rows), dim3(T), lds, st, c);
-    }
-    if (grid_sk > 0) {
+    if (c.sk_grid > 0) {
         UGSDR_HIP_TRY(lds_attr(corr1023_sk<M, QL>, lds));
-        hipLaunchKernelGGL((corr1023_sk<M, QL>), dim3(grid_sk), dim3(T), lds, st, c);
+        hipLaunchKernelGGL((corr1023_sk<M, QL>), dim3(c.sk_grid), dim3(T), lds, st, c);
+    }
+    if (c.full_rows + c.n_merge > 0) {
+        UGSDR_HIP_TRY(lds_attr(corr1023<M, QL>, lds));
+        hipLaunchKernelGGL((corr1023<M, QL>), dim3(c.full_rows + c.n_merge), dim3(T), lds, st, c);
     }
     return UGSDR_OK;
 }
@@ -406,29 +413,22 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
     const int tail_rows = c.rows - c.full_rows;
     const int G_sk = tail_rows > 0 ? std::min(corr1023_grid(c.N, cus), c.units) : 0;
     CorrArgs a = c;
+    a.sk_grid = G_sk;
+    a.n_merge = G_sk > 0 ? tail_rows : 0;
     if (G_sk > 0) {
         UGSDR_HIP_TRY(scratch.reserve(((size_t)G_sk * 2 + tail_rows) * c.L * 4));
         a.scratch = scratch.as<float2>();
     }
-    int rc;
     switch (M * 100 + QL) {
-        case 404: rc = launch_1023<4, 4>(a, G_sk, st); break;
-        case 808: rc = launch_1023<8, 8>(a, G_sk, st); break;
-        case 804: rc = launch_1023<8, 4>(a, G_sk, st); break;
-        case 1212: rc = launch_1023<12, 12>(a, G_sk, st); break;
-        case 1204: rc = launch_1023<12, 4>(a, G_sk, st); break;
-        case 1616: rc = launch_1023<16, 16>(a, G_sk, st); break;
-        case 1604: rc = launch_1023<16, 4>(a, G_sk, st); break;
+        case 404: return launch_1023<4, 4>(a, st);
+        case 808: return launch_1023<8, 8>(a, st);
+        case 804: return launch_1023<8, 4>(a, st);
+        case 1212: return launch_1023<12, 12>(a, st);
+        case 1204: return launch_1023<12, 4>(a, st);
+        case 1616: return launch_1023<16, 16>(a, st);
+        case 1604: return launch_1023<16, 4>(a, st);
         default: set_error("corr1023: unsupported length pair"); return UGSDR_ERR_UNSUPPORTED;
     }
-    if (rc) return rc;
-    if (G_sk > 0) {
-        constexpr int TM = 512;
-        const size_t lds = (size_t)(c.L + 1) / 2 * 2 * 4 + 128 * 8;
-        UGSDR_HIP_TRY(lds_attr(corr1023_merge<TM>, lds));
-        hipLaunchKernelGGL(corr1023_merge<TM>, dim3((unsigned)tail_rows), dim3(TM), lds, st, a, G_sk);
-    }
-    return UGSDR_OK;
 }
 
 }  // namespace ugsdr_hip

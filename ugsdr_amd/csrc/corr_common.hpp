@@ -52,6 +52,9 @@ struct CorrArgs {
     float inv_n;
     int full_rows;          // three-stage kernel: rows run one per workgroup
     int units;              // (row, block) units of the tail after them
+    int max_k;              // reduced path: most blocks of any request
+    int sk_grid;            // three-stage kernel: workgroups of the tail split
+    int n_merge;            // three-stage kernel: tail rows to merge
 };
 
 // the request owning a batch row (uniform per workgroup: kept in scalar registers)

@@ -99,7 +99,7 @@ __device__ void mfma_btab_build(uint4* tab, bool inv, float scale, int tid, int 
 // butterfly jj < per = n / R reads in[jj + m per], writes out[jj R + q].
 // Second (radix Rb, Ns = per): twiddle tw[m jj] (a length-n table), writes
 // out[jj + q per], natural order.  The caller synchronises before and after.
-template <int R, class H, bool FIRST, int NW>
+template <int R, class H, bool FIRST, int NW, int GPP = 2>
 __device__ __forceinline__ void mfma_stage(const uint32_t* in, int ld_in, uint32_t* out, int ld_out, int n, int nb,
                                            const float2* tw, const uint4* btab, int lane, int wave) {
     using S = MfmaShape<R>;
@@ -114,51 +114,60 @@ __device__ __forceinline__ void mfma_stage(const uint32_t* in, int ld_in, uint32
             Bre[ks][qt] = __builtin_bit_cast(typename H::v8, btab[((ks * QT + qt) * 2 + 0) * 64 + lane]);
             Bim[ks][qt] = __builtin_bit_cast(typename H::v8, btab[((ks * QT + qt) * 2 + 1) * 64 + lane]);
         }
-    for (int grp = wave; grp * 16 < Bt; grp += NW) {
-        const int j = grp * 16 + i;
-        const int jc = j < Bt ? j : 0;
-        const int seq = jc / per, jj = jc - seq * per;
-        const uint32_t* src = in + seq * ld_in + jj;
-        typename H::v8 A[KS];
+    // GPP groups per pass (grp, grp + NW, ...): their LDS loads and matrix
+    // products overlap instead of running back to back
+    for (int grp = wave; grp * 16 < Bt; grp += GPP * NW) {
+        typename H::v8 A[GPP][KS];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
+        for (int gi = 0; gi < GPP; ++gi) {
+            const int j = (grp + gi * NW) * 16 + i;
+            const int jc = j < Bt ? j : 0;
+            const int seq = jc / per, jj = jc - seq * per;
+            const uint32_t* src = in + seq * ld_in + jj;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int m = ks * 16 + 4 * g + e;
-                float2 x = make_float2(0.f, 0.f);
-                if (m < R) {
-                    x = unpack16<H>(src[m * per]);
-                    if (!FIRST && m > 0) x = cmul(x, tw[m * jj]);
-                }
-                A[ks][2 * e] = (typename H::e_t)x.x;
-                A[ks][2 * e + 1] = (typename H::e_t)x.y;
-            }
-        }
-        f32x4 Dre[QT], Dim[QT];
+            for (int ks = 0; ks < KS; ++ks) {
 #pragma unroll
-        for (int qt = 0; qt < QT; ++qt) {
-            const f32x4 z = {0.f, 0.f, 0.f, 0.f};
-            Dre[qt] = H::mfma(A[0], Bre[0][qt], z);
-            Dim[qt] = H::mfma(A[0], Bim[0][qt], z);
-#pragma unroll
-            for (int ks = 1; ks < KS; ++ks) {
-                Dre[qt] = H::mfma(A[ks], Bre[ks][qt], Dre[qt]);
-                Dim[qt] = H::mfma(A[ks], Bim[ks][qt], Dim[qt]);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int jo = grp * 16 + 4 * g + r;
-            if (jo < Bt) {
-                const int so = jo / per, jjo = jo - so * per;
-                uint32_t* dst = out + so * ld_out + (FIRST ? jjo * R : jjo);
-#pragma unroll
-                for (int qt = 0; qt < QT; ++qt) {
-                    const int q = qt * 16 + i;
-                    if (q < R) dst[FIRST ? q : q * per] = pack16<H>(make_float2(Dre[qt][r], Dim[qt][r]));
+                for (int e = 0; e < 4; ++e) {
+                    const int m = ks * 16 + 4 * g + e;
+                    float2 x = make_float2(0.f, 0.f);
+                    if (m < R) {
+                        x = unpack16<H>(src[m * per]);
+                        if (!FIRST && m > 0) x = cmul(x, tw[m * jj]);
+                    }
+                    A[gi][ks][2 * e] = (typename H::e_t)x.x;
+                    A[gi][ks][2 * e + 1] = (typename H::e_t)x.y;
                 }
             }
         }
+        f32x4 Dre[GPP][QT], Dim[GPP][QT];
+#pragma unroll
+        for (int gi = 0; gi < GPP; ++gi)
+#pragma unroll
+            for (int qt = 0; qt < QT; ++qt) {
+                const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+                Dre[gi][qt] = H::mfma(A[gi][0], Bre[0][qt], z);
+                Dim[gi][qt] = H::mfma(A[gi][0], Bim[0][qt], z);
+#pragma unroll
+                for (int ks = 1; ks < KS; ++ks) {
+                    Dre[gi][qt] = H::mfma(A[gi][ks], Bre[ks][qt], Dre[gi][qt]);
+                    Dim[gi][qt] = H::mfma(A[gi][ks], Bim[ks][qt], Dim[gi][qt]);
+                }
+            }
+#pragma unroll
+        for (int gi = 0; gi < GPP; ++gi)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int jo = (grp + gi * NW) * 16 + 4 * g + r;
+                if (jo < Bt) {
+                    const int so = jo / per, jjo = jo - so * per;
+                    uint32_t* dst = out + so * ld_out + (FIRST ? jjo * R : jjo);
+#pragma unroll
+                    for (int qt = 0; qt < QT; ++qt) {
+                        const int q = qt * 16 + i;
+                        if (q < R) dst[FIRST ? q : q * per] = pack16<H>(make_float2(Dre[gi][qt][r], Dim[gi][qt][r]));
+                    }
+                }
+            }
     }
 }
 

@@ -145,9 +145,9 @@ struct RedShape {
     }
 };
 
-template <class H, int R1a, int R1b>
-__global__ __launch_bounds__(512, 2) void corr_mfma(CorrArgs a) {
-    constexpr int T = 512, NW = T / 64, N1 = R1a * R1b, N2 = kRedN2;
+template <class H, int R1a, int R1b, int T, int GPP>
+__global__ __launch_bounds__(T, T / 256) void corr_mfma(CorrArgs a) {
+    constexpr int NW = T / 64, N1 = R1a * R1b, N2 = kRedN2;
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* bufA = reinterpret_cast<uint32_t*>(smem);
     uint32_t* bufB = bufA + kRedCap;
@@ -175,19 +175,24 @@ __global__ __launch_bounds__(512, 2) void corr_mfma(CorrArgs a) {
     const int ld1 = N1 + 1;
     const int nbat1 = (N2 + kRedCap / ld1 - 1) / (kRedCap / ld1), cb = (N2 + nbat1 - 1) / nbat1;
     const int nbat2 = (N1 + kRedCap / N2 - 1) / (kRedCap / N2), rb = (N1 + nbat2 - 1) / nbat2;
-    uint32_t* G = reinterpret_cast<uint32_t*>(a.scratch) + (int64_t)blockIdx.x * 2 * N;
-    float* S = reinterpret_cast<float*>(G + N);
+    // workgroup scratch: the phase-1 output of every block of a row (maxK
+    // slices of N 16-bit pairs), then the row's FP32 surface
+    const int64_t wg_words = (int64_t)(a.max_k + 1) * N;
+    uint32_t* Gall = reinterpret_cast<uint32_t*>(a.scratch) + (int64_t)blockIdx.x * wg_words;
+    float* S = reinterpret_cast<float*>(Gall + (int64_t)a.max_k * N);
+    constexpr int ACC = (kRedCap + T - 1) / T;  // phase-2 outputs a thread owns per row batch
     for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
         const ReqDesc q = a.reqs[find_req(a, row)];
         const int local = row - q.row_start;
         const int p = local / q.D, di = local - p * q.D;
         const int cls = q.cls[di], sh = q.shift[di];
         const gptr<uint32_t> C = as_global(reinterpret_cast<const uint32_t*>(q.specs[p]));
+        // phase 1 of every block: column DFTs, four-step twiddle, into slice k
         for (int k = 0; k < q.K; ++k) {
             const int t0 = opaque(tid);
             const gptr<uint32_t> X =
                 as_global(reinterpret_cast<const uint32_t*>(q.X) + ((int64_t)k * q.F + cls) * N);
-            const float sc = q.block_inv[k * q.F + cls] * q.spec_inv;
+            uint32_t* G = Gall + (int64_t)k * N;
             for (int c0 = 0; c0 < N2; c0 += cb) {
                 const int nb = min(cb, N2 - c0);
                 walk_cols<8>(
@@ -202,9 +207,9 @@ __global__ __launch_bounds__(512, 2) void corr_mfma(CorrArgs a) {
                         bufA[t * ld1 + n1] = pack16<H>(cmul(unpack16<H>(v.x), unpack16<H>(v.y)));
                     });
                 __syncthreads();
-                mfma_stage<R1a, H, true, NW>(bufA, ld1, bufB, N1, N1, nb, nullptr, bt1a, lane, wave);
+                mfma_stage<R1a, H, true, NW, GPP>(bufA, ld1, bufB, N1, N1, nb, nullptr, bt1a, lane, wave);
                 __syncthreads();
-                mfma_stage<R1b, H, false, NW>(bufB, N1, bufA, ld1, N1, nb, tw1, bt1b, lane, wave);
+                mfma_stage<R1b, H, false, NW, GPP>(bufB, N1, bufA, ld1, N1, nb, tw1, bt1b, lane, wave);
                 __syncthreads();
                 walk_cols<4>(
                     t0, T, nb, N1, [&](int t, int k1) { return bufA[t * ld1 + k1]; },
@@ -214,37 +219,63 @@ __global__ __launch_bounds__(512, 2) void corr_mfma(CorrArgs a) {
                     });
                 __syncthreads();
             }
-            for (int r0 = 0; r0 < N1; r0 += rb) {
-                const int nb = min(rb, N1 - r0);
-                {
-                    const gptr<uint32_t> g = as_global(G + r0 * N2);
-                    const int tot = nb * N2;
-                    for (int base = t0; base < tot; base += 8 * T) {
-                        uint32_t v[8];
+        }
+        // phase 2 per row batch over every block; the batch's surface stays
+        // in registers (thread tid owns batch outputs tid + i T)
+        for (int r0 = 0; r0 < N1; r0 += rb) {
+            const int nb = min(rb, N1 - r0);
+            const int tot = nb * N2;
+            float acc[ACC];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                            if (base + u * T < tot) v[u] = g[base + u * T];
+            for (int i = 0; i < ACC; ++i) acc[i] = 0.f;
+            for (int k = 0; k < q.K; ++k) {
+                const int t0 = opaque(tid);
+                const gptr<uint32_t> g = as_global(Gall + (int64_t)k * N + r0 * N2);
+                for (int base = t0; base < tot; base += 8 * T) {
+                    uint32_t v[8];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                            if (base + u * T < tot) bufA[base + u * T] = v[u];
+                    for (int u = 0; u < 8; ++u)
+                        if (base + u * T < tot) v[u] = g[base + u * T];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (base + u * T < tot) bufA[base + u * T] = v[u];
+                }
+                __syncthreads();
+                mfma_stage<11, H, true, NW, GPP>(bufA, N2, bufB, N2, N2, nb, nullptr, bt2a, lane, wave);
+                __syncthreads();
+                mfma_stage<31, H, false, NW, GPP>(bufB, N2, bufA, N2, N2, nb, tw2, bt2b, lane, wave);
+                __syncthreads();
+                const float sc = q.block_inv[k * q.F + cls] * q.spec_inv;
+                int t = t0 % nb, k2 = t0 / nb;
+                const int dt = T % nb, dk = T / nb;
+#pragma unroll
+                for (int i = 0; i < ACC; ++i) {
+                    if (k2 < N2) acc[i] += cabs_hw(unpack16<H>(bufA[t * N2 + k2])) * sc;
+                    t += dt;
+                    k2 += dk;
+                    if (t >= nb) {
+                        t -= nb;
+                        ++k2;
                     }
                 }
                 __syncthreads();
-                mfma_stage<11, H, true, NW>(bufA, N2, bufB, N2, N2, nb, nullptr, bt2a, lane, wave);
-                __syncthreads();
-                mfma_stage<31, H, false, NW>(bufB, N2, bufA, N2, N2, nb, tw2, bt2b, lane, wave);
-                __syncthreads();
-                const bool first = k == 0;
-                walk_cols<8>(
-                    t0, T, nb, N2,
-                    [&](int t, int k2) {
-                        const float mag = cabs_hw(unpack16<H>(bufA[t * N2 + k2])) * sc;
-                        return first ? mag : mag + S[r0 + t + N1 * k2];
-                    },
-                    [&](int t, int k2, float v) { S[r0 + t + N1 * k2] = v; });
-                __syncthreads();
+            }
+            {
+                int t = tid % nb, k2 = tid / nb;
+                const int dt = T % nb, dk = T / nb;
+#pragma unroll
+                for (int i = 0; i < ACC; ++i) {
+                    if (k2 < N2) S[r0 + t + N1 * k2] = acc[i];
+                    t += dt;
+                    k2 += dk;
+                    if (t >= nb) {
+                        t -= nb;
+                        ++k2;
+                    }
+                }
             }
         }
+        __syncthreads();
         row_stats_glob<T>(S, N, red, tid, q.rowstats + local, 1.0f);
         if (q.surfaces) {
             float* o = q.surfaces + (int64_t)local * N;
@@ -267,22 +298,30 @@ static bool red_plan(int64_t N, int& ra, int& rb_) {
     }
 }
 
-template <class H, int R1a, int R1b>
+// Two shapes (one workgroup fills a CU's LDS either way): 512 threads with
+// two butterfly groups in flight per wave, or 1024 threads with one.
+template <class H, int R1a, int R1b, int T, int GPP>
 static int launch_corr_mfma(const CorrArgs& c, int grid, hipStream_t st) {
     const size_t lds = RedShape<R1a, R1b>::lds_bytes(c.N);
-    UGSDR_HIP_TRY(set_lds_limit(reinterpret_cast<const void*>(corr_mfma<H, R1a, R1b>), lds));
-    hipLaunchKernelGGL((corr_mfma<H, R1a, R1b>), dim3(grid), dim3(512), lds, st, c);
+    UGSDR_HIP_TRY(set_lds_limit(reinterpret_cast<const void*>(corr_mfma<H, R1a, R1b, T, GPP>), lds));
+    hipLaunchKernelGGL((corr_mfma<H, R1a, R1b, T, GPP>), dim3(grid), dim3(T), lds, st, c);
     return UGSDR_OK;
 }
 
+template <class H, int R1a, int R1b>
+static int launch_corr_mfma_t(const CorrArgs& c, int grid, hipStream_t st, int threads) {
+    return threads == 512 ? launch_corr_mfma<H, R1a, R1b, 512, 2>(c, grid, st)
+                          : launch_corr_mfma<H, R1a, R1b, 1024, 1>(c, grid, st);
+}
+
 template <class H>
-static int launch_corr_mfma_n(const CorrArgs& c, int grid, hipStream_t st) {
+static int launch_corr_mfma_n(const CorrArgs& c, int grid, hipStream_t st, int threads) {
     switch (c.N / kRedN2) {
-        case 60: return launch_corr_mfma<H, 4, 15>(c, grid, st);
-        case 112: return launch_corr_mfma<H, 16, 7>(c, grid, st);
-        case 120: return launch_corr_mfma<H, 8, 15>(c, grid, st);
-        case 180: return launch_corr_mfma<H, 12, 15>(c, grid, st);
-        default: return launch_corr_mfma<H, 16, 15>(c, grid, st);
+        case 60: return launch_corr_mfma_t<H, 4, 15>(c, grid, st, threads);
+        case 112: return launch_corr_mfma_t<H, 16, 7>(c, grid, st, threads);
+        case 120: return launch_corr_mfma_t<H, 8, 15>(c, grid, st, threads);
+        case 180: return launch_corr_mfma_t<H, 12, 15>(c, grid, st, threads);
+        default: return launch_corr_mfma_t<H, 16, 15>(c, grid, st, threads);
     }
 }
 
@@ -306,7 +345,7 @@ int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
 }
 
 int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots,
-                         hipStream_t st, DevBuf& scratch, int cus) {
+                         hipStream_t st, DevBuf& scratch, int cus, int threads) {
     if (plans.empty() || plans.size() > (size_t)kMaxBatch) {
         set_error("a correlation batch holds 1 to " + std::to_string(kMaxBatch) + " requests");
         return UGSDR_ERR_ARG;
@@ -344,11 +383,14 @@ int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::ve
     c.rows = rows;
     c.n_req = (int)plans.size();
     c.inv_n = 1.0f;
+    int max_k = 1;
+    for (const AcqPlan* pl : plans) max_k = std::max(max_k, (int)pl->K);
+    c.max_k = max_k;
     const int grid = std::min(c.rows, cus);
-    UGSDR_HIP_TRY(scratch.reserve((size_t)grid * 2 * p0.N * 4));
+    UGSDR_HIP_TRY(scratch.reserve((size_t)grid * (max_k + 1) * p0.N * 4));
     c.scratch = scratch.as<float2>();
-    int rc = p0.precision == UGSDR_PREC_FP16 ? launch_corr_mfma_n<Fp16Ops>(c, grid, st)
-                                             : launch_corr_mfma_n<Bf16Ops>(c, grid, st);
+    int rc = p0.precision == UGSDR_PREC_FP16 ? launch_corr_mfma_n<Fp16Ops>(c, grid, st, threads)
+                                             : launch_corr_mfma_n<Bf16Ops>(c, grid, st, threads);
     if (rc) return rc;
     UGSDR_HIP_TRY(hipGetLastError());
     return UGSDR_OK;
@@ -382,7 +424,10 @@ int acquire_reduced(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_re
     if (int rc = enqueue_classes(pl, s, st)) return rc;
     if (int rc = enqueue_reduce_classes(pl, s, st)) return rc;
     ctx->mark(2);
-    if (int rc = enqueue_corr_reduced({&pl}, {&s}, st, ctx->scratch, device_cu_count(ctx->device))) return rc;
+    const auto tuned = ctx->tuned.find(pl.N);
+    const int threads = tuned != ctx->tuned.end() ? tuned->second : 1024;
+    if (int rc = enqueue_corr_reduced({&pl}, {&s}, st, ctx->scratch, device_cu_count(ctx->device), threads))
+        return rc;
     ctx->mark(3);
     if (int rc = enqueue_final(pl, s, st, peaks != nullptr)) return rc;
     ctx->mark(4);

@@ -149,7 +149,7 @@ static int launch_round(Server& s, Round& rd) {
     }
     if (int rc = p0.precision == UGSDR_PREC_FP32
                      ? enqueue_corr(plans, devs, st, s.scratch[rd.stream], s.cus, p0.threads)
-                     : enqueue_corr_reduced(plans, devs, st, s.scratch[rd.stream], s.cus))
+                     : enqueue_corr_reduced(plans, devs, st, s.scratch[rd.stream], s.cus, p0.threads))
         return rc;
     for (auto& jp : rd.jobs) {
         Job& j = *jp;
