@@ -798,6 +798,10 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
 int default_threads(int64_t N) {
     if (const int t = corr1023_threads(N)) return t;
     if (N <= 8192) return 256;
+    // four-step lengths: 1024 threads measured fastest on MI355X (BOC 4 ms,
+    // N = 81840: 9.9 ms against 15.3 ms at 512)
+    FourStep probe;
+    if (N > kLdsCap && fourstep_341((int)N, probe)) return 1024;
     return 512;
 }
 
@@ -819,7 +823,7 @@ std::vector<int> thread_candidates(int64_t N) {
 }
 
 size_t corr_scratch_bytes(const AcqPlan& p0, int cus) {
-    if (p0.precision != UGSDR_PREC_FP32) return (size_t)(p0.K + 1) * p0.N * 4 * (size_t)cus;
+    if (p0.precision != UGSDR_PREC_FP32) return reduced_scratch_bytes(p0.N, (int)p0.K, cus);
     if (!p0.small) return (2 * (size_t)p0.N + (p0.L + 1) / 2) * sizeof(float2) * (size_t)cus;
     if (corr1023_supports(p0.N, p0.L)) return (size_t)3 * corr1023_grid(p0.N, cus) * p0.L * 4;
     return 0;

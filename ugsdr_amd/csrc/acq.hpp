@@ -88,6 +88,7 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
 // pair has a plan, the 16-bit packing of the class spectra, and one
 // correlation launch over a batch of requests of one bucket.
 bool reduced_supported(int64_t N, int64_t L);
+size_t reduced_scratch_bytes(int64_t N, int max_k, int grid);
 int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
 int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots,
                          hipStream_t st, DevBuf& scratch, int cu_count, int threads);

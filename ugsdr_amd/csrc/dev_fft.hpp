@@ -484,13 +484,21 @@ __device__ __forceinline__ int opaque(int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
 template <class T>
 using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+using gwptr = __attribute__((address_space(1))) T*;
 #else
 template <class T>
 using gptr = const T*;
+template <class T>
+using gwptr = T*;
 #endif
 template <class T>
 __device__ __forceinline__ gptr<T> as_global(const T* p) {
     return (gptr<T>)p;
+}
+template <class T>
+__device__ __forceinline__ gwptr<T> as_global_w(T* p) {
+    return (gwptr<T>)p;
 }
 
 // |z| with the hardware square root (v_sqrt_f32, 1 ulp).  The correctly

@@ -133,149 +133,270 @@ __global__ void block_pack_kernel(const float2* X, int64_t N, const unsigned* bm
 // ------------------------------------------------------------------ correlation
 constexpr int kRedCap = 16384;   // 32-bit pairs per LDS buffer (two buffers)
 constexpr int kRedN2 = 341;      // second four-step factor, 11 * 31
+constexpr int kRedGP = 344;      // words per phase-1 output row in global memory
+                                 // (>= 341 + 3, a multiple of 4: 16-byte stores)
 
+// scratch words of one workgroup: max_k phase-1 slices of N / 341 rows,
+// then the FP32 surface
+__host__ __device__ constexpr int64_t red_wg_words(int64_t N, int max_k) {
+    return ((int64_t)max_k * (N / kRedN2) * kRedGP + N + 3) / 4 * 4;
+}
+
+// smallest v >= x with v % mod == rem
+constexpr int pad_to(int x, int mod, int rem) { return x + ((rem - x) % mod + mod) % mod; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+// LDS layouts of the four stages (SeqLayout rows: stride 1 mod 32 plus the
+// bank skew).  A stage's input element (m, jj) of a sequence (m < R, jj <
+// PER butterflies) sits at base(seq) + m * MS + jj with MS odd; the stage
+// writing that input puts its output (jjo, q) there as (m = jjo, jj = q).
+//   phase 1 (columns, length N1 = Ra Rb): in (m < Ra, jj < Rb) [MS = Rb,
+//     LD1] holds element n1 at base + n1; mid (m < Rb, jj < Ra) [SX1 =
+//     Ra | 1, LDB1]; the second stage's output k1 = jj + q Ra goes straight
+//     to global memory, row k1 of the block's slice (GP words a row);
+//   phase 2 (rows, 341 = 31 * 11): in (m < 31, jj < 11) [11, LD2] holds
+//     element n2 at base + n2; mid (m < 11, jj < 31) [31, LD2]; k2 = jj +
+//     31 q; the surface rows stage through both buffers as (k2, seq) at
+//     k2 LS + seq + skew(seq), LS % 32 == 31 (31 LS == 1 mod 32: a tile's
+//     writes are conflict free).
+// Batches hold whole tiles of 16 sequences.
 template <int R1a, int R1b>
 struct RedShape {
-    static constexpr int N1 = R1a * R1b;
+    static constexpr int N1 = R1a * R1b, N2 = kRedN2;
+    static constexpr int SK = SeqLayout<1>::SKEW;
+    static constexpr int SX1 = R1a | 1;
+    static_assert(R1b % 2 == 1, "phase-1 first radix needs an odd row");
+    static constexpr int LD1 = pad_to(N1 + SK, 32, 1), LDB1 = pad_to(R1b * SX1 + SK, 32, 1);
+    static constexpr int ROW1 = cmax(N1, R1b * SX1), LDM1 = cmax(LD1, LDB1);
+    static constexpr int CB = cmin(((kRedCap - SK - ROW1) / LDM1 + 1) / 16 * 16, (N2 + 15) / 16 * 16);
+    static constexpr int GP = kRedGP;
+    static_assert(GP >= N2 + 3 && GP % 4 == 0, "phase-1 rows take 16-byte stores of 4 columns");
+    static constexpr int LD2 = pad_to(N2 + SK, 32, 1);
+    static constexpr int RB = ((kRedCap - SK - N2) / LD2 + 1) / 16 * 16;
+    static constexpr int LS = pad_to(RB + SK, 32, 31);
+    static_assert(CB >= 16 && RB >= 16, "an LDS buffer holds less than one tile");
+    static_assert(N2 * LS <= 2 * kRedCap, "surface staging exceeds the LDS buffers");
+    // passes of the last stage per wave (its outputs stay in registers)
+    template <int T, int GPP>
+    static constexpr int passes2() {
+        return ((RB + 15) / 16 * 31 + GPP * (T / 64) - 1) / (GPP * (T / 64));
+    }
     static size_t lds_bytes(int N) {
         return (size_t)2 * kRedCap * 4 +
-               (size_t)(MfmaShape<R1a>::ENTRIES + MfmaShape<R1b>::ENTRIES + MfmaShape<11>::ENTRIES +
-                        MfmaShape<31>::ENTRIES) * 16 +
-               (size_t)(N1 + kRedN2 + kTwLo + (N + kTwLo - 1) / kTwLo) * 8 + 128 * 8;
+               (size_t)(MfmaShape<R1a>::ENTRIES + MfmaShape<R1b>::ENTRIES + MfmaShape<31>::ENTRIES +
+                        MfmaShape<11>::ENTRIES) * 16 +
+               (size_t)(2 * N1 + kRedN2 + kTwLo + (N + kTwLo - 1) / kTwLo) * 8 + 128 * 8;
     }
 };
 
+// One workgroup per row (grid-strided).  Phase 1 runs the column DFTs of
+// every block of the row (two matrix-core stages; the inter-stage twiddle
+// and the four-step twiddle are applied to FP32 stage outputs) into a
+// workgroup slice of global memory; phase 2 runs the row DFTs per batch of
+// rows over every block, the last stage accumulating |.| of its FP32
+// outputs in registers.
 template <class H, int R1a, int R1b, int T, int GPP>
 __global__ __launch_bounds__(T, T / 256) void corr_mfma(CorrArgs a) {
-    constexpr int NW = T / 64, N1 = R1a * R1b, N2 = kRedN2;
+    using L = RedShape<R1a, R1b>;
+    using LA = SeqLayout<L::LD1>;
+    using LB = SeqLayout<L::LDB1>;
+    using L2 = SeqLayout<L::LD2>;
+    constexpr int NW = T / 64, N1 = L::N1, N2 = L::N2, GP = L::GP;
+    constexpr int P2 = L::template passes2<T, GPP>();
+    constexpr int RPP1 = T / L::CB;   // product pass: column tc, rows tr + i RPP1
+    constexpr int RPP2 = T / N2;      // row copy: column tc, rows tr + i RPP2
+    static_assert(RPP1 >= 1 && RPP2 >= 1, "workgroup narrower than a batch row");
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* bufA = reinterpret_cast<uint32_t*>(smem);
     uint32_t* bufB = bufA + kRedCap;
     uint4* bt1a = reinterpret_cast<uint4*>(bufB + kRedCap);
     uint4* bt1b = bt1a + MfmaShape<R1a>::ENTRIES;
     uint4* bt2a = bt1b + MfmaShape<R1b>::ENTRIES;
-    uint4* bt2b = bt2a + MfmaShape<11>::ENTRIES;
-    float2* tw1 = reinterpret_cast<float2*>(bt2b + MfmaShape<31>::ENTRIES);
-    float2* tw2 = tw1 + N1;
+    uint4* bt2b = bt2a + MfmaShape<31>::ENTRIES;
+    float2* tw1 = reinterpret_cast<float2*>(bt2b + MfmaShape<11>::ENTRIES);
+    float2* twk = tw1 + N1;
+    float2* tw2 = twk + N1;
     float2* twlo = tw2 + N2;
     float2* twhi = twlo + kTwLo;
     const int N = a.N;
     double* red = reinterpret_cast<double*>(twhi + (N + kTwLo - 1) / kTwLo);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
     mfma_btab_build<R1a, H>(bt1a, true, 1.f / R1a, tid, T);
     mfma_btab_build<R1b, H>(bt1b, true, 1.f / R1b, tid, T);
-    mfma_btab_build<11, H>(bt2a, true, 1.f / 11, tid, T);
-    mfma_btab_build<31, H>(bt2b, true, 1.f / 31, tid, T);
+    mfma_btab_build<31, H>(bt2a, true, 1.f / 31, tid, T);
+    mfma_btab_build<11, H>(bt2b, true, 1.f / 11, tid, T);
     tw_table_build(tw1, N1, true, tid, T);
     tw_table_build(tw2, N2, true, tid, T);
     tw_build(twlo, twhi, N, tid, T);
+    for (int j = tid; j < N1; j += T) {   // exp(+2 pi i j / N), the four-step step
+        double sn, cs;
+        sincospi(2.0 * (double)j / (double)N, &sn, &cs);
+        twk[j] = make_float2((float)cs, (float)sn);
+    }
     __syncthreads();
-    // column batches with a padded stride (conflict-free strided LDS access),
-    // row batches contiguous; both balanced over their batch count
-    const int ld1 = N1 + 1;
-    const int nbat1 = (N2 + kRedCap / ld1 - 1) / (kRedCap / ld1), cb = (N2 + nbat1 - 1) / nbat1;
-    const int nbat2 = (N1 + kRedCap / N2 - 1) / (kRedCap / N2), rb = (N1 + nbat2 - 1) / nbat2;
     // workgroup scratch: the phase-1 output of every block of a row (maxK
-    // slices of N 16-bit pairs), then the row's FP32 surface
-    const int64_t wg_words = (int64_t)(a.max_k + 1) * N;
-    uint32_t* Gall = reinterpret_cast<uint32_t*>(a.scratch) + (int64_t)blockIdx.x * wg_words;
-    float* S = reinterpret_cast<float*>(Gall + (int64_t)a.max_k * N);
-    constexpr int ACC = (kRedCap + T - 1) / T;  // phase-2 outputs a thread owns per row batch
+    // slices of N1 rows k1 of GP words, columns n2 < N2 used), then the
+    // row's FP32 surface
+    uint32_t* Gall = reinterpret_cast<uint32_t*>(a.scratch) + (int64_t)blockIdx.x * red_wg_words(N, a.max_k);
+    float* S = reinterpret_cast<float*>(Gall + (int64_t)a.max_k * N1 * GP);
     for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
         const ReqDesc q = a.reqs[find_req(a, row)];
         const int local = row - q.row_start;
         const int p = local / q.D, di = local - p * q.D;
         const int cls = q.cls[di], sh = q.shift[di];
         const gptr<uint32_t> C = as_global(reinterpret_cast<const uint32_t*>(q.specs[p]));
-        // phase 1 of every block: column DFTs, four-step twiddle, into slice k
         for (int k = 0; k < q.K; ++k) {
             const int t0 = opaque(tid);
             const gptr<uint32_t> X =
                 as_global(reinterpret_cast<const uint32_t*>(q.X) + ((int64_t)k * q.F + cls) * N);
-            uint32_t* G = Gall + (int64_t)k * N;
-            for (int c0 = 0; c0 < N2; c0 += cb) {
-                const int nb = min(cb, N2 - c0);
-                walk_cols<8>(
-                    t0, T, nb, N1,
-                    [&](int t, int n1) {
-                        const int n = c0 + t + N2 * n1;
-                        int src = n + sh;
-                        src -= src >= N ? N : 0;
-                        return make_uint2(X[src], C[n]);
-                    },
-                    [&](int t, int n1, uint2 v) {
-                        bufA[t * ld1 + n1] = pack16<H>(cmul(unpack16<H>(v.x), unpack16<H>(v.y)));
-                    });
-                __syncthreads();
-                mfma_stage<R1a, H, true, NW, GPP>(bufA, ld1, bufB, N1, N1, nb, nullptr, bt1a, lane, wave);
-                __syncthreads();
-                mfma_stage<R1b, H, false, NW, GPP>(bufB, N1, bufA, ld1, N1, nb, tw1, bt1b, lane, wave);
-                __syncthreads();
-                walk_cols<4>(
-                    t0, T, nb, N1, [&](int t, int k1) { return bufA[t * ld1 + k1]; },
-                    [&](int t, int k1, uint32_t v) {
-                        const int n2 = c0 + t;
-                        G[k1 * N2 + n2] = pack16<H>(cmul(unpack16<H>(v), tw_get<true>(twlo, twhi, n2 * k1)));
-                    });
-                __syncthreads();
-            }
-        }
-        // phase 2 per row batch over every block; the batch's surface stays
-        // in registers (thread tid owns batch outputs tid + i T)
-        for (int r0 = 0; r0 < N1; r0 += rb) {
-            const int nb = min(rb, N1 - r0);
-            const int tot = nb * N2;
-            float acc[ACC];
+            const gwptr<uint32_t> G = as_global_w(Gall + (int64_t)k * N1 * GP);
+            const int tc = t0 % L::CB, tr = t0 / L::CB;
+            for (int c0 = 0; c0 < N2; c0 += L::CB) {
+                const int nb = min(L::CB, N2 - c0);
+                // x (shifted class block) times the replica spectrum: column
+                // tc of the batch, element n1 at base(tc) + n1.  The previous
+                // batch's second stage reads only the other buffer.
+                if (tr < RPP1 && tc < nb) {
+                    uint32_t* dst = bufA + LA::base(tc);
+                    const int nc = c0 + tc;
+                    for (int n1 = tr; n1 < N1; n1 += 4 * RPP1) {
+                        uint32_t xv[4], cv[4];
 #pragma unroll
-            for (int i = 0; i < ACC; ++i) acc[i] = 0.f;
-            for (int k = 0; k < q.K; ++k) {
-                const int t0 = opaque(tid);
-                const gptr<uint32_t> g = as_global(Gall + (int64_t)k * N + r0 * N2);
-                for (int base = t0; base < tot; base += 8 * T) {
-                    uint32_t v[8];
+                        for (int u = 0; u < 4; ++u) {
+                            const int m = n1 + u * RPP1;
+                            const int n = nc + N2 * (m < N1 ? m : 0);
+                            int src = n + sh;
+                            src -= src >= N ? N : 0;
+                            xv[u] = X[src];
+                            cv[u] = C[n];
+                        }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        if (base + u * T < tot) v[u] = g[base + u * T];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        if (base + u * T < tot) bufA[base + u * T] = v[u];
+                        for (int u = 0; u < 4; ++u)
+                            if (n1 + u * RPP1 < N1) dst[n1 + u * RPP1] = cmul16<H>(xv[u], cv[u]);
+                    }
                 }
                 __syncthreads();
-                mfma_stage<11, H, true, NW, GPP>(bufA, N2, bufB, N2, N2, nb, nullptr, bt2a, lane, wave);
+                // radix Ra; output (jj, q) times w_N1^(jj q) is the next input
+                mfma_tiles<R1a, H, NW, GPP, 0, R1b, R1b, LA>(
+                    bufA, nb, bt1a, lane, wave, [&](int, int, int sb, int jj, const f32x4* Dre, const f32x4* Dim) {
+                        const int s0 = sb * 16 + 4 * lg;
+                        if (li < R1a && s0 < nb) {
+                            const float2 w = tw1[jj * li];
+                            uint32_t* o = bufB + LB::base(s0) + jj * L::SX1 + li;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                if (s0 + r < nb)
+                                    o[r * L::LDB1] = pack16<H>(cmul(make_float2(Dre[0][r], Dim[0][r]), w));
+                        }
+                    });
                 __syncthreads();
-                mfma_stage<31, H, false, NW, GPP>(bufB, N2, bufA, N2, N2, nb, tw2, bt2b, lane, wave);
+                // radix Rb; output k1 = jj + q Ra of columns n2 = c0 + s0 + r
+                // times the four-step twiddle w_N^(n2 k1), four columns in one
+                // 16-byte store (past the last column it lands in row padding)
+                mfma_tiles<R1b, H, NW, GPP, 0, R1a, L::SX1, LB>(
+                    bufB, nb, bt1b, lane, wave, [&](int, int, int sb, int jj, const f32x4* Dre, const f32x4* Dim) {
+                        const int s0 = sb * 16 + 4 * lg;
+                        if (li < R1b && s0 < nb) {
+                            const int k1 = jj + li * R1a;
+                            float2 w = tw_get<true>(twlo, twhi, (c0 + s0) * k1);
+                            const float2 st = twk[k1];
+                            uint32_t v[4];
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                v[r] = pack16<H>(cmul(make_float2(Dre[0][r], Dim[0][r]), w));
+                                w = cmul(w, st);
+                            }
+                            *(gwptr<uint4>)(G + k1 * GP + c0 + s0) = make_uint4(v[0], v[1], v[2], v[3]);
+                        }
+                    });
+            }
+        }
+        __syncthreads();   // the slices are complete and visible to every wave
+        // phase 2 per batch of rows k1 = r0 + t over every block
+        const int tc2 = opaque(tid) % N2, tr2 = opaque(tid) / N2;
+        for (int r0 = 0; r0 < N1; r0 += L::RB) {
+            const int nb = min(L::RB, N1 - r0);
+            float acc[P2][GPP][4];
+#pragma unroll
+            for (int i0 = 0; i0 < P2; ++i0)
+#pragma unroll
+                for (int i1 = 0; i1 < GPP; ++i1)
+#pragma unroll
+                    for (int i2 = 0; i2 < 4; ++i2) acc[i0][i1][i2] = 0.f;
+            for (int k = 0; k < q.K; ++k) {
+                // rows r0 .. r0 + nb - 1 of slice k into base(t) + n2 (the
+                // previous block's last stage reads only the other buffer)
+                const gptr<uint32_t> g = as_global(Gall + ((int64_t)k * N1 + r0) * GP + tc2);
+                if (tr2 < RPP2)
+                    for (int t = tr2; t < nb; t += 4 * RPP2) {
+                        uint32_t v[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int tt = t + u * RPP2;
+                            v[u] = g[(tt < nb ? tt : t) * GP];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int tt = t + u * RPP2;
+                            if (tt < nb) bufA[L2::base(tt) + tc2] = v[u];
+                        }
+                    }
+                __syncthreads();
+                mfma_tiles<31, H, NW, GPP, 0, 11, 11, L2>(
+                    bufA, nb, bt2a, lane, wave, [&](int, int, int sb, int jj, const f32x4* Dre, const f32x4* Dim) {
+                        const int s0 = sb * 16 + 4 * lg;
+                        if (s0 < nb) {
+#pragma unroll
+                            for (int qt = 0; qt < 2; ++qt) {
+                                const int qq = qt * 16 + li;
+                                if (qq < 31) {
+                                    const float2 w = tw2[jj * qq];
+                                    uint32_t* o = bufB + L2::base(s0) + jj * 31 + qq;
+#pragma unroll
+                                    for (int r = 0; r < 4; ++r)
+                                        if (s0 + r < nb)
+                                            o[r * L::LD2] = pack16<H>(cmul(make_float2(Dre[qt][r], Dim[qt][r]), w));
+                                }
+                            }
+                        }
+                    });
                 __syncthreads();
                 const float sc = q.block_inv[k * q.F + cls] * q.spec_inv;
-                int t = t0 % nb, k2 = t0 / nb;
-                const int dt = T % nb, dk = T / nb;
+                mfma_tiles<11, H, NW, GPP, P2, 31, 31, L2>(
+                    bufB, nb, bt2b, lane, wave, [&](int pass, int gi, int, int, const f32x4* Dre, const f32x4* Dim) {
 #pragma unroll
-                for (int i = 0; i < ACC; ++i) {
-                    if (k2 < N2) acc[i] += cabs_hw(unpack16<H>(bufA[t * N2 + k2])) * sc;
-                    t += dt;
-                    k2 += dk;
-                    if (t >= nb) {
-                        t -= nb;
-                        ++k2;
+                        for (int r = 0; r < 4; ++r) acc[pass][gi][r] += cabs_hw(make_float2(Dre[0][r], Dim[0][r])) * sc;
+                    });
+            }
+            // the batch's surface rows through LDS as (k2, seq), then to S in
+            // natural order m = r0 + seq + N1 k2
+            __syncthreads();
+            float* stage = reinterpret_cast<float*>(bufA);
+#pragma unroll
+            for (int pass = 0; pass < P2; ++pass)
+#pragma unroll
+                for (int gi = 0; gi < GPP; ++gi) {
+                    const int t = mfma_slot_tile<NW, GPP, 31>(pass, gi, nb, wave);
+                    const int sb = t / 31, jj = t - sb * 31;
+                    const int s0 = sb * 16 + 4 * lg;
+                    if (t >= 0 && li < 11) {
+                        float* o = stage + (jj + 31 * li) * L::LS + s0 + L2::skew(s0);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (s0 + r < nb) o[r] = acc[pass][gi][r];
                     }
                 }
-                __syncthreads();
-            }
+            __syncthreads();
             {
-                int t = tid % nb, k2 = tid / nb;
-                const int dt = T % nb, dk = T / nb;
-#pragma unroll
-                for (int i = 0; i < ACC; ++i) {
-                    if (k2 < N2) S[r0 + t + N1 * k2] = acc[i];
-                    t += dt;
-                    k2 += dk;
-                    if (t >= nb) {
-                        t -= nb;
-                        ++k2;
-                    }
-                }
+                const gwptr<float> Sg = as_global_w(S + r0);
+                walk_cols<4>(
+                    opaque(tid), T, nb, N2, [&](int t, int k2) { return stage[k2 * L::LS + t + L2::skew(t)]; },
+                    [&](int t, int k2, float v) { Sg[t + N1 * k2] = v; });
             }
+            __syncthreads();
         }
-        __syncthreads();
         row_stats_glob<T>(S, N, red, tid, q.rowstats + local, 1.0f);
         if (q.surfaces) {
             float* o = q.surfaces + (int64_t)local * N;
@@ -387,13 +508,17 @@ int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::ve
     for (const AcqPlan* pl : plans) max_k = std::max(max_k, (int)pl->K);
     c.max_k = max_k;
     const int grid = std::min(c.rows, cus);
-    UGSDR_HIP_TRY(scratch.reserve((size_t)grid * (max_k + 1) * p0.N * 4));
+    UGSDR_HIP_TRY(scratch.reserve(reduced_scratch_bytes(p0.N, max_k, grid)));
     c.scratch = scratch.as<float2>();
     int rc = p0.precision == UGSDR_PREC_FP16 ? launch_corr_mfma_n<Fp16Ops>(c, grid, st, threads)
                                              : launch_corr_mfma_n<Bf16Ops>(c, grid, st, threads);
     if (rc) return rc;
     UGSDR_HIP_TRY(hipGetLastError());
     return UGSDR_OK;
+}
+
+size_t reduced_scratch_bytes(int64_t N, int max_k, int grid) {
+    return (size_t)red_wg_words(N, max_k) * 4 * (size_t)grid;
 }
 
 bool reduced_supported(int64_t N, int64_t L) {
