@@ -34,6 +34,14 @@ extern "C" {
 #define UGSDR_FMT_CI16 2 /* interleaved int16 I/Q  (Iq_16_plus_16) */
 #define UGSDR_FMT_CF32 3 /* interleaved float I/Q                  */
 #define UGSDR_FMT_R8 4   /* real int8 (Real_8), imaginary part 0   */
+/* packed recordings (FileType BbpDdc and Nt1065Grabber*; decoders
+ * src/helpers/BbpPackedSpan.hpp:9-86 and src/helpers/NtlabPackedSpan.hpp:9-99,
+ * used by SignalParametersBase::GetPartialSignal, src/signal_parameters.hpp:140-237) */
+#define UGSDR_FMT_BBP_DDC 5  /* epochs: 8-byte header (ms counter) + 2+2-bit complex samples */
+#define UGSDR_FMT_NT1065_1 6 /* one byte per sample, channel in bits 6-7 (Nt1065GrabberFirst) */
+#define UGSDR_FMT_NT1065_2 7 /* bits 4-5 (Nt1065GrabberSecond) */
+#define UGSDR_FMT_NT1065_3 8 /* bits 2-3 (Nt1065GrabberThird)  */
+#define UGSDR_FMT_NT1065_4 9 /* bits 0-1 (Nt1065GrabberFourth) */
 
 /* arithmetic of the correlation surfaces */
 #define UGSDR_PREC_FP32 0 /* plain: FP32 complex like the reference            */
@@ -119,7 +127,7 @@ int ugsdr_hip_replica_get(ugsdr_hip_ctx* ctx, uint64_t key, float* spectrum_c32,
 
 /* ------------------------------------------------------- fused acquisition */
 typedef struct {
-    const void* iq;        /* samples at fs_in in `fmt`                              */
+    const void* iq;        /* samples at fs_in in `fmt` (BBP_DDC: from an epoch header) */
     int32_t fmt;           /* UGSDR_FMT_*                                            */
     int32_t iq_on_device;  /* 1: iq is a device pointer, 0: host pointer             */
     int64_t n_samples;     /* complex samples available at iq                        */
@@ -166,6 +174,18 @@ typedef struct {
  * n_prn x surface_len, the output_peak of the selected bin. */
 int ugsdr_hip_acquire(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* req, ugsdr_acq_result* out, float* surfaces,
                       float* peaks);
+
+/* ------------------------------------------------------ recorded formats
+ * Bytes that hold n samples of fmt at fs (BBP_DDC: whole epochs, headers
+ * included, epoch length as in the file geometry of
+ * src/signal_parameters.hpp:74-80; fs must be a whole number of kHz), or a
+ * negative UGSDR_ERR_* code. */
+int64_t ugsdr_format_bytes(int fmt, int64_t n_samples, double fs);
+/* SignalParametersBase::GetPartialSignal from the start of a recording
+ * (src/signal_parameters.hpp:140-237): decodes n samples of fmt into complex
+ * float (host buffers).  BBP_DDC first checks the epoch ms counters like
+ * VerifyHeaders (signal_parameters.hpp:36-45; UGSDR_ERR_ARG on a gap). */
+int ugsdr_hip_unpack(ugsdr_hip_ctx* ctx, const void* in, int fmt, int64_t n, double fs, float* out_c32);
 
 /* ---------------------------------------- per-operator drop-ins (host data)
  * Slow path behind the Hip* CRTP classes: one call = one reference operator

@@ -203,6 +203,55 @@ double refcpu_adjust_sampling_rate(double fs, double target) {
     return fs;
 }
 
+/* ============================================================== formats == */
+int64_t refcpu_bbp_epoch_bytes(double fs) {             /* signal_parameters.hpp:74-80 */
+    double words = fs / 1e3 * 4 / 64 + 1;   /* 2+2 bits a sample, 64-bit words, header */
+    int64_t bytes = (int64_t)(words / 2) * 2;
+    if (fmod(words, 2.0) != 0.0) bytes += 2;
+    return bytes * 8;
+}
+
+int refcpu_bbp_verify_headers(const uint8_t* data, int64_t n_epochs, int64_t epoch_bytes) {
+    uint64_t last, cur;                                 /* signal_parameters.hpp:36-45 */
+    memcpy(&last, data, 8);
+    last &= (1u << 24) - 1;
+    for (int64_t i = 1; i < n_epochs; ++i) {
+        memcpy(&cur, data + i * epoch_bytes, 8);
+        cur &= (1u << 24) - 1;
+        if (cur - last != 1) return -1;
+        last = cur;
+    }
+    return 0;
+}
+
+/* one byte = two samples of signed 2-bit fields, the first declared field in
+ * the low bits (BbpPackedSpan.hpp:11-19) */
+struct rc_bbp_packed {
+    int8_t im_0 : 2;
+    int8_t re_0 : 2;
+    int8_t im_1 : 2;
+    int8_t re_1 : 2;
+};
+
+void refcpu_unpack_bbp(const uint8_t* data, int64_t n, int64_t spm, int64_t epoch_bytes, float* out) {
+    for (int64_t i = 0; i < n; ++i) {                   /* signal_parameters.hpp:225-236 */
+        int64_t e = i / spm, j = i % spm;
+        struct rc_bbp_packed pk;
+        memcpy(&pk, data + e * epoch_bytes + 8 + j / 2, 1);
+        int re = (j & 1) ? pk.re_1 : pk.re_0, im = (j & 1) ? pk.im_1 : pk.im_0;
+        out[2 * i] = (float)(re | 1);                   /* BbpPackedSpan.hpp:16-18 */
+        out[2 * i + 1] = (float)(im | 1);
+    }
+}
+
+void refcpu_unpack_nt1065(const uint8_t* data, int64_t n, int offset_bits, float* out) {
+    for (int64_t i = 0; i < n; ++i) {                   /* NtlabPackedSpan.hpp:10-21 */
+        int sign = (data[i] >> offset_bits) & 1, magnitude = (data[i] >> (offset_bits + 1)) & 1;
+        out[2 * i] = (float)((2 * sign - 1) * (1 + 2 * magnitude));
+        out[2 * i + 1] = 0.f;
+    }
+}
+
 /* ================================================================ mixer == */
 static double rc_fix_phase(double phase) {               /* mixer.hpp:20-25 */
     phase = fmod(phase, 2 * M_PI);

@@ -77,6 +77,23 @@ size_t refcpu_resample_c(const float* src, size_t n, size_t new_fs, size_t old_f
 /* FastSearchEngineBase::AdjustSamplingRate (src/acquisition/fse.hpp:222-240). */
 double refcpu_adjust_sampling_rate(double fs, double target);
 
+/* -------------------------------------------------------------- formats -- */
+/* Recorded sample formats of SignalParametersBase (src/signal_parameters.hpp:
+ * 23-32 FileType).  Pinned by bit-field layouts built independently with
+ * ctypes, the reference's Nt1065 first-sample KAT
+ * (test/integration_tests.cpp:144-176) and its BbpDdc recordings in data/. */
+/* BbpDdc epoch length in bytes at fs (signal_parameters.hpp:74-80). */
+int64_t refcpu_bbp_epoch_bytes(double fs);
+/* VerifyHeaders (signal_parameters.hpp:36-45): 0 when the 24-bit ms counters
+ * of consecutive epochs step by one, -1 otherwise. */
+int refcpu_bbp_verify_headers(const uint8_t* data, int64_t n_epochs, int64_t epoch_bytes);
+/* GetPartialSignal for BbpDdc from epoch 0 (signal_parameters.hpp:225-236
+ * with PackedSpan, src/helpers/BbpPackedSpan.hpp:9-86): n complex samples. */
+void refcpu_unpack_bbp(const uint8_t* data, int64_t n, int64_t spm, int64_t epoch_bytes, float* out);
+/* NtlabPackedSpan<offset> (src/helpers/NtlabPackedSpan.hpp:9-99): n samples,
+ * real value in the real part. */
+void refcpu_unpack_nt1065(const uint8_t* data, int64_t n, int offset_bits, float* out);
+
 /* ---------------------------------------------------------------- mixer -- */
 /* Mixer::Translate const overload (src/mixer/mixer.hpp:43-51) with the
  * IppMixer tone (src/mixer/ipp_mixer.hpp:83-88): f<0 folds to fs+f, phase is

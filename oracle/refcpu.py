@@ -62,6 +62,7 @@ P = np.ctypeslib.ndpointer
 F32 = P(dtype=np.float32, flags="C_CONTIGUOUS")
 F64 = P(dtype=np.float64, flags="C_CONTIGUOUS")
 I8 = P(dtype=np.int8, flags="C_CONTIGUOUS")
+U8 = P(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
 def _declare(L):
@@ -70,6 +71,12 @@ def _declare(L):
         getattr(L, name).restype = C.c_int
     L.refcpu_glonass_of.argtypes = [I8]
     L.refcpu_boc_expand.argtypes = [I8, C.c_size_t, I8]
+    L.refcpu_bbp_epoch_bytes.argtypes = [C.c_double]
+    L.refcpu_bbp_epoch_bytes.restype = C.c_int64
+    L.refcpu_bbp_verify_headers.argtypes = [U8, C.c_int64, C.c_int64]
+    L.refcpu_bbp_verify_headers.restype = C.c_int
+    L.refcpu_unpack_bbp.argtypes = [U8, C.c_int64, C.c_int64, C.c_int64, F32]
+    L.refcpu_unpack_nt1065.argtypes = [U8, C.c_int64, C.c_int, F32]
     L.refcpu_repeat_upsample.argtypes = [I8, C.c_size_t, C.c_size_t, F32, C.c_size_t]
     L.refcpu_upsample_f.argtypes = [F32, C.c_size_t, F32, C.c_size_t]
     L.refcpu_upsample_hold_c.argtypes = [F32, C.c_size_t, F32, C.c_size_t]
@@ -147,6 +154,34 @@ def boc_expand(chips: np.ndarray) -> np.ndarray:
     out = np.zeros(2 * len(chips), np.int8)
     lib().refcpu_boc_expand(chips, len(chips), out)
     return out
+
+
+def bbp_epoch_bytes(fs: float) -> int:
+    return int(lib().refcpu_bbp_epoch_bytes(fs))
+
+
+def bbp_verify_headers(data: np.ndarray, n_epochs: int, epoch_bytes: int) -> bool:
+    data = np.ascontiguousarray(data, np.uint8)
+    assert data.size >= n_epochs * epoch_bytes
+    return lib().refcpu_bbp_verify_headers(data, n_epochs, epoch_bytes) == 0
+
+
+def unpack_bbp(data: np.ndarray, n: int, fs: float) -> np.ndarray:
+    """GetPartialSignal for BbpDdc: n complex samples from epoch 0"""
+    data = np.ascontiguousarray(data, np.uint8)
+    spm, eb = int(round(fs / 1e3)), bbp_epoch_bytes(fs)
+    assert data.size >= -(-n // spm) * eb
+    out = np.zeros(2 * n, np.float32)
+    lib().refcpu_unpack_bbp(data, n, spm, eb, out)
+    return f2c(out)
+
+
+def unpack_nt1065(data: np.ndarray, n: int, offset_bits: int) -> np.ndarray:
+    data = np.ascontiguousarray(data, np.uint8)
+    assert data.size >= n
+    out = np.zeros(2 * n, np.float32)
+    lib().refcpu_unpack_nt1065(data, n, offset_bits, out)
+    return f2c(out)
 
 
 def repeat_upsample(code: np.ndarray, repeats: int, samples: int) -> np.ndarray:

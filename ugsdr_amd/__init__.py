@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 HIP_LIB_PATH = os.path.join(HERE, "libugsdr_hip.so")
 
 FMT_CI8, FMT_CI16, FMT_CF32, FMT_R8 = 1, 2, 3, 4
+FMT_BBP_DDC, FMT_NT1065_1, FMT_NT1065_2, FMT_NT1065_3, FMT_NT1065_4 = 5, 6, 7, 8, 9
+PACKED_FORMATS = (FMT_BBP_DDC, FMT_NT1065_1, FMT_NT1065_2, FMT_NT1065_3, FMT_NT1065_4)
 PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
 BANK_REF_SPLIT, BANK_NCO = 0, 1
 
@@ -93,6 +95,8 @@ SIGNATURES = {
     "ugsdr_hip_set_graphs": ([VP, C.c_int], C.c_int),
     "ugsdr_hip_graph_replays": ([VP], I64),
     "ugsdr_hip_set_threads": ([VP, I64, I32], C.c_int),
+    "ugsdr_format_bytes": ([C.c_int, I64, D], I64),
+    "ugsdr_hip_unpack": ([VP, VP, C.c_int, I64, D, VP], C.c_int),
     "ugsdr_code_gps_l1ca": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5i": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5q": ([C.c_int, VP], C.c_int),
@@ -192,6 +196,14 @@ def adjust_sampling_rate(fs: float, target: float) -> float:
 
 def resample_len(n: int, new_fs: int, old_fs: int) -> int:
     return lib().ugsdr_resample_len(n, new_fs, old_fs)
+
+
+def format_bytes(fmt: int, n_samples: int, fs: float) -> int:
+    """bytes holding n samples of a format (BBP_DDC: whole epochs with headers)"""
+    b = lib().ugsdr_format_bytes(fmt, n_samples, fs)
+    if b < 0:
+        _check(b)
+    return int(b)
 
 
 def results_to_dicts(out) -> list:
@@ -332,7 +344,9 @@ class Context:
             arr = np.ascontiguousarray(iq)
             req.iq = arr.ctypes.data
             req.iq_on_device = 0
-            per = {FMT_CI8: 2, FMT_CI16: 2, FMT_CF32: 2, FMT_R8: 1}[fmt]
+            if fmt in PACKED_FORMATS and n_samples is None:
+                raise ValueError("packed recordings need n_samples")
+            per = {FMT_CI8: 2, FMT_CI16: 2, FMT_CF32: 2, FMT_R8: 1}.get(fmt, 1)
             req.n_samples = int(n_samples if n_samples is not None else arr.size // per
                                 if arr.dtype != np.complex64 else arr.size)
             hold = arr
@@ -365,6 +379,18 @@ class Context:
     def acquire(self, iq, fmt: int, want_surfaces: bool = False, want_peaks: bool = False, **kw):
         req, hold = self.make_request(iq, fmt, **kw)
         return self.acquire_req(req, want_surfaces, want_peaks)
+
+    # ---- recorded formats
+    def unpack(self, raw, fmt: int, n: int, fs: float) -> np.ndarray:
+        """SignalParametersBase::GetPartialSignal from the start of a recording:
+        n samples of fmt (bytes in raw) decoded on the device to complex64"""
+        raw = np.ascontiguousarray(raw)
+        need = format_bytes(fmt, n, fs)
+        if raw.nbytes < need:
+            raise ValueError(f"{n} samples of format {fmt} need {need} bytes, got {raw.nbytes}")
+        out = np.zeros(n, np.complex64)
+        _check(lib().ugsdr_hip_unpack(self._p, raw.ctypes.data, fmt, n, fs, _ptr(out)))
+        return out
 
     # ---- per-operator drop-ins
     def mix(self, x, fs: float, f: float, phase: float = 0.0) -> np.ndarray:

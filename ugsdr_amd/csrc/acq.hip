@@ -28,6 +28,7 @@
 #include "plan.hpp"
 #include "internal.hpp"
 #include "acq.hpp"
+#include "formats.hpp"
 #include "corr_common.hpp"
 
 using namespace ugsdr_dev;
@@ -36,8 +37,7 @@ namespace ugsdr_hip {
 
 // ------------------------------------------------------------------ front end
 struct FrontArgs {
-    const void* iq;
-    int fmt;
+    SampleSource src;  // the request's samples in their recorded format
     int64_t n_in;
     int64_t n_out;
     double f_turns;   // mixing frequency / fs_in (cycles per sample), 0 = no tone
@@ -47,21 +47,6 @@ struct FrontArgs {
     int64_t down;     // lcm / fs_acq
     float2* out;
 };
-
-__device__ __forceinline__ float2 load_iq(const void* p, int fmt, int64_t i) {
-    switch (fmt) {
-        case UGSDR_FMT_CI8: {
-            const char2 v = static_cast<const char2*>(p)[i];
-            return make_float2((float)v.x, (float)v.y);
-        }
-        case UGSDR_FMT_CI16: {
-            const short2 v = static_cast<const short2*>(p)[i];
-            return make_float2((float)v.x, (float)v.y);
-        }
-        case UGSDR_FMT_CF32: return static_cast<const float2*>(p)[i];
-        default: return make_float2((float)static_cast<const signed char*>(p)[i], 0.f);
-    }
-}
 
 // e^{2 pi i t} for a phase t in cycles (FP64), reduced before the FP32 sincos
 __device__ __forceinline__ float2 phasor_turns(double t) {
@@ -75,7 +60,7 @@ __global__ void front_kernel(FrontArgs a) {
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < a.n_out;
          o += (int64_t)gridDim.x * blockDim.x) {
         if (a.down == 1 && a.up == 1) {
-            float2 v = load_iq(a.iq, a.fmt, o);
+            float2 v = load_sample(a.src, o);
             if (a.apply_tone) v = cmul(v, phasor_turns(fma((double)o, a.f_turns, a.ph_turns)));
             a.out[o] = v;
         } else {
@@ -84,7 +69,7 @@ __global__ void front_kernel(FrontArgs a) {
             for (int64_t u = u0; u < u0 + a.down; ++u) {
                 const int64_t i = u / a.up;
                 if (i >= a.n_in) break;
-                float2 v = load_iq(a.iq, a.fmt, i);
+                float2 v = load_sample(a.src, i);
                 if (a.apply_tone) v = cmul(v, phasor_turns(fma((double)i, a.f_turns, a.ph_turns)));
                 sr += v.x;
                 si += v.y;
@@ -642,7 +627,7 @@ static void doppler_classes(const double* dop, int64_t D, int64_t N, double fs, 
 static int64_t gcd64(int64_t a, int64_t b) { while (b) { int64_t t = a % b; a = b; b = t; } return a; }
 
 int64_t AcqPlan::footprint(bool surfaces) const {
-    int64_t b = n_in_used * (int64_t)bps + need * 8 + K * F * N * 8 + (int64_t)table_bytes + P * D * 32 +
+    int64_t b = in_bytes + need * 8 + K * F * N * 8 + (int64_t)table_bytes + P * D * 32 +
                 P * (int64_t)sizeof(ugsdr_acq_result) + P * L * 4;
     if (!small) b += K * F * N * 8;
     if (surfaces) b += P * D * L * 4;
@@ -653,7 +638,7 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
     pl.precision = r ? r->precision : UGSDR_PREC_FP32;
     UGSDR_CHECK_ARG(r, "null request");
     UGSDR_CHECK_ARG(r->iq && r->n_samples > 0, "empty signal");
-    UGSDR_CHECK_ARG(r->fmt >= UGSDR_FMT_CI8 && r->fmt <= UGSDR_FMT_R8, "unknown sample format");
+    if (int rc = format_geometry(r->fmt, r->fs_in, pl.spm_in, pl.epoch_bytes)) return rc;
     UGSDR_CHECK_ARG(r->n_fft > 0 && r->n_blocks > 0 && r->surface_len > 0, "bad sizes");
     UGSDR_CHECK_ARG(r->n_doppler > 0 && r->doppler, "no Doppler bins");
     UGSDR_CHECK_ARG(r->n_prn > 0 && r->replica_keys, "no replicas");
@@ -712,7 +697,11 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
     }
     pl.fmt = r->fmt;
     pl.n_in_used = std::min<int64_t>(r->n_samples, (pl.need * pl.down + pl.up - 1) / pl.up + 1);
-    pl.bps = r->fmt == UGSDR_FMT_CI8 ? 2 : r->fmt == UGSDR_FMT_CI16 ? 4 : r->fmt == UGSDR_FMT_CF32 ? 8 : 1;
+    if (pl.fmt == UGSDR_FMT_BBP_DDC && pl.n_in_used >= ((int64_t)1 << 31)) {
+        set_error("BBP_DDC requests take at most 2^31 - 1 samples");
+        return UGSDR_ERR_SIZE;
+    }
+    pl.in_bytes = format_bytes(pl.fmt, pl.n_in_used, pl.spm_in, pl.epoch_bytes);
     double f = r->mix_frequency;
     if (f < 0) f = r->fs_in + f;  // Mixer::Translate folding (mixer.hpp:46-47)
     double ph = std::fmod(r->mix_phase, 2 * M_PI);
@@ -752,8 +741,10 @@ int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hip
     // input
     const void* iq = r->iq;
     if (!r->iq_on_device) {
-        UGSDR_HIP_TRY(s.iq.reserve(pl.n_in_used * pl.bps));
-        UGSDR_HIP_TRY(hipMemcpyAsync(s.iq.ptr, r->iq, pl.n_in_used * pl.bps, hipMemcpyHostToDevice, st));
+        if (pl.fmt == UGSDR_FMT_BBP_DDC)
+            if (int rc = verify_bbp_headers(r->iq, pl.in_bytes / pl.epoch_bytes, pl.epoch_bytes)) return rc;
+        UGSDR_HIP_TRY(s.iq.reserve(pl.in_bytes));
+        UGSDR_HIP_TRY(hipMemcpyAsync(s.iq.ptr, r->iq, pl.in_bytes, hipMemcpyHostToDevice, st));
         iq = s.iq.ptr;
     }
     UGSDR_HIP_TRY(s.sig.reserve((size_t)pl.need * sizeof(float2)));
@@ -767,8 +758,10 @@ int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hip
 // the front-end kernel alone (its tables and host IQ already uploaded)
 int launch_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st) {
     FrontArgs fa{};
-    fa.iq = r->iq_on_device ? r->iq : s.iq.ptr;
-    fa.fmt = pl.fmt;
+    fa.src.p = r->iq_on_device ? r->iq : s.iq.ptr;
+    fa.src.fmt = pl.fmt;
+    fa.src.spm = pl.spm_in;
+    fa.src.epoch_bytes = pl.epoch_bytes;
     fa.n_in = pl.n_in_used;
     fa.n_out = pl.need;
     fa.apply_tone = pl.apply_tone;
@@ -973,8 +966,12 @@ int autotune_corr(ugsdr_hip_ctx* ctx, int64_t N, int* best, std::vector<float>* 
         if (best) *best = cands[0];
         return UGSDR_OK;
     }
-    // synthetic batch that fills the device: 32 PRNs x 32 Doppler bins, 1 block
+    // synthetic batch that fills the device: 32 PRNs x 32 Doppler bins, 1
+    // block; the four-step lengths take 4 blocks, since how their variants
+    // rank depends on the blocks a row accumulates (N = 81840: 512 threads win
+    // a 1-block batch, 1024 a 5-block search by 1.5x)
     const int P = 32, D = 32;
+    const int K = N > kLdsCap ? 4 : 1;
     uint64_t x = 0x9E3779B97F4A7C15ull ^ (uint64_t)N;
     auto next = [&x]() {
         x ^= x << 13;
@@ -982,7 +979,7 @@ int autotune_corr(ugsdr_hip_ctx* ctx, int64_t N, int* best, std::vector<float>* 
         x ^= x << 17;
         return x;
     };
-    std::vector<float> iq(2 * (size_t)N);
+    std::vector<float> iq(2 * (size_t)N * K);
     for (auto& v : iq) v = (float)((int)(next() % 255) - 127);
     std::vector<uint64_t> keys;
     std::vector<int8_t> chips(1023);
@@ -1007,10 +1004,10 @@ int autotune_corr(ugsdr_hip_ctx* ctx, int64_t N, int* best, std::vector<float>* 
     ugsdr_acq_request r{};
     r.iq = iq.data();
     r.fmt = UGSDR_FMT_CF32;
-    r.n_samples = N;
+    r.n_samples = N * K;
     r.fs_in = r.fs_acq = fs;
     r.n_fft = N;
-    r.n_blocks = 1;
+    r.n_blocks = K;
     r.surface_len = N;
     r.reshape = 1;
     r.coherent = 1;
@@ -1130,8 +1127,11 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
     const bool graphs = gc.enabled && !ctx->profile;
     const std::string key = graphs ? graph_key(r, surfaces != nullptr, peaks != nullptr, threads) : std::string();
     if (graphs && gc.exec && key == gc.key) {
-        if (!r->iq_on_device)
-            UGSDR_HIP_TRY(hipMemcpyAsync(s.iq.ptr, r->iq, pl.n_in_used * pl.bps, hipMemcpyHostToDevice, st));
+        if (!r->iq_on_device) {
+            if (pl.fmt == UGSDR_FMT_BBP_DDC)
+                if (int rc = verify_bbp_headers(r->iq, pl.in_bytes / pl.epoch_bytes, pl.epoch_bytes)) return rc;
+            UGSDR_HIP_TRY(hipMemcpyAsync(s.iq.ptr, r->iq, pl.in_bytes, hipMemcpyHostToDevice, st));
+        }
         UGSDR_HIP_TRY(hipGraphLaunch(gc.exec, st));
         ++gc.replays;
     } else if (graphs && key == gc.pending) {

@@ -25,7 +25,8 @@ struct AcqPlan {
     // front end
     int fmt = 0;
     int64_t up = 1, down = 1, need = 0, n_in_used = 0;
-    size_t bps = 0;
+    int spm_in = 0, epoch_bytes = 0;  // recorded-format geometry (BBP_DDC epochs)
+    int64_t in_bytes = 0;             // bytes of the n_in_used input samples
     int apply_tone = 0;
     double f_turns = 0, ph_turns = 0;
     // result scaling

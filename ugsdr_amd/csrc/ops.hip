@@ -10,6 +10,7 @@
 
 #include "ctx.hpp"
 #include "dev_fft.hpp"
+#include "formats.hpp"
 #include "internal.hpp"
 
 using namespace ugsdr_dev;
@@ -246,6 +247,56 @@ static int mix_common(ugsdr_hip_ctx* ctx, const float* in, float* out, int64_t n
     return d2h(ctx, out, ctx->aux.ptr, n * 8);
 }
 
+// ---------------------------------------------------------------- formats
+__global__ void unpack_kernel(SampleSource src, float2* out, int64_t n) {
+    GRID_STRIDE(i, n) out[i] = load_sample(src, i);
+}
+
+int format_geometry(int fmt, double fs, int& spm, int& epoch_bytes) {
+    spm = 0;
+    epoch_bytes = 0;
+    if (fmt < UGSDR_FMT_CI8 || fmt > UGSDR_FMT_NT1065_4) {
+        set_error("unknown sample format");
+        return UGSDR_ERR_ARG;
+    }
+    if (fmt != UGSDR_FMT_BBP_DDC) return UGSDR_OK;
+    const double per_ms = fs / 1e3;
+    if (!(per_ms >= 2 && per_ms < (1 << 26)) || per_ms != std::floor(per_ms)) {
+        set_error("BBP_DDC needs a sampling rate of a whole number of kHz");
+        return UGSDR_ERR_ARG;
+    }
+    spm = (int)per_ms;
+    // words = spm * 4 / 64 + 1 (two 2-bit components a sample, one header
+    // word), rounded up to an even count (signal_parameters.hpp:74-80)
+    epoch_bytes = 16 * ((spm + 16 + 31) / 32);
+    return UGSDR_OK;
+}
+
+int64_t format_bytes(int fmt, int64_t n, int spm, int epoch_bytes) {
+    switch (fmt) {
+        case UGSDR_FMT_CI8: return 2 * n;
+        case UGSDR_FMT_CI16: return 4 * n;
+        case UGSDR_FMT_CF32: return 8 * n;
+        case UGSDR_FMT_BBP_DDC: return (n + spm - 1) / spm * (int64_t)epoch_bytes;
+        default: return n;  // R8, NT1065_*
+    }
+}
+
+int verify_bbp_headers(const void* host, int64_t n_epochs, int epoch_bytes) {
+    const unsigned char* b = static_cast<const unsigned char*>(host);
+    auto counter = [&](int64_t e) {
+        uint64_t w;
+        memcpy(&w, b + e * epoch_bytes, 8);
+        return w & ((1u << 24) - 1);
+    };
+    for (int64_t e = 1; e < n_epochs; ++e)
+        if (counter(e) - counter(e - 1) != 1) {
+            set_error("Header ms count mismatch");  // signal_parameters.hpp:41-42
+            return UGSDR_ERR_ARG;
+        }
+    return UGSDR_OK;
+}
+
 }  // namespace ugsdr_hip
 
 using namespace ugsdr_hip;
@@ -441,6 +492,36 @@ int ugsdr_hip_correlate(ugsdr_hip_ctx* ctx, const float* signal, int64_t n_signa
                        n_signal, ctx->rows.as<float>());
     UGSDR_HIP_TRY(hipGetLastError());
     return d2h(ctx, out, ctx->rows.ptr, 8);
+}
+
+
+int64_t ugsdr_format_bytes(int fmt, int64_t n_samples, double fs) {
+    int spm, eb;
+    if (int rc = format_geometry(fmt, fs, spm, eb)) return rc;
+    if (n_samples < 0) {
+        set_error("negative sample count");
+        return UGSDR_ERR_ARG;
+    }
+    return format_bytes(fmt, n_samples, spm, eb);
+}
+
+int ugsdr_hip_unpack(ugsdr_hip_ctx* ctx, const void* in, int fmt, int64_t n, double fs, float* out) {
+    UGSDR_CHECK_ARG(ctx && in && out && n >= 0, "bad arguments");
+    UGSDR_CHECK_ARG(n < ((int64_t)1 << 31), "at most 2^31 - 1 samples a call");
+    SampleSource src{};
+    if (int rc = format_geometry(fmt, fs, src.spm, src.epoch_bytes)) return rc;
+    src.fmt = fmt;
+    const int64_t bytes = format_bytes(fmt, n, src.spm, src.epoch_bytes);
+    if (fmt == UGSDR_FMT_BBP_DDC)
+        if (int rc = verify_bbp_headers(in, bytes / src.epoch_bytes, src.epoch_bytes)) return rc;
+    Lock lk(ctx);
+    UGSDR_HIP_TRY(hipSetDevice(ctx->device));
+    if (int rc = h2d(ctx, ctx->aux, in, bytes)) return rc;
+    UGSDR_HIP_TRY(ctx->aux2.reserve(n * 8));
+    src.p = ctx->aux.ptr;
+    hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(n)), dim3(256), 0, ctx->stream, src, ctx->aux2.as<float2>(), n);
+    UGSDR_HIP_TRY(hipGetLastError());
+    return d2h(ctx, out, ctx->aux2.ptr, n * 8);
 }
 
 }  // extern "C"

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "acq.hpp"
+#include "formats.hpp"
 #include "corr_common.hpp"
 #include "ctx.hpp"
 #include "internal.hpp"
@@ -128,7 +129,7 @@ static int launch_round(Server& s, Round& rd) {
         ServerSlot& sl = s.slots[j.slot];
         ugsdr_acq_request r = j.req;
         if (!r.iq_on_device) {
-            const size_t bytes = (size_t)j.plan.n_in_used * j.plan.bps;
+            const size_t bytes = (size_t)j.plan.in_bytes;
             UGSDR_HIP_TRY(sl.iq_stage.reserve(bytes));
             std::memcpy(sl.iq_stage.ptr, j.iq.data(), bytes);
             r.iq = sl.iq_stage.ptr;  // pinned: the async copy does not block this thread
@@ -345,7 +346,10 @@ int ugsdr_hip_server_submit(ugsdr_hip_server* h, const ugsdr_acq_request* req, u
     j->req.doppler = j->doppler.data();
     j->req.replica_keys = j->keys.data();
     if (!req->iq_on_device) {
-        const size_t bytes = (size_t)j->plan.n_in_used * j->plan.bps;
+        const size_t bytes = (size_t)j->plan.in_bytes;
+        if (j->plan.fmt == UGSDR_FMT_BBP_DDC)
+            if (int rc = verify_bbp_headers(req->iq, j->plan.in_bytes / j->plan.epoch_bytes, j->plan.epoch_bytes))
+                return rc;
         j->iq.assign(static_cast<const char*>(req->iq), static_cast<const char*>(req->iq) + bytes);
         j->req.iq = j->iq.data();
     }
