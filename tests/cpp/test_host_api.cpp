@@ -1,8 +1,11 @@
 // C++20 host surface test: the reference's unit KATs through the Hip* CRTP
-// backends, and the reference's real_8 acquisition KAT through both the
+// backends, the reference's real_8 acquisition KAT through both the
 // per-operator FastSearchEngine<HipFseConfig> loop and the fused
-// HipFastSearchEngine.  argv[1]: path of tests/golden/real_8_first4ms.bin.
+// HipFastSearchEngine, and the fused engine straight from the packed BbpDdc
+// recording.  argv[1]: tests/golden/real_8_first4ms.bin; argv[2]:
+// tests/golden/bbp_ddc_gps_L1_first4ms.bin.
 #include <cmath>
+#include <cstdint>
 #include <complex>
 #include <cstdio>
 #include <fstream>
@@ -131,14 +134,63 @@ static int real8_kat(const char* path) {
     return 0;
 }
 
+static int bbp_recording(const char* path) {
+    const double fs = 33.25e6;
+    const int64_t spm = 33250;
+    const int64_t eb = ugsdr_format_bytes(UGSDR_FMT_BBP_DDC, spm, fs);
+    EXPECT(eb == 16640);
+    std::ifstream f(path, std::ios::binary);
+    std::vector<uint8_t> raw(4 * (std::size_t)eb);
+    f.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)raw.size());
+    EXPECT(f.gcount() == (std::streamsize)raw.size());
+    auto x = ugsdr_hip::GetPartialSignal(raw.data(), UGSDR_FMT_BBP_DDC, 4 * spm, fs);
+    // the first two samples of each epoch: sign bits 3/1 and 7/5 of the byte
+    // after the 8-byte header (src/helpers/BbpPackedSpan.hpp:11-19)
+    for (int e = 0; e < 4; ++e) {
+        const uint8_t b = raw[(std::size_t)e * eb + 8];
+        EXPECT(x[e * spm] == std::complex<float>((b & 0x08) ? -1.f : 1.f, (b & 0x02) ? -1.f : 1.f));
+        EXPECT(x[e * spm + 1] == std::complex<float>((b & 0x80) ? -1.f : 1.f, (b & 0x20) ? -1.f : 1.f));
+    }
+    auto bad = raw;
+    bad[2 * (std::size_t)eb] ^= 1;
+    bool threw = false;
+    try {
+        (void)ugsdr_hip::GetPartialSignal(bad.data(), UGSDR_FMT_BBP_DDC, 4 * spm, fs);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    // fused engine on the packed bytes with the default FseConfig rate rule
+    // (33.25 MHz -> 6.65 MHz, 4 ms coherent: N = 26600 = 2^3 5^2 7 19)
+    std::vector<ugsdr::Sv> svs;
+    std::vector<std::vector<int8_t>> codes;
+    for (int s = 0; s < 32; ++s) {
+        svs.push_back({s, ugsdr::System::Gps, ugsdr::Signal::GpsCoarseAcquisition_L1});
+        codes.push_back(l1ca(s));
+    }
+    ugsdr::HipFastSearchEngine engine(6e3, 125);
+    auto found = engine.Process(raw.data(), UGSDR_FMT_BBP_DDC, 4 * spm, fs, 0.0, svs, codes);
+    const std::map<int, double> strong = {{8, -500.0}, {22, 1500.0}};
+    int match = 0;
+    for (auto& r : found) {
+        std::printf("bbp: sv %d doppler %.0f snr %.2f\n", r.sv_number.id, r.doppler, r.GetSnr());
+        auto it = strong.find(r.sv_number.id);
+        if (it != strong.end() && std::abs(r.doppler - it->second) <= 250) ++match;
+    }
+    EXPECT(match == 2);
+    std::puts("bbp recording ok");
+    return 0;
+}
+
 int main(int argc, char** argv) {
-    if (argc < 2) {
-        std::fprintf(stderr, "usage: %s real_8_first4ms.bin\n", argv[0]);
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s real_8_first4ms.bin bbp_ddc_gps_L1_first4ms.bin\n", argv[0]);
         return 2;
     }
     try {
         if (int rc = unit_kats()) return rc;
         if (int rc = real8_kat(argv[1])) return rc;
+        if (int rc = bbp_recording(argv[2])) return rc;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "exception: %s\n", e.what());
         return 1;

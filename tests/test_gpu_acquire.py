@@ -172,7 +172,7 @@ def test_real8_kat(ctx, oracle):
         ctx.replica_drop(k)
 
 
-@pytest.mark.parametrize("n_fft", [1000, 2310, 20000])
+@pytest.mark.parametrize("n_fft", [1000, 2310, 4750, 20000, 33250])
 def test_runtime_plan_lengths(ctx, oracle, n_fft):
     """lengths without a compile-time plan (runtime radix plan / four-step)"""
     rng = np.random.default_rng(n_fft)

@@ -1,7 +1,8 @@
 """The C++20 host surface (ugsdr_amd/host/ugsdr_hip/*.hpp): the CRTP bases and
 Hip* backends compile against include/ugsdr_hip.h (CPU), and the C++ test
 program passes the reference's unit KATs and the real_8 acquisition KAT on
-the GPU through both the per-operator and the fused engine."""
+the GPU through both the per-operator and the fused engine, and finds the
+strong satellites of the packed BbpDdc recording through the fused engine."""
 import os
 import subprocess
 
@@ -44,7 +45,8 @@ def test_reference_bases_mode_compiles():
 def test_host_api_on_gpu():
     print("case C++ host API (unit KATs + real_8 per-operator vs fused)", flush=True)
     build_binary()
-    r = subprocess.run([BIN, os.path.join(ROOT, "tests", "golden", "real_8_first4ms.bin")], capture_output=True,
+    r = subprocess.run([BIN, os.path.join(ROOT, "tests", "golden", "real_8_first4ms.bin"),
+                        os.path.join(ROOT, "tests", "golden", "bbp_ddc_gps_L1_first4ms.bin")], capture_output=True,
                        text=True, timeout=600)
     print(r.stdout, r.stderr)
     assert r.returncode == 0

@@ -248,12 +248,14 @@ template <bool INV> struct Stage<5, INV> : StageOdd<5, INV> {};
 template <bool INV> struct Stage<7, INV> : StageOdd<7, INV> {};
 template <bool INV> struct Stage<11, INV> : StageOdd<11, INV> {};
 template <bool INV> struct Stage<13, INV> : StageOdd<13, INV> {};
+template <bool INV> struct Stage<19, INV> : StageOdd<19, INV> {};
 template <bool INV> struct Stage<31, INV> : StageOdd<31, INV> {};
 
 template <bool INV> struct Dft<5, INV> : DftOdd<5, INV> {};
 template <bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
 template <bool INV> struct Dft<11, INV> : DftOdd<11, INV> {};
 template <bool INV> struct Dft<13, INV> : DftOdd<13, INV> {};
+template <bool INV> struct Dft<19, INV> : DftOdd<19, INV> {};
 template <bool INV> struct Dft<31, INV> : DftOdd<31, INV> {};
 
 // 33-point DFT as a 3 x 11 prime-factor (Good-Thomas) transform, which needs
@@ -402,6 +404,7 @@ __device__ __forceinline__ void fft_stage_any(int R, float2* buf, int n, int nb,
         case 11: fft_stage<11, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
         case 13: fft_stage<13, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
         case 16: fft_stage<16, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
+        case 19: fft_stage<19, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
         case 31: fft_stage<31, T, CAP, INV, FIRST>(buf, n, nb, Ns, lo, hi, tid, ld); break;
         default: break;
     }

@@ -26,7 +26,7 @@ inline bool make_plan(int n, RadixPlan& p) {
     p.n = n;
     p.nst = 0;
     int m = n;
-    const int odd[] = {31, 13, 11, 7, 5, 3};
+    const int odd[] = {31, 19, 13, 11, 7, 5, 3};
     int tmp[16];
     int cnt = 0;
     for (int r : odd)
