@@ -187,6 +187,30 @@ int64_t ugsdr_format_bytes(int fmt, int64_t n_samples, double fs);
  * VerifyHeaders (signal_parameters.hpp:36-45; UGSDR_ERR_ARG on a gap). */
 int ugsdr_hip_unpack(ugsdr_hip_ctx* ctx, const void* in, int fmt, int64_t n, double fs, float* out_c32);
 
+/* ------------------------------------------------------ DFE channel step
+ * One digital-front-end channel (Channel, src/dfe/dfe.hpp:110-203): recorded
+ * samples in any UGSDR_FMT_* at fs_in are decoded, mixed by mixer_frequency
+ * (the file's centre frequency minus the signal's, dfe.hpp:177-180) with
+ * the channel's running phase, and resampled to fs_out.  The interference
+ * suppression option (dfe.hpp:200-201) is not part of this step. */
+typedef struct ugsdr_hip_dfe ugsdr_hip_dfe;
+/* fs_in and fs_out must be whole numbers of kHz (epochs of 1 ms). */
+int ugsdr_hip_dfe_create(ugsdr_hip_ctx* ctx, int fmt, double fs_in, double fs_out, double mixer_frequency,
+                         double phase, ugsdr_hip_dfe** out);
+/* Channel::GetSeveralEpochs (dfe.hpp:192-203) without suppression: n_epochs
+ * ms of the recording at `in` (from an epoch boundary; host or device
+ * memory) through Mixer::Translate's member overload (mixer.hpp:53-60: the
+ * stored phase, then phase += 2 pi frac(f / 1000) per call) and
+ * Resampler::Transform (resampler.hpp:39-46).  out: n_epochs * fs_out/1000
+ * complex samples, host memory or, with out_on_device = 1, device memory
+ * (e.g. the iq of a later ugsdr_hip_acquire with iq_on_device = 1; it is
+ * complete when the call returns). */
+int ugsdr_hip_dfe_process(ugsdr_hip_dfe* ch, const void* in, int in_on_device, int64_t n_epochs, float* out_c32,
+                          int out_on_device);
+/* the channel's running mixer phase (radians, not reduced) */
+int ugsdr_hip_dfe_phase(ugsdr_hip_dfe* ch, double* phase);
+int ugsdr_hip_dfe_destroy(ugsdr_hip_dfe* ch);
+
 /* ---------------------------------------- per-operator drop-ins (host data)
  * Slow path behind the Hip* CRTP classes: one call = one reference operator
  * call, data copied in and out like the ArrayFire proxy backends. */

@@ -4,6 +4,7 @@
 // HipFastSearchEngine, and the fused engine straight from the packed BbpDdc
 // recording.  argv[1]: tests/golden/real_8_first4ms.bin; argv[2]:
 // tests/golden/bbp_ddc_gps_L1_first4ms.bin.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <complex>
@@ -14,6 +15,7 @@
 #include <numeric>
 #include <vector>
 
+#include "ugsdr_hip/dfe.hpp"
 #include "ugsdr_hip/fse.hpp"
 #include "ugsdr_hip/tracking.hpp"
 
@@ -160,6 +162,25 @@ static int bbp_recording(const char* path) {
         threw = true;
     }
     EXPECT(threw);
+    // DFE channel: the recording is centred a little off L1 here (+2.5 kHz), so
+    // the mixer runs; four 1 ms epochs continue the phase like one 4 ms call
+    ugsdr::HipDfeChannel by_ms(UGSDR_FMT_BBP_DDC, fs, 1575.42e6 + 2.5e3, ugsdr::Signal::GpsCoarseAcquisition_L1, 4.2e6);
+    ugsdr::HipDfeChannel whole(UGSDR_FMT_BBP_DDC, fs, 1575.42e6 + 2.5e3, ugsdr::Signal::GpsCoarseAcquisition_L1, 4.2e6);
+    EXPECT(by_ms.central_frequency == 1575.42e6 && by_ms.spectrum_inversion);
+    EXPECT(ugsdr::CentralFrequency(ugsdr::Signal::Gps_L5I) == 1176.45e6);
+    cvec parts;
+    for (int e = 0; e < 4; ++e) {
+        auto p = by_ms.GetEpoch(raw.data() + (std::size_t)e * eb);
+        EXPECT(p.size() == 4200);
+        parts.insert(parts.end(), p.begin(), p.end());
+    }
+    auto all = whole.GetSeveralEpochs(raw.data(), 4);
+    float peak = 0.f, diff = 0.f;
+    for (std::size_t i = 0; i < all.size(); ++i) {
+        peak = std::max(peak, std::abs(all[i]));
+        diff = std::max(diff, std::abs(all[i] - parts[i]));
+    }
+    EXPECT(all.size() == parts.size() && diff <= 1e-5f * peak);
     // fused engine on the packed bytes with the default FseConfig rate rule
     // (33.25 MHz -> 6.65 MHz, 4 ms coherent: N = 26600 = 2^3 5^2 7 19)
     std::vector<ugsdr::Sv> svs;

@@ -65,6 +65,55 @@ class BankRequest(C.Structure):
     ]
 
 
+class Dfe:
+    """One DFE channel (Channel, src/dfe/dfe.hpp:110-203): recorded samples in
+    any FMT_* at fs_in, mixed by mixer_frequency with a running phase that
+    continues across calls (Mixer::Translate member overload,
+    src/mixer/mixer.hpp:53-60), resampled to fs_out."""
+
+    def __init__(self, ctx, fmt: int, fs_in: float, fs_out: float, mixer_frequency: float, phase: float = 0.0):
+        self.ctx, self.fmt, self.fs_in, self.fs_out = ctx, fmt, fs_in, fs_out
+        p = C.c_void_p()
+        _check(lib().ugsdr_hip_dfe_create(ctx._p, fmt, fs_in, fs_out, mixer_frequency, phase, C.byref(p)))
+        self._p = p
+
+    def process(self, raw, n_epochs: int, in_device_ptr: int | None = None, out_device_ptr: int | None = None):
+        """Channel::GetSeveralEpochs: n_epochs ms -> complex64 at fs_out (None
+        when written to out_device_ptr)"""
+        n_out = n_epochs * int(round(self.fs_out / 1e3))
+        if in_device_ptr is None:
+            raw = np.ascontiguousarray(raw)
+            need = format_bytes(self.fmt, n_epochs * int(round(self.fs_in / 1e3)), self.fs_in)
+            if raw.nbytes < need:
+                raise ValueError(f"{n_epochs} ms need {need} bytes, got {raw.nbytes}")
+            src, src_dev = raw.ctypes.data, 0
+        else:
+            src, src_dev = in_device_ptr, 1
+        if out_device_ptr is None:
+            out = np.zeros(n_out, np.complex64)
+            _check(lib().ugsdr_hip_dfe_process(self._p, src, src_dev, n_epochs, _ptr(out), 0))
+            return out
+        _check(lib().ugsdr_hip_dfe_process(self._p, src, src_dev, n_epochs, out_device_ptr, 1))
+        return None
+
+    @property
+    def phase(self) -> float:
+        v = C.c_double()
+        _check(lib().ugsdr_hip_dfe_phase(self._p, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self._p:
+            lib().ugsdr_hip_dfe_destroy(self._p)
+            self._p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("max_device_bytes", C.c_int64), ("max_inflight", C.c_int32), ("n_streams", C.c_int32),
                 ("max_batch", C.c_int32), ("reserved", C.c_int32)]
@@ -96,6 +145,10 @@ SIGNATURES = {
     "ugsdr_hip_graph_replays": ([VP], I64),
     "ugsdr_hip_set_threads": ([VP, I64, I32], C.c_int),
     "ugsdr_format_bytes": ([C.c_int, I64, D], I64),
+    "ugsdr_hip_dfe_create": ([VP, C.c_int, D, D, D, D, C.POINTER(VP)], C.c_int),
+    "ugsdr_hip_dfe_process": ([VP, VP, C.c_int, I64, VP, C.c_int], C.c_int),
+    "ugsdr_hip_dfe_phase": ([VP, C.POINTER(D)], C.c_int),
+    "ugsdr_hip_dfe_destroy": ([VP], C.c_int),
     "ugsdr_hip_unpack": ([VP, VP, C.c_int, I64, D, VP], C.c_int),
     "ugsdr_code_gps_l1ca": ([C.c_int, VP], C.c_int),
     "ugsdr_code_gps_l5i": ([C.c_int, VP], C.c_int),

@@ -36,18 +36,6 @@ using namespace ugsdr_dev;
 namespace ugsdr_hip {
 
 // ------------------------------------------------------------------ front end
-struct FrontArgs {
-    SampleSource src;  // the request's samples in their recorded format
-    int64_t n_in;
-    int64_t n_out;
-    double f_turns;   // mixing frequency / fs_in (cycles per sample), 0 = no tone
-    double ph_turns;  // initial phase in cycles
-    int apply_tone;
-    int64_t up;       // lcm / fs_in
-    int64_t down;     // lcm / fs_acq
-    float2* out;
-};
-
 // e^{2 pi i t} for a phase t in cycles (FP64), reduced before the FP32 sincos
 __device__ __forceinline__ float2 phasor_turns(double t) {
     t -= rint(t);
@@ -756,6 +744,13 @@ int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hip
 }
 
 // the front-end kernel alone (its tables and host IQ already uploaded)
+int launch_front_args(const FrontArgs& fa, hipStream_t st) {
+    const int blocks = (int)std::min<int64_t>((fa.n_out + 255) / 256, 4096);
+    hipLaunchKernelGGL(front_kernel, dim3(blocks), dim3(256), 0, st, fa);
+    UGSDR_HIP_TRY(hipGetLastError());
+    return UGSDR_OK;
+}
+
 int launch_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st) {
     FrontArgs fa{};
     fa.src.p = r->iq_on_device ? r->iq : s.iq.ptr;
@@ -770,10 +765,7 @@ int launch_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipS
     fa.up = pl.up;
     fa.down = pl.down;
     fa.out = s.sig.as<float2>();
-    const int blocks = (int)std::min<int64_t>((pl.need + 255) / 256, 4096);
-    hipLaunchKernelGGL(front_kernel, dim3(blocks), dim3(256), 0, st, fa);
-    UGSDR_HIP_TRY(hipGetLastError());
-    return UGSDR_OK;
+    return launch_front_args(fa, st);
 }
 
 int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {

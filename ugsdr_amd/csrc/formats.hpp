@@ -62,6 +62,23 @@ __device__ __forceinline__ float2 load_sample(const SampleSource& s, int64_t i) 
     }
 }
 
+// The front end of every path that reads recorded samples: decode, mix by a
+// tone (Mixer::Translate, src/mixer/mixer.hpp:33-51, FP64 phase), and the
+// rational resampler (Resampler::Transform, src/resample/resampler.hpp:39-46:
+// sample-and-hold up to lcm, boxcar down), one output sample per thread.
+struct FrontArgs {
+    SampleSource src;  // the samples in their recorded format
+    int64_t n_in;
+    int64_t n_out;
+    double f_turns;   // mixing frequency / fs_in (cycles per sample), 0 = no tone
+    double ph_turns;  // initial phase in cycles
+    int apply_tone;
+    int64_t up;       // lcm / fs_in
+    int64_t down;     // lcm / fs_out
+    float2* out;
+};
+int launch_front_args(const FrontArgs& fa, hipStream_t st);
+
 // Geometry of a format at input rate fs: fills spm / epoch_bytes (BBP_DDC
 // needs fs to be a whole number of kHz) and returns 0, or a UGSDR_ERR_* code
 // with the error text set.
