@@ -299,6 +299,52 @@ typedef struct {
 int ugsdr_hip_correlate_bank(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* req, ugsdr_channel_state* ch,
                              float* out_c32);
 
+/* ------------------------------------------------------- tracking loops
+ * GPU-resident Tracker::Track (src/tracking/tracker.hpp:156-197): per channel
+ * and 1 ms epoch, the epoch is mixed by the carrier NCO, early / prompt /
+ * late are correlated with TrackingParameters::CorrelateSplit against the
+ * 3-period replica table (tracking_parameters.hpp:471-493, tracker.hpp:
+ * 119-139), and the PLL/FLL and DLL updates (TrackingParameters::Pll / Dll,
+ * tracking_parameters.hpp:495-528) run on the device before the next epoch.
+ * One call runs every channel over n_blocks consecutive epochs. */
+typedef struct {
+    double code_phase;           /* samples (TrackingParameters::code_phase)     */
+    double code_frequency;       /* chips/s                                      */
+    double base_code_frequency;  /* chips/s (1.023e6 for GPS L1 C/A)            */
+    double carrier_phase;        /* radians                                      */
+    double carrier_frequency;    /* Hz                                           */
+    double carrier_phase_error;  /* last PLL discriminator, cycles               */
+    double code_nco;
+    double code_error;           /* last DLL discriminator                       */
+    float previous_prompt[2];    /* I, Q                                         */
+    int64_t epochs;              /* prompts produced so far (prompt.size())      */
+    int32_t code_period_ms;      /* GetCodePeriod (tracking_parameters.hpp:166-199) */
+    int32_t reserved;
+} ugsdr_track_state;
+
+typedef struct {
+    const void* iq;            /* n_blocks * spm samples in fmt (the channel's subband) */
+    int32_t fmt;
+    int32_t iq_on_device;
+    int64_t spm;               /* samples per 1 ms epoch                                */
+    int64_t n_blocks;
+    double fs;
+    int32_t n_channels;
+    int32_t reserved;
+    const int8_t* codes;       /* n_channels x code_len chips (+-1), host               */
+    int64_t code_len;          /* chips per code period                                 */
+    double spacing_chips;      /* early/late offset, 0.25 in Tracker (tracker.hpp:170)  */
+} ugsdr_track_request;
+
+/* Runs the loops and advances ch (n_channels states) in place.  epl:
+ * n_blocks x n_channels x 3 complex (early, prompt, late), host buffer.
+ * trace (optional, host): n_blocks x n_channels x 4 doubles, the values
+ * TrackingParameters records per epoch: -carrier_phase / 2 pi, carrier
+ * frequency, code phase, code frequency (phases, frequencies, code_phases,
+ * code_frequencies). */
+int ugsdr_hip_track(ugsdr_hip_ctx* ctx, const ugsdr_track_request* req, ugsdr_track_state* ch, float* epl_c32,
+                    double* trace);
+
 /* ---------------------------------------------------------- request server
  * Continuous-batching acquisition server (no reference counterpart: the
  * reference runs one FastSearchEngineBase::Process per epoch,

@@ -556,6 +556,66 @@ void refcpu_epl(float* x, size_t spm, double fs, double carrier_frequency, doubl
                                out + 2 * t);
 }
 
+/* loop gains (tracking_parameters.hpp:72-87) */
+static const double RC_PLL_W = 25.0 / 0.53;
+#define RC_K1_PLL (0.001 * (RC_PLL_W * RC_PLL_W) + 1.414 * RC_PLL_W)
+#define RC_K2_PLL (1.414 * RC_PLL_W)
+#define RC_K3_PLL (0.001 * (250.0 / 0.25))
+static const double RC_DLL_WN = 0.7 * 8.0 * 1.0 / (4.0 * 1.0 * 1.0 + 1.0);
+#define RC_DLL_TAU1 (0.1 / (RC_DLL_WN * RC_DLL_WN))
+#define RC_DLL_TAU2 (2.0 * 1.0 / RC_DLL_WN)
+
+static void rc_pll(refcpu_track_state* s, float pi_, float pq) {   /* tracking_parameters.hpp:495-511 */
+    double e = (pi_ != 0.0f) ? atan((double)(pq / pi_)) / (M_PI * 2.0) : 0.0;
+    float qi = s->previous_prompt[0], qq = s->previous_prompt[1];
+    float cross = pi_ * qq - qi * pq;
+    float dot = fabsf(pi_ * qi + qq * pq);
+    double ferr = (double)atan2f(cross, dot) / M_PI;
+    s->carrier_frequency += RC_K1_PLL * e - RC_K2_PLL * s->carrier_phase_error - RC_K3_PLL * ferr;
+    s->carrier_phase_error = e;
+    s->carrier_phase += e * M_PI / 2;
+    s->previous_prompt[0] = pi_;
+    s->previous_prompt[1] = pq;
+}
+
+static void rc_dll(refcpu_track_state* s, const float* e, const float* l, double fs) {  /* :513-528 */
+    float ea = hypotf(e[0], e[1]), la = hypotf(l[0], l[1]);
+    double err = 0.0;
+    if (ea + la != 0.0f) err = (ea - la) / (ea + la);
+    s->code_nco += (RC_DLL_TAU2 / RC_DLL_TAU1 * (err - s->code_error) + 0.001 / RC_DLL_TAU1 * err) *
+                   (s->base_code_frequency / 1.023e6);
+    s->code_error = err;
+    s->code_frequency = s->base_code_frequency - s->code_nco;
+    s->code_phase -= fs / 1000 / 2 * (s->code_frequency / s->base_code_frequency - 1);
+}
+
+void refcpu_track(const float* x, int64_t spm, int64_t n_blocks, double fs, const float* code3, double spacing,
+                  refcpu_track_state* st, float* epl, double* trace) {
+    float* blk = (float*)malloc(sizeof(float) * 2 * (size_t)spm);
+    const double taps[3] = {spacing, 0.0, -spacing};
+    for (int64_t b = 0; b < n_blocks; ++b) {            /* tracker.hpp:156-177 */
+        memcpy(blk, x + 2 * (size_t)b * spm, sizeof(float) * 2 * (size_t)spm);
+        double cp = st->code_phase - fs / 1e3 * (double)(st->epochs % st->code_period_ms);
+        double out[6];
+        refcpu_epl(blk, (size_t)spm, fs, st->carrier_frequency, st->carrier_phase, code3,
+                   (double)st->code_period_ms, cp, fs / st->base_code_frequency, taps, 3, out);
+        st->carrier_phase += 2 * M_PI * st->carrier_frequency / 1000.0;   /* UpdatePhase */
+        float v[6];
+        for (int i = 0; i < 6; ++i) v[i] = (float)out[i];
+        memcpy(epl + 6 * (size_t)b, v, sizeof v);
+        rc_pll(st, v[2], v[3]);
+        rc_dll(st, v, v + 4, fs);
+        st->epochs += 1;
+        if (trace) {
+            trace[4 * b] = -st->carrier_phase / (2 * M_PI);
+            trace[4 * b + 1] = st->carrier_frequency;
+            trace[4 * b + 2] = st->code_phase;
+            trace[4 * b + 3] = st->code_frequency;
+        }
+    }
+    free(blk);
+}
+
 typedef struct {
     const float* x; int64_t spm, n_blocks; double fs;
     const refcpu_channel* ch; int n_ch; const int8_t* codes; int64_t code_len;

@@ -185,6 +185,29 @@ void refcpu_epl(float* x, size_t spm, double fs, double carrier_frequency, doubl
                 const float* code3, double code_period_ms, double code_phase,
                 double samples_per_chip, const double* taps_chips, int n_taps, double* out);
 
+/* Tracking loop state of one channel (TrackingParameters,
+ * src/tracking/tracking_parameters.hpp:100-135); the layout of
+ * ugsdr_track_state. */
+typedef struct {
+    double code_phase, code_frequency, base_code_frequency;
+    double carrier_phase, carrier_frequency, carrier_phase_error;
+    double code_nco, code_error;
+    float previous_prompt[2];
+    int64_t epochs;
+    int32_t code_period_ms, reserved;
+} refcpu_track_state;
+/* Tracker::Track for one channel (src/tracking/tracker.hpp:156-177):
+ * per epoch b of x (n_blocks x spm complex): the epoch's code phase
+ * (code_phase - fs/1e3 * (epochs % code_period)), refcpu_epl at taps
+ * {+spacing, 0, -spacing} chips with the carrier phase advanced after the
+ * mix (UpdatePhase), the correlations rounded to complex<float>, then
+ * TrackingParameters::Pll and ::Dll (tracking_parameters.hpp:495-528) in
+ * their float/double mix.  code3: the 3-period replica table.  epl:
+ * n_blocks x 3 complex float; trace (optional): n_blocks x 4 doubles
+ * (-carrier_phase/2pi, carrier frequency, code phase, code frequency). */
+void refcpu_track(const float* x, int64_t spm, int64_t n_blocks, double fs, const float* code3, double spacing,
+                  refcpu_track_state* st, float* epl, double* trace);
+
 /* Correlator bank, NCO mode (no reference counterpart; the reference's
  * per-sample replica is an integer shift of the 3-period table).  For each
  * channel c and block b, with block-start carrier phase theta (cycles) and

@@ -65,6 +65,13 @@ I8 = P(dtype=np.int8, flags="C_CONTIGUOUS")
 U8 = P(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
+class TrackState(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("code_phase", "code_frequency", "base_code_frequency", "carrier_phase",
+                                          "carrier_frequency", "carrier_phase_error", "code_nco", "code_error")] + [
+        ("previous_prompt", C.c_float * 2), ("epochs", C.c_int64), ("code_period_ms", C.c_int32),
+        ("reserved", C.c_int32)]
+
+
 def _declare(L):
     for name in ("refcpu_gps_l1ca", "refcpu_gps_l5i", "refcpu_gps_l5q"):
         getattr(L, name).argtypes = [C.c_int, I8]
@@ -77,6 +84,8 @@ def _declare(L):
     L.refcpu_bbp_verify_headers.restype = C.c_int
     L.refcpu_unpack_bbp.argtypes = [U8, C.c_int64, C.c_int64, C.c_int64, F32]
     L.refcpu_unpack_nt1065.argtypes = [U8, C.c_int64, C.c_int, F32]
+    L.refcpu_track.argtypes = [F32, C.c_int64, C.c_int64, C.c_double, F32, C.c_double, C.POINTER(TrackState), F32,
+                               C.c_void_p]
     L.refcpu_repeat_upsample.argtypes = [I8, C.c_size_t, C.c_size_t, F32, C.c_size_t]
     L.refcpu_upsample_f.argtypes = [F32, C.c_size_t, F32, C.c_size_t]
     L.refcpu_upsample_hold_c.argtypes = [F32, C.c_size_t, F32, C.c_size_t]
@@ -182,6 +191,29 @@ def unpack_nt1065(data: np.ndarray, n: int, offset_bits: int) -> np.ndarray:
     out = np.zeros(2 * n, np.float32)
     lib().refcpu_unpack_nt1065(data, n, offset_bits, out)
     return f2c(out)
+
+
+def track_state(code_phase: float, carrier_frequency: float, base_code_frequency: float = 1.023e6,
+                code_period_ms: int = 1, carrier_phase: float = 0.0) -> TrackState:
+    """TrackingParameters right after the acquisition hand-off"""
+    st = TrackState()
+    st.code_phase, st.carrier_frequency, st.carrier_phase = code_phase, carrier_frequency, carrier_phase
+    st.code_frequency = st.base_code_frequency = base_code_frequency
+    st.code_period_ms = code_period_ms
+    return st
+
+
+def track(x: np.ndarray, spm: int, n_blocks: int, fs: float, code3: np.ndarray, state: TrackState,
+          spacing: float = 0.25):
+    """Tracker::Track for one channel; state is advanced in place.  Returns
+    (epl [n_blocks, 3] complex64, trace [n_blocks, 4])"""
+    xb = c2f(np.ascontiguousarray(x[:spm * n_blocks], np.complex64))
+    code3 = np.ascontiguousarray(code3, np.float32)
+    assert code3.size == 3 * spm * state.code_period_ms
+    epl = np.zeros(6 * n_blocks, np.float32)
+    trace = np.zeros((n_blocks, 4), np.float64)
+    lib().refcpu_track(xb, spm, n_blocks, fs, code3, spacing, C.byref(state), epl, trace.ctypes.data)
+    return f2c(epl).reshape(n_blocks, 3), trace
 
 
 def repeat_upsample(code: np.ndarray, repeats: int, samples: int) -> np.ndarray:

@@ -203,6 +203,41 @@ static int bbp_recording(const char* path) {
     return 0;
 }
 
+// HipTracker: a clean GPS L1 C/A signal (sv 4, +1500.3 Hz, delay 1234
+// samples at 4.092 MHz) seeded 40 Hz and 0.6 samples off; after 1 s the
+// carrier is within 3 Hz and the prompt's energy is in I.
+static int tracker_lock() {
+    const double fs = 4.092e6, f0 = 1500.3, tau = 1234.0;
+    const int spm = 4092, n_ms = 1000;
+    const auto code = l1ca(4);
+    std::vector<float> iq(2 * (std::size_t)spm * n_ms);
+    for (int k = 0; k < spm * n_ms; ++k) {
+        const long chip = (long)std::floor((k - tau) * 1.023e6 / fs);
+        const double c = code[(std::size_t)(((chip % 1023) + 1023) % 1023)];
+        const double ph = 2 * M_PI * f0 * k / fs + 0.3;
+        iq[2 * (std::size_t)k] = (float)(c * std::cos(ph));
+        iq[2 * (std::size_t)k + 1] = (float)(c * std::sin(ph));
+    }
+    ugsdr::AcquisitionResult<float> acq;
+    acq.sv_number = {4, ugsdr::System::Gps, ugsdr::Signal::GpsCoarseAcquisition_L1};
+    acq.doppler = f0 + 40.0;
+    acq.code_offset = tau + 0.6;
+    ugsdr::HipTracker tracker(fs, {acq}, {code});
+    tracker.Track(iq.data(), UGSDR_FMT_CF32, 600);
+    tracker.Track(iq.data() + 2 * (std::size_t)spm * 600, UGSDR_FMT_CF32, 400);   // the loops carry on across calls
+    const auto& ch = tracker.GetTrackingParameters()[0];
+    EXPECT(ch.prompt.size() == 1000 && ch.state.epochs == 1000);
+    EXPECT(std::abs(ch.frequencies.back() - f0) < 3.0);
+    double i_sum = 0, q_sum = 0;
+    for (std::size_t b = 800; b < 1000; ++b) {
+        i_sum += std::abs(ch.prompt[b].real());
+        q_sum += std::abs(ch.prompt[b].imag());
+    }
+    EXPECT(q_sum < 0.15 * i_sum);
+    std::printf("tracker: %.2f Hz after 1 s, code phase %.2f\n", ch.frequencies.back(), ch.code_phases.back());
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s real_8_first4ms.bin bbp_ddc_gps_L1_first4ms.bin\n", argv[0]);
@@ -212,6 +247,7 @@ int main(int argc, char** argv) {
         if (int rc = unit_kats()) return rc;
         if (int rc = real8_kat(argv[1])) return rc;
         if (int rc = bbp_recording(argv[2])) return rc;
+        if (int rc = tracker_lock()) return rc;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "exception: %s\n", e.what());
         return 1;

@@ -1,0 +1,111 @@
+"""GPU-resident tracking loops (ugsdr_hip_track) through the C ABI.
+
+* Parity with the oracle (refcpu_track, Tracker::TrackSingleSatellite with
+  Pll / Dll): eight channels on one signal holding eight satellites, every
+  epoch's early / prompt / late within 2e-4 of the prompt's scale and the
+  loop trace (carrier frequency, code phase, carrier phase) within 1e-3 Hz,
+  1e-3 samples and 1e-5 cycles.  The correlations are FP32 sums in another
+  order on each side; the loop state is FP64 on both.
+* The device loops lock from acquisition-size errors like the oracle's
+  (tests/test_tracking.py: carrier pulled in, prompt in I, early and late at
+  3/4 of the prompt, code phase in [delay, delay + 1)).
+* The hand-off: a fused acquisition of the same signal seeds the states
+  (code_phase = code_offset, carrier = Doppler bin; TrackingParameters'
+  constructor, tracking_parameters.hpp:118-124) and every detected
+  satellite locks.
+"""
+import numpy as np
+import pytest
+
+import ugsdr_amd as U
+from oracle import refcpu as R
+
+from test_tracking import FS, SPM, code3_table, l1_signal
+
+pytestmark = pytest.mark.gpu
+
+SATS = [(0, 1500.3, 1234.0), (3, -2210.7, 77.0), (5, 3120.2, 4000.0), (8, -450.9, 2048.0),
+        (11, 890.1, 15.0), (14, -3999.4, 3071.0), (19, 2501.6, 512.0), (27, -1275.2, 3600.0)]
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def multi_signal(n_ms, seed=9):
+    x = np.zeros(n_ms * SPM, np.complex128)
+    for i, (sv, f, tau) in enumerate(SATS):
+        x += l1_signal(sv, n_ms, f, tau, phase0=0.4 * i, amp=1.0, sigma=0.0)
+    rng = np.random.default_rng(seed)
+    x += 1.5 * (rng.normal(size=x.size) + 1j * rng.normal(size=x.size))
+    return x.astype(np.complex64)
+
+
+def test_track_matches_oracle(ctx, oracle):
+    n_ms = 300
+    x = multi_signal(n_ms)
+    errs = [(30.0, 0.7), (-45.0, -1.1), (12.0, 0.2), (-20.0, 1.4), (50.0, -0.5), (-8.0, 0.9), (35.0, 1.1),
+            (-25.0, -0.3)]
+    states = [U.track_state(tau + dc, f + df) for (sv, f, tau), (df, dc) in zip(SATS, errs)]
+    codes = [U.gps_l1ca(sv) for sv, _, _ in SATS]
+    say(f"case track 8 channels x {n_ms} epochs @ 4.092 MHz vs oracle")
+    iq = x.view(np.float32)
+    epl, tr = ctx.track(iq, U.FMT_CF32, spm=SPM, n_blocks=n_ms, fs=FS, states=states, codes=codes)
+    for c, ((sv, f, tau), (df, dc)) in enumerate(zip(SATS, errs)):
+        st = R.track_state(tau + dc, f + df)
+        repl, rtr = R.track(x, SPM, n_ms, FS, code3_table(sv), st)
+        scale = np.max(np.abs(repl[:, 1]))
+        e_epl = np.max(np.abs(epl[:, c, :] - repl)) / scale
+        e_f = np.max(np.abs(tr[:, c, 1] - rtr[:, 1]))
+        e_cp = np.max(np.abs(tr[:, c, 2] - rtr[:, 2]))
+        e_ph = np.max(np.abs(tr[:, c, 0] - rtr[:, 0]))
+        say(f"  sv {sv}: epl {e_epl:.2e}, carrier {e_f:.2e} Hz, code {e_cp:.2e} samples, phase {e_ph:.2e} cycles")
+        assert e_epl < 2e-4 and e_f < 1e-3 and e_cp < 1e-3 and e_ph < 1e-5
+        assert states[c].epochs == n_ms and abs(states[c].carrier_frequency - st.carrier_frequency) < 1e-3
+
+
+def test_track_locks_like_oracle(ctx):
+    n_ms = 1000
+    sv, f0, tau = 4, 1500.3, 1234.0
+    x = l1_signal(sv, n_ms, f0, tau)
+    starts = [(40.0, 0.6), (-60.0, -1.2), (25.0, 1.6)]
+    states = [U.track_state(tau + dc, f0 + df) for df, dc in starts]
+    say("case track lock from acquisition-size errors, 3 channels x 1000 epochs")
+    epl, tr = ctx.track(x.view(np.float32), U.FMT_CF32, spm=SPM, n_blocks=n_ms, fs=FS, states=states,
+                        codes=[U.gps_l1ca(sv)] * 3)
+    for c, st in enumerate(states):
+        P = epl[-200:, c, 1]
+        assert abs(tr[-1, c, 1] - f0) < 3.0
+        assert np.mean(np.abs(P.imag)) < 0.15 * np.mean(np.abs(P.real))
+        assert abs(np.mean(np.abs(P)) / SPM - 1) < 0.01
+        for tap in (0, 2):
+            assert abs(np.mean(np.abs(epl[-200:, c, tap])) / np.mean(np.abs(P)) - 0.75) < 0.02
+        assert tau - 0.25 <= st.code_phase <= tau + 1.25
+
+
+def test_acquisition_hands_off_to_tracking(ctx):
+    n_ms = 1000
+    x = multi_signal(n_ms, seed=21)
+    keys = [ctx.replica_put(U.gps_l1ca(s), 1, SPM) for s in range(32)]
+    say("case acquisition (4 ms, 32 PRN, 125 Hz bins) -> tracking hand-off -> 1000 epochs")
+    try:
+        res, _, _ = ctx.acquire(x[:4 * SPM].view(np.float32), U.FMT_CF32, n_samples=4 * SPM, fs_in=FS, fs_acq=FS,
+                                n_fft=SPM, n_blocks=4, surface_len=SPM, doppler=U.doppler_bins(5000, 125),
+                                keys=keys)
+    finally:
+        for k in keys:
+            ctx.replica_drop(k)
+    found = [p for p, r in enumerate(res) if r["detected"]]
+    assert sorted(found) == sorted(sv for sv, _, _ in SATS)
+    states = [U.track_state(res[p]["code_offset"], res[p]["doppler"]) for p in found]
+    epl, tr = ctx.track(x.view(np.float32), U.FMT_CF32, spm=SPM, n_blocks=n_ms, fs=FS, states=states,
+                        codes=[U.gps_l1ca(p) for p in found])
+    truth = {sv: (f, tau) for sv, f, tau in SATS}
+    for c, p in enumerate(found):
+        f0, tau = truth[p]
+        P = epl[-200:, c, 1]
+        say(f"  sv {p}: acq {res[p]['doppler']:.0f} Hz / {res[p]['code_offset']:.0f} -> "
+            f"{tr[-1, c, 1]:.1f} Hz / {tr[-1, c, 2]:.2f}")
+        assert abs(tr[-1, c, 1] - f0) < 5.0
+        assert np.mean(np.abs(P.imag)) < 0.2 * np.mean(np.abs(P.real))
+        assert tau - 0.5 <= tr[-1, c, 2] <= tau + 1.5

@@ -114,6 +114,31 @@ class Dfe:
         self.close()
 
 
+class TrackState(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("code_phase", "code_frequency", "base_code_frequency", "carrier_phase",
+                                          "carrier_frequency", "carrier_phase_error", "code_nco", "code_error")] + [
+        ("previous_prompt", C.c_float * 2), ("epochs", C.c_int64), ("code_period_ms", C.c_int32),
+        ("reserved", C.c_int32)]
+
+
+class TrackRequest(C.Structure):
+    _fields_ = [
+        ("iq", C.c_void_p), ("fmt", C.c_int32), ("iq_on_device", C.c_int32), ("spm", C.c_int64),
+        ("n_blocks", C.c_int64), ("fs", C.c_double), ("n_channels", C.c_int32), ("reserved", C.c_int32),
+        ("codes", C.POINTER(C.c_int8)), ("code_len", C.c_int64), ("spacing_chips", C.c_double),
+    ]
+
+
+def track_state(code_phase: float, carrier_frequency: float, base_code_frequency: float = 1.023e6,
+                code_period_ms: int = 1, carrier_phase: float = 0.0) -> TrackState:
+    """TrackingParameters right after the acquisition hand-off"""
+    st = TrackState()
+    st.code_phase, st.carrier_frequency, st.carrier_phase = code_phase, carrier_frequency, carrier_phase
+    st.code_frequency = st.base_code_frequency = base_code_frequency
+    st.code_period_ms = code_period_ms
+    return st
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("max_device_bytes", C.c_int64), ("max_inflight", C.c_int32), ("n_streams", C.c_int32),
                 ("max_batch", C.c_int32), ("reserved", C.c_int32)]
@@ -145,6 +170,7 @@ SIGNATURES = {
     "ugsdr_hip_graph_replays": ([VP], I64),
     "ugsdr_hip_set_threads": ([VP, I64, I32], C.c_int),
     "ugsdr_format_bytes": ([C.c_int, I64, D], I64),
+    "ugsdr_hip_track": ([VP, VP, VP, VP, VP], C.c_int),
     "ugsdr_hip_dfe_create": ([VP, C.c_int, D, D, D, D, C.POINTER(VP)], C.c_int),
     "ugsdr_hip_dfe_process": ([VP, VP, C.c_int, I64, VP, C.c_int], C.c_int),
     "ugsdr_hip_dfe_phase": ([VP, C.POINTER(D)], C.c_int),
@@ -432,6 +458,36 @@ class Context:
     def acquire(self, iq, fmt: int, want_surfaces: bool = False, want_peaks: bool = False, **kw):
         req, hold = self.make_request(iq, fmt, **kw)
         return self.acquire_req(req, want_surfaces, want_peaks)
+
+    # ---- tracking loops
+    def track(self, iq, fmt: int, *, spm: int, n_blocks: int, fs: float, states, codes, spacing: float = 0.25,
+              iq_device_ptr: int | None = None, want_trace: bool = True):
+        """Tracker::Track on the device for every channel; states (a TrackState
+        array or list) are advanced in place.  Returns (epl [n_blocks, n_ch, 3]
+        complex64, trace [n_blocks, n_ch, 4] or None)"""
+        n_ch = len(states)
+        arr = (TrackState * n_ch)(*states) if not isinstance(states, C.Array) else states
+        codes = np.ascontiguousarray(np.stack([np.asarray(c, np.int8) for c in codes]))
+        req = TrackRequest()
+        if iq_device_ptr is not None:
+            req.iq, req.iq_on_device = iq_device_ptr, 1
+            hold = None
+        else:
+            hold = np.ascontiguousarray(iq)
+            req.iq, req.iq_on_device = hold.ctypes.data, 0
+        req.fmt, req.spm, req.n_blocks, req.fs = fmt, spm, n_blocks, fs
+        req.n_channels = n_ch
+        req.codes = codes.ctypes.data_as(C.POINTER(C.c_int8))
+        req.code_len = codes.shape[1]
+        req.spacing_chips = spacing
+        epl = np.zeros((n_blocks, n_ch, 3), np.complex64)
+        trace = np.zeros((n_blocks, n_ch, 4), np.float64) if want_trace else None
+        _check(lib().ugsdr_hip_track(self._p, C.byref(req), arr, _ptr(epl),
+                                     trace.ctypes.data if trace is not None else None))
+        if not isinstance(states, C.Array):
+            for i in range(n_ch):
+                states[i] = arr[i]
+        return epl, trace
 
     # ---- recorded formats
     def unpack(self, raw, fmt: int, n: int, fs: float) -> np.ndarray:

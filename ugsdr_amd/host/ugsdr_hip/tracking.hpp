@@ -1,7 +1,8 @@
 // Tracking-side surface: the TrackingParametersConfig type list, the
 // CorrelateSplit / AddWithPhase tap of the reference written over any
-// Config::CorrelatorType, and HipCorrelatorBank, which evaluates every
-// channel's taps over many 1 ms blocks in one C-ABI call.
+// Config::CorrelatorType, HipCorrelatorBank, which evaluates every channel's
+// taps over many 1 ms blocks in one C-ABI call, and HipTracker, the closed
+// tracking loops of every channel run on the device.
 #pragma once
 
 #include <cmath>
@@ -13,6 +14,7 @@
 
 #include "backends.hpp"
 #include "context.hpp"
+#include "fse.hpp"
 
 namespace ugsdr {
 
@@ -113,6 +115,91 @@ private:
                                                   reinterpret_cast<float*>(out.data())));
         return out;
     }
+};
+
+// Tracker (src/tracking/tracker.hpp:99-197) for the channels of one signal at
+// one sampling rate: the loops (PLL/FLL and DLL of TrackingParameters) run on
+// the device, and each channel keeps the per-epoch records TrackingParameters
+// keeps (early / prompt / late, phases, frequencies, code phases, code
+// frequencies).
+class HipTracker {
+public:
+    struct Channel {
+        Sv sv;
+        ugsdr_track_state state{};
+        std::vector<std::complex<float>> early, prompt, late;
+        std::vector<double> phases, frequencies, code_phases, code_frequencies;
+    };
+
+    // TrackingParameters(acquisition, ...) (tracking_parameters.hpp:118-140)
+    // for signals whose code period is code_period_ms: code phase from the
+    // acquisition's code offset, carrier from its Doppler.
+    HipTracker(double sampling_rate, const std::vector<AcquisitionResult<float>>& acquisitions,
+               const std::vector<std::vector<int8_t>>& codes, double base_code_frequency = 1.023e6,
+               int code_period_ms = 1, ugsdr_hip::HipContext& ctx = ugsdr_hip::HipContext::Default())
+        : fs(sampling_rate), context(&ctx), chips(codes) {
+        for (const auto& a : acquisitions) {
+            Channel c;
+            c.sv = a.sv_number;
+            c.state.code_phase = a.code_offset;
+            c.state.carrier_frequency = a.doppler;
+            c.state.code_frequency = c.state.base_code_frequency = base_code_frequency;
+            c.state.code_period_ms = code_period_ms;
+            channels.push_back(std::move(c));
+        }
+    }
+
+    // Tracker::Track over n_epochs 1 ms epochs of interleaved samples in fmt
+    // at the tracker's rate (host, or device memory with on_device)
+    void Track(const void* samples, int fmt, std::size_t n_epochs, bool on_device = false, double spacing = 0.25) {
+        const std::size_t n = channels.size();
+        if (!n) return;
+        const auto spm = (int64_t)(fs / 1e3);
+        std::vector<ugsdr_track_state> st(n);
+        std::vector<int8_t> codes;
+        for (std::size_t i = 0; i < n; ++i) {
+            st[i] = channels[i].state;
+            codes.insert(codes.end(), chips[i].begin(), chips[i].end());
+        }
+        ugsdr_track_request r{};
+        r.iq = samples;
+        r.fmt = fmt;
+        r.iq_on_device = on_device ? 1 : 0;
+        r.spm = spm;
+        r.n_blocks = (int64_t)n_epochs;
+        r.fs = fs;
+        r.n_channels = (int32_t)n;
+        r.codes = codes.data();
+        r.code_len = (int64_t)chips[0].size();
+        r.spacing_chips = spacing;
+        std::vector<std::complex<float>> epl(n_epochs * n * 3);
+        std::vector<double> trace(n_epochs * n * 4);
+        ugsdr_hip::check(ugsdr_hip_track(context->get(), &r, st.data(), reinterpret_cast<float*>(epl.data()),
+                                         trace.data()));
+        for (std::size_t i = 0; i < n; ++i) {
+            Channel& c = channels[i];
+            c.state = st[i];
+            for (std::size_t b = 0; b < n_epochs; ++b) {
+                const auto* e = &epl[(b * n + i) * 3];
+                const auto* t = &trace[(b * n + i) * 4];
+                c.early.push_back(e[0]);
+                c.prompt.push_back(e[1]);
+                c.late.push_back(e[2]);
+                c.phases.push_back(t[0]);
+                c.frequencies.push_back(t[1]);
+                c.code_phases.push_back(t[2]);
+                c.code_frequencies.push_back(t[3]);
+            }
+        }
+    }
+
+    const std::vector<Channel>& GetTrackingParameters() const { return channels; }
+
+private:
+    double fs;
+    ugsdr_hip::HipContext* context;
+    std::vector<std::vector<int8_t>> chips;
+    std::vector<Channel> channels;
 };
 
 }  // namespace ugsdr
