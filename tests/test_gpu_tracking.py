@@ -109,3 +109,38 @@ def test_acquisition_hands_off_to_tracking(ctx):
         assert abs(tr[-1, c, 1] - f0) < 5.0
         assert np.mean(np.abs(P.imag)) < 0.2 * np.mean(np.abs(P.real))
         assert tau - 0.5 <= tr[-1, c, 2] <= tau + 1.5
+
+
+@pytest.mark.parametrize("fmt,fs", [(U.FMT_CI16, 4.093e6), (U.FMT_CI8, 4.094e6), (U.FMT_R8, 4.095e6)])
+def test_track_formats_and_ragged_epochs(ctx, oracle, fmt, fs):
+    """integer sample formats (the compiled CI8 / CI16 loaders and the
+    run-time decoder behind R8) at rates whose epochs are not whole groups of
+    four samples (4093, 4094, 4095 per ms): the same bar against the oracle
+    fed the decoded samples"""
+    spm, n_ms = int(fs / 1e3), 200
+    sv, f0, tau = 6, -1730.4, 2900.0
+    x = l1_signal(sv, n_ms, f0, tau, fs=fs)
+    if fmt == U.FMT_CI16:
+        raw = np.empty(2 * x.size, np.int16)
+        raw[0::2], raw[1::2] = np.round(100 * x.real), np.round(100 * x.imag)
+        dec = (raw[0::2] + 1j * raw[1::2]).astype(np.complex64)
+    elif fmt == U.FMT_CI8:
+        raw = np.empty(2 * x.size, np.int8)
+        raw[0::2], raw[1::2] = np.clip(np.round(30 * x.real), -127, 127), np.clip(np.round(30 * x.imag), -127, 127)
+        dec = (raw[0::2].astype(np.float32) + 1j * raw[1::2].astype(np.float32)).astype(np.complex64)
+    else:
+        raw = np.clip(np.round(30 * x.real), -127, 127).astype(np.int8)
+        dec = raw.astype(np.float32).astype(np.complex64)
+    starts = [(30.0, 0.4), (-45.0, -0.8), (10.0, 1.3)]
+    states = [U.track_state(tau + dc, f0 + df) for df, dc in starts]
+    say(f"case track fmt {fmt}, {spm} samples per epoch, 3 channels x {n_ms} epochs vs oracle")
+    epl, tr = ctx.track(raw, fmt, spm=spm, n_blocks=n_ms, fs=fs, states=states, codes=[U.gps_l1ca(sv)] * 3)
+    for c, (df, dc) in enumerate(starts):
+        st = R.track_state(tau + dc, f0 + df)
+        repl, rtr = R.track(dec, spm, n_ms, fs, code3_table(sv, spm), st)
+        scale = np.max(np.abs(repl[:, 1]))
+        e_epl = np.max(np.abs(epl[:, c, :] - repl)) / scale
+        e_f = np.max(np.abs(tr[:, c, 1] - rtr[:, 1]))
+        e_cp = np.max(np.abs(tr[:, c, 2] - rtr[:, 2]))
+        say(f"  channel {c}: epl {e_epl:.2e}, carrier {e_f:.2e} Hz, code {e_cp:.2e} samples")
+        assert e_epl < 2e-4 and e_f < 1e-3 and e_cp < 1e-3

@@ -113,6 +113,14 @@ struct ugsdr_hip_ctx {
             key.clear();
         }
     } graph;
+    // tracking replica tables (ugsdr_hip_track), kept while the channels'
+    // codes and the sample rate stay the same between calls
+    struct TrackTables {
+        ugsdr_hip::DevBuf buf;
+        std::vector<int8_t> codes;
+        int64_t spm = 0, period = 0, len3p = 0;
+        int n_ch = 0;
+    } track;
     // autotuner: transform length -> correlation workgroup size
     std::map<int64_t, int> tuned;
     bool autotune = true;

@@ -62,6 +62,24 @@ __device__ __forceinline__ float2 load_sample(const SampleSource& s, int64_t i) 
     }
 }
 
+// load_sample with the format fixed at compile time for the interleaved I/Q
+// formats (straight-line loads the compiler can issue back to back); any
+// other FMT value decodes through the run-time switch.
+template <int FMT>
+__device__ __forceinline__ float2 load_sample_fmt(const SampleSource& s, int64_t i) {
+    if constexpr (FMT == UGSDR_FMT_CI8) {
+        const char2 v = static_cast<const char2*>(s.p)[i];
+        return make_float2((float)v.x, (float)v.y);
+    } else if constexpr (FMT == UGSDR_FMT_CI16) {
+        const short2 v = static_cast<const short2*>(s.p)[i];
+        return make_float2((float)v.x, (float)v.y);
+    } else if constexpr (FMT == UGSDR_FMT_CF32) {
+        return static_cast<const float2*>(s.p)[i];
+    } else {
+        return load_sample(s, i);
+    }
+}
+
 // The front end of every path that reads recorded samples: decode, mix by a
 // tone (Mixer::Translate, src/mixer/mixer.hpp:33-51, FP64 phase), and the
 // rational resampler (Resampler::Transform, src/resample/resampler.hpp:39-46:

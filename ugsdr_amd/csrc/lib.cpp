@@ -42,6 +42,7 @@ int64_t ugsdr_hip_ctx::device_bytes() const {
     for (const DevBuf* d : {&iq, &sig, &classes, &scratch, &rows, &results, &surf, &tables, &aux, &aux2})
         b += (int64_t)d->bytes;
     b += slot.bytes();
+    b += (int64_t)track.buf.bytes;
     for (const auto& kv : replicas)
         b += kv.second.n_fft * (kv.second.precision == UGSDR_PREC_FP32 ? 8 : 4);
     return b;
@@ -94,6 +95,7 @@ int ugsdr_hip_ctx_destroy(ugsdr_hip_ctx* ctx) {
             d->release();
         ctx->graph.drop();
         ctx->slot.release_all();
+        ctx->track.buf.release();
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

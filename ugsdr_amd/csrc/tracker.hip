@@ -13,6 +13,7 @@
 //     epoch uses the new state without a trip to the host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -49,18 +50,21 @@ struct TrackArgs {
     int n_ch;
     double fs;
     double spacing;              // early/late offset in chips
-    const float* code3;          // n_ch x code3_len replica tables
-    int64_t code3_len;
+    const float* tabs;           // per channel: 4 alignment copies of the 3-period replica
+    int64_t len3p;               // floats per copy (multiple of 4)
     ugsdr_track_state* state;    // n_ch
     float2* epl;                 // n_blocks x n_ch x 3
     double* trace;               // n_blocks x n_ch x 4, or null
 };
 
+// What every thread needs to correlate one epoch, written by wave 0.
 struct EpochPlan {
-    int first_len[3], first_phase[3], second_phase[3];
-    double rel[3];
-    double df, ph_turns;
-    int mix;
+    int fl[3];        // first-span length of each tap (CorrelateSplit)
+    int fa[3];        // 4-aligned table offset of the tap: fp - (fp & 3), fp = ceil(2P - phase)
+    int al[3];        // fp & 3: the alignment copy that makes fa + 4g a 16-byte boundary
+    double rel[3];    // fractional position of the split in the epoch (AddWithPhase)
+    double df, pt;    // mixing tone: turns per sample and start phase in turns (df = 0: no mixing)
+    float2 w1, wg;    // e^{j 2 pi df}, e^{j 2 pi 4 T df}
 };
 
 // TrackingParameters::Pll (tracking_parameters.hpp:495-511).  The prompt is
@@ -91,120 +95,293 @@ __device__ __forceinline__ void dll_update(ugsdr_track_state& s, float2 e, float
     s.code_phase -= fs / 1000 / 2 * (s.code_frequency / s.base_code_frequency - 1);
 }
 
+// Plans the next epoch from the state and advances the carrier phase, run
+// by all of wave 0: TrackSingleSatellite's code phase (tracker.hpp:165) and
+// the tone of Translate(x, fs, -f, -phase) (mixer.hpp:33-41) on lane 0, then
+// UpdatePhase (tracking_parameters.hpp:163-165); CorrelateSplit's split of
+// tap t = early / prompt / late (tracking_parameters.hpp:471-486) on lane t;
+// the tone's per-sample and per-group phasors on lanes 3 and 4.
+__device__ __noinline__ void plan_epoch(ugsdr_track_state& s, EpochPlan& plan, int spm, double fs, double spacing,
+                                        int lane) {
+    double cp = 0.0, off = 0.0, df = 0.0;
+    int period = 1;
+    if (lane == 0) {
+        period = s.code_period_ms;
+        const int64_t k = period == 1 ? 0 : s.epochs % period;
+        cp = s.code_phase - fs / 1e3 * (double)k;
+        off = fs / s.base_code_frequency * spacing;  // GetSamplesPerChip() * spacing
+        double f = -s.carrier_frequency;
+        if (f == 0.0) {
+            plan.pt = 0.0;  // Translate leaves the epoch as it is
+        } else {
+            // -phase reduced to [0, 2 pi) (floor form: within an ulp of the
+            // phase of fmod)
+            const double x = -s.carrier_phase;
+            double phase = x - 2.0 * kPi * floor(x / (2.0 * kPi));
+            if (phase < 0) phase += 2.0 * kPi;
+            if (phase >= 2.0 * kPi) phase -= 2.0 * kPi;
+            if (f < 0) f = fs + f;
+            df = f / fs;
+            plan.pt = phase / (2.0 * kPi);
+        }
+        plan.df = df;
+        s.carrier_phase += 2 * kPi * s.carrier_frequency / 1000.0;  // UpdatePhase
+    }
+    cp = __shfl(cp, 0, 64);
+    off = __shfl(off, 0, 64);
+    df = __shfl(df, 0, 64);
+    period = __shfl(period, 0, 64);
+    if (lane < 3) {
+        const double cps = (double)spm * period;
+        double p = cp + (double)(1 - lane) * off;  // cp + off, cp, cp - off
+        while (p < 0) p += cps;
+        if (p > cps) p = fmod(p, cps);
+        const int fp = (int)ceil(2 * cps - p);
+        int fl = (int)ceil(p);
+        while (fl >= spm) fl -= spm;
+        // p mod spm, exactly: q spm is exact and p - q spm is exact for
+        // p in [q spm, (q + 1) spm)
+        double q = floor(p / spm);
+        double r = p - q * spm;
+        if (r < 0) r += spm;
+        if (r >= spm) r -= spm;
+        plan.fl[lane] = fl;
+        plan.al[lane] = fp & 3;
+        plan.fa[lane] = fp - (fp & 3);
+        plan.rel[lane] = r / spm;
+    } else if (lane == 3 || lane == 4) {
+        float2 w = make_float2(1.f, 0.f);
+        if (df != 0.0) {
+            double t = lane == 3 ? df : 4.0 * kTrackT * df;
+            t -= rint(t);
+            double sn, cs;
+            sincos(2.0 * kPi * t, &sn, &cs);
+            w = make_float2((float)cs, (float)sn);
+        }
+        if (lane == 3)
+            plan.w1 = w;
+        else
+            plan.wg = w;
+    }
+}
+
+// Wave 0 after the correlation of epoch b: lane t joins tap t's two spans
+// over the waves (AddWithPhase, tracking_parameters.hpp:89-98), lane 0 runs
+// the PLL/FLL and DLL updates and the trace, then the next epoch is planned.
+__device__ __noinline__ void finish_epoch(ugsdr_track_state& s, EpochPlan& plan, const float4* red, float2* o,
+                                          double* tr, bool last, int spm, double fs, double spacing, int lane) {
+    float2 e = make_float2(0.f, 0.f);
+    if (lane < 3) {
+        float2 lhs = make_float2(0.f, 0.f), rhs = lhs;
+        for (int w = 0; w < kTrackT / 64; ++w) {
+            const float4 v = red[w * 3 + lane];
+            lhs = cadd(lhs, make_float2(v.x, v.y));
+            rhs = cadd(rhs, make_float2(v.z, v.w));
+        }
+        const double rel = plan.rel[lane];
+        if (rel < 0.5) {
+            const float2 tmp = lhs;
+            lhs = rhs;
+            rhs = tmp;
+        }
+        const double wl = (double)lhs.x / rel, wr = (double)rhs.x / (1.0 - rel);
+        e = fabs(wl + wr) > fabs(wl) ? cadd(lhs, rhs) : csub(lhs, rhs);
+        o[lane] = e;
+    }
+    const float2 early = make_float2(__shfl(e.x, 0, 64), __shfl(e.y, 0, 64));
+    const float2 prompt = make_float2(__shfl(e.x, 1, 64), __shfl(e.y, 1, 64));
+    const float2 late = make_float2(__shfl(e.x, 2, 64), __shfl(e.y, 2, 64));
+    if (lane == 0) {
+        pll_update(s, prompt);
+        dll_update(s, early, late, fs);
+        ++s.epochs;
+        if (tr) {
+            tr[0] = -s.carrier_phase / (2 * kPi);
+            tr[1] = s.carrier_frequency;
+            tr[2] = s.code_phase;
+            tr[3] = s.code_frequency;
+        }
+    }
+    if (!last) plan_epoch(s, plan, spm, fs, spacing, lane);
+}
+
+// Sums the six complex partials of every lane over the wave by halving:
+// after the exchanges over lane bits 5 and 4 each quarter of the wave holds
+// three of the twelve components, summed over its partners, and a butterfly
+// over bits 3..0 completes them.  Lane 16 q ends with components 3q..3q+2.
+__device__ __forceinline__ void wave_reduce12(float (&v)[12], int lane, float (&out)[3]) {
+    float h[6];
+    const bool b5 = lane & 32;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const float keep = b5 ? v[6 + i] : v[i], give = b5 ? v[i] : v[6 + i];
+        h[i] = keep + __shfl_xor(give, 32, 64);
+    }
+    const bool b4 = lane & 16;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float keep = b4 ? h[3 + i] : h[i], give = b4 ? h[i] : h[3 + i];
+        out[i] = keep + __shfl_xor(give, 16, 64);
+    }
+#pragma unroll
+    for (int m = 8; m > 0; m >>= 1)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out[i] += __shfl_xor(out[i], m, 64);
+}
+
+template <int FMT>
 __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
     const int c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     __shared__ EpochPlan plan;
-    __shared__ float2 red[(kTrackT / 64) * 6];
+    __shared__ float4 red[(kTrackT / 64) * 3];
     __shared__ ugsdr_track_state s;
-    const float* code3 = a.code3 + (int64_t)c * a.code3_len;
+    const float* tabs = a.tabs + (int64_t)c * 4 * a.len3p;
+    const int spm = (int)a.spm, nfull = spm >> 2;
     if (tid == 0) s = a.state[c];
+    if (wave == 0) plan_epoch(s, plan, spm, a.fs, a.spacing, lane);
+    __syncthreads();
     for (int b = 0; b < a.n_blocks; ++b) {
-        if (tid == 0) {
-            const int64_t spm = a.spm;
-            const double cps = (double)spm * s.code_period_ms;
-            // TrackSingleSatellite (tracker.hpp:165): the code phase of this epoch
-            const double cp = s.code_phase - a.fs / 1e3 * (double)(s.epochs % s.code_period_ms);
-            const double off = a.fs / s.base_code_frequency * a.spacing;  // GetSamplesPerChip() * spacing
-            const double taps[3] = {cp + off, cp, cp - off};
-            for (int t = 0; t < 3; ++t) {
-                double p = taps[t];
-                while (p < 0) p += cps;
-                if (p > cps) p = fmod(p, cps);
-                plan.first_len[t] = (int)((int64_t)ceil(p) % spm);
-                plan.first_phase[t] = (int)(int64_t)ceil(2 * cps - p);
-                plan.second_phase[t] = plan.first_phase[t] + plan.first_len[t];
-                plan.rel[t] = fmod(p, (double)spm) / (double)spm;
-            }
-            // Translate(x, fs, -f_carr, -phase_carr), in place
-            double phase = fmod(-s.carrier_phase, 2.0 * kPi);
-            if (phase < 0) phase += 2.0 * kPi;
-            double f = -s.carrier_frequency;
-            plan.mix = f != 0.0;
-            if (f < 0) f = a.fs + f;
-            plan.df = f / a.fs;
-            plan.ph_turns = phase / (2.0 * kPi);
-            s.carrier_phase += 2 * kPi * s.carrier_frequency / 1000.0;  // UpdatePhase
-        }
-        __syncthreads();
-        float2 acc[6];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) acc[i] = make_float2(0.f, 0.f);
         const int64_t base = (int64_t)b * a.spm;
-        const double df = plan.df, pt = plan.ph_turns;
-        const int mix = plan.mix;
-        int fl[3], fp[3], sp[3];
+        int fl[3];
+        const float4* tab[3];
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
-            fl[t] = plan.first_len[t];
-            fp[t] = plan.first_phase[t];
-            sp[t] = plan.second_phase[t];
+            fl[t] = plan.fl[t];
+            tab[t] = reinterpret_cast<const float4*>(tabs + plan.al[t] * a.len3p + plan.fa[t]);
         }
-        for (int n = tid; n < a.spm; n += kTrackT) {
-            float2 x = load_sample(a.src, base + n);
-            if (mix) {
+        const float2 wg = plan.wg, w1 = plan.w1;
+        const double df = plan.df, pt = plan.pt;
+        float2 z[4];
+        {
+            double ph = fma(4.0 * tid, df, pt);
+            ph -= rint(ph);
+            float sn, cs;
+            sincosf((float)(2.0 * kPi * ph), &sn, &cs);
+            z[0] = df == 0.0 ? make_float2(1.f, 0.f) : make_float2(cs, sn);
+            z[1] = cmul(z[0], w1);
+            z[2] = cmul(z[1], w1);
+            z[3] = cmul(z[2], w1);
+        }
+        float2 tot[3], snap[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) tot[t] = snap[t] = make_float2(0.f, 0.f);
+        // groups of four whole samples, the next group's loads in flight while
+        // the current one is summed (the prefetch index is clamped in range)
+        float2 xn[4];
+        float4 cn[3];
+        auto fetch = [&](int g) {
+            g = min(g, nfull - 1);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xn[u] = load_sample_fmt<FMT>(a.src, base + 4 * g + u);
+#pragma unroll
+            for (int t = 0; t < 3; ++t) cn[t] = tab[t][g];
+        };
+        fetch(tid);
+        for (int g = tid; g < nfull; g += kTrackT) {
+            float2 x[4];
+            float4 cv[3];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = xn[u];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) cv[t] = cn[t];
+            fetch(g + kTrackT);
+            const int n0 = 4 * g;
+            float2 xm[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                xm[u] = cmul(x[u], z[u]);
+                z[u] = cmul(z[u], wg);
+            }
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const float cu[4] = {cv[t].x, cv[t].y, cv[t].z, cv[t].w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    tot[t].x = fmaf(xm[u].x, cu[u], tot[t].x);
+                    tot[t].y = fmaf(xm[u].y, cu[u], tot[t].y);
+                }
+                if (n0 + 4 <= fl[t]) {
+                    snap[t] = tot[t];
+                } else if (n0 < fl[t]) {
+                    // the split falls inside this group: the first span ends at fl
+                    float2 sp = tot[t];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (n0 + u >= fl[t]) {
+                            sp.x = fmaf(-xm[u].x, cu[u], sp.x);
+                            sp.y = fmaf(-xm[u].y, cu[u], sp.y);
+                        }
+                    snap[t] = sp;
+                }
+            }
+        }
+        // the epoch's last spm mod 4 samples, one per thread; every group of
+        // such a thread lies before them, so its snapshot is its running sum
+        // whenever the sample itself is in the first span
+        if (tid < spm - 4 * nfull) {
+            const int n = 4 * nfull + tid;
+            float2 xm = load_sample_fmt<FMT>(a.src, base + n);
+            if (df != 0.0) {
                 double ph = fma((double)n, df, pt);
                 ph -= rint(ph);
                 float sn, cs;
                 sincosf((float)(2.0 * kPi * ph), &sn, &cs);
-                x = cmul(x, make_float2(cs, sn));
+                xm = cmul(xm, make_float2(cs, sn));
             }
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
-                const bool first = n < fl[t];
-                const float cv = code3[first ? fp[t] + n : sp[t] + (n - fl[t])];
-                float2& ac = acc[2 * t + (first ? 0 : 1)];
-                ac.x = fmaf(x.x, cv, ac.x);
-                ac.y = fmaf(x.y, cv, ac.y);
+                const float cu = tabs[plan.fa[t] + plan.al[t] + n];
+                tot[t].x = fmaf(xm.x, cu, tot[t].x);
+                tot[t].y = fmaf(xm.y, cu, tot[t].y);
+                if (n < fl[t]) snap[t] = tot[t];
             }
         }
+        float v[12], part[3];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            acc[i].x = wave_sum(acc[i].x);
-            acc[i].y = wave_sum(acc[i].y);
-            if (lane == 0) red[wave * 6 + i] = acc[i];
+        for (int t = 0; t < 3; ++t) {
+            v[4 * t] = snap[t].x;
+            v[4 * t + 1] = snap[t].y;
+            v[4 * t + 2] = tot[t].x - snap[t].x;
+            v[4 * t + 3] = tot[t].y - snap[t].y;
+        }
+        wave_reduce12(v, lane, part);
+        if ((lane & 15) == 0) {
+            float* r = reinterpret_cast<float*>(red) + wave * 12 + 3 * (lane >> 4);
+            r[0] = part[0];
+            r[1] = part[1];
+            r[2] = part[2];
         }
         __syncthreads();
-        if (tid == 0) {
-            float2 part[6];
-            for (int i = 0; i < 6; ++i) {
-                part[i] = make_float2(0.f, 0.f);
-                for (int w = 0; w < kTrackT / 64; ++w) part[i] = cadd(part[i], red[w * 6 + i]);
-            }
-            float2 epl[3];
-            for (int t = 0; t < 3; ++t) {
-                float2 lhs = part[2 * t], rhs = part[2 * t + 1];
-                if (plan.rel[t] < 0.5) {
-                    const float2 tmp = lhs;
-                    lhs = rhs;
-                    rhs = tmp;
-                }
-                const double wl = (double)lhs.x / plan.rel[t], wr = (double)rhs.x / (1.0 - plan.rel[t]);
-                epl[t] = fabs(wl + wr) > fabs(wl) ? cadd(lhs, rhs) : csub(lhs, rhs);
-            }
-            float2* o = a.epl + ((int64_t)b * a.n_ch + c) * 3;
-            o[0] = epl[0];
-            o[1] = epl[1];
-            o[2] = epl[2];
-            pll_update(s, epl[1]);
-            dll_update(s, epl[0], epl[2], a.fs);
-            ++s.epochs;
-            if (a.trace) {
-                double* tr = a.trace + ((int64_t)b * a.n_ch + c) * 4;
-                tr[0] = -s.carrier_phase / (2 * kPi);
-                tr[1] = s.carrier_frequency;
-                tr[2] = s.code_phase;
-                tr[3] = s.code_frequency;
-            }
-        }
+        if (wave == 0)
+            finish_epoch(s, plan, red, a.epl + ((int64_t)b * a.n_ch + c) * 3,
+                         a.trace ? a.trace + ((int64_t)b * a.n_ch + c) * 4 : nullptr, b + 1 == a.n_blocks, spm, a.fs,
+                         a.spacing, lane);
         __syncthreads();
     }
+    // no plan is made past the last epoch: the state is the one after n_blocks
     if (tid == 0) a.state[c] = s;
+}
+
+// The 3-period replica table of Codes::FillCodesImpl (tracker.hpp:39-51):
+// entry i of the 3 P spm table is chip floor(i L / (P spm)) mod L, the
+// floor-index upsampling of the code repeated three times.  Copy a holds
+// entry (j + a) mod 3 P spm at j, so that the reads of a tap whose table
+// offset is a mod 4 start on 16-byte boundaries.
+__global__ void track_tables_kernel(const int8_t* codes, int64_t code_len, int64_t per_period, int64_t len3,
+                                    int64_t len3p, int n_ch, float* tabs) {
+    const int64_t total = (int64_t)n_ch * 4 * len3p;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = k % len3p, ca = k / len3p;
+        const int64_t al = ca & 3, ch = ca >> 2;
+        const int64_t i = (j + al) % len3;
+        tabs[k] = (float)codes[ch * code_len + (i * code_len / per_period) % code_len];
+    }
 }
 
 static int track_impl(ugsdr_hip_ctx* ctx, const ugsdr_track_request* r, ugsdr_track_state* ch, float* epl,
                       double* trace) {
     UGSDR_CHECK_ARG(r && ch && epl, "null argument");
-    UGSDR_CHECK_ARG(r->iq && r->spm > 0 && r->n_blocks > 0 && r->fs > 0, "bad signal");
+    UGSDR_CHECK_ARG(r->iq && r->spm >= 4 && r->n_blocks > 0 && r->fs > 0, "bad signal");
     UGSDR_CHECK_ARG(r->n_channels > 0 && r->codes && r->code_len > 0, "bad channels");
     UGSDR_CHECK_ARG(r->spacing_chips >= 0 && r->spacing_chips < 1, "bad early/late spacing");
     UGSDR_CHECK_ARG(r->n_blocks < (1 << 30) && r->spm < (1 << 24) && r->spm * r->n_blocks < ((int64_t)1 << 31),
@@ -214,6 +391,7 @@ static int track_impl(ugsdr_hip_ctx* ctx, const ugsdr_track_request* r, ugsdr_tr
         UGSDR_CHECK_ARG(ch[c].code_period_ms == period && period >= 1, "channels of one call share a code period");
         UGSDR_CHECK_ARG(ch[c].base_code_frequency > 0, "bad code frequency");
     }
+    UGSDR_CHECK_ARG(3 * r->spm * period + 8 < ((int64_t)1 << 31), "replica table beyond 2^31 samples");
     SampleSource src{};
     if (int rc = format_geometry(r->fmt, r->fs, src.spm, src.epoch_bytes)) return rc;
     src.fmt = r->fmt;
@@ -229,20 +407,34 @@ static int track_impl(ugsdr_hip_ctx* ctx, const ugsdr_track_request* r, ugsdr_tr
     } else {
         src.p = r->iq;
     }
-    // 3-period replica tables at fs (Codes::FillCodesImpl, tracker.hpp:39-51):
-    // floor-index upsampling of the chips repeated three times
-    const int64_t cps = r->spm * period, len3 = 3 * cps, chips3 = 3 * r->code_len;
-    std::vector<float> code3((size_t)r->n_channels * len3);
-    for (int c = 0; c < r->n_channels; ++c)
-        for (int64_t i = 0; i < len3; ++i)
-            code3[(size_t)c * len3 + i] =
-                (float)r->codes[(size_t)c * r->code_len + ((i * chips3 / len3) % chips3) % r->code_len];
-    const size_t b_code = code3.size() * 4, b_state = (size_t)r->n_channels * sizeof(ugsdr_track_state);
-    const size_t o_state = (b_code + 255) / 256 * 256;
-    UGSDR_HIP_TRY(ctx->tables.reserve(o_state + b_state));
+    // replica tables, rebuilt on the device when the codes or the rate change
+    const int64_t cps = r->spm * period, len3 = 3 * cps, len3p = (len3 + 4 + 3) / 4 * 4;
+    auto& tt = ctx->track;
+    const size_t n_codes = (size_t)r->n_channels * r->code_len;
+    const size_t b_tabs = (size_t)r->n_channels * 4 * len3p * 4, o_codes = (b_tabs + 255) / 256 * 256;
+    const bool same = tt.n_ch == r->n_channels && tt.spm == r->spm && tt.period == period &&
+                      tt.len3p == len3p && tt.codes.size() == n_codes &&
+                      std::equal(tt.codes.begin(), tt.codes.end(), r->codes);
+    if (!same) {
+        tt.n_ch = 0;
+        UGSDR_HIP_TRY(tt.buf.reserve(o_codes + n_codes));
+        UGSDR_HIP_TRY(hipMemcpyAsync(tt.buf.as<char>() + o_codes, r->codes, n_codes, hipMemcpyHostToDevice, st));
+        const int64_t total = (int64_t)r->n_channels * 4 * len3p;
+        const unsigned grid = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(track_tables_kernel, dim3(grid), dim3(256), 0, st,
+                           reinterpret_cast<const int8_t*>(tt.buf.as<char>() + o_codes), r->code_len, cps, len3,
+                           len3p, r->n_channels, tt.buf.as<float>());
+        UGSDR_HIP_TRY(hipGetLastError());
+        tt.codes.assign(r->codes, r->codes + n_codes);
+        tt.n_ch = r->n_channels;
+        tt.spm = r->spm;
+        tt.period = period;
+        tt.len3p = len3p;
+    }
+    const size_t b_state = (size_t)r->n_channels * sizeof(ugsdr_track_state);
+    UGSDR_HIP_TRY(ctx->tables.reserve(b_state));
     char* d = ctx->tables.as<char>();
-    UGSDR_HIP_TRY(hipMemcpyAsync(d, code3.data(), b_code, hipMemcpyHostToDevice, st));
-    UGSDR_HIP_TRY(hipMemcpyAsync(d + o_state, ch, b_state, hipMemcpyHostToDevice, st));
+    UGSDR_HIP_TRY(hipMemcpyAsync(d, ch, b_state, hipMemcpyHostToDevice, st));
     const size_t b_epl = (size_t)r->n_blocks * r->n_channels * 3 * sizeof(float2);
     const size_t b_trace = trace ? (size_t)r->n_blocks * r->n_channels * 4 * sizeof(double) : 0;
     const size_t o_trace = (b_epl + 255) / 256 * 256;
@@ -254,17 +446,37 @@ static int track_impl(ugsdr_hip_ctx* ctx, const ugsdr_track_request* r, ugsdr_tr
     a.n_ch = r->n_channels;
     a.fs = r->fs;
     a.spacing = r->spacing_chips;
-    a.code3 = reinterpret_cast<const float*>(d);
-    a.code3_len = len3;
-    a.state = reinterpret_cast<ugsdr_track_state*>(d + o_state);
+    a.tabs = tt.buf.as<float>();
+    a.len3p = len3p;
+    a.state = reinterpret_cast<ugsdr_track_state*>(d);
     a.epl = ctx->results.as<float2>();
     a.trace = trace ? reinterpret_cast<double*>(ctx->results.as<char>() + o_trace) : nullptr;
-    hipLaunchKernelGGL(track_kernel, dim3((unsigned)r->n_channels), dim3(kTrackT), 0, st, a);
+    ctx->mark(0);
+    switch (r->fmt) {
+        case UGSDR_FMT_CI8:
+            hipLaunchKernelGGL(track_kernel<UGSDR_FMT_CI8>, dim3((unsigned)r->n_channels), dim3(kTrackT), 0, st, a);
+            break;
+        case UGSDR_FMT_CI16:
+            hipLaunchKernelGGL(track_kernel<UGSDR_FMT_CI16>, dim3((unsigned)r->n_channels), dim3(kTrackT), 0, st, a);
+            break;
+        case UGSDR_FMT_CF32:
+            hipLaunchKernelGGL(track_kernel<UGSDR_FMT_CF32>, dim3((unsigned)r->n_channels), dim3(kTrackT), 0, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL(track_kernel<-1>, dim3((unsigned)r->n_channels), dim3(kTrackT), 0, st, a);
+            break;
+    }
     UGSDR_HIP_TRY(hipGetLastError());
+    ctx->mark(1);
     UGSDR_HIP_TRY(hipMemcpyAsync(epl, a.epl, b_epl, hipMemcpyDeviceToHost, st));
     if (trace) UGSDR_HIP_TRY(hipMemcpyAsync(trace, a.trace, b_trace, hipMemcpyDeviceToHost, st));
     UGSDR_HIP_TRY(hipMemcpyAsync(ch, a.state, b_state, hipMemcpyDeviceToHost, st));
     UGSDR_HIP_TRY(hipStreamSynchronize(st));
+    if (ctx->profile) {
+        // the loop kernel's time goes in the correlation slot
+        ctx->last_ms[0] = ctx->last_ms[1] = ctx->last_ms[3] = 0.f;
+        UGSDR_HIP_TRY(hipEventElapsedTime(&ctx->last_ms[2], ctx->ev[0], ctx->ev[1]));
+    }
     return UGSDR_OK;
 }
 
