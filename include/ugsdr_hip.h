@@ -148,7 +148,14 @@ typedef struct {
     int64_t n_prn;
     double threshold;      /* 3.3 in the reference (fse.hpp:150)                     */
     int32_t precision;     /* UGSDR_PREC_*                                           */
-    int32_t reserved;
+    int32_t hypotheses;    /* 0 or 1: one result per replica.  H > 1: each run of H
+                            * consecutive replicas is one satellite's code
+                            * hypotheses (the Galileo E1 sign permutations of
+                            * fse.hpp:374-389): one result per run, the hypothesis
+                            * of largest level (the first on ties, max_element),
+                            * detected when its SNR clears the threshold.  n_prn
+                            * must be a multiple of H; surfaces and peaks stay per
+                            * replica.                                              */
 } ugsdr_acq_request;
 
 typedef struct {
@@ -170,7 +177,9 @@ typedef struct {
  * (src/acquisition/fse.hpp:242-276, generalised to K blocks): mix, resample,
  * per (PRN, Doppler, block) circular correlation against the cached replica
  * spectrum, |.| and fold, sum over blocks, first-index max, mean, n-1 sigma,
- * strict-< Doppler selection, threshold.  out: n_prn results.  surfaces
+ * strict-< Doppler selection, threshold.  out: n_prn results (n_prn / H with
+ * hypotheses H > 1: FastSearchEngineBase::ProcessGalileo's selection,
+ * fse.hpp:356-396).  surfaces
  * (optional host buffer): n_prn x n_doppler x surface_len.  peaks (optional):
  * n_prn x surface_len, the output_peak of the selected bin. */
 int ugsdr_hip_acquire(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* req, ugsdr_acq_result* out, float* surfaces,

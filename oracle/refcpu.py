@@ -358,6 +358,44 @@ def acquire(signal: np.ndarray, replicas: np.ndarray, *, n_fft: int, n_blocks: i
     return res, surf, peaks
 
 
+def reduce_hypotheses(results: list, h: int, threshold: float = 3.3) -> list:
+    """FastSearchEngineBase::ProcessGalileo's choice (src/acquisition/fse.hpp:
+    382-388): per run of h results, std::max_element by level (the first of
+    the largest), pushed when its SNR clears the threshold"""
+    out = []
+    for g in range(len(results) // h):
+        run = results[g * h:(g + 1) * h]
+        best = 0
+        for k in range(1, h):
+            if run[best]["level"] < run[k]["level"]:
+                best = k
+        r = dict(run[best])
+        r["detected"] = int(r["snr"] > threshold)
+        r["hypothesis"] = best
+        out.append(r)
+    return out
+
+
+def process_galileo(signal: np.ndarray, chips_list, *, spm: int, fs_in: float, fs_acq: float, doppler,
+                    intermediate_frequency: float = 0.0, threshold: float = 3.3, threads: int | None = None):
+    """FastSearchEngineBase::ProcessGalileo (src/acquisition/fse.hpp:356-396)
+    on a signal already at fs_acq: per satellite the 4 ms code is upsampled to
+    4 spm samples (Upsampler::Transform, floor index), each of the four sign
+    permutations (:374-379) multiplies its milliseconds, ProcessBpsk<false>
+    (no fold, no threshold) runs every hypothesis, and the largest level wins"""
+    perms = [[1, 1, 1, 1], [1, 1, 1, -1], [1, 1, -1, -1], [1, -1, -1, -1]]
+    n = 4 * spm
+    reps = []
+    for chips in chips_list:
+        base = repeat_upsample(np.asarray(chips, np.int8), 1, n)
+        for p in perms:
+            reps.append(base * np.repeat(np.asarray(p, np.float32), spm))
+    res, _, _ = acquire(signal, np.stack(reps), n_fft=n, n_blocks=1, surface_len=n, fs_in=fs_in, fs_acq=fs_acq,
+                        doppler=doppler, reshape=False, intermediate_frequency=intermediate_frequency,
+                        threshold=threshold, threads=threads)
+    return reduce_hypotheses(res, 4, threshold)
+
+
 def correlate(signal: np.ndarray, code: np.ndarray) -> complex:
     if len(signal) != len(code):
         raise RuntimeError("Size mismatch")

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <complex>
 #include <cstdio>
+#include <random>
 #include <fstream>
 #include <map>
 #include <utility>
@@ -238,6 +239,54 @@ static int tracker_lock() {
     return 0;
 }
 
+// ProcessGalileo: two satellites whose milliseconds carry sign permutations
+// 2 and 3 of the four, plus one absent code; 4 ms at 4.092 MHz, complex
+// float.  Both are found at their planted code phase, the absent one is not.
+static int galileo_hypotheses() {
+    const double fs = 4.092e6;
+    const int spm = 4092, n = 4 * spm;
+    const int perms[4][4] = {{1, 1, 1, 1}, {1, 1, 1, -1}, {1, 1, -1, -1}, {1, -1, -1, -1}};
+    std::mt19937 gen(42);
+    std::vector<std::vector<int8_t>> codes(3);
+    for (auto& c : codes) {
+        c.resize(8184);
+        for (std::size_t i = 0; i < 4092; ++i) {
+            const int8_t v = (gen() & 1) ? 1 : -1;
+            c[2 * i] = v;
+            c[2 * i + 1] = (int8_t)-v;
+        }
+    }
+    const int planted[2] = {2, 3}, delay[2] = {777, 12001};
+    const double dop[2] = {1250.0, -2750.0};
+    std::vector<std::complex<float>> x(n);
+    std::normal_distribution<double> noise(0.0, 1.0);
+    for (int k = 0; k < n; ++k) {
+        std::complex<double> v(noise(gen), noise(gen));
+        for (int s = 0; s < 2; ++s) {
+            const int m = ((k - delay[s]) % n + n) % n;
+            const double c = codes[s][(std::size_t)m * 8184 / n] * perms[planted[s]][m / spm];
+            v += 0.1 * c * std::polar(1.0, 2 * M_PI * dop[s] * k / fs);
+        }
+        x[k] = std::complex<float>(v);
+    }
+    ugsdr::HipFseOptions opt;
+    opt.coherent_ms = 4;
+    opt.target_sampling_rate = fs;
+    ugsdr::HipFastSearchEngine engine(5000, 250, opt);
+    std::vector<ugsdr::Sv> svs(3);
+    for (int s = 0; s < 3; ++s) svs[s] = {s, ugsdr::System::Galileo, ugsdr::Signal::Galileo_E1b};
+    const auto found = engine.ProcessGalileo(x.data(), UGSDR_FMT_CF32, n, fs, 0.0, svs, codes);
+    EXPECT(found.size() == 2);
+    for (int s = 0; s < 2 && s < (int)found.size(); ++s) {
+        EXPECT(found[s].sv_number.id == s);
+        EXPECT(std::abs(found[s].code_offset - delay[s]) < 0.5);
+        EXPECT(std::abs(found[s].doppler - dop[s]) < 1.0);
+        std::printf("galileo: sv %d code offset %.0f doppler %.0f snr %.2f\n", s, found[s].code_offset,
+                    found[s].doppler, found[s].GetSnr());
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s real_8_first4ms.bin bbp_ddc_gps_L1_first4ms.bin\n", argv[0]);
@@ -248,6 +297,7 @@ int main(int argc, char** argv) {
         if (int rc = real8_kat(argv[1])) return rc;
         if (int rc = bbp_recording(argv[2])) return rc;
         if (int rc = tracker_lock()) return rc;
+        if (int rc = galileo_hypotheses()) return rc;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "exception: %s\n", e.what());
         return 1;

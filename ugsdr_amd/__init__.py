@@ -37,7 +37,7 @@ class AcqRequest(C.Structure):
         ("surface_len", C.c_int64), ("reshape", C.c_int32), ("coherent", C.c_int32),
         ("doppler", C.POINTER(C.c_double)), ("n_doppler", C.c_int64),
         ("replica_keys", C.POINTER(C.c_uint64)), ("n_prn", C.c_int64), ("threshold", C.c_double),
-        ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("precision", C.c_int32), ("hypotheses", C.c_int32),
     ]
 
 
@@ -269,6 +269,25 @@ def boc(chips: np.ndarray) -> np.ndarray:
     return out
 
 
+# The per-ms sign permutations FastSearchEngineBase::ProcessGalileo tries on
+# the 4 ms E1 code (src/acquisition/fse.hpp:374-379)
+E1_SIGN_PERMUTATIONS = np.array([[1, 1, 1, 1], [1, 1, 1, -1], [1, 1, -1, -1], [1, -1, -1, -1]], np.int8)
+
+
+def e1_sign_hypotheses(chips: np.ndarray, n_fft: int) -> np.ndarray:
+    """one satellite's four replicas (fse.hpp:371-385): the code upsampled to
+    n_fft samples by floor index (Upsampler::Transform), then millisecond i
+    (n_fft / 4 samples) multiplied by sign i of each permutation.  Acquire them
+    as consecutive replicas with hypotheses=4."""
+    if n_fft % 4:
+        raise ValueError("the 4 ms code needs n_fft divisible by 4")
+    chips = np.asarray(chips, np.float32)
+    i = np.arange(n_fft, dtype=np.int64)
+    rep = chips[(i * len(chips)) // n_fft]
+    signs = np.repeat(E1_SIGN_PERMUTATIONS.astype(np.float32), n_fft // 4, axis=1)
+    return np.ascontiguousarray(rep[None, :] * signs)
+
+
 def adjust_sampling_rate(fs: float, target: float) -> float:
     return lib().ugsdr_adjust_sampling_rate(fs, target)
 
@@ -410,7 +429,7 @@ class Context:
                      surface_len: int, doppler, keys, reshape: bool = True, coherent: bool = True,
                      mix_frequency: float = 0.0, mix_phase: float = 0.0, intermediate_frequency: float = 0.0,
                      threshold: float = 3.3, precision: int = PREC_FP32, n_samples: int | None = None,
-                     iq_device_ptr: int | None = None):
+                     iq_device_ptr: int | None = None, hypotheses: int = 0):
         dop = np.ascontiguousarray(doppler, np.float64)
         keys = np.ascontiguousarray(keys, np.uint64)
         req = AcqRequest()
@@ -441,12 +460,14 @@ class Context:
         req.n_prn = len(keys)
         req.threshold = threshold
         req.precision = precision
+        req.hypotheses = hypotheses
         return req, (hold, dop, keys)
 
     def acquire_req(self, req, want_surfaces: bool = False, want_peaks: bool = False, raw: bool = False):
-        """results as dicts per PRN (raw: the ugsdr_acq_result ctypes array, no conversion)"""
+        """results as dicts per PRN, or per satellite with hypotheses > 1 (raw:
+        the ugsdr_acq_result ctypes array, no conversion)"""
         P, D, L = req.n_prn, req.n_doppler, req.surface_len
-        out = (AcqResult * P)()
+        out = (AcqResult * (P // max(1, req.hypotheses)))()
         surf = np.zeros((P, D, L), np.float32) if want_surfaces else None
         peaks = np.zeros((P, L), np.float32) if want_peaks else None
         _check(lib().ugsdr_hip_acquire(self._p, C.byref(req), out,
@@ -640,7 +661,7 @@ class Server:
     def submit(self, req: AcqRequest) -> int:
         t = C.c_uint64()
         _check(lib().ugsdr_hip_server_submit(self._p, C.byref(req), C.byref(t)))
-        self._n_prn[t.value] = req.n_prn
+        self._n_prn[t.value] = req.n_prn // max(1, req.hypotheses)
         return t.value
 
     def poll(self, ticket: int, wait_ms: int = -1):

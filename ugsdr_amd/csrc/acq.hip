@@ -630,6 +630,8 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
     UGSDR_CHECK_ARG(r->n_fft > 0 && r->n_blocks > 0 && r->surface_len > 0, "bad sizes");
     UGSDR_CHECK_ARG(r->n_doppler > 0 && r->doppler, "no Doppler bins");
     UGSDR_CHECK_ARG(r->n_prn > 0 && r->replica_keys, "no replicas");
+    UGSDR_CHECK_ARG(r->hypotheses >= 0 && (r->hypotheses <= 1 || r->n_prn % r->hypotheses == 0),
+                    "n_prn must be a multiple of the hypothesis count");
     UGSDR_CHECK_ARG(r->fs_in > 0 && r->fs_acq > 0, "bad sampling rate");
     if (r->n_fft % r->surface_len) {
         set_error("Signal size is not divided by block size");
@@ -1076,7 +1078,7 @@ static std::string graph_key(const ugsdr_acq_request* r, bool surfaces, bool pea
     if (!c.iq_on_device) c.iq = nullptr;
     c.doppler = nullptr;
     c.replica_keys = nullptr;
-    c.reserved = 0;
+    c.hypotheses = 0;
     std::string k(reinterpret_cast<const char*>(&c), sizeof c);
     k.append(reinterpret_cast<const char*>(r->doppler), (size_t)r->n_doppler * sizeof(double));
     k.append(reinterpret_cast<const char*>(r->replica_keys), (size_t)r->n_prn * sizeof(uint64_t));
@@ -1168,6 +1170,17 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
     return UGSDR_OK;
 }
 
+void reduce_hypotheses(const ugsdr_acq_result* in, int64_t n, int h, double threshold, ugsdr_acq_result* out) {
+    for (int64_t g = 0; g < n / h; ++g) {
+        const ugsdr_acq_result* run = in + g * h;
+        int best = 0;
+        for (int k = 1; k < h; ++k)
+            if (run[best].level < run[k].level) best = k;
+        out[g] = run[best];
+        out[g].detected = run[best].snr > threshold ? 1 : 0;
+    }
+}
+
 }  // namespace ugsdr_hip
 
 extern "C" int ugsdr_hip_acquire(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* req, ugsdr_acq_result* out,
@@ -1178,7 +1191,15 @@ extern "C" int ugsdr_hip_acquire(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* re
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
     UGSDR_HIP_TRY(hipSetDevice(ctx->device));
-    return ugsdr_hip::acquire_impl(ctx, req, out, surfaces, peaks);
+    if (!req || req->hypotheses <= 1 || !out) return ugsdr_hip::acquire_impl(ctx, req, out, surfaces, peaks);
+    if (req->n_prn <= 0 || req->n_prn % req->hypotheses) {
+        ugsdr_hip::set_error("n_prn must be a multiple of the hypothesis count");
+        return UGSDR_ERR_ARG;
+    }
+    std::vector<ugsdr_acq_result> all((size_t)req->n_prn);
+    if (int rc = ugsdr_hip::acquire_impl(ctx, req, all.data(), surfaces, peaks)) return rc;
+    ugsdr_hip::reduce_hypotheses(all.data(), req->n_prn, req->hypotheses, req->threshold, out);
+    return UGSDR_OK;
 }
 
 extern "C" int ugsdr_hip_autotune(ugsdr_hip_ctx* ctx, int64_t n_fft, int32_t* threads, float* ms_out, int32_t n_ms) {

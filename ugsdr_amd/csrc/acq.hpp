@@ -89,6 +89,11 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
 // pair has a plan, the 16-bit packing of the class spectra, and one
 // correlation launch over a batch of requests of one bucket.
 bool reduced_supported(int64_t N, int64_t L);
+// ProcessGalileo's choice among a satellite's code hypotheses
+// (fse.hpp:382-388): per run of h consecutive results, the first of largest
+// level, detected when its SNR clears the threshold.  in: n results, out:
+// n / h results (h > 1).
+void reduce_hypotheses(const ugsdr_acq_result* in, int64_t n, int h, double threshold, ugsdr_acq_result* out);
 size_t reduced_scratch_bytes(int64_t N, int max_k, int grid);
 int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
 int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots,

@@ -400,7 +400,12 @@ int ugsdr_hip_server_poll(ugsdr_hip_server* h, uint64_t ticket, int wait_ms, ugs
         set_error(j->message);
         return j->error;
     }
-    if (out) std::memcpy(out, j->results.data(), j->results.size() * sizeof(ugsdr_acq_result));
+    if (out) {
+        if (j->req.hypotheses > 1)
+            reduce_hypotheses(j->results.data(), (int64_t)j->results.size(), j->req.hypotheses, j->req.threshold, out);
+        else
+            std::memcpy(out, j->results.data(), j->results.size() * sizeof(ugsdr_acq_result));
+    }
     return UGSDR_OK;
 }
 

@@ -233,12 +233,88 @@ public:
         return dst;
     }
 
+    // FastSearchEngineBase::ProcessGalileo (fse.hpp:356-396): every
+    // satellite's 4 ms E1 code (BOC-expanded chips, codes[i]) in the four
+    // per-ms sign permutations of fse.hpp:374-379, searched unfolded over
+    // 4 ms in one fused call; per satellite the hypothesis of largest level is
+    // kept when its SNR clears the threshold.
+    std::vector<AcquisitionResult<float>> ProcessGalileo(const void* iq, int fmt, std::size_t n_samples, double fs,
+                                                         double intermediate_frequency, const std::vector<Sv>& svs,
+                                                         const std::vector<std::vector<int8_t>>& codes) {
+        static constexpr int kPermutations[4][4] = {{1, 1, 1, 1}, {1, 1, 1, -1}, {1, 1, -1, -1}, {1, -1, -1, -1}};
+        const double new_fs = AdjustSamplingRate(fs, options.target_sampling_rate);
+        const auto spm = (std::size_t)(new_fs / 1e3);
+        const std::size_t n_fft = 4 * spm;
+        std::vector<double> doppler;
+        for (double d = -doppler_range; d <= doppler_range; d += doppler_step) doppler.push_back(d);
+        std::vector<uint64_t> replica_keys;
+        std::vector<float> rep(n_fft);
+        for (std::size_t i = 0; i < svs.size(); ++i) {
+            const auto& c = codes[i];
+            for (int h = 0; h < 4; ++h) {
+                // Upsampler::Transform (floor index), then ms j times its sign
+                for (std::size_t k = 0; k < n_fft; ++k)
+                    rep[k] = (float)(c[(k * c.size()) / n_fft] * kPermutations[h][k / spm]);
+                replica_keys.push_back(SampleKey(rep, c, h));
+            }
+        }
+        ugsdr_acq_request r{};
+        r.iq = iq;
+        r.fmt = fmt;
+        r.n_samples = (int64_t)n_samples;
+        r.fs_in = fs;
+        r.fs_acq = new_fs;
+        r.mix_frequency = -intermediate_frequency;
+        r.intermediate_frequency = intermediate_frequency;
+        r.n_fft = (int64_t)n_fft;
+        r.n_blocks = 1;
+        r.surface_len = (int64_t)n_fft;
+        r.reshape = 0;    // ProcessBpsk<false>
+        r.coherent = 1;
+        r.doppler = doppler.data();
+        r.n_doppler = (int64_t)doppler.size();
+        r.replica_keys = replica_keys.data();
+        r.n_prn = (int64_t)replica_keys.size();
+        r.threshold = options.threshold;
+        r.precision = options.precision;
+        r.hypotheses = 4;
+        std::vector<ugsdr_acq_result> out(svs.size());
+        ugsdr_hip::check(ugsdr_hip_acquire(context->get(), &r, out.data(), nullptr, nullptr));
+        std::vector<AcquisitionResult<float>> dst;
+        for (std::size_t i = 0; i < svs.size(); ++i) {
+            if (!out[i].detected) continue;
+            AcquisitionResult<float> a;
+            a.sv_number = svs[i];
+            a.doppler = out[i].doppler;
+            a.code_offset = out[i].code_offset;
+            a.level = out[i].level;
+            a.sigma = out[i].sigma;
+            a.intermediate_frequency = intermediate_frequency;
+            dst.push_back(std::move(a));
+        }
+        std::sort(dst.begin(), dst.end(), [](auto& a, auto& b) { return a.sv_number.id < b.sv_number.id; });
+        return dst;
+    }
+
 private:
     double doppler_range;
     double doppler_step;
     Options options;
     ugsdr_hip::HipContext* context;
     std::vector<std::pair<std::vector<int64_t>, uint64_t>> keys;
+
+    // replica from samples, cached by (code, hypothesis, length, precision)
+    uint64_t SampleKey(const std::vector<float>& samples, const std::vector<int8_t>& code, int hypothesis) {
+        std::vector<int64_t> id{-1 - hypothesis, (int64_t)samples.size(), options.precision};
+        id.insert(id.end(), code.begin(), code.end());
+        for (auto& kv : keys)
+            if (kv.first == id) return kv.second;
+        uint64_t k = 0;
+        ugsdr_hip::check(ugsdr_hip_replica_put_samples(context->get(), samples.data(), (int64_t)samples.size(),
+                                                       options.precision, &k));
+        keys.emplace_back(std::move(id), k);
+        return k;
+    }
 
     uint64_t Key(const std::vector<int8_t>& code, std::size_t repeats, std::size_t n_fft) {
         std::vector<int64_t> id{(int64_t)repeats, (int64_t)n_fft, options.precision};

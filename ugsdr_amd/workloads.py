@@ -248,6 +248,46 @@ def bank_l5(seed: int = 0xC4, n_ch: int = 64, n_blocks: int = 1000, fs: float = 
     return BankWorkload("BANK_L5", fs, spm, n_blocks, iq, codes, chans, taps, sats)
 
 
+@dataclass
+class GalileoWorkload:
+    name: str
+    fs: float
+    n_fft: int
+    iq: np.ndarray             # interleaved int16 I/Q, 4 ms
+    codes: list                # per satellite: BOC(1,1)-expanded 4092-chip code (8184)
+    planted: list              # (satellite, permutation, delay in samples, Doppler)
+    doppler: np.ndarray
+
+    @property
+    def cells(self) -> int:    # satellites x permutations x Doppler x code phase
+        return len(self.codes) * 4 * len(self.doppler) * self.n_fft
+
+
+def galileo_e1(seed: int = 0xE1, n_sv: int = 50, n_present: int = 6, fs: float = 8.192e6) -> GalileoWorkload:
+    """E1-like code-hypothesis search: n_sv seeded 4092-chip codes (BOC(1,1)),
+    n_present of them in 4 ms of int16 IQ at fs, each with one of the four
+    per-ms sign permutations, C/N0 U[44, 50] dB-Hz, Doppler on the +-5 kHz /
+    250 Hz grid, sigma 300 LSB"""
+    from . import boc, e1_sign_hypotheses
+    rng = np.random.default_rng(seed)
+    n = int(round(4 * fs / 1e3))
+    codes = [boc(random_code(seed * 7919 + s, 4092)) for s in range(n_sv)]
+    sigma = 300.0
+    k = np.arange(n)
+    x = sigma * (rng.normal(size=n) + 1j * rng.normal(size=n))
+    planted = []
+    for s in rng.choice(n_sv, n_present, replace=False):
+        h, tau, f = int(rng.integers(0, 4)), int(rng.integers(0, n)), 250.0 * int(rng.integers(-20, 21))
+        amp = sigma * np.sqrt(2 * 10 ** (rng.uniform(44, 50) / 10) / fs)
+        rep = e1_sign_hypotheses(codes[s], n)[h]
+        x += amp * np.roll(rep, tau) * np.exp(1j * (2 * np.pi * f * k / fs + rng.uniform(0, 2 * np.pi)))
+        planted.append((int(s), h, tau, f))
+    iq = np.empty(2 * n, np.int16)
+    iq[0::2] = np.clip(np.round(x.real), -32767, 32767)
+    iq[1::2] = np.clip(np.round(x.imag), -32767, 32767)
+    return GalileoWorkload("GALILEO_E1_HYP", fs, n, iq, codes, sorted(planted), doppler_bins(5000, 250))
+
+
 CONFIGS = {
     "L1CA_1MS": l1ca_1ms,
     "L1CA_4MS": l1ca_4ms,
