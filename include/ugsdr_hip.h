@@ -185,6 +185,36 @@ typedef struct {
 int ugsdr_hip_acquire(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* req, ugsdr_acq_result* out, float* surfaces,
                       float* peaks);
 
+/* ------------------------------------------------------ anti-jamming (STFT)
+ * The interference suppression of the digital front end
+ * (src/antijamming/jse.hpp, jamming_detection.hpp, src/math/stft.hpp). */
+#define UGSDR_STFT_BLACKMAN_HARRIS 0 /* IppStft: 4-term Blackman-Harris over N-1 (ipp_stft.hpp:26-45) */
+#define UGSDR_STFT_HANN 1            /* SequentialStft: periodic Hann (stft.hpp:31-37)                 */
+#define UGSDR_JAMMER_NARROWBAND 1
+#define UGSDR_JAMMER_WIDEBAND 2
+typedef struct {
+    int32_t type;           /* UGSDR_JAMMER_* (DetectionResult::InterferenceType) */
+    int32_t frequency_bin;  /* STFT bin, 0..127                                    */
+    double frequency;       /* bin * fs / 128                                      */
+    double jammer_to_noise; /* mean J/N over the columns that voted for it         */
+} ugsdr_jammer;
+/* ShortTimeFourierTransform::Transform (stft.hpp:16-20,40-58): (n - 96) / 32
+ * columns of 128 windowed samples at stride 32, forward DFT without scaling.
+ * out: n_cols x 128 complex float (host buffers). */
+int ugsdr_hip_stft(ugsdr_hip_ctx* ctx, const float* in_c32, int64_t n, int window, float* out_c32);
+/* the container-of-columns overload, OverlapAndAdd (stft.hpp:60-72): inverse
+ * DFT (1/128) of each column added at stride 32; out: 128 + 32 (n_cols - 1)
+ * complex float */
+int ugsdr_hip_istft(ugsdr_hip_ctx* ctx, const float* cols_c32, int64_t n_cols, float* out_c32);
+/* JammingSuppressionEngine::Process (jse.hpp:93-103) in place on n complex
+ * float samples at fs (host, or device memory with on_device): STFT
+ * detection (JammingDetector::Process), then nothing, the whole-signal
+ * notch of +-128 DFT bins per narrowband detection, or the STFT excision of
+ * cells with |X| >= 20 when any detection is wideband.  Up to max_out
+ * detections go to out; *n_out gets their count. */
+int ugsdr_hip_jse_process(ugsdr_hip_ctx* ctx, float* io_c32, int64_t n, double fs, int window, int on_device,
+                          ugsdr_jammer* out, int32_t max_out, int32_t* n_out);
+
 /* ------------------------------------------------------ recorded formats
  * Bytes that hold n samples of fmt at fs (BBP_DDC: whole epochs, headers
  * included, epoch length as in the file geometry of

@@ -435,3 +435,166 @@ def bank_nco(x: np.ndarray, spm: int, n_blocks: int, fs: float, channels, codes:
     lib().refcpu_bank_nco(xb, spm, n_blocks, fs, ch, n_ch, codes, codes.shape[1], taps, len(taps),
                           threads or min(os.cpu_count() or 1, 16), out)
     return out.view(np.complex128).reshape(n_blocks, n_ch, len(taps))
+
+
+# ======================================================= anti-jamming (STFT)
+# Restatement of ShortTimeFourierTransform (src/math/stft.hpp, ipp_stft.hpp),
+# JammingDetector (src/antijamming/jamming_detection.hpp) and
+# JammingSuppressionEngine (src/antijamming/jse.hpp).  Samples and STFT cells
+# are complex64 like the reference's std::complex<float>; the DFTs run in
+# FP64 (numpy pocketfft) and are rounded to complex64.
+
+STFT_WINDOW, STFT_OVERLAP = 128, 96            # stft.hpp:18 defaults
+WINDOW_BLACKMAN_HARRIS, WINDOW_HANN = 0, 1
+
+
+def stft_window(kind: int, n: int = STFT_WINDOW) -> np.ndarray:
+    """IppStft's 4-term Blackman-Harris over n - 1 (ipp_stft.hpp:26-35,
+    accumulated into a float term by term) or SequentialStft's periodic Hann
+    (stft.hpp:31-37)"""
+    w = np.zeros(n, np.float32)
+    if kind == WINDOW_BLACKMAN_HARRIS:
+        a = (0.3635819, 0.4891775, 0.1365995, 0.0106411)
+        for i in range(n):
+            acc = np.float32(0.0)          # float += double: summed in double, rounded per term
+            for j in range(4):
+                acc = np.float32(float(acc) + a[j] * (-1.0) ** j * float(np.cos(2.0 * j * np.pi * i / (n - 1))))
+            w[i] = acc
+    else:
+        for i in range(n):
+            w[i] = np.float32(0.5 * (1 - np.cos(2.0 * np.pi * i / n)))
+    return w
+
+
+def stft(x: np.ndarray, kind: int) -> np.ndarray:
+    """SplitIntoColumns (stft.hpp:40-58): (n - 96) / 32 columns of 128
+    samples at stride 32, windowed, forward DFT without scaling"""
+    x = np.asarray(x, np.complex64)
+    stride = STFT_WINDOW - STFT_OVERLAP
+    ncol = (len(x) - STFT_OVERLAP) // stride
+    w = stft_window(kind)
+    idx = np.arange(ncol)[:, None] * stride + np.arange(STFT_WINDOW)[None, :]
+    cols = (x[idx] * w[None, :]).astype(np.complex64)
+    return np.fft.fft(cols.astype(np.complex128), axis=1).astype(np.complex64)
+
+
+def istft(cols: np.ndarray) -> np.ndarray:
+    """OverlapAndAdd (stft.hpp:60-72): the inverse DFT (1/128) of each column
+    added at stride 32, in column order; 128 + 32 (ncol - 1) samples"""
+    cols = np.asarray(cols, np.complex64)
+    stride = STFT_WINDOW - STFT_OVERLAP
+    out = np.zeros(STFT_WINDOW + (len(cols) - 1) * stride, np.complex64)
+    t = np.fft.ifft(cols.astype(np.complex128), axis=1).astype(np.complex64)
+    for i in range(len(cols)):
+        out[i * stride:i * stride + STFT_WINDOW] += t[i]
+    return out
+
+
+def null_region_detector(index: int, null: int, data: np.ndarray) -> None:
+    """JammingDetector::NullRegion (jamming_detection.hpp:105-113), size_t
+    arithmetic: start and finish taken modulo the size"""
+    n = len(data)
+    start = (index - null + n) % n
+    finish = (index + null + n) % n
+    if finish < start:
+        finish += n
+    for i in range(start, finish + 1):
+        data[i % n] = 0
+
+
+def null_region_jse(index: int, null: int, data: np.ndarray) -> None:
+    """JammingSuppressionEngine::NullRegion (jse.hpp:50-59), signed
+    arithmetic: start = index - null + size, finish = index + null"""
+    n = len(data)
+    start = index - null + n
+    finish = index + null
+    if finish < start:
+        finish += n
+    for i in range(start, finish + 1):
+        data[i % n] = 0
+
+
+def jse_intermediate(cols: np.ndarray, fs: float, max_cnt: int = 8, null: int = 3, js_threshold: float = 6.0):
+    """IntermediateDetection (jamming_detection.hpp:115-137): per column the
+    eight strongest bins, each nulled +-3 after it is taken, J/N = value /
+    sigma of the nulled magnitudes, candidates under 6 dropped (order kept).
+    Returns per column a list of (bin, frequency, j/n)."""
+    out = []
+    for col in cols:
+        mag = np.abs(col).astype(np.float32)
+        cand = []
+        for _ in range(max_cnt):
+            idx = int(np.argmax(mag))           # first index of the maximum
+            cand.append((idx, idx * fs / len(mag), float(mag[idx])))
+            null_region_detector(idx, null, mag)
+        _, sd = mean_std(mag)
+        cand = [(b, f, v / sd) for b, f, v in cand]
+        out.append([c for c in cand if not (c[2] < js_threshold)])
+    return out
+
+
+NARROWBAND, WIDEBAND = 1, 2
+
+
+def jse_analyze(intermediate, probability_threshold: float = 0.9):
+    """AnalyzeStftData (jamming_detection.hpp:139-190), including its integer
+    mean count, its frequency-ordered map and its wideband rule (cnt >=
+    size_t(map size * 0.9), so a one-entry map also reports wideband).
+    Returns a list of (type, bin, frequency, j/n)."""
+    dst = []
+    mean_jammers = sum(len(el) for el in intermediate) // len(intermediate)
+    for i in range(mean_jammers):
+        occ = {}                              # frequency -> [count, bin, frequency, j/n sum]
+        quantity = 0.0
+        for el in intermediate:
+            if len(el) <= i:
+                continue
+            b, f, jn = el[i]
+            e = occ.setdefault(f, [0.0, 0, 0.0, 0.0])
+            e[0] += 1
+            e[1], e[2] = b, f
+            e[3] += jn
+            quantity += 1
+        for e in occ.values():
+            e[0] /= quantity
+            e[3] /= quantity
+        keys = sorted(occ)
+        cnt_wideband = 0
+        wb = occ[keys[0]]
+        for k in keys:
+            e = occ[k]
+            if e[0] >= probability_threshold:
+                dst.append((NARROWBAND, e[1], e[2], e[3]))
+                break
+            if e[0] >= probability_threshold / len(occ):
+                cnt_wideband += 1
+                wb = e
+        if cnt_wideband >= int(len(occ) * probability_threshold):
+            dst.append((WIDEBAND, wb[1], wb[2], wb[3]))
+    return dst
+
+
+def jse_process(x: np.ndarray, fs: float, kind: int, null_window: int = 128, abs_threshold: float = 20.0):
+    """JammingSuppressionEngine::Process (jse.hpp:93-103): detect; no
+    detections leave the samples; only narrowband ones notch +-128 bins of
+    the whole-signal DFT around each (FilterInFreqencyDomain, :61-69);
+    otherwise STFT cells with |X| >= 20 are zeroed and the columns overlap-
+    added back, resized to the input length (FilterWideband, :71-87).
+    Returns (samples, detections)."""
+    x = np.asarray(x, np.complex64).copy()
+    cols = stft(x, kind)
+    det = jse_analyze(jse_intermediate(cols, fs))
+    if not det:
+        return x, det
+    if all(d[0] != WIDEBAND for d in det):
+        X = np.fft.fft(x.astype(np.complex128)).astype(np.complex64)
+        for _, _, f, _ in det:
+            null_region_jse(int(f / fs * len(X)), null_window, X)
+        return np.fft.ifft(X.astype(np.complex128)).astype(np.complex64), det
+    cols = cols.copy()
+    cols[np.abs(cols) >= abs_threshold] = 0
+    y = istft(cols)
+    out = np.zeros(len(x), np.complex64)
+    m = min(len(x), len(y))
+    out[:m] = y[:m]
+    return out, det

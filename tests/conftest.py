@@ -49,6 +49,16 @@ class DeviceArray:
                                    1) == 0   # hipMemcpyHostToDevice
         assert self._hip.hipDeviceSynchronize() == 0
 
+    def get(self):
+        """a host copy of the device contents"""
+        import ctypes as C
+
+        import numpy as np
+
+        out = np.empty_like(self.host)
+        assert self._hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(out.nbytes), 2) == 0
+        return out
+
     def close(self):
         if self.ptr:
             self._hip.hipFree(self.ptr)

@@ -18,6 +18,8 @@ FMT_CI8, FMT_CI16, FMT_CF32, FMT_R8 = 1, 2, 3, 4
 FMT_BBP_DDC, FMT_NT1065_1, FMT_NT1065_2, FMT_NT1065_3, FMT_NT1065_4 = 5, 6, 7, 8, 9
 PACKED_FORMATS = (FMT_BBP_DDC, FMT_NT1065_1, FMT_NT1065_2, FMT_NT1065_3, FMT_NT1065_4)
 PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
+STFT_BLACKMAN_HARRIS, STFT_HANN = 0, 1
+JAMMER_NARROWBAND, JAMMER_WIDEBAND = 1, 2
 BANK_REF_SPLIT, BANK_NCO = 0, 1
 
 ERRORS = {-1: "ARG", -2: "HIP", -3: "SIZE", -4: "UNSUPPORTED", -5: "NOMEM", -6: "STATE"}
@@ -121,6 +123,11 @@ class TrackState(C.Structure):
         ("reserved", C.c_int32)]
 
 
+class Jammer(C.Structure):
+    _fields_ = [("type", C.c_int32), ("frequency_bin", C.c_int32), ("frequency", C.c_double),
+                ("jammer_to_noise", C.c_double)]
+
+
 class TrackRequest(C.Structure):
     _fields_ = [
         ("iq", C.c_void_p), ("fmt", C.c_int32), ("iq_on_device", C.c_int32), ("spm", C.c_int64),
@@ -171,6 +178,9 @@ SIGNATURES = {
     "ugsdr_hip_set_threads": ([VP, I64, I32], C.c_int),
     "ugsdr_format_bytes": ([C.c_int, I64, D], I64),
     "ugsdr_hip_track": ([VP, VP, VP, VP, VP], C.c_int),
+    "ugsdr_hip_stft": ([VP, VP, I64, C.c_int, VP], C.c_int),
+    "ugsdr_hip_istft": ([VP, VP, I64, VP], C.c_int),
+    "ugsdr_hip_jse_process": ([VP, VP, I64, D, C.c_int, C.c_int, VP, I32, C.POINTER(I32)], C.c_int),
     "ugsdr_hip_dfe_create": ([VP, C.c_int, D, D, D, D, C.POINTER(VP)], C.c_int),
     "ugsdr_hip_dfe_process": ([VP, VP, C.c_int, I64, VP, C.c_int], C.c_int),
     "ugsdr_hip_dfe_phase": ([VP, C.POINTER(D)], C.c_int),
@@ -509,6 +519,38 @@ class Context:
             for i in range(n_ch):
                 states[i] = arr[i]
         return epl, trace
+
+    # ---- anti-jamming (STFT)
+    def stft(self, x, window: int = STFT_BLACKMAN_HARRIS) -> np.ndarray:
+        """ShortTimeFourierTransform::Transform: (n - 96) // 32 columns of 128"""
+        x = _c64(x)
+        ncol = (len(x) - 96) // 32
+        out = np.zeros((max(ncol, 0), 128), np.complex64)
+        _check(lib().ugsdr_hip_stft(self._p, _ptr(x), len(x), window, _ptr(out)))
+        return out
+
+    def istft(self, cols) -> np.ndarray:
+        """OverlapAndAdd of the columns: 128 + 32 (n_cols - 1) samples"""
+        cols = np.ascontiguousarray(cols, np.complex64)
+        out = np.zeros(128 + 32 * (len(cols) - 1), np.complex64)
+        _check(lib().ugsdr_hip_istft(self._p, _ptr(cols), len(cols), _ptr(out)))
+        return out
+
+    def jse_process(self, x, fs: float, window: int = STFT_BLACKMAN_HARRIS, device_ptr: int | None = None,
+                    n: int | None = None):
+        """JammingSuppressionEngine::Process.  Host samples: returns (cleaned
+        copy, detections); device_ptr (n complex64 in device memory): cleaned
+        in place, returns (None, detections).  Detections are tuples (type,
+        bin, frequency, j/n)."""
+        det = (Jammer * 16)()
+        cnt = C.c_int32()
+        if device_ptr is not None:
+            _check(lib().ugsdr_hip_jse_process(self._p, device_ptr, int(n), fs, window, 1, det, 16, C.byref(cnt)))
+            y = None
+        else:
+            y = _c64(x).copy()
+            _check(lib().ugsdr_hip_jse_process(self._p, _ptr(y), len(y), fs, window, 0, det, 16, C.byref(cnt)))
+        return y, [(d.type, d.frequency_bin, d.frequency, d.jammer_to_noise) for d in det[:min(cnt.value, 16)]]
 
     # ---- recorded formats
     def unpack(self, raw, fmt: int, n: int, fs: float) -> np.ndarray:

@@ -9,6 +9,9 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
                  float* peaks);
 int acquire_reduced(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_result* out, float* surfaces,
                     float* peaks);
+// DFT of d (n complex, device) in place on ctx's stream, tmp n complex of
+// scratch; the inverse divides by n (DiscreteFourierTransform conventions)
+int dft_device(ugsdr_hip_ctx* ctx, float2* d, float2* tmp, int64_t n, int inverse);
 // conj(DFT(replica)) of length n_fft into out (device), from chips or samples
 int replica_spectrum(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chips, int64_t repeats, const float* samples,
                      int64_t n_fft, float2* out);

@@ -207,7 +207,7 @@ static void factor(int64_t n, std::vector<int64_t>& f) {
 }
 
 // forward DFT of d (n complex, device) in place using tmp (n complex)
-static int dft_device(ugsdr_hip_ctx* ctx, float2* d, float2* tmp, int64_t n, int inverse) {
+int dft_device(ugsdr_hip_ctx* ctx, float2* d, float2* tmp, int64_t n, int inverse) {
     if (n <= 1) return UGSDR_OK;
     if (inverse) hipLaunchKernelGGL(conj_scale_kernel, dim3(grid_for(n)), dim3(256), 0, ctx->stream, d, n, 1.0f);
     std::vector<int64_t> f;
