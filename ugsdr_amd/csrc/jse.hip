@@ -20,7 +20,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <map>
 #include <vector>
 
 #include "ctx.hpp"
@@ -64,9 +63,13 @@ __device__ __forceinline__ void dft128(float2 (&v)[16], float2* lds, int t) {
     Dft<16, INV>::run(v);
 }
 
-template <int WINDOWED_DETECT>
+// Per column with DETECT: kept[c] = {count, bins of the candidates whose J/N
+// is not under 6, in extraction order, 7 spare bytes}; jn[r * ncol + c] =
+// J/N of the r-th kept candidate (round-major, so the host reads only the
+// rounds it needs).
+template <int DETECT>
 __global__ __launch_bounds__(256) void stft_kernel(const float2* x, int64_t ncol, const float* window,
-                                                   float2* cols, int* cand_bin, float* cand_val, double* sigma) {
+                                                   float2* cols, uint8_t* kept, double* jn) {
     __shared__ float2 lds[kColsPerBlock][8 * kLdsPitch];
     const int g = threadIdx.x >> 4, t = threadIdx.x & 15;
     const int64_t c = (int64_t)blockIdx.x * kColsPerBlock + g;
@@ -87,12 +90,15 @@ __global__ __launch_bounds__(256) void stft_kernel(const float2* x, int64_t ncol
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) cols[cc * kWin + k1 + 8 * k2] = v[k2];
     }
-    if (!WINDOWED_DETECT) return;
+    if (!DETECT) return;
     // IntermediateDetection (jamming_detection.hpp:115-137) on lanes 0..7 of
     // the group (lanes 8..15 mirror them and write nothing)
     float mag[16];
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) mag[k2] = sqrtf(v[k2].x * v[k2].x + v[k2].y * v[k2].y);
+    int cbin[kCand];
+    float cval[kCand];
+#pragma unroll
     for (int r = 0; r < kCand; ++r) {
         float best = mag[0];
         int bi = k1;
@@ -111,10 +117,8 @@ __global__ __launch_bounds__(256) void stft_kernel(const float2* x, int64_t ncol
                 bi = oi;
             }
         }
-        if (owner && k1 == 0) {
-            cand_bin[cc * kCand + r] = bi;
-            cand_val[cc * kCand + r] = best;
-        }
+        cbin[r] = bi;
+        cval[r] = best;
         // NullRegion(index, 3, magnitude): the seven bins around it, circularly
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) {
@@ -133,7 +137,24 @@ __global__ __launch_bounds__(256) void stft_kernel(const float2* x, int64_t ncol
     for (int k2 = 0; k2 < 16; ++k2) q += ((double)mag[k2] - mean) * ((double)mag[k2] - mean);
 #pragma unroll
     for (int off = 4; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
-    if (owner && k1 == 0) sigma[cc] = sqrt(q / (kWin - 1));
+    if (owner && k1 == 0) {
+        // J/N = value / sigma; candidates under 6 are dropped in order
+        // (jamming_detection.hpp:130-136)
+        const double sd = sqrt(q / (kWin - 1));
+        uint8_t rec[16] = {};
+        int cnt = 0;
+#pragma unroll
+        for (int r = 0; r < kCand; ++r) {
+            const double j = (double)cval[r] / sd;
+            if (!(j < 6.0)) {
+                rec[1 + cnt] = (uint8_t)cbin[r];
+                jn[(int64_t)cnt * ncol + cc] = j;
+                ++cnt;
+            }
+        }
+        rec[0] = (uint8_t)cnt;
+        *reinterpret_cast<uint4*>(kept + cc * 16) = *reinterpret_cast<const uint4*>(rec);
+    }
 }
 
 // inverse DFT (1/128) of each column in place; cells with |X| >= threshold
@@ -197,51 +218,51 @@ void make_window(int kind, float* w) {
     }
 }
 
-struct Internal {
-    int bin;
-    double frequency, jn;
-};
-
-// AnalyzeStftData (jamming_detection.hpp:139-190), its integer mean count,
-// frequency-ordered map and size_t(size * 0.9) wideband rule included
-std::vector<ugsdr_jammer> analyze(const std::vector<std::vector<Internal>>& inter) {
+// AnalyzeStftData (jamming_detection.hpp:139-190) over the kept candidates:
+// kept[16 c] = count, kept[16 c + 1 + r] = bin of the r-th; jn[r ncol + c].
+// Its frequency-ordered map becomes a table over the 128 bins (frequency =
+// bin fs / 128 rises with the bin), its sums run in column order, and its
+// integer mean count and size_t(size * 0.9) wideband rule are kept.
+std::vector<ugsdr_jammer> analyze(const uint8_t* kept, const double* jn, int64_t ncol, size_t rounds, double fs) {
     constexpr double kProb = 0.9;
     std::vector<ugsdr_jammer> dst;
-    size_t mean = 0;
-    for (auto& el : inter) mean += el.size();
-    mean /= inter.size();
-    for (size_t i = 0; i < mean; ++i) {
-        std::map<double, std::pair<double, Internal>> occ;
+    for (size_t i = 0; i < rounds; ++i) {
+        double count[kWin] = {}, jsum[kWin] = {};
         double quantity = 0;
-        for (auto& el : inter) {
-            if (el.size() <= i) continue;
-            auto& e = occ[el[i].frequency];
-            e.first += 1;
-            e.second.frequency = el[i].frequency;
-            e.second.bin = el[i].bin;
-            e.second.jn += el[i].jn;
+        for (int64_t c = 0; c < ncol; ++c) {
+            if (kept[(size_t)c * 16] <= i) continue;
+            const int b = kept[(size_t)c * 16 + 1 + i];
+            count[b] += 1;
+            jsum[b] += jn[i * (size_t)ncol + (size_t)c];
             ++quantity;
         }
-        for (auto& kv : occ) {
-            kv.second.first /= quantity;
-            kv.second.second.jn /= quantity;
-        }
+        size_t size = 0;
+        for (int b = 0; b < kWin; ++b) size += count[b] > 0;
+        auto entry = [&](int b) {
+            return ugsdr_jammer{0, b, b * fs / kWin, jsum[b] / quantity};
+        };
         size_t cnt_wideband = 0;
-        const Internal* wb = &occ.begin()->second.second;
-        for (auto& kv : occ) {
-            const double p = kv.second.first;
+        int wb = -1;
+        for (int b = 0; b < kWin; ++b) {
+            if (count[b] == 0) continue;
+            if (wb < 0) wb = b;  // the map's first entry
+            const double p = count[b] / quantity;
             if (p >= kProb) {
-                dst.push_back({UGSDR_JAMMER_NARROWBAND, kv.second.second.bin, kv.second.second.frequency,
-                               kv.second.second.jn});
+                ugsdr_jammer d = entry(b);
+                d.type = UGSDR_JAMMER_NARROWBAND;
+                dst.push_back(d);
                 break;
             }
-            if (p >= kProb / (double)occ.size()) {
+            if (p >= kProb / (double)size) {
                 ++cnt_wideband;
-                wb = &kv.second.second;
+                wb = b;
             }
         }
-        if (cnt_wideband >= (size_t)(occ.size() * kProb))
-            dst.push_back({UGSDR_JAMMER_WIDEBAND, wb->bin, wb->frequency, wb->jn});
+        if (cnt_wideband >= (size_t)(size * kProb)) {
+            ugsdr_jammer d = entry(wb);
+            d.type = UGSDR_JAMMER_WIDEBAND;
+            dst.push_back(d);
+        }
     }
     return dst;
 }
@@ -265,7 +286,7 @@ int stft_impl(ugsdr_hip_ctx* ctx, const float* in, int64_t n, int window, float*
     UGSDR_HIP_TRY(ctx->sig.reserve((size_t)ncol * kWin * 8));
     hipLaunchKernelGGL(stft_kernel<0>, dim3((unsigned)((ncol + kColsPerBlock - 1) / kColsPerBlock)), dim3(256), 0,
                        ctx->stream, ctx->aux.as<float2>(), ncol, ctx->tables.as<float>(), ctx->sig.as<float2>(),
-                       nullptr, nullptr, nullptr);
+                       nullptr, nullptr);
     UGSDR_HIP_TRY(hipGetLastError());
     UGSDR_HIP_TRY(hipMemcpyAsync(out, ctx->sig.ptr, (size_t)ncol * kWin * 8, hipMemcpyDeviceToHost, ctx->stream));
     UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -303,34 +324,30 @@ int jse_impl(ugsdr_hip_ctx* ctx, float* io, int64_t n, double fs, int window, in
         x = ctx->aux.as<float2>();
     }
     UGSDR_HIP_TRY(ctx->sig.reserve((size_t)ncol * kWin * 8));
-    const size_t b_bin = (size_t)ncol * kCand * 4, o_val = (b_bin + 255) / 256 * 256;
-    const size_t o_sig = (o_val + b_bin + 255) / 256 * 256;
-    UGSDR_HIP_TRY(ctx->results.reserve(o_sig + (size_t)ncol * 8));
-    int* d_bin = ctx->results.as<int>();
-    float* d_val = reinterpret_cast<float*>(ctx->results.as<char>() + o_val);
-    double* d_sig = reinterpret_cast<double*>(ctx->results.as<char>() + o_sig);
+    const size_t b_kept = (size_t)ncol * 16, o_jn = (b_kept + 255) / 256 * 256;
+    UGSDR_HIP_TRY(ctx->results.reserve(o_jn + (size_t)ncol * kCand * 8));
+    uint8_t* d_kept = ctx->results.as<uint8_t>();
+    double* d_jn = reinterpret_cast<double*>(ctx->results.as<char>() + o_jn);
     const unsigned blocks = (unsigned)((ncol + kColsPerBlock - 1) / kColsPerBlock);
     ctx->mark(0);
     hipLaunchKernelGGL(stft_kernel<1>, dim3(blocks), dim3(256), 0, st, x, ncol, ctx->tables.as<float>(),
-                       ctx->sig.as<float2>(), d_bin, d_val, d_sig);
+                       ctx->sig.as<float2>(), d_kept, d_jn);
     UGSDR_HIP_TRY(hipGetLastError());
     ctx->mark(1);
-    std::vector<int> bins((size_t)ncol * kCand);
-    std::vector<float> vals((size_t)ncol * kCand);
-    std::vector<double> sig((size_t)ncol);
-    UGSDR_HIP_TRY(hipMemcpyAsync(bins.data(), d_bin, b_bin, hipMemcpyDeviceToHost, st));
-    UGSDR_HIP_TRY(hipMemcpyAsync(vals.data(), d_val, b_bin, hipMemcpyDeviceToHost, st));
-    UGSDR_HIP_TRY(hipMemcpyAsync(sig.data(), d_sig, (size_t)ncol * 8, hipMemcpyDeviceToHost, st));
+    // the kept candidates' bins first; their J/N only for the rounds the
+    // mean count asks for
+    std::vector<uint8_t> kept(b_kept);
+    UGSDR_HIP_TRY(hipMemcpyAsync(kept.data(), d_kept, b_kept, hipMemcpyDeviceToHost, st));
     UGSDR_HIP_TRY(hipStreamSynchronize(st));
-    // the candidates' J/N, those under 6 dropped in order (:130-136)
-    std::vector<std::vector<Internal>> inter((size_t)ncol);
-    for (int64_t c = 0; c < ncol; ++c)
-        for (int r = 0; r < kCand; ++r) {
-            const int b = bins[(size_t)c * kCand + r];
-            const double jn = (double)vals[(size_t)c * kCand + r] / sig[(size_t)c];
-            if (!(jn < 6.0)) inter[(size_t)c].push_back({b, b * fs / kWin, jn});
-        }
-    const std::vector<ugsdr_jammer> det = analyze(inter);
+    size_t total = 0;
+    for (int64_t c = 0; c < ncol; ++c) total += kept[(size_t)c * 16];
+    const size_t rounds = total / (size_t)ncol;
+    std::vector<double> jn(rounds * (size_t)ncol);
+    if (rounds) {
+        UGSDR_HIP_TRY(hipMemcpyAsync(jn.data(), d_jn, jn.size() * 8, hipMemcpyDeviceToHost, st));
+        UGSDR_HIP_TRY(hipStreamSynchronize(st));
+    }
+    const std::vector<ugsdr_jammer> det = analyze(kept.data(), jn.data(), ncol, rounds, fs);
     if (n_out) *n_out = (int32_t)det.size();
     for (size_t i = 0; i < det.size() && (int32_t)i < max_out; ++i) out[i] = det[i];
     ctx->mark(2);

@@ -132,6 +132,22 @@ public:
     }
 };
 
+// ShortTimeFourierTransform (src/math/stft.hpp:13-29): columns of a signal,
+// or the overlap-add of columns back into a signal
+template <typename StftImpl>
+class ShortTimeFourierTransform {
+public:
+    template <ComplexContainer T>
+    static auto Transform(const T& src, std::size_t window_length = 128, std::size_t overlap_length = 96) {
+        return StftImpl::Process(src, window_length, overlap_length);
+    }
+    template <Container T>
+    requires ComplexContainer<typename T::value_type>
+    static auto Transform(const T& src, std::size_t window_length = 128, std::size_t overlap_length = 96) {
+        return StftImpl::Process(src, window_length, overlap_length);
+    }
+};
+
 template <typename AbsImpl>
 class Abs {
 public:

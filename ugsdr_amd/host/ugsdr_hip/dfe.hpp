@@ -12,6 +12,7 @@
 
 #include "context.hpp"
 #include "fse.hpp"
+#include "jse.hpp"
 
 namespace ugsdr {
 
@@ -55,11 +56,15 @@ public:
 
     // Channel(signal_params, signal, new_sampling_rate) (dfe.hpp:163-174): the
     // mixer moves the file's centre frequency onto the signal's carrier.
+    // mitigation: Config::IsMitigationEnabled() (dfe.hpp:200-201), the
+    // interference suppression run on each call's output
     HipDfeChannel(int fmt, double file_sampling_rate, double file_central_frequency, Signal signal,
-                  double new_sampling_rate, ugsdr_hip::HipContext& ctx = ugsdr_hip::HipContext::Default())
+                  double new_sampling_rate, ugsdr_hip::HipContext& ctx = ugsdr_hip::HipContext::Default(),
+                  bool mitigation = false)
         : sampling_rate(new_sampling_rate), central_frequency(CentralFrequency(signal)),
           spectrum_inversion(file_central_frequency - central_frequency > 1),
-          output_per_ms((std::size_t)(new_sampling_rate / 1e3)) {
+          output_per_ms((std::size_t)(new_sampling_rate / 1e3)), mitigate(mitigation),
+          jamming_suppressor(new_sampling_rate, UGSDR_STFT_BLACKMAN_HARRIS, ctx) {
         ugsdr_hip::check(ugsdr_hip_dfe_create(ctx.get(), fmt, file_sampling_rate, new_sampling_rate,
                                               file_central_frequency - central_frequency, 0.0, &ch));
     }
@@ -72,13 +77,16 @@ public:
     std::vector<std::complex<float>> GetSeveralEpochs(const void* bytes, std::size_t epoch_cnt) {
         std::vector<std::complex<float>> out(epoch_cnt * output_per_ms);
         ugsdr_hip::check(ugsdr_hip_dfe_process(ch, bytes, 0, (int64_t)epoch_cnt, reinterpret_cast<float*>(out.data()), 0));
+        if (mitigate) jamming_suppressor.Process(out);
         return out;
     }
     // the same, written to device memory for a later device-side search
     void GetSeveralEpochsOnDevice(const void* bytes, std::size_t epoch_cnt, std::complex<float>* device_out) {
         ugsdr_hip::check(
             ugsdr_hip_dfe_process(ch, bytes, 0, (int64_t)epoch_cnt, reinterpret_cast<float*>(device_out), 1));
+        if (mitigate) jamming_suppressor.ProcessOnDevice(device_out, epoch_cnt * output_per_ms);
     }
+    const auto& GetDetectionResults() const { return jamming_suppressor.GetDetectionResults(); }
     std::vector<std::complex<float>> GetEpoch(const void* bytes) { return GetSeveralEpochs(bytes, 1); }
 
     double GetPhase() const {
@@ -89,6 +97,8 @@ public:
 
 private:
     std::size_t output_per_ms = 0;
+    bool mitigate = false;
+    HipJammingSuppressionEngine jamming_suppressor;
     ugsdr_hip_dfe* ch = nullptr;
 };
 
