@@ -45,6 +45,8 @@ def parse():
                     help="storage of the measured search (fp16/bf16: the reduced form)")
     ap.add_argument("--threads", type=int, default=0,
                     help="pin the correlation variant (workgroup size) instead of autotuning")
+    ap.add_argument("--leg", default="", choices=["", "tracking", "jse", "galileo"],
+                    help="run only one of the other workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (rocprofv3 counter calibration)")
     return ap.parse_args()
@@ -153,6 +155,43 @@ def time_search(ctx, w, steps: int, warmup: int, precision: int = 0):
     return dt, stages, res
 
 
+def galileo_workload(ctx):
+    """Galileo E1 code hypotheses: every satellite's four sign permutations in
+    one fused search, one result per satellite (ProcessGalileo's selection)"""
+    import torch
+
+    import ugsdr_amd as U
+    from ugsdr_amd import workloads as W
+
+    gw = W.galileo_e1()
+    gkeys = [ctx.replica_put_samples(r) for c in gw.codes for r in U.e1_sign_hypotheses(c, gw.n_fft)]
+    dev = torch.from_numpy(gw.iq).to("cuda:0")
+    req, hold = ctx.make_request(None, U.FMT_CI16, fs_in=gw.fs, fs_acq=gw.fs, n_fft=gw.n_fft, n_blocks=1,
+                                 surface_len=gw.n_fft, doppler=gw.doppler, keys=gkeys, reshape=False,
+                                 n_samples=gw.n_fft, iq_device_ptr=dev.data_ptr(), hypotheses=4)
+    for _ in range(3):
+        ctx.acquire_req(req, raw=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        res, _, _ = ctx.acquire_req(req, raw=True)
+    dt = (time.perf_counter() - t0) / 10
+    ctx.set_profiling(True)
+    ctx.acquire_req(req)
+    stages = ctx.last_times()
+    ctx.set_profiling(False)
+    for k in gkeys:
+        ctx.replica_drop(k)
+    res = U.results_to_dicts(res)
+    found = sorted(i for i, r in enumerate(res) if r["detected"])
+    return {"GALILEO_E1_HYP": {"ms_per_search": 1e3 * dt, "cells_per_s": gw.cells / dt, "cells": gw.cells,
+                               "satellites": len(gw.codes), "hypotheses": 4, "n_fft": gw.n_fft,
+                               "stage_ms": stages, "detected_sv": found,
+                               "planted_found_at_delay": sum(1 for s, h, tau, f in gw.planted
+                                                             if res[s]["detected"] and res[s]["index"] == tau),
+                               "planted": len(gw.planted)}}
+
+
 def extra_workloads(ctx):
     """cells/s of the other acquisition workloads and the correlator bank rate"""
     import numpy as np
@@ -189,35 +228,7 @@ def extra_workloads(ctx):
                                                      "values": [float(v) for v in d_db]},
                     "planted_index_and_bin_equal_fp32": same,
                     "speedup_vs_fp32": out["BOC_4MS"]["ms_per_search"] / (1e3 * dt)}
-    # Galileo E1 code hypotheses: every satellite's four sign permutations in
-    # one fused search, one result per satellite (ProcessGalileo's selection)
-    gw = W.galileo_e1()
-    gkeys = [ctx.replica_put_samples(r) for c in gw.codes for r in U.e1_sign_hypotheses(c, gw.n_fft)]
-    dev = torch.from_numpy(gw.iq).to("cuda:0")
-    req, hold = ctx.make_request(None, U.FMT_CI16, fs_in=gw.fs, fs_acq=gw.fs, n_fft=gw.n_fft, n_blocks=1,
-                                 surface_len=gw.n_fft, doppler=gw.doppler, keys=gkeys, reshape=False,
-                                 n_samples=gw.n_fft, iq_device_ptr=dev.data_ptr(), hypotheses=4)
-    for _ in range(3):
-        ctx.acquire_req(req, raw=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(10):
-        res, _, _ = ctx.acquire_req(req, raw=True)
-    dt = (time.perf_counter() - t0) / 10
-    ctx.set_profiling(True)
-    ctx.acquire_req(req)
-    stages = ctx.last_times()
-    ctx.set_profiling(False)
-    for k in gkeys:
-        ctx.replica_drop(k)
-    res = U.results_to_dicts(res)
-    found = sorted(i for i, r in enumerate(res) if r["detected"])
-    out["GALILEO_E1_HYP"] = {"ms_per_search": 1e3 * dt, "cells_per_s": gw.cells / dt, "cells": gw.cells,
-                             "satellites": len(gw.codes), "hypotheses": 4, "n_fft": gw.n_fft,
-                             "stage_ms": stages, "detected_sv": found,
-                             "planted_found_at_delay": sum(1 for s, h, tau, f in gw.planted
-                                                           if res[s]["detected"] and res[s]["index"] == tau),
-                             "planted": len(gw.planted)}
+    out.update(galileo_workload(ctx))
     bw = W.bank_l5()
     iq_dev = torch.from_numpy(bw.iq).to("cuda:0")
     ctx.correlate_bank(None, U.FMT_CI16, spm=bw.spm, n_blocks=2, fs=bw.fs, channels=bw.channels, codes=bw.codes,
@@ -478,6 +489,18 @@ def main():
 
     import ugsdr_amd as U
     from ugsdr_amd import workloads as W
+
+    if args.leg:
+        ctx = U.Context(0)
+        if args.leg == "tracking":
+            bw = W.bank_l5()
+            out = {"TRACK_L5_64CH": tracking_workload(ctx, bw, torch.from_numpy(bw.iq).to("cuda:0"), cpu=False)}
+        elif args.leg == "jse":
+            out = jse_workload(ctx)
+        else:
+            out = galileo_workload(ctx)
+        print(json.dumps(out))
+        return
 
     if args.copy_calibration_mib:
         # a copy kernel of known size: FETCH_SIZE / WRITE_SIZE per dispatch of it
