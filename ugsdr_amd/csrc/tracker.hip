@@ -67,15 +67,39 @@ struct EpochPlan {
     float2 w1, wg;    // e^{j 2 pi df}, e^{j 2 pi 4 T df}
 };
 
+// Constants of a channel's loop, set once per launch (the reciprocals turn
+// the per-epoch divisions by constants into multiplications)
+struct ChanConst {
+    double fs_ms;       // fs / 1e3
+    double half_fs_ms;  // fs / 1000 / 2
+    double off;         // GetSamplesPerChip() * spacing
+    double base_ratio;  // base code frequency / 1.023e6
+    double inv_base;    // 1 / base code frequency
+    double inv_fs;      // 1 / fs
+    double fs;
+    int period;         // code period in ms
+};
+
+constexpr double kInv2Pi = 1.0 / (2.0 * kPi), kInvPi = 1.0 / kPi;
+
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 // TrackingParameters::Pll (tracking_parameters.hpp:495-511).  The prompt is
 // complex<float>: the I/Q ratio and the cross / dot products are float, the
 // discriminators and the state FP64.
 __device__ __forceinline__ void pll_update(ugsdr_track_state& s, float2 p) {
-    const double e = p.x != 0.f ? atan((double)(p.y / p.x)) / (kPi * 2.0) : 0.0;
+    const double e = p.x != 0.f ? atan((double)(p.y / p.x)) * kInv2Pi : 0.0;
     const float qx = s.previous_prompt[0], qy = s.previous_prompt[1];
     const float cross = p.x * qy - qx * p.y;
     const float dot = fabsf(p.x * qx + qy * p.y);
-    const double ferr = (double)atan2f(cross, dot) / kPi;
+    const double ferr = (double)atan2f(cross, dot) * kInvPi;
     s.carrier_frequency += gains::kPll1 * e - gains::kPll2 * s.carrier_phase_error - gains::kPll3 * ferr;
     s.carrier_phase_error = e;
     s.carrier_phase += e * kPi / 2;
@@ -85,94 +109,34 @@ __device__ __forceinline__ void pll_update(ugsdr_track_state& s, float2 p) {
 
 // TrackingParameters::Dll (tracking_parameters.hpp:513-528): normalised
 // early-minus-late envelope in float, NCO and code phase in FP64
-__device__ __forceinline__ void dll_update(ugsdr_track_state& s, float2 e, float2 l, double fs) {
+__device__ __forceinline__ void dll_update(ugsdr_track_state& s, float2 e, float2 l, const ChanConst& cc) {
     const float ea = hypotf(e.x, e.y), la = hypotf(l.x, l.y);
     double err = 0.0;
     if (ea + la != 0.f) err = (ea - la) / (ea + la);
-    s.code_nco += (gains::kDll1 * (err - s.code_error) + gains::kDll2 * err) * (s.base_code_frequency / 1.023e6);
+    s.code_nco += (gains::kDll1 * (err - s.code_error) + gains::kDll2 * err) * cc.base_ratio;
     s.code_error = err;
     s.code_frequency = s.base_code_frequency - s.code_nco;
-    s.code_phase -= fs / 1000 / 2 * (s.code_frequency / s.base_code_frequency - 1);
+    s.code_phase -= cc.half_fs_ms * (s.code_frequency * cc.inv_base - 1);
 }
 
-// Plans the next epoch from the state and advances the carrier phase, run
-// by all of wave 0: TrackSingleSatellite's code phase (tracker.hpp:165) and
-// the tone of Translate(x, fs, -f, -phase) (mixer.hpp:33-41) on lane 0, then
-// UpdatePhase (tracking_parameters.hpp:163-165); CorrelateSplit's split of
-// tap t = early / prompt / late (tracking_parameters.hpp:471-486) on lane t;
-// the tone's per-sample and per-group phasors on lanes 3 and 4.
-__device__ __noinline__ void plan_epoch(ugsdr_track_state& s, EpochPlan& plan, int spm, double fs, double spacing,
+// Wave 0 between epochs, all 64 lanes.  Unless `first`: lane t joins tap
+// t's two spans over the waves (AddWithPhase, tracking_parameters.hpp:89-98)
+// and lane 0 runs the PLL/FLL and DLL updates and the trace.  Unless
+// `last`: the next epoch is planned - TrackSingleSatellite's code phase
+// (tracker.hpp:165) and the tone of Translate(x, fs, -f, -phase)
+// (mixer.hpp:33-41) on lane 0, then UpdatePhase (tracking_parameters.hpp:
+// 163-165); CorrelateSplit's split of tap t = early / prompt / late
+// (tracking_parameters.hpp:471-486) on lane t; the tone's per-sample and
+// per-group phasors on lanes 3 and 4.  The state goes through registers of
+// lane 0 and back to LDS once.
+__device__ __noinline__ void epoch_tail(ugsdr_track_state& s_lds, EpochPlan& plan, const ChanConst& cc_lds,
+                                        const float4* red, float2* o, double* tr, bool first, bool last, int spm,
                                         int lane) {
-    double cp = 0.0, off = 0.0, df = 0.0;
-    int period = 1;
-    if (lane == 0) {
-        period = s.code_period_ms;
-        const int64_t k = period == 1 ? 0 : s.epochs % period;
-        cp = s.code_phase - fs / 1e3 * (double)k;
-        off = fs / s.base_code_frequency * spacing;  // GetSamplesPerChip() * spacing
-        double f = -s.carrier_frequency;
-        if (f == 0.0) {
-            plan.pt = 0.0;  // Translate leaves the epoch as it is
-        } else {
-            // -phase reduced to [0, 2 pi) (floor form: within an ulp of the
-            // phase of fmod)
-            const double x = -s.carrier_phase;
-            double phase = x - 2.0 * kPi * floor(x / (2.0 * kPi));
-            if (phase < 0) phase += 2.0 * kPi;
-            if (phase >= 2.0 * kPi) phase -= 2.0 * kPi;
-            if (f < 0) f = fs + f;
-            df = f / fs;
-            plan.pt = phase / (2.0 * kPi);
-        }
-        plan.df = df;
-        s.carrier_phase += 2 * kPi * s.carrier_frequency / 1000.0;  // UpdatePhase
-    }
-    cp = __shfl(cp, 0, 64);
-    off = __shfl(off, 0, 64);
-    df = __shfl(df, 0, 64);
-    period = __shfl(period, 0, 64);
-    if (lane < 3) {
-        const double cps = (double)spm * period;
-        double p = cp + (double)(1 - lane) * off;  // cp + off, cp, cp - off
-        while (p < 0) p += cps;
-        if (p > cps) p = fmod(p, cps);
-        const int fp = (int)ceil(2 * cps - p);
-        int fl = (int)ceil(p);
-        while (fl >= spm) fl -= spm;
-        // p mod spm, exactly: q spm is exact and p - q spm is exact for
-        // p in [q spm, (q + 1) spm)
-        double q = floor(p / spm);
-        double r = p - q * spm;
-        if (r < 0) r += spm;
-        if (r >= spm) r -= spm;
-        plan.fl[lane] = fl;
-        plan.al[lane] = fp & 3;
-        plan.fa[lane] = fp - (fp & 3);
-        plan.rel[lane] = r / spm;
-    } else if (lane == 3 || lane == 4) {
-        float2 w = make_float2(1.f, 0.f);
-        if (df != 0.0) {
-            double t = lane == 3 ? df : 4.0 * kTrackT * df;
-            t -= rint(t);
-            double sn, cs;
-            sincos(2.0 * kPi * t, &sn, &cs);
-            w = make_float2((float)cs, (float)sn);
-        }
-        if (lane == 3)
-            plan.w1 = w;
-        else
-            plan.wg = w;
-    }
-}
-
-// Wave 0 after the correlation of epoch b: lane t joins tap t's two spans
-// over the waves (AddWithPhase, tracking_parameters.hpp:89-98), lane 0 runs
-// the PLL/FLL and DLL updates and the trace, then the next epoch is planned.
-__device__ __noinline__ void finish_epoch(ugsdr_track_state& s, EpochPlan& plan, const float4* red, float2* o,
-                                          double* tr, bool last, int spm, double fs, double spacing, int lane) {
+    const ChanConst cc = cc_lds;
     float2 e = make_float2(0.f, 0.f);
-    if (lane < 3) {
+    if (!first && lane < 3) {
         float2 lhs = make_float2(0.f, 0.f), rhs = lhs;
+#pragma unroll
         for (int w = 0; w < kTrackT / 64; ++w) {
             const float4 v = red[w * 3 + lane];
             lhs = cadd(lhs, make_float2(v.x, v.y));
@@ -188,27 +152,92 @@ __device__ __noinline__ void finish_epoch(ugsdr_track_state& s, EpochPlan& plan,
         e = fabs(wl + wr) > fabs(wl) ? cadd(lhs, rhs) : csub(lhs, rhs);
         o[lane] = e;
     }
-    const float2 early = make_float2(__shfl(e.x, 0, 64), __shfl(e.y, 0, 64));
-    const float2 prompt = make_float2(__shfl(e.x, 1, 64), __shfl(e.y, 1, 64));
-    const float2 late = make_float2(__shfl(e.x, 2, 64), __shfl(e.y, 2, 64));
+    const float2 early = make_float2(lane_value(e.x, 0), lane_value(e.y, 0));
+    const float2 prompt = make_float2(lane_value(e.x, 1), lane_value(e.y, 1));
+    const float2 late = make_float2(lane_value(e.x, 2), lane_value(e.y, 2));
+    double cp = 0.0, df = 0.0;
     if (lane == 0) {
-        pll_update(s, prompt);
-        dll_update(s, early, late, fs);
-        ++s.epochs;
-        if (tr) {
-            tr[0] = -s.carrier_phase / (2 * kPi);
-            tr[1] = s.carrier_frequency;
-            tr[2] = s.code_phase;
-            tr[3] = s.code_frequency;
+        ugsdr_track_state s = s_lds;
+        if (!first) {
+            pll_update(s, prompt);
+            dll_update(s, early, late, cc);
+            ++s.epochs;
+            if (tr) {
+                tr[0] = -s.carrier_phase * kInv2Pi;
+                tr[1] = s.carrier_frequency;
+                tr[2] = s.code_phase;
+                tr[3] = s.code_frequency;
+            }
         }
+        if (!last) {
+            const int64_t k = cc.period == 1 ? 0 : s.epochs % cc.period;
+            cp = s.code_phase - cc.fs_ms * (double)k;
+            double f = -s.carrier_frequency;
+            double pt = 0.0;  // f == 0: Translate leaves the epoch as it is
+            if (f != 0.0) {
+                // -phase reduced to [0, 2 pi) (floor form: within an ulp of fmod's)
+                const double x = -s.carrier_phase;
+                double phase = x - 2.0 * kPi * floor(x * kInv2Pi);
+                if (phase < 0) phase += 2.0 * kPi;
+                if (phase >= 2.0 * kPi) phase -= 2.0 * kPi;
+                if (f < 0) f = cc.fs + f;
+                df = f * cc.inv_fs;
+                pt = phase * kInv2Pi;
+            }
+            plan.df = df;
+            plan.pt = pt;
+            s.carrier_phase += 2 * kPi * s.carrier_frequency / 1000.0;  // UpdatePhase
+        }
+        s_lds = s;
     }
-    if (!last) plan_epoch(s, plan, spm, fs, spacing, lane);
+    if (last) return;
+    cp = lane_value(cp, 0);
+    df = lane_value(df, 0);
+    if (lane < 3) {
+        const double cps = (double)spm * cc.period;
+        double p = cp + (double)(1 - lane) * cc.off;  // cp + off, cp, cp - off
+        while (p < 0) p += cps;
+        if (p > cps) p = fmod(p, cps);
+        const int fp = (int)ceil(2 * cps - p);
+        int fl = (int)ceil(p);
+        while (fl >= spm) fl -= spm;
+        // p mod spm, exactly: q spm is exact and p - q spm is exact for
+        // p in [q spm, (q + 1) spm)
+        const double q = floor(p / spm);
+        double r = p - q * spm;
+        if (r < 0) r += spm;
+        if (r >= spm) r -= spm;
+        plan.fl[lane] = fl;
+        plan.al[lane] = fp & 3;
+        plan.fa[lane] = fp - (fp & 3);
+        plan.rel[lane] = r / spm;
+    } else if (lane == 3 || lane == 4) {
+        float2 w = make_float2(1.f, 0.f);
+        if (df != 0.0) {
+            double t = lane == 3 ? df : 4.0 * kTrackT * df;
+            t -= rint(t);
+            float sn, cs;
+            sincospif((float)(2.0 * t), &sn, &cs);
+            w = make_float2(cs, sn);
+        }
+        if (lane == 3)
+            plan.w1 = w;
+        else
+            plan.wg = w;
+    }
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
 }
 
 // Sums the six complex partials of every lane over the wave by halving:
-// after the exchanges over lane bits 5 and 4 each quarter of the wave holds
-// three of the twelve components, summed over its partners, and a butterfly
-// over bits 3..0 completes them.  Lane 16 q ends with components 3q..3q+2.
+// after the exchanges over lane bits 5 and 4 (shuffles) each row of 16 lanes
+// holds three of the twelve components, summed over its partners, and DPP
+// exchanges inside the row (xor 1 and xor 2 as quad permutations, then the
+// half-row and row mirrors) complete them.  Every lane of row q ends with
+// components 3q..3q+2.
 __device__ __forceinline__ void wave_reduce12(float (&v)[12], int lane, float (&out)[3]) {
     float h[6];
     const bool b5 = lane & 32;
@@ -224,9 +253,13 @@ __device__ __forceinline__ void wave_reduce12(float (&v)[12], int lane, float (&
         out[i] = keep + __shfl_xor(give, 16, 64);
     }
 #pragma unroll
-    for (int m = 8; m > 0; m >>= 1)
+    for (int i = 0; i < 3; ++i) out[i] += dpp_move<0xB1>(out[i]);   // quad_perm [1,0,3,2]
 #pragma unroll
-        for (int i = 0; i < 3; ++i) out[i] += __shfl_xor(out[i], m, 64);
+    for (int i = 0; i < 3; ++i) out[i] += dpp_move<0x4E>(out[i]);   // quad_perm [2,3,0,1]
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] += dpp_move<0x141>(out[i]);  // row_half_mirror
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] += dpp_move<0x140>(out[i]);  // row_mirror
 }
 
 template <int FMT>
@@ -237,8 +270,20 @@ __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
     __shared__ ugsdr_track_state s;
     const float* tabs = a.tabs + (int64_t)c * 4 * a.len3p;
     const int spm = (int)a.spm, nfull = spm >> 2;
-    if (tid == 0) s = a.state[c];
-    if (wave == 0) plan_epoch(s, plan, spm, a.fs, a.spacing, lane);
+    __shared__ ChanConst cc;
+    if (tid == 0) {
+        s = a.state[c];
+        cc.fs = a.fs;
+        cc.fs_ms = a.fs / 1e3;
+        cc.half_fs_ms = a.fs / 1000 / 2;
+        cc.off = a.fs / s.base_code_frequency * a.spacing;
+        cc.base_ratio = s.base_code_frequency / 1.023e6;
+        cc.inv_base = 1.0 / s.base_code_frequency;
+        cc.inv_fs = 1.0 / a.fs;
+        cc.period = s.code_period_ms;
+    }
+    __syncthreads();
+    if (wave == 0) epoch_tail(s, plan, cc, red, nullptr, nullptr, true, false, spm, lane);
     __syncthreads();
     for (int b = 0; b < a.n_blocks; ++b) {
         const int64_t base = (int64_t)b * a.spm;
@@ -256,7 +301,7 @@ __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
             double ph = fma(4.0 * tid, df, pt);
             ph -= rint(ph);
             float sn, cs;
-            sincosf((float)(2.0 * kPi * ph), &sn, &cs);
+            sincospif((float)(2.0 * ph), &sn, &cs);
             z[0] = df == 0.0 ? make_float2(1.f, 0.f) : make_float2(cs, sn);
             z[1] = cmul(z[0], w1);
             z[2] = cmul(z[1], w1);
@@ -325,7 +370,7 @@ __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
                 double ph = fma((double)n, df, pt);
                 ph -= rint(ph);
                 float sn, cs;
-                sincosf((float)(2.0 * kPi * ph), &sn, &cs);
+                sincospif((float)(2.0 * ph), &sn, &cs);
                 xm = cmul(xm, make_float2(cs, sn));
             }
 #pragma unroll
@@ -336,7 +381,7 @@ __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
                 if (n < fl[t]) snap[t] = tot[t];
             }
         }
-        float v[12], part[3];
+        float v[12];
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             v[4 * t] = snap[t].x;
@@ -344,6 +389,7 @@ __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
             v[4 * t + 2] = tot[t].x - snap[t].x;
             v[4 * t + 3] = tot[t].y - snap[t].y;
         }
+        float part[3];
         wave_reduce12(v, lane, part);
         if ((lane & 15) == 0) {
             float* r = reinterpret_cast<float*>(red) + wave * 12 + 3 * (lane >> 4);
@@ -353,9 +399,9 @@ __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
         }
         __syncthreads();
         if (wave == 0)
-            finish_epoch(s, plan, red, a.epl + ((int64_t)b * a.n_ch + c) * 3,
-                         a.trace ? a.trace + ((int64_t)b * a.n_ch + c) * 4 : nullptr, b + 1 == a.n_blocks, spm, a.fs,
-                         a.spacing, lane);
+            epoch_tail(s, plan, cc, red, a.epl + ((int64_t)b * a.n_ch + c) * 3,
+                       a.trace ? a.trace + ((int64_t)b * a.n_ch + c) * 4 : nullptr, false, b + 1 == a.n_blocks, spm,
+                       lane);
         __syncthreads();
     }
     // no plan is made past the last epoch: the state is the one after n_blocks
