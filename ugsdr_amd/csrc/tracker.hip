@@ -1,16 +1,19 @@
 // GPU-resident tracking loops (Tracker::Track, src/tracking/tracker.hpp:156-197).
 //
-// One workgroup per channel walks the epochs in order.  Per 1 ms epoch:
-//   - thread 0 plans it: the code phase of the epoch (TrackSingleSatellite,
-//     tracker.hpp:165), the early / prompt / late split points
-//     (CorrelateSplit, tracking_parameters.hpp:471-486) and the carrier tone
-//     of the in-place Mixer::Translate (mixer.hpp:33-41), then advances the
-//     carrier phase (UpdatePhase, tracking_parameters.hpp:163-165);
-//   - every thread mixes its samples and accumulates the six partial
-//     correlations against the 3-period replica table;
-//   - thread 0 combines them (AddWithPhase, tracking_parameters.hpp:89-98)
-//     and runs the PLL/FLL and DLL updates (:495-528) in FP64, so the next
-//     epoch uses the new state without a trip to the host.
+// One workgroup per channel walks the epochs of a call in order.  Per 1 ms
+// epoch:
+//   - wave 0 has planned it: the code phase of the epoch (TrackSingleSatellite,
+//     tracker.hpp:165), the early / prompt / late split points and replica
+//     offsets (CorrelateSplit, tracking_parameters.hpp:471-486) and the tone
+//     of the in-place Mixer::Translate (mixer.hpp:33-41), after which the
+//     carrier phase advances (UpdatePhase, tracking_parameters.hpp:163-165);
+//   - every thread mixes groups of four samples and accumulates the three
+//     taps against the 3-period replica table, a snapshot of each running sum
+//     marking the tap's split point;
+//   - the sums are reduced over each wave and, in LDS, over the waves; wave 0
+//     joins each tap's two spans (AddWithPhase, tracking_parameters.hpp:89-98),
+//     runs the PLL/FLL and DLL updates (:495-528) in FP64 and plans the next
+//     epoch, so no epoch waits for the host.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
