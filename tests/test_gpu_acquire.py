@@ -172,9 +172,10 @@ def test_real8_kat(ctx, oracle):
         ctx.replica_drop(k)
 
 
-@pytest.mark.parametrize("n_fft", [1000, 2310, 4750, 20000, 33250])
+@pytest.mark.parametrize("n_fft", [1000, 2310, 4750, 20000, 33250, 32768, 65536])
 def test_runtime_plan_lengths(ctx, oracle, n_fft):
-    """lengths without a compile-time plan (runtime radix plan / four-step)"""
+    """lengths without a compile-time plan (runtime radix plan / four-step),
+    and the power-of-two four-step lengths with compile-time factor plans"""
     rng = np.random.default_rng(n_fft)
     fs = n_fft * 1000.0
     code = np.where(rng.random(n_fft // 2) < 0.5, -1, 1).astype(np.int8)

@@ -472,6 +472,8 @@ using P112 = PlanCT<16, 7>;
 using P120 = PlanCT<8, 5, 3>;
 using P180 = PlanCT<5, 4, 3, 3>;
 using P240 = PlanCT<16, 5, 3>;
+using P128 = PlanCT<16, 8>;
+using P256 = PlanCT<16, 16>;
 
 template <int T, int CAP, class P>
 static int launch_fwd_small(FwdArgs& a, int blocks, hipStream_t st) {
@@ -503,10 +505,23 @@ static bool fourstep_341(int N, FourStep& f) {
     return true;
 }
 
+// four-step splits whose two factors both have compile-time plans: N1 * 341
+// as above, and the powers of two 2^15 = 128 * 256 and 2^16 = 256 * 256
+static bool fourstep_ct(int N, FourStep& f) {
+    if (fourstep_341(N, f)) return true;
+    if (N != 32768 && N != 65536) return false;
+    f.n = N;
+    f.n1 = N / 256;
+    f.n2 = 256;
+    make_plan(f.n1, f.p1);
+    make_plan(f.n2, f.p2);
+    return true;
+}
+
 static bool plan_transform(int N, bool& small, RadixPlan& plan, FourStep& fs) {
     small = N <= kLdsCap;
     if (small) return make_plan(N, plan);
-    return fourstep_341(N, fs) || make_fourstep(N, fs);
+    return fourstep_ct(N, fs) || make_fourstep(N, fs);
 }
 
 // forward transform(s): `blocks` transforms of length a.N written to a.out
@@ -532,8 +547,10 @@ static int launch_forward(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratc
         UGSDR_HIP_TRY(scratch.reserve((size_t)blocks * a.N * sizeof(float2)));
         a.scratch = scratch.as<float2>();
         FourStep probe;
-        const bool ct = fourstep_341(a.N, probe);
+        const bool ct = fourstep_ct(a.N, probe);
         if (!ct) rc = launch_fwd_large<PlanRT, PlanRT>(a, blocks, st);
+        else if (a.fs.n2 == 256) rc = a.fs.n1 == 128 ? launch_fwd_large<P128, P256>(a, blocks, st)
+                                                       : launch_fwd_large<P256, P256>(a, blocks, st);
         else switch (a.fs.n1) {
             case 60: rc = launch_fwd_large<P60, P341>(a, blocks, st); break;
             case 112: rc = launch_fwd_large<P112, P341>(a, blocks, st); break;
@@ -786,9 +803,9 @@ int default_threads(int64_t N) {
     if (const int t = corr1023_threads(N)) return t;
     if (N <= 8192) return 256;
     // four-step lengths: 1024 threads measured fastest on MI355X (BOC 4 ms,
-    // N = 81840: 9.9 ms against 15.3 ms at 512)
+    // N = 81840: 15.1 ms against 19.8 ms at 512)
     FourStep probe;
-    if (N > kLdsCap && fourstep_341((int)N, probe)) return 1024;
+    if (N > kLdsCap && fourstep_ct((int)N, probe)) return 1024;
     return 512;
 }
 
@@ -803,7 +820,7 @@ std::vector<int> thread_candidates(int64_t N) {
         case 16368: return {t1023, 512, 1024};
         default: {
             FourStep probe;
-            if (N > kLdsCap && fourstep_341((int)N, probe)) return {512, 1024};
+            if (N > kLdsCap && fourstep_ct((int)N, probe)) return {512, 1024};
             return {default_threads(N)};
         }
     }
@@ -911,8 +928,15 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
         c.scratch = scratch.as<float2>();
         FourStep probe;
         const bool big = T == 1024;
-        if (!fourstep_341(c.N, probe)) rc = launch_corr_large<512, PlanRT, PlanRT>(c, grid, st);
-        else switch (c.fs.n1) {
+        if (!fourstep_ct(c.N, probe)) rc = launch_corr_large<512, PlanRT, PlanRT>(c, grid, st);
+        else if (c.fs.n2 == 256) {
+            if (c.fs.n1 == 128)
+                rc = big ? launch_corr_large<1024, P128, P256>(c, grid, st)
+                         : launch_corr_large<512, P128, P256>(c, grid, st);
+            else
+                rc = big ? launch_corr_large<1024, P256, P256>(c, grid, st)
+                         : launch_corr_large<512, P256, P256>(c, grid, st);
+        } else switch (c.fs.n1) {
             case 60: rc = big ? launch_corr_large<1024, P60, P341>(c, grid, st)
                               : launch_corr_large<512, P60, P341>(c, grid, st); break;
             case 112: rc = big ? launch_corr_large<1024, P112, P341>(c, grid, st)
