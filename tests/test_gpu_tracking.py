@@ -144,3 +144,35 @@ def test_track_formats_and_ragged_epochs(ctx, oracle, fmt, fs):
         e_cp = np.max(np.abs(tr[:, c, 2] - rtr[:, 2]))
         say(f"  channel {c}: epl {e_epl:.2e}, carrier {e_f:.2e} Hz, code {e_cp:.2e} samples")
         assert e_epl < 2e-4 and e_f < 1e-3 and e_cp < 1e-3
+
+
+def test_track_four_ms_code_matches_oracle(ctx, oracle):
+    """a 4 ms code period (Galileo E1-like BOC(1,1), 8184 half-chips: the
+    epoch's code phase steps back by a ms of samples each epoch within the
+    period, tracker.hpp:165) and a wider early/late spacing of 0.5 chip"""
+    from test_galileo import e1_chips
+
+    spm, n_ms, fs = SPM, 240, FS
+    rng = np.random.default_rng(44)
+    sats = [(e1_chips(201), 700.4, 5000.0), (e1_chips(202), -1900.8, 12345.0)]
+    k = np.arange(n_ms * spm)
+    x = 1.2 * (rng.normal(size=k.size) + 1j * rng.normal(size=k.size))
+    for chips, f, tau in sats:
+        idx = np.floor((k - tau) * 2.046e6 / fs).astype(np.int64) % 8184
+        x = x + chips[idx].astype(np.float64) * np.exp(2j * np.pi * f * k / fs + 0.7j)
+    x = x.astype(np.complex64)
+    states = [U.track_state(tau + 0.6, f + 25.0, code_period_ms=4) for _, f, tau in sats]
+    say(f"case track 4 ms BOC codes, spacing 0.5, 2 channels x {n_ms} epochs vs oracle")
+    epl, tr = ctx.track(x.view(np.float32), U.FMT_CF32, spm=spm, n_blocks=n_ms, fs=fs, states=states,
+                        codes=[c for c, _, _ in sats], spacing=0.5)
+    for c, (chips, f, tau) in enumerate(sats):
+        st = R.track_state(tau + 0.6, f + 25.0, code_period_ms=4)
+        repl, rtr = R.track(x, spm, n_ms, fs, R.repeat_upsample(chips, 3, 3 * 4 * spm), st, spacing=0.5)
+        scale = np.max(np.abs(repl[:, 1]))
+        e_epl = np.max(np.abs(epl[:, c, :] - repl)) / scale
+        e_f = np.max(np.abs(tr[:, c, 1] - rtr[:, 1]))
+        e_cp = np.max(np.abs(tr[:, c, 2] - rtr[:, 2]))
+        say(f"  channel {c}: epl {e_epl:.2e}, carrier {e_f:.2e} Hz, code {e_cp:.2e} samples, "
+            f"final {tr[-1, c, 1]:.1f} Hz")
+        assert e_epl < 2e-4 and e_f < 1e-3 and e_cp < 1e-3
+        assert states[c].epochs == n_ms
