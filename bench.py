@@ -45,7 +45,7 @@ def parse():
                     help="storage of the measured search (fp16/bf16: the reduced form)")
     ap.add_argument("--threads", type=int, default=0,
                     help="pin the correlation variant (workgroup size) instead of autotuning")
-    ap.add_argument("--leg", default="", choices=["", "tracking", "jse", "galileo"],
+    ap.add_argument("--leg", default="", choices=["", "tracking", "jse", "galileo", "galileo_c3"],
                     help="run only one of the other workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (rocprofv3 counter calibration)")
@@ -155,20 +155,29 @@ def time_search(ctx, w, steps: int, warmup: int, precision: int = 0):
     return dt, stages, res
 
 
-def galileo_workload(ctx):
+def galileo_workload(ctx, c3: bool = False):
     """Galileo E1 code hypotheses: every satellite's four sign permutations in
-    one fused search, one result per satellite (ProcessGalileo's selection)"""
+    one fused search, one result per satellite (ProcessGalileo's selection).
+    c3: the BOC acquisition configuration's shape (36 satellites, 4 ms x 5
+    blocks at 20.46 MHz, +-6 kHz / 125 Hz) in the reduced FP16 form"""
     import torch
 
     import ugsdr_amd as U
     from ugsdr_amd import workloads as W
 
-    gw = W.galileo_e1()
-    gkeys = [ctx.replica_put_samples(r) for c in gw.codes for r in U.e1_sign_hypotheses(c, gw.n_fft)]
+    if c3:
+        gw = W.galileo_e1(seed=0xE3, n_sv=36, fs=20.46e6, n_blocks=5, doppler_range=6000, doppler_step=125,
+                          name="GALILEO_E1_HYP_C3_FP16")
+        prec = U.PREC_FP16
+    else:
+        gw = W.galileo_e1()
+        prec = U.PREC_FP32
+    gkeys = [ctx.replica_put_samples(r, prec) for c in gw.codes for r in U.e1_sign_hypotheses(c, gw.n_fft)]
     dev = torch.from_numpy(gw.iq).to("cuda:0")
-    req, hold = ctx.make_request(None, U.FMT_CI16, fs_in=gw.fs, fs_acq=gw.fs, n_fft=gw.n_fft, n_blocks=1,
+    req, hold = ctx.make_request(None, U.FMT_CI16, fs_in=gw.fs, fs_acq=gw.fs, n_fft=gw.n_fft, n_blocks=gw.n_blocks,
                                  surface_len=gw.n_fft, doppler=gw.doppler, keys=gkeys, reshape=False,
-                                 n_samples=gw.n_fft, iq_device_ptr=dev.data_ptr(), hypotheses=4)
+                                 n_samples=gw.n_fft * gw.n_blocks, iq_device_ptr=dev.data_ptr(), hypotheses=4,
+                                 precision=prec)
     for _ in range(3):
         ctx.acquire_req(req, raw=True)
     torch.cuda.synchronize()
@@ -184,12 +193,12 @@ def galileo_workload(ctx):
         ctx.replica_drop(k)
     res = U.results_to_dicts(res)
     found = sorted(i for i, r in enumerate(res) if r["detected"])
-    return {"GALILEO_E1_HYP": {"ms_per_search": 1e3 * dt, "cells_per_s": gw.cells / dt, "cells": gw.cells,
-                               "satellites": len(gw.codes), "hypotheses": 4, "n_fft": gw.n_fft,
-                               "stage_ms": stages, "detected_sv": found,
-                               "planted_found_at_delay": sum(1 for s, h, tau, f in gw.planted
-                                                             if res[s]["detected"] and res[s]["index"] == tau),
-                               "planted": len(gw.planted)}}
+    return {gw.name: {"ms_per_search": 1e3 * dt, "cells_per_s": gw.cells / dt, "cells": gw.cells,
+                      "satellites": len(gw.codes), "hypotheses": 4, "n_fft": gw.n_fft, "blocks": gw.n_blocks,
+                      "precision": {0: "fp32", 1: "fp16"}[prec], "stage_ms": stages, "detected_sv": found,
+                      "planted_found_at_delay": sum(1 for s, h, tau, f in gw.planted
+                                                    if res[s]["detected"] and res[s]["index"] == tau),
+                      "planted": len(gw.planted)}}
 
 
 def extra_workloads(ctx):
@@ -229,6 +238,7 @@ def extra_workloads(ctx):
                     "planted_index_and_bin_equal_fp32": same,
                     "speedup_vs_fp32": out["BOC_4MS"]["ms_per_search"] / (1e3 * dt)}
     out.update(galileo_workload(ctx))
+    out.update(galileo_workload(ctx, c3=True))
     bw = W.bank_l5()
     iq_dev = torch.from_numpy(bw.iq).to("cuda:0")
     ctx.correlate_bank(None, U.FMT_CI16, spm=bw.spm, n_blocks=2, fs=bw.fs, channels=bw.channels, codes=bw.codes,
@@ -498,7 +508,7 @@ def main():
         elif args.leg == "jse":
             out = jse_workload(ctx)
         else:
-            out = galileo_workload(ctx)
+            out = galileo_workload(ctx, c3=args.leg == "galileo_c3")
         print(json.dumps(out))
         return
 

@@ -110,3 +110,31 @@ def test_hypotheses_refusals(ctx):
     finally:
         for k in keys:
             ctx.replica_drop(k)
+
+
+def test_hypotheses_reduced_precision_match_fp32(ctx):
+    """the grouping on the reduced (FP16 storage, matrix-core) search at the
+    BOC configuration's length (4 ms at 20.46 MHz, N = 81840): the same
+    satellites, code phases and Doppler bins as the FP32 search, peak
+    metric within 0.1 dB"""
+    from ugsdr_amd import workloads as W
+
+    gw = W.galileo_e1(seed=0xE5, n_sv=6, n_present=3, fs=20.46e6, n_blocks=1, doppler_range=1000,
+                      doppler_step=250, name="test")
+    say("case Galileo hypotheses, FP16 vs FP32, 6 satellites x 4 permutations x 9 bins, N = 81840")
+    out = {}
+    for prec in (U.PREC_FP32, U.PREC_FP16):
+        keys = [ctx.replica_put_samples(r, prec) for c in gw.codes for r in U.e1_sign_hypotheses(c, gw.n_fft)]
+        try:
+            res, _, _ = ctx.acquire(gw.iq, U.FMT_CI16, fs_in=gw.fs, fs_acq=gw.fs, n_fft=gw.n_fft, n_blocks=1,
+                                    surface_len=gw.n_fft, doppler=gw.doppler, keys=keys, reshape=False,
+                                    hypotheses=4, precision=prec)
+        finally:
+            for k in keys:
+                ctx.replica_drop(k)
+        out[prec] = res
+    for a, b in zip(out[U.PREC_FP16], out[U.PREC_FP32]):
+        assert a["detected"] == b["detected"] and a["index"] == b["index"] and a["doppler_bin"] == b["doppler_bin"]
+        assert abs(20 * np.log10(a["snr"] / b["snr"])) < 0.1
+    for s, h, tau, f in gw.planted:
+        assert out[U.PREC_FP16][s]["detected"] and out[U.PREC_FP16][s]["index"] == tau

@@ -257,35 +257,41 @@ class GalileoWorkload:
     codes: list                # per satellite: BOC(1,1)-expanded 4092-chip code (8184)
     planted: list              # (satellite, permutation, delay in samples, Doppler)
     doppler: np.ndarray
+    n_blocks: int = 1          # 4 ms blocks summed after |.|
 
     @property
     def cells(self) -> int:    # satellites x permutations x Doppler x code phase
         return len(self.codes) * 4 * len(self.doppler) * self.n_fft
 
 
-def galileo_e1(seed: int = 0xE1, n_sv: int = 50, n_present: int = 6, fs: float = 8.192e6) -> GalileoWorkload:
+def galileo_e1(seed: int = 0xE1, n_sv: int = 50, n_present: int = 6, fs: float = 8.192e6, n_blocks: int = 1,
+               doppler_range: float = 5000, doppler_step: float = 250,
+               name: str = "GALILEO_E1_HYP") -> GalileoWorkload:
     """E1-like code-hypothesis search: n_sv seeded 4092-chip codes (BOC(1,1)),
-    n_present of them in 4 ms of int16 IQ at fs, each with one of the four
-    per-ms sign permutations, C/N0 U[44, 50] dB-Hz, Doppler on the +-5 kHz /
-    250 Hz grid, sigma 300 LSB"""
+    n_present of them in n_blocks x 4 ms of int16 IQ at fs, each with one of
+    the four per-ms sign permutations (the same in every 4 ms block), C/N0
+    U[44, 50] dB-Hz, Doppler on the search grid, sigma 300 LSB"""
     from . import boc, e1_sign_hypotheses
     rng = np.random.default_rng(seed)
     n = int(round(4 * fs / 1e3))
     codes = [boc(random_code(seed * 7919 + s, 4092)) for s in range(n_sv)]
     sigma = 300.0
-    k = np.arange(n)
-    x = sigma * (rng.normal(size=n) + 1j * rng.normal(size=n))
+    k = np.arange(n * n_blocks)
+    x = sigma * (rng.normal(size=k.size) + 1j * rng.normal(size=k.size))
     planted = []
+    span = int(doppler_range // doppler_step)
     for s in rng.choice(n_sv, n_present, replace=False):
-        h, tau, f = int(rng.integers(0, 4)), int(rng.integers(0, n)), 250.0 * int(rng.integers(-20, 21))
+        h, tau = int(rng.integers(0, 4)), int(rng.integers(0, n))
+        f = doppler_step * int(rng.integers(-span, span + 1))
         amp = sigma * np.sqrt(2 * 10 ** (rng.uniform(44, 50) / 10) / fs)
-        rep = e1_sign_hypotheses(codes[s], n)[h]
-        x += amp * np.roll(rep, tau) * np.exp(1j * (2 * np.pi * f * k / fs + rng.uniform(0, 2 * np.pi)))
+        rep = np.tile(np.roll(e1_sign_hypotheses(codes[s], n)[h], tau), n_blocks)
+        x += amp * rep * np.exp(1j * (2 * np.pi * f * k / fs + rng.uniform(0, 2 * np.pi)))
         planted.append((int(s), h, tau, f))
-    iq = np.empty(2 * n, np.int16)
+    iq = np.empty(2 * k.size, np.int16)
     iq[0::2] = np.clip(np.round(x.real), -32767, 32767)
     iq[1::2] = np.clip(np.round(x.imag), -32767, 32767)
-    return GalileoWorkload("GALILEO_E1_HYP", fs, n, iq, codes, sorted(planted), doppler_bins(5000, 250))
+    return GalileoWorkload(name, fs, n, iq, codes, sorted(planted), doppler_bins(doppler_range, doppler_step),
+                           n_blocks)
 
 
 CONFIGS = {
