@@ -32,6 +32,14 @@ def test_dft(ctx, n):
     assert np.max(np.abs(ctx.dft(x, True) - refi)) < 2e-6 * np.abs(refi).max()
 
 
+def test_dft_refuses_large_prime_factor(ctx):
+    """a stage's cost is n times its radix: lengths with a prime factor over
+    4096 are refused instead of running a quadratic stage"""
+    with pytest.raises(U.UgsdrError):
+        ctx.dft(np.zeros(4099, np.complex64))
+    assert ctx.dft(np.ones(4093, np.complex64))[0] == pytest.approx(4093, rel=1e-5)
+
+
 def test_dft_tone_kat(ctx):
     x = np.exp(2j * np.pi * np.arange(1000) * 0.1).astype(np.complex64)   # basic_tests.cpp:482-517
     y = ctx.dft(x)

@@ -209,9 +209,16 @@ static void factor(int64_t n, std::vector<int64_t>& f) {
 // forward DFT of d (n complex, device) in place using tmp (n complex)
 int dft_device(ugsdr_hip_ctx* ctx, float2* d, float2* tmp, int64_t n, int inverse) {
     if (n <= 1) return UGSDR_OK;
-    if (inverse) hipLaunchKernelGGL(conj_scale_kernel, dim3(grid_for(n)), dim3(256), 0, ctx->stream, d, n, 1.0f);
     std::vector<int64_t> f;
     factor(n, f);
+    // each stage costs n * radix: a length with a large prime factor would
+    // make one stage quadratic in that factor
+    if (f.back() > 4096) {
+        set_error("DFT length " + std::to_string(n) + " has the prime factor " + std::to_string(f.back()) +
+                  " (at most 4096 supported)");
+        return UGSDR_ERR_UNSUPPORTED;
+    }
+    if (inverse) hipLaunchKernelGGL(conj_scale_kernel, dim3(grid_for(n)), dim3(256), 0, ctx->stream, d, n, 1.0f);
     float2* a = d;
     float2* b = tmp;
     int64_t Ns = 1;
