@@ -12,6 +12,13 @@ latency, the correlation kernel's HBM roofline figure (algorithmic 8 bytes per
 cell-block over the kernel's HIP-event duration) and the CPU baseline (the
 oracle's FP32 restatement of the reference's per-PRN/per-Doppler loop, timed on
 this host's cores on the same input).
+
+`other_workloads` adds the other acquisition configurations (1 ms and 4 ms
+L1 C/A, BOC 4 ms in FP32 and in the reduced FP16/BF16 form, Galileo E1 code
+hypotheses), the correlator bank, closed-loop tracking, searches straight from
+a packed BbpDdc recording, the DFE channel step and the anti-jamming front
+end; `serving` and `serving_boc_fp16` are the request server under Poisson
+arrivals.  `--leg tracking|jse|galileo|galileo_c3` runs one of those alone.
 """
 from __future__ import annotations
 
