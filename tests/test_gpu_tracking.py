@@ -176,3 +176,28 @@ def test_track_four_ms_code_matches_oracle(ctx, oracle):
             f"final {tr[-1, c, 1]:.1f} Hz")
         assert e_epl < 2e-4 and e_f < 1e-3 and e_cp < 1e-3
         assert states[c].epochs == n_ms
+
+
+@pytest.mark.parametrize("n_ch", [300, 600])
+def test_track_more_channels_than_cus(ctx, oracle, n_ch):
+    """more channels than the GPU's CUs (the two- and four-channels-per-CU
+    workgroup shapes) over 120 epochs; every 37th channel against the oracle"""
+    n_ms = 120
+    x = multi_signal(n_ms, seed=31)
+    rng = np.random.default_rng(5)
+    chans = []
+    for i in range(n_ch):
+        sv, f, tau = SATS[i % len(SATS)]
+        chans.append((sv, f + rng.uniform(-40, 40), tau + rng.uniform(-0.8, 0.8)))
+    states = [U.track_state(tau, f) for _, f, tau in chans]
+    say(f"case track {n_ch} channels x {n_ms} epochs (more channels than CUs)")
+    epl, tr = ctx.track(x.view(np.float32), U.FMT_CF32, spm=SPM, n_blocks=n_ms, fs=FS, states=states,
+                        codes=[U.gps_l1ca(sv) for sv, _, _ in chans])
+    for c in range(0, n_ch, 37):
+        sv, f, tau = chans[c]
+        st = R.track_state(tau, f)
+        repl, rtr = R.track(x, SPM, n_ms, FS, code3_table(sv), st)
+        scale = np.max(np.abs(repl[:, 1]))
+        assert np.max(np.abs(epl[:, c, :] - repl)) / scale < 2e-4
+        assert np.max(np.abs(tr[:, c, 1] - rtr[:, 1])) < 1e-3
+        assert np.max(np.abs(tr[:, c, 2] - rtr[:, 2])) < 1e-3

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <vector>
 
+#include "acq.hpp"
 #include "ctx.hpp"
 #include "dev_fft.hpp"
 #include "formats.hpp"
@@ -29,6 +30,10 @@ using namespace ugsdr_dev;
 
 namespace ugsdr_hip {
 
+// workgroup sizes: 1024 threads per channel while every channel has a CU of
+// its own, 512 (two channels per CU) up to twice as many channels as CUs,
+// 256 (four per CU) beyond; each CU then overlaps one channel's serial loop
+// step with the others' correlations
 constexpr int kTrackT = 1024;
 
 // Loop filter gains of TrackingParameters (tracking_parameters.hpp:72-87):
@@ -132,6 +137,7 @@ __device__ __forceinline__ void dll_update(ugsdr_track_state& s, float2 e, float
 // (tracking_parameters.hpp:471-486) on lane t; the tone's per-sample and
 // per-group phasors on lanes 3 and 4.  The state goes through registers of
 // lane 0 and back to LDS once.
+template <int T>
 __device__ __noinline__ void epoch_tail(ugsdr_track_state& s_lds, EpochPlan& plan, const ChanConst& cc_lds,
                                         const float4* red, float2* o, double* tr, bool first, bool last, int spm,
                                         int lane) {
@@ -140,7 +146,7 @@ __device__ __noinline__ void epoch_tail(ugsdr_track_state& s_lds, EpochPlan& pla
     if (!first && lane < 3) {
         float2 lhs = make_float2(0.f, 0.f), rhs = lhs;
 #pragma unroll
-        for (int w = 0; w < kTrackT / 64; ++w) {
+        for (int w = 0; w < T / 64; ++w) {
             const float4 v = red[w * 3 + lane];
             lhs = cadd(lhs, make_float2(v.x, v.y));
             rhs = cadd(rhs, make_float2(v.z, v.w));
@@ -217,7 +223,7 @@ __device__ __noinline__ void epoch_tail(ugsdr_track_state& s_lds, EpochPlan& pla
     } else if (lane == 3 || lane == 4) {
         float2 w = make_float2(1.f, 0.f);
         if (df != 0.0) {
-            double t = lane == 3 ? df : 4.0 * kTrackT * df;
+            double t = lane == 3 ? df : 4.0 * T * df;
             t -= rint(t);
             float sn, cs;
             sincospif((float)(2.0 * t), &sn, &cs);
@@ -265,11 +271,11 @@ __device__ __forceinline__ void wave_reduce12(float (&v)[12], int lane, float (&
     for (int i = 0; i < 3; ++i) out[i] += dpp_move<0x140>(out[i]);  // row_mirror
 }
 
-template <int FMT>
-__global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
+template <int FMT, int T>
+__global__ __launch_bounds__(T) void track_kernel(TrackArgs a) {
     const int c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     __shared__ EpochPlan plan;
-    __shared__ float4 red[(kTrackT / 64) * 3];
+    __shared__ float4 red[(T / 64) * 3];
     __shared__ ugsdr_track_state s;
     const float* tabs = a.tabs + (int64_t)c * 4 * a.len3p;
     const int spm = (int)a.spm, nfull = spm >> 2;
@@ -286,7 +292,7 @@ __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
         cc.period = s.code_period_ms;
     }
     __syncthreads();
-    if (wave == 0) epoch_tail(s, plan, cc, red, nullptr, nullptr, true, false, spm, lane);
+    if (wave == 0) epoch_tail<T>(s, plan, cc, red, nullptr, nullptr, true, false, spm, lane);
     __syncthreads();
     for (int b = 0; b < a.n_blocks; ++b) {
         const int64_t base = (int64_t)b * a.spm;
@@ -325,14 +331,14 @@ __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
             for (int t = 0; t < 3; ++t) cn[t] = tab[t][g];
         };
         fetch(tid);
-        for (int g = tid; g < nfull; g += kTrackT) {
+        for (int g = tid; g < nfull; g += T) {
             float2 x[4];
             float4 cv[3];
 #pragma unroll
             for (int u = 0; u < 4; ++u) x[u] = xn[u];
 #pragma unroll
             for (int t = 0; t < 3; ++t) cv[t] = cn[t];
-            fetch(g + kTrackT);
+            fetch(g + T);
             const int n0 = 4 * g;
             float2 xm[4];
 #pragma unroll
@@ -402,7 +408,7 @@ __global__ __launch_bounds__(kTrackT) void track_kernel(TrackArgs a) {
         }
         __syncthreads();
         if (wave == 0)
-            epoch_tail(s, plan, cc, red, a.epl + ((int64_t)b * a.n_ch + c) * 3,
+            epoch_tail<T>(s, plan, cc, red, a.epl + ((int64_t)b * a.n_ch + c) * 3,
                        a.trace ? a.trace + ((int64_t)b * a.n_ch + c) * 4 : nullptr, false, b + 1 == a.n_blocks, spm,
                        lane);
         __syncthreads();
@@ -425,6 +431,18 @@ __global__ void track_tables_kernel(const int8_t* codes, int64_t code_len, int64
         const int64_t i = (j + al) % len3;
         tabs[k] = (float)codes[ch * code_len + (i * code_len / per_period) % code_len];
     }
+}
+
+template <int T>
+static int launch_track(int fmt, int n_ch, const TrackArgs& a, hipStream_t st) {
+    const dim3 grid((unsigned)n_ch), block(T);
+    switch (fmt) {
+        case UGSDR_FMT_CI8: hipLaunchKernelGGL((track_kernel<UGSDR_FMT_CI8, T>), grid, block, 0, st, a); break;
+        case UGSDR_FMT_CI16: hipLaunchKernelGGL((track_kernel<UGSDR_FMT_CI16, T>), grid, block, 0, st, a); break;
+        case UGSDR_FMT_CF32: hipLaunchKernelGGL((track_kernel<UGSDR_FMT_CF32, T>), grid, block, 0, st, a); break;
+        default: hipLaunchKernelGGL((track_kernel<-1, T>), grid, block, 0, st, a); break;
+    }
+    return UGSDR_OK;
 }
 
 static int track_impl(ugsdr_hip_ctx* ctx, const ugsdr_track_request* r, ugsdr_track_state* ch, float* epl,
@@ -501,20 +519,11 @@ static int track_impl(ugsdr_hip_ctx* ctx, const ugsdr_track_request* r, ugsdr_tr
     a.epl = ctx->results.as<float2>();
     a.trace = trace ? reinterpret_cast<double*>(ctx->results.as<char>() + o_trace) : nullptr;
     ctx->mark(0);
-    switch (r->fmt) {
-        case UGSDR_FMT_CI8:
-            hipLaunchKernelGGL(track_kernel<UGSDR_FMT_CI8>, dim3((unsigned)r->n_channels), dim3(kTrackT), 0, st, a);
-            break;
-        case UGSDR_FMT_CI16:
-            hipLaunchKernelGGL(track_kernel<UGSDR_FMT_CI16>, dim3((unsigned)r->n_channels), dim3(kTrackT), 0, st, a);
-            break;
-        case UGSDR_FMT_CF32:
-            hipLaunchKernelGGL(track_kernel<UGSDR_FMT_CF32>, dim3((unsigned)r->n_channels), dim3(kTrackT), 0, st, a);
-            break;
-        default:
-            hipLaunchKernelGGL(track_kernel<-1>, dim3((unsigned)r->n_channels), dim3(kTrackT), 0, st, a);
-            break;
-    }
+    const int cus = device_cu_count(ctx->device);
+    const int rc = r->n_channels <= cus       ? launch_track<kTrackT>(r->fmt, r->n_channels, a, st)
+                   : r->n_channels <= 2 * cus ? launch_track<kTrackT / 2>(r->fmt, r->n_channels, a, st)
+                                              : launch_track<kTrackT / 4>(r->fmt, r->n_channels, a, st);
+    if (rc) return rc;
     UGSDR_HIP_TRY(hipGetLastError());
     ctx->mark(1);
     UGSDR_HIP_TRY(hipMemcpyAsync(epl, a.epl, b_epl, hipMemcpyDeviceToHost, st));
