@@ -18,7 +18,8 @@ L1 C/A, BOC 4 ms in FP32 and in the reduced FP16/BF16 form, Galileo E1 code
 hypotheses), the correlator bank, closed-loop tracking, searches straight from
 a packed BbpDdc recording, the DFE channel step and the anti-jamming front
 end; `serving` and `serving_boc_fp16` are the request server under Poisson
-arrivals.  `--leg tracking|jse|galileo|galileo_c3` runs one of those alone.
+arrivals.  `--leg tracking|tracking_many|jse|galileo|galileo_c3` runs one of
+those alone.
 """
 from __future__ import annotations
 
@@ -52,7 +53,7 @@ def parse():
                     help="storage of the measured search (fp16/bf16: the reduced form)")
     ap.add_argument("--threads", type=int, default=0,
                     help="pin the correlation variant (workgroup size) instead of autotuning")
-    ap.add_argument("--leg", default="", choices=["", "tracking", "jse", "galileo", "galileo_c3"],
+    ap.add_argument("--leg", default="", choices=["", "tracking", "tracking_many", "jse", "galileo", "galileo_c3"],
                     help="run only one of the other workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (rocprofv3 counter calibration)")
@@ -260,6 +261,7 @@ def extra_workloads(ctx):
                       "blocks": bw.n_blocks, "channels": len(bw.channels), "taps": len(bw.taps),
                       "prompt_over_very_early_fraction": float(np.mean(p > np.abs(o[:, :, 0])))}
     out["TRACK_L5_64CH"] = tracking_workload(ctx, bw, iq_dev)
+    out.update(tracking_many_workload(ctx))
     return out
 
 
@@ -320,6 +322,47 @@ def tracking_workload(ctx, bw, iq_dev, cpu: bool = True):
                              "sample": "channel 0, 20 epochs"}
         out["speedup_vs_cpu_oracle_1core"] = out["channel_epochs_per_s"] / out["cpu_oracle"]["channel_epochs_per_s"]
     return out
+
+
+def tracking_many_workload(ctx):
+    """many-channel tracking at a low rate: 512 channels (two per CU) on a
+    GPS L1 C/A signal of 8 seeded satellites (int16 at 4.092 MHz, C/N0
+    44-48 dB-Hz), 64 channels per satellite started at different
+    acquisition-size errors, 1000 epochs per call"""
+    import numpy as np
+    import torch
+
+    import ugsdr_amd as U
+    from ugsdr_amd import workloads as W
+
+    rng = np.random.default_rng(0x7A1)
+    fs, spm, n_ep, n_ch = 4.092e6, 4092, 1000, 512
+    codes = W.l1ca_codes(32)
+    sats = W._draw_sats(rng, 8, 32, lambda p: codes[p], 1.023e6, W.L1_HZ, 1023, 44, 48)
+    iq = W.synthesize(sats, fs, spm * n_ep, 100.0, 0x7A1, 16)
+    dev = torch.from_numpy(iq).to("cuda:0")
+    chans = [(s, s.doppler + rng.uniform(-60, 60), (s.code_phase * 4.0) % spm + rng.uniform(-2, 2))
+             for i in range(n_ch) for s in [sats[i % 8]]]
+
+    def fresh():
+        return [U.track_state(cp, f) for _, f, cp in chans]
+
+    kw = dict(spm=spm, n_blocks=n_ep, fs=fs, codes=[codes[s.prn] for s, _, _ in chans], iq_device_ptr=dev.data_ptr())
+    ctx.track(None, U.FMT_CI16, states=fresh(), **kw)
+    ctx.set_profiling(True)
+    kern = []
+    for _ in range(3):
+        epl, tr = ctx.track(None, U.FMT_CI16, states=fresh(), **kw)
+        kern.append(ctx.last_times()["corr"] * 1e-3)
+    ctx.set_profiling(False)
+    k = statistics.median(kern)
+    P = epl[-200:, :, 1]
+    locked = sum(1 for c, (s, _, _) in enumerate(chans)
+                 if abs(tr[-1, c, 1] - s.doppler) < 5.0
+                 and np.mean(np.abs(P[:, c].imag)) < 0.3 * np.mean(np.abs(P[:, c].real)))
+    return {"TRACK_L1_512CH": {"channels": n_ch, "epochs": n_ep, "fs": fs, "kernel_ms_per_call": 1e3 * k,
+                               "channel_epochs_per_s": n_ch * n_ep / k, "us_per_epoch": 1e6 * k / n_ep,
+                               "realtime_factor": n_ep * 1e-3 / k, "locked_channels": locked}}
 
 
 def jse_workload(ctx):
@@ -512,6 +555,8 @@ def main():
         if args.leg == "tracking":
             bw = W.bank_l5()
             out = {"TRACK_L5_64CH": tracking_workload(ctx, bw, torch.from_numpy(bw.iq).to("cuda:0"), cpu=False)}
+        elif args.leg == "tracking_many":
+            out = tracking_many_workload(ctx)
         elif args.leg == "jse":
             out = jse_workload(ctx)
         else:
