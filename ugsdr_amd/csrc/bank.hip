@@ -42,6 +42,7 @@ struct BankArgs {
     const signed char* codes;    // n_ch x code_len
     int64_t code_len;
     double taps[kMaxTaps];       // chips
+    int tap_margin;              // nco mode: chips of wrap padding on each side of the LDS code table
     // ref_split mode
     const float* code3;          // n_ch x code3_len
     int64_t code3_len;
@@ -80,12 +81,25 @@ __device__ __forceinline__ void bank_reduce_store(float2 (&acc)[NT], int n_taps,
     }
 }
 
+// One workgroup per (channel, block).  The code sits in LDS with `margin`
+// chips of wrap padding on each side, so a tap's chip is base + tap's
+// integer part + (frac(u) >= the tap's threshold), with no modulo per tap:
+// floor(u + tap) = floor(u) + floor(tap) + [frac(u) >= 1 - frac(tap)].  The
+// carrier is an FP64 phasor per thread advanced by e^(j 2 pi T df) per
+// sample slot (the same rate as the FP32 vector unit on this GPU), started
+// from the thread's own FP64 phase.
 __global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
     const int c = blockIdx.x, b = blockIdx.y;  // the channels of one block run together (IQ block reused from L2)
     extern __shared__ __align__(16) unsigned char smem[];
     signed char* code = reinterpret_cast<signed char*>(smem);
-    float2* red = reinterpret_cast<float2*>(smem + ((a.code_len + 15) / 16) * 16);
-    for (int64_t i = threadIdx.x; i < a.code_len; i += kBankT) code[i] = a.codes[(int64_t)c * a.code_len + i];
+    const int Li = (int)a.code_len, M = a.tap_margin;
+    float2* red = reinterpret_cast<float2*>(smem + ((Li + 2 * M + 15) / 16) * 16);
+    for (int i = threadIdx.x; i < Li + 2 * M; i += kBankT) {
+        int j = i - M;
+        j += j < 0 ? Li : 0;
+        j -= j >= Li ? Li : 0;
+        code[i] = a.codes[(int64_t)c * a.code_len + j];
+    }
     const double f = a.ch[4 * c], th0 = a.ch[4 * c + 1], rate = a.ch[4 * c + 2], tau0 = a.ch[4 * c + 3];
     const double L = (double)a.code_len;
     const double n0 = (double)b * (double)a.spm;
@@ -94,7 +108,26 @@ __global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
     double tau = fma(rate, n0 / a.fs, tau0);
     tau = fmod(tau, L);
     const double df = f / a.fs, dr = rate / a.fs, invL = 1.0 / L;
-    const int Li = (int)a.code_len;
+    int ti[kMaxTaps];
+    double thr[kMaxTaps];
+#pragma unroll
+    for (int t = 0; t < kMaxTaps; ++t) {
+        const double tp = t < a.n_taps ? a.taps[t] : 0.0, fl = floor(tp);
+        ti[t] = (int)fl + M;
+        thr[t] = 1.0 - (tp - fl);
+    }
+    // e^(-j 2 pi ph) for this thread's first sample and its step per slot
+    double zc, zs, wc, ws;
+    {
+        double ph = fma((double)threadIdx.x, df, theta);
+        ph -= rint(ph);
+        sincospi(2.0 * ph, &zs, &zc);
+        zs = -zs;
+        double st = (double)kBankT * df;
+        st -= rint(st);
+        sincospi(2.0 * st, &ws, &wc);
+        ws = -ws;
+    }
     float2 acc[kMaxTaps];
 #pragma unroll
     for (int t = 0; t < kMaxTaps; ++t) acc[t] = make_float2(0.f, 0.f);
@@ -104,23 +137,23 @@ __global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
                                                : (const void*)(static_cast<const float2*>(a.iq) + b * a.spm);
     for (int64_t n = threadIdx.x; n < a.spm; n += kBankT) {
         const float2 x = bank_load(blk, a.fmt, n);
-        // carrier phase in cycles (FP64), reduced to [-1/2, 1/2]; the hardware
-        // sin/cos take revolutions
-        double ph = fma((double)n, df, theta);
-        ph -= rint(ph);
-        const float s = __builtin_amdgcn_sinf((float)ph), co = __builtin_amdgcn_cosf((float)ph);
+        const float co = (float)zc, s = (float)zs;
         // x * exp(-j 2 pi ph)
-        const float2 w = make_float2(fmaf(x.x, co, x.y * s), fmaf(x.y, co, -x.x * s));
+        const float2 w = make_float2(fmaf(x.x, co, -x.y * s), fmaf(x.y, co, x.x * s));
+        const double nzc = fma(zc, wc, -zs * ws);
+        zs = fma(zc, ws, zs * wc);
+        zc = nzc;
+        // code phase in chips (FP64); floor(u) mod L once per sample
         const double u = fma((double)n, dr, tau);
+        const double fu = floor(u);
+        const double frac = u - fu;
+        int base = (int)fma(-L, floor(fu * invL), fu);
+        base -= base >= Li ? Li : 0;
+        base += base < 0 ? Li : 0;
 #pragma unroll
         for (int t = 0; t < kMaxTaps; ++t) {
             if (t >= a.n_taps) break;
-            // chip index floor(u + tap) mod L, reduced in FP64 (no integer division)
-            double r = floor(u + a.taps[t]);
-            r = fma(-L, floor(r * invL), r);
-            int idx = (int)r;
-            idx -= idx >= Li ? Li : 0;
-            idx += idx < 0 ? Li : 0;
+            const int idx = base + ti[t] + (frac >= thr[t] ? 1 : 0);
             const float cv = (float)code[idx];
             acc[t].x = fmaf(w.x, cv, acc[t].x);
             acc[t].y = fmaf(w.y, cv, acc[t].y);
@@ -250,7 +283,12 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
         a.ch = ctx->tables.as<double>();
         a.codes = reinterpret_cast<const signed char*>(ctx->tables.as<char>() + chs.size() * 8);
         a.code_len = r->code_len;
-        const size_t lds = ((r->code_len + 15) / 16) * 16 + (kBankT / 64) * kMaxTaps * sizeof(float2);
+        // wrap padding: the largest |floor(tap)| + 1 chips on each side
+        double reach = 0;
+        for (int t = 0; t < r->n_taps; ++t) reach = std::max(reach, std::fabs(std::floor(r->taps_chips[t])) + 1);
+        UGSDR_CHECK_ARG(reach < (double)r->code_len, "taps reach beyond one code period");
+        a.tap_margin = (int)reach;
+        const size_t lds = ((r->code_len + 2 * a.tap_margin + 15) / 16) * 16 + (kBankT / 64) * kMaxTaps * sizeof(float2);
         if (lds > 160 * 1024) {
             set_error("code too long for the LDS code table");
             return UGSDR_ERR_UNSUPPORTED;
