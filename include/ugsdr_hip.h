@@ -211,7 +211,9 @@ int ugsdr_hip_istft(ugsdr_hip_ctx* ctx, const float* cols_c32, int64_t n_cols, f
  * detection (JammingDetector::Process), then nothing, the whole-signal
  * notch of +-128 DFT bins per narrowband detection, or the STFT excision of
  * cells with |X| >= 20 when any detection is wideband.  Up to max_out
- * detections go to out; *n_out gets their count. */
+ * detections go to out; *n_out gets their count.  The notch's whole-signal
+ * DFT takes lengths whose prime factors are at most 4096 (UGSDR_ERR_UNSUPPORTED
+ * otherwise, samples untouched). */
 int ugsdr_hip_jse_process(ugsdr_hip_ctx* ctx, float* io_c32, int64_t n, double fs, int window, int on_device,
                           ugsdr_jammer* out, int32_t max_out, int32_t* n_out);
 
