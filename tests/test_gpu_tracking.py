@@ -201,3 +201,31 @@ def test_track_more_channels_than_cus(ctx, oracle, n_ch):
         assert np.max(np.abs(epl[:, c, :] - repl)) / scale < 2e-4
         assert np.max(np.abs(tr[:, c, 1] - rtr[:, 1])) < 1e-3
         assert np.max(np.abs(tr[:, c, 2] - rtr[:, 2])) < 1e-3
+
+
+def test_dfe_output_tracked_on_device(ctx):
+    """the DFE channel's output left in device memory feeds the loops
+    directly (iq_on_device): the same result as tracking its host copy"""
+    from conftest import DeviceArray
+    from test_formats import BBP_FS, BBP_L1
+
+    raw = np.fromfile(BBP_L1, np.uint8)
+    fs, n_ms = 4.2e6, 4
+    spm = int(fs / 1e3)
+    buf = DeviceArray(np.zeros(n_ms * spm, np.complex64))
+    say("case DFE (BbpDdc recording) -> device buffer -> tracking loops")
+    try:
+        with U.Dfe(ctx, U.FMT_BBP_DDC, BBP_FS, fs, 0.0) as ch:
+            ch.process(raw, n_ms, out_device_ptr=buf.ptr.value)
+        host = buf.get()
+        starts = [(1375.0, 1000.0), (-625.0, 3000.0)]
+        s_dev = [U.track_state(cp, f) for f, cp in starts]
+        s_host = [U.track_state(cp, f) for f, cp in starts]
+        codes = [U.gps_l1ca(22), U.gps_l1ca(8)]
+        epl_d, tr_d = ctx.track(None, U.FMT_CF32, spm=spm, n_blocks=n_ms, fs=fs, states=s_dev, codes=codes,
+                                iq_device_ptr=buf.ptr.value)
+        epl_h, tr_h = ctx.track(host.view(np.float32), U.FMT_CF32, spm=spm, n_blocks=n_ms, fs=fs, states=s_host,
+                                codes=codes)
+    finally:
+        buf.close()
+    assert np.array_equal(epl_d, epl_h) and np.array_equal(tr_d, tr_h)
