@@ -69,7 +69,8 @@ int64_t ugsdr_hip_ctx_device_bytes(ugsdr_hip_ctx* ctx);
  * harness hook that replaces its disabled google-benchmark section
  * (benchmark/benchmark.cpp:482-539).  ms_out[4] = front end (unpack, mix,
  * resample), forward class FFTs, correlation, finalize of the last acquire;
- * after ugsdr_hip_track, ms_out[2] is the loop kernel and the others are 0. */
+ * after ugsdr_hip_track or ugsdr_hip_correlate_bank, ms_out[2] is that call's
+ * kernel and the others are 0. */
 int ugsdr_hip_ctx_set_profiling(ugsdr_hip_ctx* ctx, int enable);
 int ugsdr_hip_ctx_last_times(ugsdr_hip_ctx* ctx, float* ms_out4);
 

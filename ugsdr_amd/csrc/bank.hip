@@ -294,8 +294,10 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
             return UGSDR_ERR_UNSUPPORTED;
         }
         UGSDR_HIP_TRY(set_lds_limit(reinterpret_cast<const void*>(bank_nco_kernel), lds));
+        ctx->mark(0);
         hipLaunchKernelGGL(bank_nco_kernel, grid, dim3(kBankT), lds, ctx->stream, a);
         UGSDR_HIP_TRY(hipGetLastError());
+        ctx->mark(1);
     } else {
         UGSDR_CHECK_ARG(r->samples_per_chip > 0 && r->code_period_ms > 0, "bad replica rate");
         // 3-period replica table, Codes::FillCodesImpl (src/tracking/tracker.hpp:39-51)
@@ -327,11 +329,18 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
         a.code3_len = len3;
         a.split = reinterpret_cast<const SplitTap*>(d + o2);
         a.carrier = reinterpret_cast<const double*>(d + o3);
+        ctx->mark(0);
         hipLaunchKernelGGL(bank_split_kernel, grid, dim3(kBankT), 0, ctx->stream, a);
         UGSDR_HIP_TRY(hipGetLastError());
+        ctx->mark(1);
     }
     UGSDR_HIP_TRY(hipMemcpyAsync(out, a.out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
     UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->profile) {
+        // the bank kernel's time goes in the correlation slot
+        ctx->last_ms[0] = ctx->last_ms[1] = ctx->last_ms[3] = 0.f;
+        UGSDR_HIP_TRY(hipEventElapsedTime(&ctx->last_ms[2], ctx->ev[0], ctx->ev[1]));
+    }
     // advance the channel states past the processed blocks
     for (int c = 0; c < r->n_channels; ++c) {
         if (r->mode == UGSDR_BANK_NCO) {
