@@ -91,8 +91,9 @@ int ugsdr_hip_set_graphs(ugsdr_hip_ctx* ctx, int enable);
 int64_t ugsdr_hip_graph_replays(ugsdr_hip_ctx* ctx);
 /* The correlation variants built for a transform length, each named by its
  * workgroup size (a 1023*M length also has the three-stage kernel, named by
- * its own size: 192, 320, 448 or 576).  Writes up to n of them to threads_out
- * and returns how many exist (no reference counterpart). */
+ * its own size: 192 at N = 4092, 512 at N = 8184, 12276 and 16368, where it
+ * replaces the generic kernel of that size).  Writes up to n of them to
+ * threads_out and returns how many exist (no reference counterpart). */
 int ugsdr_hip_corr_variants(int64_t n_fft, int32_t* threads_out, int32_t n);
 /* Pins the correlation variant for a length on this context (threads from
  * ugsdr_hip_corr_variants; 0 unpins, returning to the autotuner/default). */

@@ -810,14 +810,16 @@ int default_threads(int64_t N) {
 }
 
 // The three-stage 1023*M kernel is identified by its own workgroup size
-// (192, 320, 448 or 576 threads); the others select corr_small shapes.
+// (192 threads at N = 4092, 512 at 8184, 12276 and 16368, where it takes
+// the place of the generic kernel of that size); the others select
+// corr_small shapes.
 std::vector<int> thread_candidates(int64_t N) {
     const int t1023 = corr1023_threads(N);
     switch (N) {
-        case 4092:
-        case 8184: return {t1023, 256, 512};
+        case 4092: return {t1023, 256, 512};
+        case 8184: return {t1023, 256};
         case 12276:
-        case 16368: return {t1023, 512, 1024};
+        case 16368: return {t1023, 1024};
         default: {
             FourStep probe;
             if (N > kLdsCap && fourstep_ct((int)N, probe)) return {512, 1024};

@@ -2,11 +2,14 @@
 // 1 ms at 4.092, 8.184, 12.276 and 16.368 MHz, or 4 ms at 4.092 MHz).
 //
 // The inverse FFT of one row block runs as three Stockham stages,
-// N = 31 * 33 * M, with one radix-31 butterfly per thread (N / 31 threads,
-// rounded up to whole waves):
-//   stage 1 (radix 31, no twiddles): the thread loads its 31 inputs
+// N = 31 * 33 * M, on 512 threads (M >= 8) or N / 31 rounded up to whole
+// waves (M = 4):
+//   stage 1 (radix 31, no twiddles) on the matrix cores: a wave takes 16
+//           butterflies at a time, each lane loads 8 of their inputs
 //           x[i] = X[(i + shift) mod N] * conj(C)[i] straight from global
-//           memory into registers, so the product never round-trips LDS;
+//           memory (the product never round-trips LDS), forms the symmetric
+//           pair sums and differences, and four 16x16x16 FP32 MFMA products
+//           against the cos / sin tables give the 31 outputs;
 //   stage 2 (radix 33 = 3 x 11 prime factor): twiddles w_1023^(r k) come
 //           from a 31 x 33 table, one LDS read each;
 //   stage 3 (radix M): butterfly j owns outputs m = j + q * 1023, q < M.
@@ -35,10 +38,23 @@ struct C1023Shape {
     static constexpr int N = kC1023 * M;
     static constexpr int B1 = 33 * M;   // stage-1 butterflies
     static constexpr int B2 = 31 * M;   // stage-2 butterflies
-    static constexpr int T = (B1 + 63) / 64 * 64;  // whole waves
+    // threads: whole waves, at least one per stage-2 butterfly.  M >= 8
+    // runs one workgroup per CU (LDS bound) as 8 waves, two per SIMD, so
+    // every phase splits evenly over the four SIMDs; M = 4 runs three
+    // workgroups of 3 waves per CU.
+    static constexpr int T = M >= 8 ? 512 : (B1 + 63) / 64 * 64;
+    static constexpr int WAVES_PER_EU = M >= 8 ? 2 : 3;
+    static_assert(T >= B2, "stage 2 runs one butterfly per thread");
+    // the next block's first stage-1 gathers are issued during stage 3 where
+    // the register budget (two waves per SIMD) holds them
+    static constexpr bool PREFETCH = M >= 8;
     static constexpr int NSLOT = (kC1023 + T - 1) / T;  // stage-3 butterflies per thread
-    // LDS: data, stage-2 table [31][33], stage-3 table [1023], reduction scratch
-    static constexpr size_t lds_bytes = (size_t)(N + 2 * kC1023) * sizeof(float2) + 128 * sizeof(double);
+    static constexpr int G1 = (B1 + 15) / 16;      // stage-1 groups of 16 butterflies
+    static constexpr int GW = (G1 + T / 64 - 1) / (T / 64);  // groups per wave
+    // LDS: data, stage-2 table [31][33], stage-3 table [1023], reduction
+    // scratch, stage-1 operand table [4][64]
+    static constexpr size_t lds_bytes =
+        (size_t)(N + 2 * kC1023) * sizeof(float2) + 128 * sizeof(double) + 256 * sizeof(float2);
 };
 
 // w^r for r = 0..R-1 from w^1 by a shallow product tree (depth <= 4 products
@@ -62,10 +78,12 @@ __device__ __forceinline__ void twiddle_powers(float2 w1, float2 (&w)[R]) {
     }
 }
 
-// LDS of the three-stage kernels: data, stage-2 table, stage-3 table, reduction scratch
+// LDS of the three-stage kernels: data, stage-2 table, stage-3 table,
+// reduction scratch, stage-1 operand table.  Ends with a barrier: stage 1
+// reads its table before the first barrier of the block loop.
 template <int M>
 struct C1023Lds {
-    float2 *buf, *tw2, *tw3;
+    float2 *buf, *tw2, *tw3, *tw1;
     double* red;
     __device__ C1023Lds(unsigned char* smem, int tid) {
         constexpr int N = C1023Shape<M>::N, T = C1023Shape<M>::T;
@@ -73,6 +91,7 @@ struct C1023Lds {
         tw2 = buf + N;         // [k * 33 + r] = exp(+2 pi i r k / 1023)
         tw3 = tw2 + kC1023;    // [j] = exp(+2 pi i j / N)
         red = reinterpret_cast<double*>(tw3 + kC1023);
+        tw1 = reinterpret_cast<float2*>(red + 128);  // [s * 64 + lane]: see stage 1
         for (int i = tid; i < kC1023; i += T) {
             const int k = i / 33, r = i - k * 33;
             double s, c;
@@ -81,8 +100,23 @@ struct C1023Lds {
             sincospi(2.0 * (double)i / (double)N, &s, &c);
             tw3[i] = make_float2((float)c, (float)s);
         }
+        // the A operand of stage 1's step s in lane l: (cos, sin)(2 pi k p / 31)
+        // for output k = l % 16 and pair p = 4 s + l / 16
+        for (int i = tid; i < 256; i += T) {
+            const int l = i & 63, k = l & 15, p = 4 * (i >> 6) + (l >> 4);
+            double s, c;
+            sincospi(2.0 * (double)((k * p) % 31) / 31.0, &s, &c);
+            tw1[i] = make_float2((float)c, (float)s);
+        }
+        __syncthreads();
     }
 };
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 
 // Blocks k0 .. k1-1 of one row into the thread's surface samples
 // S[s][u] = surface[tid + s T + u 1023].
@@ -96,30 +130,98 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
     float2* buf = Ld.buf;
     const float2* tw2 = Ld.tw2;
     const float2* tw3 = Ld.tw3;
+    const float2* tw1 = Ld.tw1;
+    using S1 = C1023Shape<M>;
+    constexpr int W = T / 64, G1 = S1::G1, GW = S1::GW;
+    // stage-1 gathers of group gi of block kb: 8 inputs per lane (see stage 1)
+    auto gather = [&](int kb, int gi, float2 (&x)[8], float2 (&c)[8]) {
+        const gptr<float2> Xb = as_global(q.X + ((int64_t)kb * q.F + cls) * N);
+        const int tt = opaque(tid);
+        const int lane = tt & 63, col = lane & 15, kq = lane >> 4;
+        const int wv = __builtin_amdgcn_readfirstlane(tt >> 6);
+        const int g = min(wv + gi * W, G1 - 1);
+        const int j = min(g * 16 + col, B1 - 1);
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            const int pp = 4 * st + kq;
+            const unsigned i1 = j + pp * B1;
+            const unsigned i2 = j + (pp == 0 ? 0 : 31 - pp) * B1;
+            unsigned s1 = i1 + sh, s2 = i2 + sh;
+            s1 -= s1 >= (unsigned)N ? N : 0;
+            s2 -= s2 >= (unsigned)N ? N : 0;
+            x[2 * st] = Xb[s1];
+            c[2 * st] = C[i1];
+            x[2 * st + 1] = Xb[s2];
+            c[2 * st + 1] = C[i2];
+        }
+    };
+    // with PREFETCH, group 0 of the next block is gathered during this
+    // block's stage 3
+    float2 xv[2][8], cv[2][8];
+    if constexpr (S1::PREFETCH) {
+        if (k0 < k1) gather(k0, 0, xv[0], cv[0]);
+    }
     for (int k = k0; k < k1; ++k) {
         const int t = opaque(tid);
-        const gptr<float2> Xk = as_global(q.X + ((int64_t)k * q.F + cls) * N);
-        // ---- stage 1: radix 31 from global memory
-        // (every thread computes, idle ones on a clamped index, so the
-        // registers are defined on all paths; only the stores are guarded)
+        if constexpr (!S1::PREFETCH) gather(k, 0, xv[0], cv[0]);
+        // ---- stage 1: radix 31 on the matrix cores.  Group g of the wave
+        // holds butterflies j = 16 g + l % 16.  With the symmetric form's
+        // s_p = x_p + x_{31-p}, d_p = x_p - x_{31-p} (p = 1..15; s_0 = x_0,
+        // d_0 = 0) as the B operand (pair p = 4 s + l / 16 at step s) and
+        // cos / sin(2 pi k p / 31) as the A operand, four 16x16x16 products
+        // give a_k = sum_p cos * s_p and b_k = sum_p sin * d_p for k < 16 in
+        // lane l's registers r (k = 4 (l / 16) + r, butterfly l % 16), and
+        // X_k = a_k + i b_k, X_{31-k} = a_k - i b_k.
+        // The gathers of group gi + 1 are issued before group gi's products
+        // (group 0's at the top of the block, or with PREFETCH during the
+        // previous block's stage 3), on straight-line
+        // code: a wave's surplus group slot (the group count is not a
+        // multiple of the wave count) and the lanes of the last group past
+        // B1 gather a clamped index, skip the products and store nothing.
         {
-            const bool act = t < B1;
-            const int j = act ? t : 0;
-            float2 v[31];
-            // gathers in two halves: the second half's addresses wait for the
-            // first half's products, bounding the registers loads hold
-            int jb = j;
+            const int lane = t & 63, col = lane & 15, kq = lane >> 4;
+            const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+            f32x4 acc[GW][4];
 #pragma unroll
-            for (int r = 0; r < 31; ++r) {
-                if (r == 16) jb = after(jb, v[15].x);
-                const int i = jb + r * B1;
-                int src = i + sh;
-                src -= src >= N ? N : 0;
-                v[r] = cmul(Xk[src], C[i]);
+            for (int gi = 0; gi < GW; ++gi) {
+                if (gi + 1 < GW) gather(k, gi + 1, xv[(gi + 1) & 1], cv[(gi + 1) & 1]);
+                f32x4 ar = {0.f, 0.f, 0.f, 0.f}, ai = ar, br = ar, bi = ar;
+                if (wv + gi * W < G1) {
+#pragma unroll
+                    for (int st = 0; st < 4; ++st) {
+                        const float2 x1 = cmul(xv[gi & 1][2 * st], cv[gi & 1][2 * st]);
+                        float2 x2 = cmul(xv[gi & 1][2 * st + 1], cv[gi & 1][2 * st + 1]);
+                        if (st == 0 && kq == 0) x2 = make_float2(0.f, 0.f);
+                        const float2 sp = cadd(x1, x2), dp = csub(x1, x2);
+                        const float2 cs = tw1[st * 64 + lane];
+                        ar = mfma4(cs.x, sp.x, ar);
+                        ai = mfma4(cs.x, sp.y, ai);
+                        br = mfma4(cs.y, dp.x, br);
+                        bi = mfma4(cs.y, dp.y, bi);
+                    }
+                }
+                acc[gi][0] = ar;
+                acc[gi][1] = ai;
+                acc[gi][2] = br;
+                acc[gi][3] = bi;
             }
-            DftOdd<31, true>::prep(v);
             __syncthreads();
-            if (act) DftOdd<31, true>::store(v, buf + j * 31, 1);
+#pragma unroll
+            for (int gi = 0; gi < GW; ++gi) {
+                const int g = wv + gi * W;
+                const int j = g * 16 + col;
+                if (g < G1 && j < B1) {
+                    float2* dst = buf + j * 31;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int kk = 4 * kq + r;
+                        const float a_re = acc[gi][0][r], a_im = acc[gi][1][r];
+                        const float b_re = acc[gi][2][r], b_im = acc[gi][3][r];
+                        dst[kk] = make_float2(a_re - b_im, a_im + b_re);
+                        if (kk > 0) dst[31 - kk] = make_float2(a_re + b_im, a_im - b_re);
+                    }
+                }
+            }
         }
         __syncthreads();
         // ---- stage 2: radix 33 (3 x 11), Ns = 31
@@ -146,6 +248,9 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
         }
         __syncthreads();
         // ---- stage 3: radix M, Ns = 1023, accumulate |.| per owned output
+        if constexpr (S1::PREFETCH) {
+            if (k + 1 < k1) gather(k + 1, 0, xv[0], cv[0]);
+        }
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s) {
             // one slot at a time: the next slot's loads wait for this one's sums
@@ -282,7 +387,7 @@ __device__ __forceinline__ void corr1023_merge_row(const CorrArgs& a, int tail, 
 // partial surfaces (launched before this kernel), so that short work runs
 // beside the first round of rows rather than after the last.
 template <int M, int QL>
-__global__ __launch_bounds__(C1023Shape<M>::T, 3) void corr1023(CorrArgs a) {
+__global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void corr1023(CorrArgs a) {
     using S_ = C1023Shape<M>;
     constexpr int NSLOT = S_::NSLOT;
     static_assert(M % QL == 0, "surface length must divide the transform");
@@ -316,7 +421,7 @@ __global__ __launch_bounds__(C1023Shape<M>::T, 3) void corr1023(CorrArgs a) {
 // first and the last row of a share can be cut).  corr1023_merge finishes
 // the rows.
 template <int M, int QL>
-__global__ __launch_bounds__(C1023Shape<M>::T, 3) void corr1023_sk(CorrArgs a) {
+__global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void corr1023_sk(CorrArgs a) {
     using S_ = C1023Shape<M>;
     constexpr int T = S_::T, NSLOT = S_::NSLOT, L = kC1023 * QL;
     extern __shared__ __align__(16) unsigned char smem[];
