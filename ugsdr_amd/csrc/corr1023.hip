@@ -114,6 +114,12 @@ struct C1023Lds {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// a load at a 32-bit byte offset from a wave-uniform base: one VGPR offset
+// on the scalar base address, no 64-bit address arithmetic per lane
+__device__ __forceinline__ float2 ld_off(gptr<float2> base, uint32_t byte_off) {
+    return *reinterpret_cast<gptr<float2>>(reinterpret_cast<gptr<char>>(base) + byte_off);
+}
+
 __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
@@ -149,10 +155,10 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
             unsigned s1 = i1 + sh, s2 = i2 + sh;
             s1 -= s1 >= (unsigned)N ? N : 0;
             s2 -= s2 >= (unsigned)N ? N : 0;
-            x[2 * st] = Xb[s1];
-            c[2 * st] = C[i1];
-            x[2 * st + 1] = Xb[s2];
-            c[2 * st + 1] = C[i2];
+            x[2 * st] = ld_off(Xb, s1 * 8u);
+            c[2 * st] = ld_off(C, i1 * 8u);
+            x[2 * st + 1] = ld_off(Xb, s2 * 8u);
+            c[2 * st + 1] = ld_off(C, i2 * 8u);
         }
     };
     // with PREFETCH, group 0 of the next block is gathered during this
@@ -185,8 +191,9 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
 #pragma unroll
             for (int gi = 0; gi < GW; ++gi) {
                 if (gi + 1 < GW) gather(k, gi + 1, xv[(gi + 1) & 1], cv[(gi + 1) & 1]);
-                f32x4 ar = {0.f, 0.f, 0.f, 0.f}, ai = ar, br = ar, bi = ar;
-                if (wv + gi * W < G1) {
+                // slots below G1 on every wave need no check
+                if ((gi + 1) * W <= G1 || wv + gi * W < G1) {
+                    f32x4 ar = {0.f, 0.f, 0.f, 0.f}, ai = ar, br = ar, bi = ar;
 #pragma unroll
                     for (int st = 0; st < 4; ++st) {
                         const float2 x1 = cmul(xv[gi & 1][2 * st], cv[gi & 1][2 * st]);
@@ -199,11 +206,11 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
                         br = mfma4(cs.y, dp.x, br);
                         bi = mfma4(cs.y, dp.y, bi);
                     }
+                    acc[gi][0] = ar;
+                    acc[gi][1] = ai;
+                    acc[gi][2] = br;
+                    acc[gi][3] = bi;
                 }
-                acc[gi][0] = ar;
-                acc[gi][1] = ai;
-                acc[gi][2] = br;
-                acc[gi][3] = bi;
             }
             __syncthreads();
 #pragma unroll
