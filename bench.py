@@ -550,8 +550,11 @@ def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0):
 
 
 def corr_kernel_name(n_fft: int, surface_len: int, threads: int) -> str:
-    """the correlation kernel a (length, variant) pair runs"""
-    if n_fft % 1023 == 0 and threads == -(-(33 * (n_fft // 1023)) // 64) * 64:
+    """the correlation kernel a (length, variant) pair runs: the first variant
+    of a 1023*M length (M = 4, 8, 12, 16) is the three-stage kernel"""
+    import ugsdr_amd as U
+
+    if n_fft % 1023 == 0 and n_fft // 1023 in (4, 8, 12, 16) and threads == U.corr_variants(n_fft)[0]:
         return f"corr1023<{n_fft // 1023},{surface_len // 1023}> (31 x 33 x {n_fft // 1023}, {threads} threads)"
     if n_fft <= 16384:
         return f"corr_small<{threads},...> (N={n_fft})"
