@@ -94,7 +94,8 @@ __device__ __forceinline__ float2 fwd_source(const FwdArgs& a, int blk, int64_t 
         const float2 x = a.sig[(int64_t)k * a.N + n];
         const double phi = a.class_frac[c];
         if (phi == 0.0) return x;
-        return cmul(x, phasor_turns(-phi * (double)n / (double)a.N));
+        // the reciprocal is loop-invariant in the callers' sample loops
+        return cmul(x, phasor_turns(-phi * ((double)n * (1.0 / (double)a.N))));
     }
     if (a.mode == 1) {
         const int64_t m = ((n * a.len) / a.N) % a.len;
