@@ -388,11 +388,16 @@ struct FinalArgs {
     int L;
 };
 
-__global__ void finalize_kernel(FinalArgs a) {
+// The selection runs in bin order on thread 0 (strict <, runner-up as it
+// goes); the row statistics reach it through LDS, 256 bins at a time,
+// loaded by the whole workgroup.
+__global__ __launch_bounds__(256) void finalize_kernel(FinalArgs a) {
     const int p = blockIdx.x;
     __shared__ int sel;
+    __shared__ RowStat rs[256];
+    ugsdr_acq_result r;
+    double runner = 0.0;
     if (threadIdx.x == 0) {
-        ugsdr_acq_result r;
         r.level = 0.0;
         r.sigma = 1.0;
         r.mean = 0.0;
@@ -402,25 +407,35 @@ __global__ void finalize_kernel(FinalArgs a) {
         r.runner_up_code = 0.0;
         r.index = 0;
         r.doppler_bin = -1;
-        double runner = 0.0;
-        for (int di = 0; di < a.D; ++di) {
-            const RowStat s = a.rowstats[(int64_t)p * a.D + di];
-            const double level = (double)s.level;
-            if (r.level < level) {
-                if (r.doppler_bin >= 0 && r.level > runner) runner = r.level;
-                r.level = level;
-                r.mean = s.mean;
-                r.stddev = s.stddev;
-                r.sigma = (double)(s.stddev + s.mean);
-                r.index = s.index;
-                r.code_offset = a.ratio * (double)s.index;
-                r.doppler = a.doppler[di] + a.intermediate_frequency;
-                r.doppler_bin = di;
-                r.runner_up_code = s.second;
-            } else if (level > runner) {
-                runner = level;
+    }
+    for (int d0 = 0; d0 < a.D; d0 += 256) {
+        const int nd = min(256, a.D - d0);
+        __syncthreads();
+        if ((int)threadIdx.x < nd) rs[threadIdx.x] = a.rowstats[(int64_t)p * a.D + d0 + threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 0; i < nd; ++i) {
+                const int di = d0 + i;
+                const RowStat s = rs[i];
+                const double level = (double)s.level;
+                if (r.level < level) {
+                    if (r.doppler_bin >= 0 && r.level > runner) runner = r.level;
+                    r.level = level;
+                    r.mean = s.mean;
+                    r.stddev = s.stddev;
+                    r.sigma = (double)(s.stddev + s.mean);
+                    r.index = s.index;
+                    r.code_offset = a.ratio * (double)s.index;
+                    r.doppler = a.doppler[di] + a.intermediate_frequency;
+                    r.doppler_bin = di;
+                    r.runner_up_code = s.second;
+                } else if (level > runner) {
+                    runner = level;
+                }
             }
         }
+    }
+    if (threadIdx.x == 0) {
         r.runner_up_doppler = runner;
         r.snr = r.level / r.sigma;
         r.detected = (!a.reshape || r.snr > a.threshold) ? 1 : 0;
