@@ -137,8 +137,7 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
     const float2* tw2 = Ld.tw2;
     const float2* tw3 = Ld.tw3;
     const float2* tw1 = Ld.tw1;
-    using S1 = C1023Shape<M>;
-    constexpr int W = T / 64, G1 = S1::G1, GW = S1::GW;
+    constexpr int W = T / 64, G1 = S_::G1, GW = S_::GW;
     // stage-1 gathers of group gi of block kb: 8 inputs per lane (see stage 1)
     auto gather = [&](int kb, int gi, float2 (&x)[8], float2 (&c)[8]) {
         const gptr<float2> Xb = as_global(q.X + ((int64_t)kb * q.F + cls) * N);
@@ -164,12 +163,12 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
     // with PREFETCH, group 0 of the next block is gathered during this
     // block's stage 3
     float2 xv[2][8], cv[2][8];
-    if constexpr (S1::PREFETCH) {
+    if constexpr (S_::PREFETCH) {
         if (k0 < k1) gather(k0, 0, xv[0], cv[0]);
     }
     for (int k = k0; k < k1; ++k) {
         const int t = opaque(tid);
-        if constexpr (!S1::PREFETCH) gather(k, 0, xv[0], cv[0]);
+        if constexpr (!S_::PREFETCH) gather(k, 0, xv[0], cv[0]);
         // ---- stage 1: radix 31 on the matrix cores.  Group g of the wave
         // holds butterflies j = 16 g + l % 16.  With the symmetric form's
         // s_p = x_p + x_{31-p}, d_p = x_p - x_{31-p} (p = 1..15; s_0 = x_0,
@@ -255,7 +254,7 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
         }
         __syncthreads();
         // ---- stage 3: radix M, Ns = 1023, accumulate |.| per owned output
-        if constexpr (S1::PREFETCH) {
+        if constexpr (S_::PREFETCH) {
             if (k + 1 < k1) gather(k + 1, 0, xv[0], cv[0]);
         }
 #pragma unroll
