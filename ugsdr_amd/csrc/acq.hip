@@ -550,8 +550,8 @@ static int launch_forward(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratc
     }
     if (small) {
         switch (a.N) {
-            case 4092: rc = launch_fwd_small<256, 4096, P4092>(a, blocks, st); break;
-            case 8184: rc = launch_fwd_small<256, 8192, P8184>(a, blocks, st); break;
+            case 4092: rc = launch_fwd_small<512, 4096, P4092>(a, blocks, st); break;
+            case 8184: rc = launch_fwd_small<512, 8192, P8184>(a, blocks, st); break;
             case 12276: rc = launch_fwd_small<1024, kLdsCap, P12276>(a, blocks, st); break;
             case 16368: rc = launch_fwd_small<1024, kLdsCap, P16368>(a, blocks, st); break;
             default:
