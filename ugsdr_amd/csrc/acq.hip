@@ -490,6 +490,9 @@ using P180 = PlanCT<5, 4, 3, 3>;
 using P240 = PlanCT<16, 5, 3>;
 using P128 = PlanCT<16, 8>;
 using P256 = PlanCT<16, 16>;
+using P4200 = PlanCT<8, 7, 5, 5, 3>;  // 1 ms at 4.2 MHz (the BbpDdc recordings' rate rule)
+using P152 = PlanCT<19, 8>;
+using P175 = PlanCT<7, 5, 5>;
 
 template <int T, int CAP, class P>
 static int launch_fwd_small(FwdArgs& a, int blocks, hipStream_t st) {
@@ -522,9 +525,19 @@ static bool fourstep_341(int N, FourStep& f) {
 }
 
 // four-step splits whose two factors both have compile-time plans: N1 * 341
-// as above, and the powers of two 2^15 = 128 * 256 and 2^16 = 256 * 256
+// as above, the powers of two 2^15 = 128 * 256 and 2^16 = 256 * 256, and
+// 26600 = 152 * 175 (4 ms at 6.65 MHz, the BbpDdc recordings' acquisition
+// rate; the same split the runtime planner picks)
 static bool fourstep_ct(int N, FourStep& f) {
     if (fourstep_341(N, f)) return true;
+    if (N == 26600) {
+        f.n = N;
+        f.n1 = 152;
+        f.n2 = 175;
+        make_plan(f.n1, f.p1);
+        make_plan(f.n2, f.p2);
+        return true;
+    }
     if (N != 32768 && N != 65536) return false;
     f.n = N;
     f.n1 = N / 256;
@@ -552,6 +565,7 @@ static int launch_forward(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratc
         switch (a.N) {
             case 4092: rc = launch_fwd_small<512, 4096, P4092>(a, blocks, st); break;
             case 8184: rc = launch_fwd_small<512, 8192, P8184>(a, blocks, st); break;
+            case 4200: rc = launch_fwd_small<256, 8192, P4200>(a, blocks, st); break;
             case 12276: rc = launch_fwd_small<1024, kLdsCap, P12276>(a, blocks, st); break;
             case 16368: rc = launch_fwd_small<1024, kLdsCap, P16368>(a, blocks, st); break;
             default:
@@ -565,6 +579,7 @@ static int launch_forward(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratc
         FourStep probe;
         const bool ct = fourstep_ct(a.N, probe);
         if (!ct) rc = launch_fwd_large<PlanRT, PlanRT>(a, blocks, st);
+        else if (a.fs.n2 == 175) rc = launch_fwd_large<P152, P175>(a, blocks, st);
         else if (a.fs.n2 == 256) rc = a.fs.n1 == 128 ? launch_fwd_large<P128, P256>(a, blocks, st)
                                                        : launch_fwd_large<P256, P256>(a, blocks, st);
         else switch (a.fs.n1) {
@@ -934,6 +949,7 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
                 rc = Ts == 1024 ? launch_corr_small<1024, kLdsCap, P16368>(c, grid, st)
                                : launch_corr_small<512, kLdsCap, P16368>(c, grid, st);
                 break;
+            case 4200: rc = launch_corr_small<256, 8192, P4200>(c, grid, st); break;
             default:
                 if (c.N <= 4096) rc = launch_corr_small<256, 4096, PlanRT>(c, grid, st);
                 else if (c.N <= 8192) rc = launch_corr_small<256, 8192, PlanRT>(c, grid, st);
@@ -947,6 +963,8 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
         FourStep probe;
         const bool big = T == 1024;
         if (!fourstep_ct(c.N, probe)) rc = launch_corr_large<512, PlanRT, PlanRT>(c, grid, st);
+        else if (c.fs.n2 == 175)
+            rc = big ? launch_corr_large<1024, P152, P175>(c, grid, st) : launch_corr_large<512, P152, P175>(c, grid, st);
         else if (c.fs.n2 == 256) {
             if (c.fs.n1 == 128)
                 rc = big ? launch_corr_large<1024, P128, P256>(c, grid, st)
