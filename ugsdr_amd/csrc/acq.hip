@@ -565,7 +565,7 @@ static int launch_forward(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratc
         switch (a.N) {
             case 4092: rc = launch_fwd_small<512, 4096, P4092>(a, blocks, st); break;
             case 8184: rc = launch_fwd_small<512, 8192, P8184>(a, blocks, st); break;
-            case 4200: rc = launch_fwd_small<256, 8192, P4200>(a, blocks, st); break;
+            case 4200: rc = launch_fwd_small<512, 8192, P4200>(a, blocks, st); break;
             case 12276: rc = launch_fwd_small<1024, kLdsCap, P12276>(a, blocks, st); break;
             case 16368: rc = launch_fwd_small<1024, kLdsCap, P16368>(a, blocks, st); break;
             default:
@@ -848,6 +848,7 @@ std::vector<int> thread_candidates(int64_t N) {
     const int t1023 = corr1023_threads(N);
     switch (N) {
         case 4092: return {t1023, 256, 512};
+        case 4200: return {256, 512};
         case 8184: return {t1023, 256};
         case 12276:
         case 16368: return {t1023, 1024};
@@ -949,7 +950,10 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
                 rc = Ts == 1024 ? launch_corr_small<1024, kLdsCap, P16368>(c, grid, st)
                                : launch_corr_small<512, kLdsCap, P16368>(c, grid, st);
                 break;
-            case 4200: rc = launch_corr_small<256, 8192, P4200>(c, grid, st); break;
+            case 4200:
+                rc = Ts == 512 ? launch_corr_small<512, 8192, P4200>(c, grid, st)
+                              : launch_corr_small<256, 8192, P4200>(c, grid, st);
+                break;
             default:
                 if (c.N <= 4096) rc = launch_corr_small<256, 4096, PlanRT>(c, grid, st);
                 else if (c.N <= 8192) rc = launch_corr_small<256, 8192, PlanRT>(c, grid, st);
