@@ -290,6 +290,47 @@ void refcpu_mixer_member_translate_c(float* x, size_t n, double fs, double f, do
     *phase += 2 * M_PI * m;
 }
 
+/* TableMixer sine table (table_mixer.hpp:31-47): 5/4 of a period in 64 steps
+ * per period, FP64 sine rounded once to float. */
+void refcpu_table_mixer_table(float* table80) {
+    const double step = 5 * M_PI / 80.0 / 2;
+    for (int i = 0; i < 80; ++i) table80[i] = (float)(1.0 * sin((double)i * step));
+}
+
+/* TableMixer::Process (table_mixer.hpp:78-101) with its NCO (nco.hpp:17-20):
+ * the products are rounded separately (no fused multiply-add), as the
+ * reference's complex<float> multiply is. */
+__attribute__((optimize("fp-contract=off")))
+static void rc_table_mul(float* x, size_t n, double fs, double f, double phase) {
+    float table[80];
+    refcpu_table_mixer_table(table);
+    const uint32_t adder = (uint32_t)(uint64_t)(4294967296.0 * f / fs);
+    uint32_t holder = (uint32_t)(uint64_t)(4294967296.0 * phase / (2 * M_PI));
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t idx = holder >> 26;             /* table_mixer.hpp:53 */
+        holder += adder;                               /* 32-bit accumulator wraps */
+        const float c = table[idx + 16], s = table[idx];   /* cos at +1/4 period */
+        const float a = x[2 * i], b = x[2 * i + 1];
+        const float ac = a * c, bs = b * s, as = a * s, bc = b * c;
+        x[2 * i] = ac - bs;
+        x[2 * i + 1] = as + bc;
+    }
+}
+
+void refcpu_table_mix_c(float* x, size_t n, double fs, double f, double phase) {
+    phase = rc_fix_phase(phase);                         /* mixer.hpp:35-40 */
+    if (fabs(f) == 0.0) return;
+    if (f < 0) f = fs + f;
+    rc_table_mul(x, n, fs, f, phase);
+}
+
+void refcpu_table_mixer_member_c(float* x, size_t n, double fs, double f, double* phase) {
+    refcpu_table_mix_c(x, n, fs, f, *phase);             /* mixer.hpp:53-60 */
+    double m = fmod(f / 1000.0, 1.0);
+    if (m < 0) m += 1;
+    *phase += 2 * M_PI * m;
+}
+
 /* ======================================================= matched filter == */
 int refcpu_mf_prepare_f(const float* code, size_t n, float* spec) {
     for (size_t i = 0; i < n; ++i) { spec[2 * i] = code[i]; spec[2 * i + 1] = 0.f; }

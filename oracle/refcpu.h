@@ -104,6 +104,16 @@ void refcpu_mix_inplace_c(float* x, size_t n, double fs, double f, double phase)
 /* Member Translate (src/mixer/mixer.hpp:53-60): mixes with the stored
  * (fs, f, *phase) and advances *phase by 2 pi frac(f/1000). */
 void refcpu_mixer_member_translate_c(float* x, size_t n, double fs, double f, double* phase);
+/* TableMixer (src/mixer/table_mixer.hpp:27-101, src/mixer/nco.hpp:8-22), the
+ * mixer of the reference's non-IPP configurations: 32-bit phase accumulator
+ * with adder = floor(2^32 f/fs) and start floor(2^32 phase/2pi), the top 6
+ * bits index an 80-entry float sine table (5/4 period), cos = entry + 16.
+ * refcpu_table_mix_c is Mixer::Translate's in-place overload over it
+ * (mixer.hpp:33-41), refcpu_table_mixer_member_c the member overload
+ * (mixer.hpp:53-60).  Bit-exact with the reference build (oracle/_ref). */
+void refcpu_table_mixer_table(float* table80);
+void refcpu_table_mix_c(float* x, size_t n, double fs, double f, double phase);
+void refcpu_table_mixer_member_c(float* x, size_t n, double fs, double f, double* phase);
 
 /* ------------------------------------------------------------------ fft -- */
 /* Arbitrary-N DFT, forward unscaled, inverse scaled by 1/N

@@ -101,6 +101,9 @@ def _declare(L):
     L.refcpu_mix_inplace_c.argtypes = [F32, C.c_size_t, C.c_double, C.c_double, C.c_double]
     L.refcpu_mixer_member_translate_c.argtypes = [F32, C.c_size_t, C.c_double, C.c_double,
                                                   C.POINTER(C.c_double)]
+    L.refcpu_table_mixer_table.argtypes = [F32]
+    L.refcpu_table_mix_c.argtypes = [F32, C.c_size_t, C.c_double, C.c_double, C.c_double]
+    L.refcpu_table_mixer_member_c.argtypes = [F32, C.c_size_t, C.c_double, C.c_double, C.POINTER(C.c_double)]
     L.refcpu_fft_d.argtypes = [F64, C.c_size_t, C.c_int]
     L.refcpu_fft_f.argtypes = [F32, C.c_size_t, C.c_int]
     L.refcpu_mf_prepare_f.argtypes = [F32, C.c_size_t, F32]
@@ -272,6 +275,25 @@ def mixer_member_translate(x: np.ndarray, fs: float, f: float, phase: float):
     b = c2f(x)
     ph = C.c_double(phase)
     lib().refcpu_mixer_member_translate_c(b, len(x), fs, f, C.byref(ph))
+    return f2c(b), ph.value
+
+
+def table_mixer_table() -> np.ndarray:
+    t = np.zeros(80, np.float32)
+    lib().refcpu_table_mixer_table(t)
+    return t
+
+
+def table_mix(x: np.ndarray, fs: float, f: float, phase: float = 0.0) -> np.ndarray:
+    b = c2f(x)
+    lib().refcpu_table_mix_c(b, len(x), fs, f, phase)
+    return f2c(b)
+
+
+def table_mixer_member(x: np.ndarray, fs: float, f: float, phase: float):
+    b = c2f(x)
+    ph = C.c_double(phase)
+    lib().refcpu_table_mixer_member_c(b, len(x), fs, f, C.byref(ph))
     return f2c(b), ph.value
 
 
