@@ -264,6 +264,16 @@ int ugsdr_hip_mix(ugsdr_hip_ctx* ctx, const float* in_c32, float* out_c32, int64
                   double phase);
 /* Mixer::Translate in-place overload (mixer.hpp:33-41): no-op when f == 0 */
 int ugsdr_hip_mix_inplace(ugsdr_hip_ctx* ctx, float* io_c32, int64_t n, double fs, double f, double phase);
+/* Mixer::Translate in-place overload (mixer.hpp:33-41) over TableMixer, the
+ * mixer of the reference's non-IPP configurations (src/mixer/table_mixer.hpp:
+ * 27-105, src/mixer/nco.hpp:8-22): a 32-bit phase accumulator started at
+ * floor(2^32 phase / 2 pi) and advanced by floor(2^32 f / fs) per sample, whose
+ * top 6 bits index an 80-entry float sine table (cos 16 entries on).  No-op
+ * when f == 0; f < 0 folds to fs + f (UGSDR_ERR_ARG below -fs).  io holds n
+ * complex float samples in host memory, or device memory with on_device = 1
+ * (complete when the call returns).  Bit-exact with the reference build. */
+int ugsdr_hip_table_mix(ugsdr_hip_ctx* ctx, float* io_c32, int64_t n, double fs, double f, double phase,
+                        int on_device);
 /* DiscreteFourierTransform::Transform (src/math/ipp_dft.hpp:106-122): any n,
  * forward unscaled, inverse scaled 1/n */
 int ugsdr_hip_dft(ugsdr_hip_ctx* ctx, const float* in_c32, float* out_c32, int64_t n, int inverse);

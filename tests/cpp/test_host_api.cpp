@@ -10,6 +10,8 @@
 #include <complex>
 #include <cstdio>
 #include <random>
+#include <stdexcept>
+#include <string>
 #include <fstream>
 #include <map>
 #include <utility>
@@ -332,9 +334,96 @@ static int antijamming() {
     return 0;
 }
 
+
+static std::vector<uint8_t> read_file(const char* path, std::size_t n) {
+    std::ifstream f(path, std::ios::binary);
+    std::vector<uint8_t> raw(n);
+    f.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)raw.size());
+    if (f.gcount() != (std::streamsize)raw.size()) throw std::runtime_error(std::string("short read: ") + path);
+    return raw;
+}
+
+// FastSearchEngineBase::ProcessGps (fse.hpp:327-335) through the fused engine:
+// * with an L1 band that holds satellites, only L1 C/A results come back;
+// * with only the L5 recording (66.5 MHz, 1176.45 MHz), L1 is skipped and
+//   L5I finds satellites 2, 5, 8 and 25 at 16.625 MHz (AcquireL5, complex fold);
+// * with a synthetic L5 band that holds only an L5Q signal, L5I finds nothing
+//   and the L5Q fallback finds it at its planted code phase and Doppler.
+static int process_gps(const char* l1_path, const char* l5_path) {
+    std::vector<ugsdr::Sv> svs;
+    for (int s = 0; s < 32; ++s) svs.push_back({s, ugsdr::System::Gps, ugsdr::Signal::GpsCoarseAcquisition_L1});
+    ugsdr::HipFastSearchEngine engine(5e3, 200);
+    using Band = ugsdr::HipFastSearchEngine::Subband;
+
+    const auto l1_raw = read_file(l1_path, 4 * 16640);
+    const auto l5_raw = read_file(l5_path, 4 * 33264);
+    Band l1{l1_raw.data(), UGSDR_FMT_BBP_DDC, 4 * 33250, 33.25e6, 1575.42e6};
+    Band l5{l5_raw.data(), UGSDR_FMT_BBP_DDC, 4 * 66500, 66.5e6, 1176.45e6};
+    auto both = engine.ProcessGps(l1, l5, svs);
+    EXPECT(!both.empty());
+    for (auto& r : both) EXPECT(r.sv_number.signal == ugsdr::Signal::GpsCoarseAcquisition_L1);
+
+    auto only_l5 = engine.ProcessGps(Band{}, l5, svs);
+    std::vector<int> ids;
+    for (auto& r : only_l5) {
+        EXPECT(r.sv_number.signal == ugsdr::Signal::Gps_L5I);
+        ids.push_back(r.sv_number.id);
+        std::printf("gps L5I: sv %d doppler %.0f code offset %.0f snr %.2f\n", r.sv_number.id, r.doppler,
+                    r.code_offset, r.GetSnr());
+    }
+    EXPECT((ids == std::vector<int>{2, 5, 8, 25}));
+
+    // L5Q only: sv 6, 4001 samples (2000.5 chips) in, +1800 Hz, 20.46 MHz complex float, 4 ms
+    const double fs = 20.46e6, dop = 1800.0;
+    const int n = 4 * 20460, delay = 4001;
+    std::vector<int8_t> q(10230);
+    ugsdr_hip::check(ugsdr_code_gps_l5q(6, q.data()));
+    std::mt19937 gen(9);
+    std::normal_distribution<double> noise(0.0, 1.0);
+    std::vector<std::complex<float>> x(n);
+    for (int k = 0; k < n; ++k) {
+        const int m = ((k - delay) % 20460 + 20460) % 20460;
+        x[k] = std::complex<float>(std::complex<double>(noise(gen), noise(gen)) +
+                                   0.08 * q[(std::size_t)m / 2] * std::polar(1.0, 2 * M_PI * dop * k / fs));
+    }
+    Band synth{x.data(), UGSDR_FMT_CF32, (std::size_t)n, fs, 1176.45e6};
+    auto only_q = engine.ProcessGps(Band{}, synth, svs);
+    EXPECT(only_q.size() == 1);
+    if (!only_q.empty()) {
+        EXPECT(only_q[0].sv_number.id == 6 && only_q[0].sv_number.signal == ugsdr::Signal::Gps_L5Q);
+        EXPECT(only_q[0].code_offset == delay && only_q[0].doppler == dop);
+        std::printf("gps L5Q fallback: sv %d doppler %.0f code offset %.0f snr %.2f\n", only_q[0].sv_number.id,
+                    only_q[0].doppler, only_q[0].code_offset, only_q[0].GetSnr());
+    }
+    std::puts("process gps ok");
+    return 0;
+}
+
+// HipTableMixer through the reference's Mixer surface: the in-place static
+// overload and the member overload over two calls equal one call with the
+// member's phase advance (TableMixer, src/mixer/table_mixer.hpp).
+static int table_mixer() {
+    const double fs = 4.092e6, f = 1250.0;
+    std::vector<std::complex<float>> a(8184, {1.0f, 0.0f});
+    ugsdr::HipTableMixer::Translate(a, fs, f, 0.0);
+    // fs/4 exactly: cos/sin at quarter turns
+    std::vector<std::complex<float>> q(8, {1.0f, 0.0f});
+    ugsdr::HipTableMixer::Translate(q, 4.0, 1.0, 0.0);
+    EXPECT(std::abs(q[1] - std::complex<float>(0.f, 1.f)) < 1e-6f && std::abs(q[2] + 1.0f) < 1e-6f);
+    ugsdr::HipTableMixer m(fs, f, 0.0);
+    std::vector<std::complex<float>> b1(4092, {1.0f, 0.0f}), b2(4092, {1.0f, 0.0f});
+    m.Translate(b1);
+    m.Translate(b2);
+    EXPECT(b1[0] == a[0] && b1[4091] == a[4091]);
+    EXPECT(std::abs(b2[0] - std::polar(1.0f, (float)(2 * M_PI * 0.25))) < 0.1f);   // phase 2 pi * frac(1.25)
+    std::puts("table mixer ok");
+    return 0;
+}
+
 int main(int argc, char** argv) {
-    if (argc < 3) {
-        std::fprintf(stderr, "usage: %s real_8_first4ms.bin bbp_ddc_gps_L1_first4ms.bin\n", argv[0]);
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s real_8_first4ms.bin bbp_ddc_gps_L1_first4ms.bin bbp_ddc_gps_L5_first4ms.bin\n",
+                     argv[0]);
         return 2;
     }
     try {
@@ -344,6 +433,8 @@ int main(int argc, char** argv) {
         if (int rc = tracker_lock()) return rc;
         if (int rc = galileo_hypotheses()) return rc;
         if (int rc = antijamming()) return rc;
+        if (int rc = table_mixer()) return rc;
+        if (int rc = process_gps(argv[2], argv[3])) return rc;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "exception: %s\n", e.what());
         return 1;

@@ -232,6 +232,51 @@ def test_corr_variants_match_oracle(ctx, oracle, n_fft, surface_len, coherent):
         ctx.replica_drop(k)
 
 
+def planted_case(oracle, rng, n_fft, surface_len, n_chips, fs, n_blocks=2, shift=777, f_dop=1000.0, n_prn=3):
+    """n_prn random +-1 codes of n_chips per surface_len samples, the second
+    one planted at code phase `shift` and Doppler f_dop in complex AWGN"""
+    repeats = n_fft // surface_len
+    codes = [np.where(rng.random(n_chips) < 0.5, -1, 1).astype(np.int8) for _ in range(n_prn)]
+    reps = np.stack([oracle.repeat_upsample(c, repeats, n_fft) for c in codes])
+    n = n_blocks * n_fft
+    t = np.arange(n)
+    x = (rng.normal(0, 20, n) + 1j * rng.normal(0, 20, n)).astype(np.complex64)
+    x += (2.0 * np.roll(np.tile(reps[1], n_blocks), shift) * np.exp(2j * np.pi * f_dop * t / fs)).astype(np.complex64)
+    return codes, reps, x
+
+
+# every transform length the workloads and recordings use beyond the C/A
+# lengths above: the FP32 four-step plans N1 x 341 (N1 = 60, 120, 180, 240:
+# L5-like 1-3 ms at 20.46 MHz and the BOC configuration), and the BbpDdc
+# recording's N = 4200 (4.2 MHz, 1 ms) and 26600 (6.65 MHz, 4 ms) plans
+@pytest.mark.parametrize("n_fft,surface_len,n_chips,fs", [
+    (20460, 20460, 10230, 20.46e6), (40920, 40920, 10230, 20.46e6), (40920, 20460, 10230, 20.46e6),
+    (61380, 61380, 10230, 20.46e6), (61380, 20460, 10230, 20.46e6), (81840, 81840, 8184, 20.46e6),
+    (4200, 4200, 1023, 4.2e6), (26600, 26600, 1023, 6.65e6), (26600, 6650, 1023, 6.65e6)])
+def test_fourstep_and_recording_lengths_every_variant(ctx, oracle, n_fft, surface_len, n_chips, fs):
+    rng = np.random.default_rng(n_fft * 7 + surface_len)
+    dop = U.doppler_bins(2000, 500)
+    codes, reps, x = planted_case(oracle, rng, n_fft, surface_len, n_chips, fs)
+    repeats = n_fft // surface_len
+    keys = [ctx.replica_put(c, repeats, n_fft) for c in codes]
+    iq = np.ascontiguousarray(x).view(np.float32)
+    ref, rsurf, _ = oracle.acquire(x, reps, n_fft=n_fft, n_blocks=2, surface_len=surface_len, fs_in=fs, fs_acq=fs,
+                                   doppler=dop, precision=64, want_surfaces=True)
+    variants = U.corr_variants(n_fft)
+    assert variants
+    for t in variants:
+        say(f"case length N={n_fft} L={surface_len} K=2 threads={t}")
+        ctx.set_threads(n_fft, t)
+        gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft,
+                                   n_blocks=2, surface_len=surface_len, doppler=dop, keys=keys)
+        assert_parity(gpu, ref, f"N={n_fft}/L={surface_len}/T={t}")
+        assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
+        assert gpu[1]["index"] == 777 % surface_len and gpu[1]["doppler"] == 1000.0
+    ctx.set_threads(n_fft, 0)
+    for k in keys:
+        ctx.replica_drop(k)
+
+
 @pytest.mark.parametrize("n_prn,n_bins,n_blocks", [(3, 9, 3), (4, 70, 4)])
 def test_corr1023_tail_rows_split_over_workgroups(ctx, oracle, n_prn, n_bins, n_blocks):
     """N = 16368 with several blocks per row: whole rounds of rows run one row

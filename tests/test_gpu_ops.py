@@ -21,6 +21,50 @@ def test_mixer(ctx, oracle):
     np.testing.assert_array_equal(ctx.mix_inplace(x, 4e6, 0.0, 1.0), x)   # mixer.hpp:36-37
 
 
+TABLE_MIXER_CASES = [
+    (4.092e6, 1.25e3, 0.0), (4.092e6, -3.7e3, 1.0), (40.92e6, 5.0e3, -2.5), (16.368e6, 4.1234e6, 9.0),
+    (2.0e6, 1.0e6, 0.0), (4.2e6, 0.0, 1.0), (1.0e6, 1.3e6, 0.3), (33.25e6, -8.4205e6, 6.283185307179586),
+]
+
+
+@pytest.mark.parametrize("fs,f,phase", TABLE_MIXER_CASES)
+def test_table_mixer_bit_exact(ctx, oracle, fs, f, phase):
+    """HipTableMixer against the oracle's TableMixer, itself bit-exact with the
+    reference's own TableMixer (tests/test_ref_pins.py): every output bit equal"""
+    print(f"case ops/table_mixer fs={fs} f={f} phase={phase}", flush=True)
+    rng = np.random.default_rng(int(fs) % 997)
+    x = (100 * (rng.normal(size=70001) + 1j * rng.normal(size=70001))).astype(np.complex64)
+    got = ctx.table_mix(x, fs, f, phase)
+    np.testing.assert_array_equal(got.view(np.uint32), oracle.table_mix(x, fs, f, phase).view(np.uint32))
+    from oracle import refsrc
+    if refsrc.available():
+        np.testing.assert_array_equal(got.view(np.uint32), refsrc.table_mix(x, fs, f, phase).view(np.uint32))
+
+
+def test_table_mixer_member_and_device(ctx, oracle):
+    """member Translate (mixer.hpp:53-60) over five 1 ms calls, and the device
+    memory form, equal the oracle bit for bit"""
+    from conftest import DeviceArray
+    print("case ops/table_mixer member + device", flush=True)
+    fs, f, phase, n = 4.092e6, 2.375e3, 0.25, 4092
+    rng = np.random.default_rng(11)
+    x = (30 * (rng.normal(size=5 * n) + 1j * rng.normal(size=5 * n))).astype(np.complex64)
+    ph_g = ph_o = phase
+    for c in range(5):
+        g, ph_g = ctx.table_mixer_member(x[c * n:(c + 1) * n], fs, f, ph_g)
+        o, ph_o = oracle.table_mixer_member(x[c * n:(c + 1) * n], fs, f, ph_o)
+        np.testing.assert_array_equal(g.view(np.uint32), o.view(np.uint32))
+        assert ph_g == ph_o
+    d = DeviceArray(x)
+    try:
+        ctx.table_mix_device(d.ptr.value, len(x), fs, -f, 1.5)
+        np.testing.assert_array_equal(d.get().view(np.uint32), oracle.table_mix(x, fs, -f, 1.5).view(np.uint32))
+    finally:
+        d.close()
+    with pytest.raises(U.UgsdrError):
+        ctx.table_mix(x[:8], fs, -3 * fs, 0.0)
+
+
 @pytest.mark.parametrize("n", [511, 1000, 4092, 16368, 38192])
 def test_dft(ctx, n):
     print(f"case ops/dft n={n}", flush=True)

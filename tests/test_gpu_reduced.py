@@ -95,6 +95,42 @@ def test_reduced_boc_vs_fp64_and_fp32(ctx, oracle, prec):
         ctx.replica_drop(k)
 
 
+@pytest.mark.parametrize("prec", [U.PREC_FP16, U.PREC_BF16])
+@pytest.mark.parametrize("n_fft", [20460, 38192, 40920, 61380])
+def test_reduced_fourstep_lengths_vs_fp64(ctx, oracle, prec, n_fft):
+    """the other matrix-core plans (N1 = 60, 112, 120, 180 times 341): L5-like
+    1-3 ms at 20.46 MHz and the real_8 recording's 38192, 2 blocks, 9 bins,
+    3 PRNs with one planted"""
+    rng = np.random.default_rng(n_fft + prec)
+    fs = 20.46e6 if n_fft % 20460 == 0 else 9.548e6
+    n_chips = 10230 * (n_fft // 20460) if n_fft % 20460 == 0 else 4092
+    dop = U.doppler_bins(2000, 500)
+    codes = [np.where(rng.random(n_chips) < 0.5, -1, 1).astype(np.int8) for _ in range(3)]
+    reps = np.stack([oracle.repeat_upsample(c, 1, n_fft) for c in codes])
+    t = np.arange(2 * n_fft)
+    x = (rng.normal(0, 20, 2 * n_fft) + 1j * rng.normal(0, 20, 2 * n_fft)).astype(np.complex64)
+    x += (2.0 * np.roll(np.tile(reps[1], 2), 4321) * np.exp(2j * np.pi * 1000 * t / fs)).astype(np.complex64)
+    iq = np.ascontiguousarray(x).view(np.float32)
+    say(f"case reduced N={n_fft} {NAME[prec]}")
+    kw = dict(fs_in=fs, fs_acq=fs, n_fft=n_fft, n_blocks=2, surface_len=n_fft, doppler=dop)
+    k32 = [ctx.replica_put(c, 1, n_fft) for c in codes]
+    kr = [ctx.replica_put(c, 1, n_fft, prec) for c in codes]
+    g32, s32, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, keys=k32, **kw)
+    gr, sr, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, keys=kr, precision=prec, **kw)
+    ref, s64, _ = oracle.acquire(x, reps, n_fft=n_fft, n_blocks=2, surface_len=n_fft, fs_in=fs, fs_acq=fs,
+                                 doppler=dop, precision=64, want_surfaces=True)
+    err_r = np.linalg.norm(sr - s64) / np.linalg.norm(s64)
+    say(f"  surface rel err vs FP64: {err_r:.2e}; FP32 {np.linalg.norm(s32 - s64) / np.linalg.norm(s64):.2e}")
+    assert err_r <= SURF_TOL[prec]
+    d_db = 20 * np.log10(gr[1]["snr"] / g32[1]["snr"])
+    say(f"  planted: idx {gr[1]['index']} bin {gr[1]['doppler_bin']} PNR delta vs FP32 {d_db:+.4f} dB")
+    assert abs(d_db) <= PNR_DB
+    assert gr[1]["index"] == g32[1]["index"] == ref[1]["index"] == 4321
+    assert gr[1]["doppler_bin"] == g32[1]["doppler_bin"] == ref[1]["doppler_bin"]
+    for k in k32 + kr:
+        ctx.replica_drop(k)
+
+
 def test_reduced_rejects_unsupported(ctx):
     """lengths without a matrix-core plan and folded surfaces report UNSUPPORTED"""
     w = W.l1ca_1ms()

@@ -7,6 +7,7 @@ point raises if it cannot be loaded — there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -199,6 +200,7 @@ SIGNATURES = {
     "ugsdr_hip_acquire": ([VP, C.POINTER(AcqRequest), C.POINTER(AcqResult), VP, VP], C.c_int),
     "ugsdr_hip_mix": ([VP, VP, VP, I64, D, D, D], C.c_int),
     "ugsdr_hip_mix_inplace": ([VP, VP, I64, D, D, D], C.c_int),
+    "ugsdr_hip_table_mix": ([VP, VP, I64, D, D, D, C.c_int], C.c_int),
     "ugsdr_hip_dft": ([VP, VP, VP, I64, C.c_int], C.c_int),
     "ugsdr_hip_dft_reduced": ([VP, C.c_int, C.c_int, VP, VP, I64, I64], C.c_int),
     "ugsdr_hip_mf_prepare": ([VP, VP, VP, I64], C.c_int),
@@ -575,6 +577,26 @@ class Context:
         x = _c64(x).copy()
         _check(lib().ugsdr_hip_mix_inplace(self._p, _ptr(x), len(x), fs, f, phase))
         return x
+
+    def table_mix(self, x, fs: float, f: float, phase: float = 0.0) -> np.ndarray:
+        """TableMixer through Mixer::Translate's in-place overload
+        (src/mixer/table_mixer.hpp:78-101, mixer.hpp:33-41)."""
+        x = _c64(x).copy()
+        _check(lib().ugsdr_hip_table_mix(self._p, _ptr(x), len(x), fs, f, phase, 0))
+        return x
+
+    def table_mix_device(self, ptr: int, n: int, fs: float, f: float, phase: float = 0.0) -> None:
+        """the same on n complex float samples in device memory, in place"""
+        _check(lib().ugsdr_hip_table_mix(self._p, ptr, int(n), fs, f, phase, 1))
+
+    def table_mixer_member(self, x, fs: float, f: float, phase: float):
+        """Mixer<TableMixer>'s member Translate (mixer.hpp:53-60): mixes with
+        the stored phase, returns (samples, phase + 2 pi frac(f/1000))."""
+        y = self.table_mix(x, fs, f, phase)
+        m = math.fmod(f / 1000.0, 1.0)
+        if m < 0:
+            m += 1
+        return y, phase + 2 * math.pi * m
 
     def dft(self, x, inverse: bool = False) -> np.ndarray:
         x = _c64(x)

@@ -35,6 +35,33 @@ __global__ void mix_kernel(const float2* in, float2* out, int64_t n, double f_tu
     }
 }
 
+// TableMixer's 80-entry sine table (src/mixer/table_mixer.hpp:31-47), built on
+// the host in FP64 and rounded once to float, so that the device uses the very
+// values the reference's host build computes.
+struct SineTable80 {
+    float v[80];
+};
+
+// TableMixer::Process (table_mixer.hpp:78-101) in place: sample i takes the
+// top 6 bits of the 32-bit accumulator holder0 + i * adder (nco.hpp:17-20,
+// wrapping), cos at entry + 16 and sin at the entry; the complex product is
+// formed from separately rounded products, as the reference's
+// complex<float> multiply is (no fused multiply-add).
+__global__ void table_mix_kernel(float2* io, int64_t n, uint32_t holder0, uint32_t adder, SineTable80 tab) {
+#pragma clang fp contract(off)
+    __shared__ float t[80];
+    if (threadIdx.x < 80) t[threadIdx.x] = tab.v[threadIdx.x];
+    __syncthreads();
+    GRID_STRIDE(i, n) {
+        const uint32_t h = holder0 + (uint32_t)(uint64_t)i * adder;
+        const uint32_t idx = h >> 26;
+        const float c = t[idx + 16], s = t[idx];
+        const float2 x = io[i];
+        const float ac = x.x * c, bs = x.y * s, as = x.x * s, bc = x.y * c;
+        io[i] = make_float2(ac - bs, as + bc);
+    }
+}
+
 // One generic Stockham stage, any radix R, one thread per output element:
 // out[(j/Ns)*Ns*R + j%Ns + q*Ns] = sum_r in[j + r*N/R] exp(-2 pi i r (j%Ns + q Ns)/(Ns R))
 __global__ void dft_stage_kernel(const float2* in, float2* out, int64_t N, int64_t R, int64_t Ns) {
@@ -254,6 +281,38 @@ static int mix_common(ugsdr_hip_ctx* ctx, const float* in, float* out, int64_t n
     return d2h(ctx, out, ctx->aux.ptr, n * 8);
 }
 
+// Mixer<TableMixer>::Translate, in-place overload (mixer.hpp:33-41): FixPhase,
+// f == 0 returns early, f < 0 folds to fs + f, then TableMixer::Process.
+static int table_mix_common(ugsdr_hip_ctx* ctx, float* io, int64_t n, double fs, double f, double phase,
+                            int on_device) {
+    UGSDR_CHECK_ARG(ctx && io && fs > 0 && n >= 0, "bad table mixer arguments");
+    phase = std::fmod(phase, 2 * M_PI);
+    if (phase < 0) phase += 2 * M_PI;
+    if (std::fabs(f) == 0.0) return UGSDR_OK;
+    if (f < 0) f = fs + f;
+    UGSDR_CHECK_ARG(f >= 0, "table mixer frequency below -fs");
+    SineTable80 tab;
+    const double step = 5 * M_PI / 80.0 / 2;   // table_mixer.hpp:37
+    for (int i = 0; i < 80; ++i) tab.v[i] = (float)(1.0 * std::sin((double)i * step));
+    // NumericallyControlledOscillator<32> (nco.hpp:17-20): truncating casts,
+    // the 32-bit holder keeps the low 32 bits
+    const uint32_t adder = (uint32_t)(uint64_t)(4294967296.0 * f / fs);
+    const uint32_t holder = (uint32_t)(uint64_t)(4294967296.0 * phase / (2 * M_PI));
+    Lock lk(ctx);
+    UGSDR_HIP_TRY(hipSetDevice(ctx->device));
+    float2* d = reinterpret_cast<float2*>(io);
+    if (!on_device) {
+        if (int rc = h2d(ctx, ctx->aux, io, n * 8)) return rc;
+        d = ctx->aux.as<float2>();
+    }
+    if (n > 0)
+        hipLaunchKernelGGL(table_mix_kernel, dim3(grid_for(n)), dim3(256), 0, ctx->stream, d, n, holder, adder, tab);
+    UGSDR_HIP_TRY(hipGetLastError());
+    if (!on_device) return d2h(ctx, io, ctx->aux.ptr, n * 8);
+    UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return UGSDR_OK;
+}
+
 // ---------------------------------------------------------------- formats
 __global__ void unpack_kernel(SampleSource src, float2* out, int64_t n) {
     GRID_STRIDE(i, n) out[i] = load_sample(src, i);
@@ -316,6 +375,11 @@ int ugsdr_hip_mix(ugsdr_hip_ctx* ctx, const float* in, float* out, int64_t n, do
 
 int ugsdr_hip_mix_inplace(ugsdr_hip_ctx* ctx, float* io, int64_t n, double fs, double f, double phase) {
     return mix_common(ctx, io, io, n, fs, f, phase, true);
+}
+
+int ugsdr_hip_table_mix(ugsdr_hip_ctx* ctx, float* io, int64_t n, double fs, double f, double phase,
+                        int on_device) {
+    return table_mix_common(ctx, io, n, fs, f, phase, on_device);
 }
 
 int ugsdr_hip_dft(ugsdr_hip_ctx* ctx, const float* in, float* out, int64_t n, int inverse) {

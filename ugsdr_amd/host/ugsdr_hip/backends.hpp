@@ -7,6 +7,7 @@
 #include <complex>
 #include <cstddef>
 #include <span>
+#include <type_traits>
 #include <vector>
 
 #include "bases.hpp"
@@ -18,6 +19,19 @@ namespace hip_detail {
 inline ugsdr_hip_ctx* ctx() { return ugsdr_hip::HipContext::Default().get(); }
 inline const float* cf(const std::complex<float>* p) { return reinterpret_cast<const float*>(p); }
 inline float* cf(std::complex<float>* p) { return reinterpret_cast<float*>(p); }
+
+// The Hip* backends compute in FP32 (complex<float>), the reference's float
+// path.  The reference's typed tests also instantiate its operators over
+// double (test/basic_tests.cpp); those instantiations have no device
+// counterpart here and stop at compile time with this message instead of
+// narrowing to float in silence.
+template <typename T>
+constexpr bool is_fp32 = std::is_same_v<T, float>;
+template <typename T>
+void fp32_only() {
+    static_assert(is_fp32<T>, "ugsdr Hip* backends compute in FP32 (std::complex<float>); "
+                              "instantiations over double have no device counterpart");
+}
 }  // namespace hip_detail
 
 // IppMixer::Process counterpart (src/mixer/ipp_mixer.hpp:47-88); frequency is
@@ -40,9 +54,48 @@ protected:
                                        (int64_t)src.size(), fs, f, phase));
         return dst;
     }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static void Process(std::vector<std::complex<U>>&, double, double, double = 0) {
+        hip_detail::fp32_only<U>();
+    }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static std::vector<std::complex<U>> Process(const std::vector<std::complex<U>>& src, double, double, double = 0) {
+        hip_detail::fp32_only<U>();
+        return src;
+    }
 
 public:
     HipMixer(double sampling_freq, double frequency, double phase) : Mixer<HipMixer>(sampling_freq, frequency, phase) {}
+};
+
+// TableMixer counterpart (src/mixer/table_mixer.hpp:27-105), the mixer of the
+// reference's non-IPP tracking and DFE configurations: 32-bit NCO and the
+// 80-entry float sine table, bit-exact with the reference.  Like TableMixer it
+// has only the in-place Process, so only Mixer's in-place and member
+// Translate overloads instantiate.
+class HipTableMixer : public Mixer<HipTableMixer> {
+protected:
+    friend class Mixer<HipTableMixer>;
+
+    static void Process(std::vector<std::complex<float>>& src_dst, double fs, double f, double phase = 0) {
+        ugsdr_hip::check(ugsdr_hip_table_mix(hip_detail::ctx(), hip_detail::cf(src_dst.data()),
+                                             (int64_t)src_dst.size(), fs, f, phase, 0));
+    }
+    static void Process(std::span<std::complex<float>> src_dst, double fs, double f, double phase = 0) {
+        ugsdr_hip::check(ugsdr_hip_table_mix(hip_detail::ctx(), hip_detail::cf(src_dst.data()),
+                                             (int64_t)src_dst.size(), fs, f, phase, 0));
+    }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static void Process(std::vector<std::complex<U>>&, double, double, double = 0) {
+        hip_detail::fp32_only<U>();
+    }
+
+public:
+    HipTableMixer(double sampling_freq, double frequency, double phase)
+        : Mixer<HipTableMixer>(sampling_freq, frequency, phase) {}
 };
 
 // IppMatchedFilter counterpart (src/matched_filter/ipp_matched_filter.hpp:60-90)
@@ -69,6 +122,16 @@ protected:
     static void Process(std::vector<std::complex<float>>& src_dst, const std::vector<T>& impulse_response) {
         ProcessOptimized(src_dst, Prepare(impulse_response));
     }
+    template <typename U, typename S>
+        requires(!hip_detail::is_fp32<U>)
+    static void ProcessOptimized(std::vector<std::complex<U>>&, const S&) {
+        hip_detail::fp32_only<U>();
+    }
+    template <typename U, typename T>
+        requires(!hip_detail::is_fp32<U>)
+    static void Process(std::vector<std::complex<U>>&, const std::vector<T>&) {
+        hip_detail::fp32_only<U>();
+    }
 };
 
 // IppAbs counterpart (src/math/ipp_abs.hpp:25-33)
@@ -79,6 +142,12 @@ protected:
         std::vector<float> dst(src.size());
         ugsdr_hip::check(ugsdr_hip_abs(hip_detail::ctx(), hip_detail::cf(src.data()), dst.data(), (int64_t)src.size()));
         return dst;
+    }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static std::vector<U> Process(const std::vector<std::complex<U>>& src) {
+        hip_detail::fp32_only<U>();
+        return std::vector<U>(src.size());
     }
 };
 
@@ -105,6 +174,16 @@ protected:
         Process(dst, block);
         return dst;
     }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static void Process(std::vector<U>&, std::size_t) {
+        hip_detail::fp32_only<U>();
+    }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static void Process(std::vector<std::complex<U>>&, std::size_t) {
+        hip_detail::fp32_only<U>();
+    }
 };
 
 // IppMaxIndex counterpart (src/math/ipp_max_index.hpp:27-39): first index of the max
@@ -120,6 +199,12 @@ protected:
         r.index = (std::size_t)i;
         return r;
     }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static MaxIndex::Result<U> Process(const std::vector<U>&) {
+        hip_detail::fp32_only<U>();
+        return {};
+    }
 };
 
 // IppMeanStdDev counterpart (src/math/ipp_mean_stddev.hpp:27-39): sample sigma
@@ -130,6 +215,12 @@ protected:
         MeanStdDev::Result<float> r;
         ugsdr_hip::check(ugsdr_hip_mean_std(hip_detail::ctx(), src.data(), (int64_t)src.size(), &r.mean, &r.sigma));
         return r;
+    }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static MeanStdDev::Result<U> Process(const std::vector<U>&) {
+        hip_detail::fp32_only<U>();
+        return {};
     }
 };
 
@@ -150,61 +241,34 @@ protected:
         Process(dst, new_fs, old_fs);
         return dst;
     }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static void Process(std::vector<std::complex<U>>&, std::size_t, std::size_t) {
+        hip_detail::fp32_only<U>();
+    }
+    template <typename U>
+        requires(!hip_detail::is_fp32<U>)
+    static std::vector<std::complex<U>> Process(const std::vector<std::complex<U>>& src, std::size_t, std::size_t) {
+        hip_detail::fp32_only<U>();
+        return src;
+    }
 };
 
 // SequentialUpsampler counterpart (src/resample/upsampler.hpp:31-37)
 class HipUpsampler : public Upsampler<HipUpsampler> {
 protected:
     friend class Upsampler<HipUpsampler>;
+    // codes of any integer or float element type; double values would be
+    // narrowed on the way through the device, so they stop at compile time
     template <typename T>
     static void Process(std::vector<T>& src_dst, std::size_t samples) {
+        if constexpr (std::is_floating_point_v<T>) hip_detail::fp32_only<T>();
         std::vector<float> in(src_dst.begin(), src_dst.end()), out(samples);
         ugsdr_hip::check(ugsdr_hip_upsample(hip_detail::ctx(), in.data(), (int64_t)in.size(), out.data(),
                                             (int64_t)samples));
         src_dst.assign(out.begin(), out.end());
     }
 };
-
-// IppStft / SequentialStft counterpart (src/math/ipp_stft.hpp:14-96,
-// src/math/stft.hpp:31-88): the window is IppStft's Blackman-Harris or
-// SequentialStft's Hann; the device transform has the reference's default
-// geometry (128-sample windows, 96 overlap).
-template <int Window = UGSDR_STFT_BLACKMAN_HARRIS>
-class HipStftT : public ShortTimeFourierTransform<HipStftT<Window>> {
-protected:
-    friend class ShortTimeFourierTransform<HipStftT<Window>>;
-    static void CheckGeometry(std::size_t window_length, std::size_t overlap_length) {
-        if (window_length != 128 || overlap_length != 96)
-            throw std::runtime_error("HipStft runs 128-sample windows with 96 overlap");
-    }
-    static auto Process(const std::vector<std::complex<float>>& src, std::size_t window_length,
-                        std::size_t overlap_length) {
-        CheckGeometry(window_length, overlap_length);
-        const std::size_t ncol = src.size() >= 128 ? (src.size() - 96) / 32 : 0;
-        std::vector<std::complex<float>> flat(ncol * 128);
-        ugsdr_hip::check(ugsdr_hip_stft(hip_detail::ctx(), hip_detail::cf(src.data()), (int64_t)src.size(), Window,
-                                        hip_detail::cf(flat.data())));
-        std::vector<std::vector<std::complex<float>>> dst(ncol);
-        for (std::size_t c = 0; c < ncol; ++c) dst[c].assign(flat.begin() + c * 128, flat.begin() + (c + 1) * 128);
-        return dst;
-    }
-    static auto Process(const std::vector<std::vector<std::complex<float>>>& src, std::size_t window_length,
-                        std::size_t overlap_length) {
-        CheckGeometry(window_length, overlap_length);
-        std::vector<std::complex<float>> flat;
-        for (auto& col : src) {
-            if (col.size() != 128) throw std::runtime_error("Size mismatch");
-            flat.insert(flat.end(), col.begin(), col.end());
-        }
-        std::vector<std::complex<float>> dst(src.empty() ? 0 : 128 + (src.size() - 1) * 32);
-        if (!src.empty())
-            ugsdr_hip::check(ugsdr_hip_istft(hip_detail::ctx(), hip_detail::cf(flat.data()), (int64_t)src.size(),
-                                             hip_detail::cf(dst.data())));
-        return dst;
-    }
-};
-using HipStft = HipStftT<UGSDR_STFT_BLACKMAN_HARRIS>;
-using HipSequentialStft = HipStftT<UGSDR_STFT_HANN>;
 
 // IppCorrelator counterpart (src/correlator/ipp_correlator.hpp:25-38)
 class HipCorrelator : public Correlator<HipCorrelator> {
@@ -223,6 +287,7 @@ protected:
     }
     template <typename T1, typename T2>
     static auto Process(const T1& signal, const T2& code) {
+        hip_detail::fp32_only<typename T1::value_type::value_type>();
         return Process(std::span<const std::complex<float>>(signal.data(), signal.size()),
                        std::span<const typename T2::value_type>(code.data(), code.size()));
     }

@@ -13,41 +13,13 @@
 #include <type_traits>
 #include <vector>
 
+#include "acquisition_result.hpp"
 #include "backends.hpp"
 #include "context.hpp"
 
 namespace ugsdr {
 
 #ifndef UGSDR_HIP_REFERENCE_BASES
-enum class System : std::uint32_t { Gps = 0, Glonass, Galileo, BeiDou, NavIC, Sbas, Qzss };
-enum class Signal : std::uint32_t {
-    GpsCoarseAcquisition_L1, Gps_L2CM, Gps_L5I, Gps_L5Q, GlonassCivilFdma_L1, GlonassCivilFdma_L2, Galileo_E1b,
-    Galileo_E1c, Galileo_E5aI, Galileo_E5aQ, Galileo_E5bI, Galileo_E5bQ, Galileo_E6b, Galileo_E6c, BeiDou_B1I,
-    BeiDou_B1C, NavIC_L5, SbasCoarseAcquisition_L1, Sbas_L5I, Sbas_L5Q, QzssCoarseAcquisition_L1, Qzss_L1S,
-    Qzss_L2CM, Qzss_L5I, Qzss_L5Q,
-};
-// satellite id (src/common.hpp:147-154)
-struct Sv {
-    std::int32_t id = 0;
-    System system = System::Gps;
-    Signal signal = Signal::GpsCoarseAcquisition_L1;
-};
-
-// src/acquisition/acquisition_result.hpp:12-31
-template <typename T>
-struct AcquisitionResult final {
-    Sv sv_number;
-    double doppler = 0;
-    double code_offset = 0;
-    double level = 0;
-    double sigma = 1.0;
-    double intermediate_frequency = 0;
-    std::vector<T> output_peak;
-    auto operator<=>(const AcquisitionResult& rhs) const { return level <=> rhs.level; }
-    auto GetSnr() const { return level / sigma; }
-    auto GetAcquiredSignalType() const { return sv_number.signal; }
-};
-
 // backend type list with is-base-of gates (src/acquisition/fse.hpp:36-67)
 template <auto target_sampling_rate, typename MixerT, typename UpsamplerT, typename MatchedFilterT, typename AbsT,
           typename ReshapeAndSumT, typename MaxIndexT, typename MeanStdDevT, typename ResamplerT>
@@ -183,54 +155,41 @@ public:
                                                   double intermediate_frequency, const std::vector<Sv>& svs,
                                                   const std::vector<std::vector<int8_t>>& codes,
                                                   std::size_t code_ms = 1) {
-        const double new_fs = AdjustSamplingRate(fs, options.target_sampling_rate);
-        const auto spm = (std::size_t)(new_fs / 1e3);
-        const std::size_t n_fft = options.coherent_ms * spm;
-        std::vector<double> doppler;
-        for (double d = -doppler_range; d <= doppler_range; d += doppler_step) doppler.push_back(d);
-        std::vector<uint64_t> prn_keys;
-        for (std::size_t i = 0; i < svs.size(); ++i) prn_keys.push_back(Key(codes[i], options.coherent_ms / code_ms,
-                                                                            n_fft));
-        ugsdr_acq_request r{};
-        r.iq = iq;
-        r.fmt = fmt;
-        r.iq_on_device = 0;
-        r.n_samples = (int64_t)n_samples;
-        r.fs_in = fs;
-        r.fs_acq = new_fs;
-        r.mix_frequency = -intermediate_frequency;  // Translate(signal, fs, -IF) (fse.hpp:289)
-        r.intermediate_frequency = intermediate_frequency;
-        r.n_fft = (int64_t)n_fft;
-        r.n_blocks = (int64_t)options.noncoherent_blocks;
-        r.surface_len = (int64_t)(options.reshape ? spm : n_fft);
-        r.reshape = options.reshape;
-        r.coherent = options.coherent;
-        r.doppler = doppler.data();
-        r.n_doppler = (int64_t)doppler.size();
-        r.replica_keys = prn_keys.data();
-        r.n_prn = (int64_t)prn_keys.size();
-        r.threshold = options.threshold;
-        r.precision = options.precision;
-        std::vector<ugsdr_acq_result> out(svs.size());
-        std::vector<float> peaks(options.output_peak ? svs.size() * (std::size_t)r.surface_len : 0);
-        ugsdr_hip::check(ugsdr_hip_acquire(context->get(), &r, out.data(), nullptr,
-                                           options.output_peak ? peaks.data() : nullptr));
+        return Run(iq, fmt, n_samples, fs, intermediate_frequency, options.target_sampling_rate, options.coherent,
+                   svs, codes, code_ms);
+    }
+
+    // One band of the digital front end (SignalEpoch::GetSubband with the
+    // channel's rate and centre frequency, src/dfe/dfe.hpp); a band the
+    // front end does not have is left empty (DigitalFrontend::HasSignal).
+    struct Subband {
+        const void* iq = nullptr;
+        int fmt = UGSDR_FMT_CF32;
+        std::size_t n_samples = 0;
+        double sampling_rate = 0;
+        double central_frequency = 0;
+        bool Present() const { return iq != nullptr && n_samples > 0; }
+    };
+
+    // FastSearchEngineBase::ProcessGps (src/acquisition/fse.hpp:327-335): GPS
+    // L1 C/A (AcquireGoldCodesL1, fse.hpp:278-300) and, only when that finds
+    // nothing, L5I and then L5Q (AcquireL5<signal, coherent=false>,
+    // fse.hpp:302-325): the IF measured from 1176.45 MHz, the rate from
+    // AdjustSamplingRate(fs, 20.46 MHz), the 4 ms correlation folded to 1 ms
+    // as complex values before |.|.  svs: the satellites tried (their id is
+    // the 0-based PRN); results carry the signal that found them.
+    std::vector<AcquisitionResult<float>> ProcessGps(const Subband& l1, const Subband& l5,
+                                                     const std::vector<Sv>& svs) {
         std::vector<AcquisitionResult<float>> dst;
-        for (std::size_t i = 0; i < svs.size(); ++i) {
-            if (!out[i].detected) continue;
-            AcquisitionResult<float> a;
-            a.sv_number = svs[i];
-            a.doppler = out[i].doppler;
-            a.code_offset = out[i].code_offset;
-            a.level = out[i].level;
-            a.sigma = out[i].sigma;
-            a.intermediate_frequency = intermediate_frequency;
-            if (options.output_peak)
-                a.output_peak.assign(peaks.begin() + i * r.surface_len, peaks.begin() + (i + 1) * r.surface_len);
-            dst.push_back(std::move(a));
-        }
-        std::sort(dst.begin(), dst.end(), [](auto& a, auto& b) { return a.sv_number.id < b.sv_number.id; });
-        return dst;
+        if (l1.Present())
+            dst = Acquire(l1, 1575.42e6, options.target_sampling_rate, true, Signal::GpsCoarseAcquisition_L1,
+                          ugsdr_code_gps_l1ca, 1023, svs);
+        if (!dst.empty() || !l5.Present()) return dst;
+        dst = Acquire(l5, 1176.45e6, acquisition_sampling_rate_L5, false, Signal::Gps_L5I, ugsdr_code_gps_l5i,
+                      10230, svs);
+        if (!dst.empty()) return dst;
+        return Acquire(l5, 1176.45e6, acquisition_sampling_rate_L5, false, Signal::Gps_L5Q, ugsdr_code_gps_l5q,
+                       10230, svs);
     }
 
     // FastSearchEngineBase::ProcessGalileo (fse.hpp:356-396): every
@@ -297,9 +256,83 @@ public:
     }
 
 private:
+    static constexpr double acquisition_sampling_rate_L5 = 20.46e6;   // fse.hpp:152
+
     double doppler_range;
     double doppler_step;
     Options options;
+
+    // AcquireGoldCodesL1 / AcquireL5 of one signal: IF = -(centre - carrier),
+    // the mix by -IF, the rate rule and the fused search over every satellite
+    std::vector<AcquisitionResult<float>> Acquire(const Subband& band, double carrier, double target_rate,
+                                                  bool coherent, Signal signal, int (*code_fn)(int, int8_t*),
+                                                  std::size_t code_len, const std::vector<Sv>& svs) {
+        std::vector<Sv> tried = svs;
+        std::vector<std::vector<int8_t>> codes;
+        for (auto& sv : tried) {
+            sv.signal = signal;
+            std::vector<int8_t> c(code_len);
+            ugsdr_hip::check(code_fn(sv.id, c.data()));
+            codes.push_back(std::move(c));
+        }
+        const double intermediate_frequency = -(band.central_frequency - carrier);
+        return Run(band.iq, band.fmt, band.n_samples, band.sampling_rate, intermediate_frequency, target_rate,
+                   coherent, tried, codes, 1);
+    }
+
+    std::vector<AcquisitionResult<float>> Run(const void* iq, int fmt, std::size_t n_samples, double fs,
+                                              double intermediate_frequency, double target_rate, bool coherent,
+                                              const std::vector<Sv>& svs,
+                                              const std::vector<std::vector<int8_t>>& codes, std::size_t code_ms) {
+        const double new_fs = AdjustSamplingRate(fs, target_rate);
+        const auto spm = (std::size_t)(new_fs / 1e3);
+        const std::size_t n_fft = options.coherent_ms * spm;
+        std::vector<double> doppler;
+        for (double d = -doppler_range; d <= doppler_range; d += doppler_step) doppler.push_back(d);
+        std::vector<uint64_t> prn_keys;
+        for (std::size_t i = 0; i < svs.size(); ++i) prn_keys.push_back(Key(codes[i], options.coherent_ms / code_ms,
+                                                                            n_fft));
+        ugsdr_acq_request r{};
+        r.iq = iq;
+        r.fmt = fmt;
+        r.iq_on_device = 0;
+        r.n_samples = (int64_t)n_samples;
+        r.fs_in = fs;
+        r.fs_acq = new_fs;
+        r.mix_frequency = -intermediate_frequency;  // Translate(signal, fs, -IF) (fse.hpp:289)
+        r.intermediate_frequency = intermediate_frequency;
+        r.n_fft = (int64_t)n_fft;
+        r.n_blocks = (int64_t)options.noncoherent_blocks;
+        r.surface_len = (int64_t)(options.reshape ? spm : n_fft);
+        r.reshape = options.reshape;
+        r.coherent = coherent;
+        r.doppler = doppler.data();
+        r.n_doppler = (int64_t)doppler.size();
+        r.replica_keys = prn_keys.data();
+        r.n_prn = (int64_t)prn_keys.size();
+        r.threshold = options.threshold;
+        r.precision = options.precision;
+        std::vector<ugsdr_acq_result> out(svs.size());
+        std::vector<float> peaks(options.output_peak ? svs.size() * (std::size_t)r.surface_len : 0);
+        ugsdr_hip::check(ugsdr_hip_acquire(context->get(), &r, out.data(), nullptr,
+                                           options.output_peak ? peaks.data() : nullptr));
+        std::vector<AcquisitionResult<float>> dst;
+        for (std::size_t i = 0; i < svs.size(); ++i) {
+            if (!out[i].detected) continue;
+            AcquisitionResult<float> a;
+            a.sv_number = svs[i];
+            a.doppler = out[i].doppler;
+            a.code_offset = out[i].code_offset;
+            a.level = out[i].level;
+            a.sigma = out[i].sigma;
+            a.intermediate_frequency = intermediate_frequency;
+            if (options.output_peak)
+                a.output_peak.assign(peaks.begin() + i * r.surface_len, peaks.begin() + (i + 1) * r.surface_len);
+            dst.push_back(std::move(a));
+        }
+        std::sort(dst.begin(), dst.end(), [](auto& a, auto& b) { return a.sv_number.id < b.sv_number.id; });
+        return dst;
+    }
     ugsdr_hip::HipContext* context;
     std::vector<std::pair<std::vector<int64_t>, uint64_t>> keys;
 

@@ -7,9 +7,52 @@
 #include <cstddef>
 #include <vector>
 
+#include "backends.hpp"
 #include "context.hpp"
 
 namespace ugsdr {
+
+// IppStft / SequentialStft counterpart (src/math/ipp_stft.hpp:14-96,
+// src/math/stft.hpp:31-88): the window is IppStft's Blackman-Harris or
+// SequentialStft's Hann; the device transform has the reference's default
+// geometry (128-sample windows, 96 overlap).
+template <int Window = UGSDR_STFT_BLACKMAN_HARRIS>
+class HipStftT : public ShortTimeFourierTransform<HipStftT<Window>> {
+protected:
+    friend class ShortTimeFourierTransform<HipStftT<Window>>;
+    static void CheckGeometry(std::size_t window_length, std::size_t overlap_length) {
+        if (window_length != 128 || overlap_length != 96)
+            throw std::runtime_error("HipStft runs 128-sample windows with 96 overlap");
+    }
+    static auto Process(const std::vector<std::complex<float>>& src, std::size_t window_length,
+                        std::size_t overlap_length) {
+        CheckGeometry(window_length, overlap_length);
+        const std::size_t ncol = src.size() >= 128 ? (src.size() - 96) / 32 : 0;
+        std::vector<std::complex<float>> flat(ncol * 128);
+        ugsdr_hip::check(ugsdr_hip_stft(hip_detail::ctx(), hip_detail::cf(src.data()), (int64_t)src.size(), Window,
+                                        hip_detail::cf(flat.data())));
+        std::vector<std::vector<std::complex<float>>> dst(ncol);
+        for (std::size_t c = 0; c < ncol; ++c) dst[c].assign(flat.begin() + c * 128, flat.begin() + (c + 1) * 128);
+        return dst;
+    }
+    static auto Process(const std::vector<std::vector<std::complex<float>>>& src, std::size_t window_length,
+                        std::size_t overlap_length) {
+        CheckGeometry(window_length, overlap_length);
+        std::vector<std::complex<float>> flat;
+        for (auto& col : src) {
+            if (col.size() != 128) throw std::runtime_error("Size mismatch");
+            flat.insert(flat.end(), col.begin(), col.end());
+        }
+        std::vector<std::complex<float>> dst(src.empty() ? 0 : 128 + (src.size() - 1) * 32);
+        if (!src.empty())
+            ugsdr_hip::check(ugsdr_hip_istft(hip_detail::ctx(), hip_detail::cf(flat.data()), (int64_t)src.size(),
+                                             hip_detail::cf(dst.data())));
+        return dst;
+    }
+};
+using HipStft = HipStftT<UGSDR_STFT_BLACKMAN_HARRIS>;
+using HipSequentialStft = HipStftT<UGSDR_STFT_HANN>;
+
 
 class HipJammingSuppressionEngine {
 public:

@@ -14,7 +14,7 @@
 
 #include "backends.hpp"
 #include "context.hpp"
-#include "fse.hpp"
+#include "acquisition_result.hpp"
 
 namespace ugsdr {
 
