@@ -869,11 +869,11 @@ size_t corr_scratch_bytes(const AcqPlan& p0, int cus) {
 
 int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots, hipStream_t st,
                  DevBuf& scratch, int cus, int threads) {
-    const AcqPlan& p0 = *plans[0];
-    if (plans.empty() || plans.size() > (size_t)kMaxBatch) {
+    if (plans.empty() || plans.size() > (size_t)kMaxBatch || plans.size() != slots.size()) {
         set_error("a correlation batch holds 1 to " + std::to_string(kMaxBatch) + " requests");
         return UGSDR_ERR_ARG;
     }
+    const AcqPlan& p0 = *plans[0];
     CorrArgs c{};
     ReqDesc* reqs = c.reqs;
     int rows = 0, units = 0, maxK = 1;
