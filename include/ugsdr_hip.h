@@ -187,6 +187,23 @@ typedef struct {
 int ugsdr_hip_acquire(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* req, ugsdr_acq_result* out, float* surfaces,
                       float* peaks);
 
+/* The same search for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Arguments and results are
+ * those of ugsdr_hip_acquire for the FP32 plain form (UGSDR_ERR_UNSUPPORTED
+ * for reduced precision).  Every device buffer of the search is carved out of
+ * one fixed 64 MiB device array per GPU, held until the process ends; the
+ * call runs no hipMalloc / hipFree, graph instantiation or autotuning (it uses
+ * the context's tuned workgroup size if one exists, else the default).  A
+ * search whose buffers exceed the array (e.g. with full surfaces of a large
+ * search) returns UGSDR_ERR_NOMEM.  Calls on one GPU serialize on a lock
+ * inside the library because they share its array, and the array keeps the
+ * last call's contents (no kernel reads what it has not written in the same
+ * call).  Host-side bookkeeping of the request (its plan, a few KB) uses the
+ * ordinary host heap. */
+int ugsdr_hip_acquire_noalloc(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* req, ugsdr_acq_result* out,
+                              float* surfaces, float* peaks);
+
 /* ------------------------------------------------------ anti-jamming (STFT)
  * The interference suppression of the digital front end
  * (src/antijamming/jse.hpp, jamming_detection.hpp, src/math/stft.hpp). */

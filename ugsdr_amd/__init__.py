@@ -198,6 +198,7 @@ SIGNATURES = {
     "ugsdr_hip_replica_drop": ([VP, C.c_uint64], C.c_int),
     "ugsdr_hip_replica_get": ([VP, C.c_uint64, VP, I64], C.c_int),
     "ugsdr_hip_acquire": ([VP, C.POINTER(AcqRequest), C.POINTER(AcqResult), VP, VP], C.c_int),
+    "ugsdr_hip_acquire_noalloc": ([VP, C.POINTER(AcqRequest), C.POINTER(AcqResult), VP, VP], C.c_int),
     "ugsdr_hip_mix": ([VP, VP, VP, I64, D, D, D], C.c_int),
     "ugsdr_hip_mix_inplace": ([VP, VP, I64, D, D, D], C.c_int),
     "ugsdr_hip_table_mix": ([VP, VP, I64, D, D, D, C.c_int], C.c_int),
@@ -475,14 +476,17 @@ class Context:
         req.hypotheses = hypotheses
         return req, (hold, dop, keys)
 
-    def acquire_req(self, req, want_surfaces: bool = False, want_peaks: bool = False, raw: bool = False):
+    def acquire_req(self, req, want_surfaces: bool = False, want_peaks: bool = False, raw: bool = False,
+                    noalloc: bool = False):
         """results as dicts per PRN, or per satellite with hypotheses > 1 (raw:
-        the ugsdr_acq_result ctypes array, no conversion)"""
+        the ugsdr_acq_result ctypes array, no conversion); noalloc runs the
+        search through ugsdr_hip_acquire_noalloc"""
         P, D, L = req.n_prn, req.n_doppler, req.surface_len
         out = (AcqResult * (P // max(1, req.hypotheses)))()
         surf = np.zeros((P, D, L), np.float32) if want_surfaces else None
         peaks = np.zeros((P, L), np.float32) if want_peaks else None
-        _check(lib().ugsdr_hip_acquire(self._p, C.byref(req), out,
+        fn = lib().ugsdr_hip_acquire_noalloc if noalloc else lib().ugsdr_hip_acquire
+        _check(fn(self._p, C.byref(req), out,
                                        _ptr(surf) if surf is not None else None,
                                        _ptr(peaks) if peaks is not None else None))
         res = out if raw else results_to_dicts(out)
@@ -491,6 +495,12 @@ class Context:
     def acquire(self, iq, fmt: int, want_surfaces: bool = False, want_peaks: bool = False, **kw):
         req, hold = self.make_request(iq, fmt, **kw)
         return self.acquire_req(req, want_surfaces, want_peaks)
+
+    def acquire_noalloc(self, iq, fmt: int, want_surfaces: bool = False, want_peaks: bool = False, **kw):
+        """the same search through the entry point that allocates no device
+        memory (its fixed workspace; FP32 only)"""
+        req, hold = self.make_request(iq, fmt, **kw)
+        return self.acquire_req(req, want_surfaces, want_peaks, noalloc=True)
 
     # ---- tracking loops
     def track(self, iq, fmt: int, *, spm: int, n_blocks: int, fs: float, states, codes, spacing: float = 0.25,

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
-HIP_SOURCES = ["acq.hip", "corr1023.hip", "ops.hip", "bank.hip", "reduced.hip", "server.hip", "dfe.hip", "tracker.hip", "jse.hip", "lib.cpp"]
+HIP_SOURCES = ["acq.hip", "corr1023.hip", "ops.hip", "bank.hip", "reduced.hip", "server.hip", "dfe.hip", "tracker.hip", "jse.hip", "noalloc.hip", "lib.cpp"]
 
 
 def _run(cmd):

@@ -15,6 +15,9 @@ int dft_device(ugsdr_hip_ctx* ctx, float2* d, float2* tmp, int64_t n, int invers
 // conj(DFT(replica)) of length n_fft into out (device), from chips or samples
 int replica_spectrum(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chips, int64_t repeats, const float* samples,
                      int64_t n_fft, float2* out);
+// resolves the no-alloc workspace's device address on `device` (current),
+// once per device (noalloc.hip)
+int noalloc_scratch_init(int device);
 
 }  // namespace ugsdr_hip
 

@@ -67,6 +67,7 @@ int ugsdr_hip_ctx_create(int device, ugsdr_hip_ctx** out) {
         return UGSDR_ERR_ARG;
     }
     UGSDR_HIP_TRY(hipSetDevice(device));
+    if (int rc = noalloc_scratch_init(device)) return rc;
     auto* ctx = new ugsdr_hip_ctx();
     ctx->device = device;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
