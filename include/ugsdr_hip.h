@@ -124,6 +124,9 @@ int ugsdr_hip_replica_put(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chi
  * permutations of src/acquisition/fse.hpp:374-385) */
 int ugsdr_hip_replica_put_samples(ugsdr_hip_ctx* ctx, const float* samples, int64_t n_fft, int precision,
                                   uint64_t* key);
+/* frees a cached spectrum; UGSDR_ERR_STATE while queued or running server
+ * requests read it (ugsdr_hip_server_submit pins the request's keys until its
+ * result is ready) */
 int ugsdr_hip_replica_drop(ugsdr_hip_ctx* ctx, uint64_t key);
 /* copies the stored spectrum (n_fft complex float) back, for inspection */
 int ugsdr_hip_replica_get(ugsdr_hip_ctx* ctx, uint64_t key, float* spectrum_c32, int64_t n_fft);
@@ -444,7 +447,8 @@ typedef struct {
 int ugsdr_hip_server_create(ugsdr_hip_ctx* ctx, const ugsdr_server_config* cfg, ugsdr_hip_server** out);
 /* Validates and copies the request (Doppler list, keys and host IQ), so the
  * caller may reuse its buffers at once; a device IQ pointer must stay valid
- * until the ticket completes.  Replica keys must not be dropped meanwhile. */
+ * until the ticket completes.  The request's replica keys stay pinned
+ * (ugsdr_hip_replica_drop refuses them) until its result is ready. */
 int ugsdr_hip_server_submit(ugsdr_hip_server* srv, const ugsdr_acq_request* req, uint64_t* ticket);
 /* UGSDR_OK: finished, out[n_prn] filled, latency (submit to completion) in ms,
  * ticket released; UGSDR_PENDING: still running after wait_ms; < 0: the

@@ -391,3 +391,36 @@ def test_graph_replay_matches_eager(ctx):
     assert res2 == ref[1][0]
     for k in keys:
         ctx.replica_drop(k)
+
+
+def test_concurrent_callers_on_one_context(ctx):
+    """the reference calls its operators from par_unseq worker threads
+    (fse.hpp:294): 8 host threads share one context, each running 4 searches
+    of its own (different PRN subsets, Doppler spans and inputs); every result
+    equals the same search run serially"""
+    from concurrent.futures import ThreadPoolExecutor
+    w = W.l1ca_1ms()
+    wide = W.l1ca_wide()
+    say("case 8 threads x 4 searches on one context (L1CA_1MS and L1CA_WIDE shapes)")
+    keys1 = put_replicas(ctx, w)
+    keysw = put_replicas(ctx, wide)
+    jobs = []
+    for t in range(8):
+        for i in range(4):
+            k = (t * 4 + i) % 7
+            if i % 2:
+                jobs.append((wide, keysw[k:k + 9], U.doppler_bins(5000 - 250 * k, 250)))
+            else:
+                jobs.append((w, keys1[k:k + 12], U.doppler_bins(5000 - 500 * k, 500)))
+
+    def run(job):
+        wl, keys, dop = job
+        return gpu_run(ctx, wl, keys, dop=dop)[0]
+
+    serial = [run(j) for j in jobs]
+    with ThreadPoolExecutor(8) as ex:
+        parallel = list(ex.map(run, jobs))
+    for a, b in zip(parallel, serial):
+        assert a == b
+    for k in keys1 + keysw:
+        ctx.replica_drop(k)

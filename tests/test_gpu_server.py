@@ -74,3 +74,22 @@ def test_server_budget_and_errors(ctx):
             assert_same(o, outs[0])
         with pytest.raises(U.UgsdrError, match="STATE"):
             srv.poll(ts[0], 0)
+
+
+def test_server_pins_replicas(ctx):
+    """a replica read by queued or running requests cannot be dropped; once
+    their results are collected it can"""
+    print("case server replica pin", flush=True)
+    mix = W.ServeMix(ctx, seed=0xC7, pool=1)
+    r = mix.draw()
+    req, hold = mix.request(r)
+    key = int(req.replica_keys[0])
+    with U.Server(ctx, max_inflight=1, max_batch=1) as srv:
+        ts = [srv.submit(req) for _ in range(8)]
+        with pytest.raises(U.UgsdrError, match="STATE"):
+            ctx.replica_drop(key)
+        for t in ts:
+            assert srv.poll(t, -1) is not None
+    ctx.replica_drop(key)
+    with pytest.raises(U.UgsdrError, match="STATE"):
+        ctx.replica_drop(key)

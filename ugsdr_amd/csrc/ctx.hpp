@@ -68,6 +68,7 @@ struct Replica {
     int64_t n_fft = 0;
     int precision = UGSDR_PREC_FP32;
     float scale = 1.0f;    // reduced precision: stored = spectrum * scale
+    int64_t users = 0;     // queued or running server requests that read it
 };
 
 // device buffers one acquisition request holds while in flight

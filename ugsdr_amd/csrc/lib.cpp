@@ -311,6 +311,10 @@ int ugsdr_hip_replica_drop(ugsdr_hip_ctx* ctx, uint64_t key) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->replicas.find(key);
     if (it == ctx->replicas.end()) { set_error("unknown replica key"); return UGSDR_ERR_STATE; }
+    if (it->second.users > 0) {
+        set_error("replica in use by queued or running server requests");
+        return UGSDR_ERR_STATE;
+    }
     UGSDR_HIP_TRY(hipSetDevice(ctx->device));
     UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
     UGSDR_HIP_TRY(hipFree(it->second.spec));

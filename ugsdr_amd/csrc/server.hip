@@ -112,6 +112,22 @@ static hipEvent_t take_event(Server& s) {
     return e;
 }
 
+// A queued or running request pins the replica spectra it reads, so that
+// ugsdr_hip_replica_drop refuses to free them under it (caller holds ctx->mu).
+static void pin_replicas(ugsdr_hip_ctx* ctx, const std::vector<uint64_t>& keys, int64_t delta) {
+    for (uint64_t k : keys) {
+        auto it = ctx->replicas.find(k);
+        if (it != ctx->replicas.end()) it->second.users += delta;
+    }
+}
+
+// unpins the replicas of finished jobs (takes ctx->mu; call without s.mu)
+static void unpin_jobs(ugsdr_hip_ctx* ctx, const std::vector<std::shared_ptr<Job>>& done) {
+    if (done.empty()) return;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (auto& j : done) pin_replicas(ctx, j->keys, -1);
+}
+
 static void fail_job(Job& j, int rc) {
     j.state = 3;
     j.error = rc;
@@ -141,9 +157,12 @@ static int launch_round(Server& s, Round& rd) {
         plans.push_back(&j.plan);
         devs.push_back(&sl.dev);
     }
-    // the per-stream batch buffers may only grow once the stream's earlier rounds are done
+    // the per-stream batch buffers may only grow once the stream's earlier
+    // rounds are done; sized for the batch's largest request (the bucket key
+    // leaves the block count K free, and the reduced form's scratch grows with K)
     const AcqPlan& p0 = *plans[0];
-    const size_t need_scratch = corr_scratch_bytes(p0, s.cus);
+    size_t need_scratch = 0;
+    for (const AcqPlan* p : plans) need_scratch = std::max(need_scratch, corr_scratch_bytes(*p, s.cus));
     if (s.scratch[rd.stream].bytes < need_scratch) {
         UGSDR_HIP_TRY(hipStreamSynchronize(st));
         UGSDR_HIP_TRY(s.scratch[rd.stream].reserve(need_scratch));
@@ -206,6 +225,7 @@ static void scheduler_loop(Server* sp) {
                 rounds.push_back(std::move(rd));
             }
         lk.unlock();
+        std::vector<std::shared_ptr<Job>> failed;
         for (auto& rd : rounds) {
             int rc = rd.done ? launch_round(s, rd) : UGSDR_ERR_HIP;
             std::lock_guard<std::mutex> g(s.mu);
@@ -217,6 +237,7 @@ static void scheduler_loop(Server* sp) {
                     s.free_slots.push_back(j->slot);
                     s.inflight_bytes -= j->bytes;
                     ++s.st.failed;
+                    failed.push_back(j);
                 }
                 if (rd.done) s.event_pool.push_back(rd.done);
                 s.cv_done.notify_all();
@@ -227,6 +248,7 @@ static void scheduler_loop(Server* sp) {
             s.st.max_batch_seen = std::max<int64_t>(s.st.max_batch_seen, (int64_t)rd.jobs.size());
             s.inflight.push_back(std::move(rd));
         }
+        unpin_jobs(s.ctx, failed);
         lk.lock();
     }
 }
@@ -236,6 +258,7 @@ static void completer_loop(Server* sp) {
     (void)hipSetDevice(s.ctx->device);
     while (true) {
         bool any = false;
+        std::vector<std::shared_ptr<Job>> finished;
         {
             std::unique_lock<std::mutex> lk(s.mu);
             if (s.sched_done && s.inflight.empty()) break;
@@ -262,6 +285,7 @@ static void completer_loop(Server* sp) {
                     }
                     s.free_slots.push_back(j.slot);
                     s.inflight_bytes -= j.bytes;
+                    finished.push_back(jp);
                 }
                 s.event_pool.push_back(it->done);
                 it = s.inflight.erase(it);
@@ -272,6 +296,7 @@ static void completer_loop(Server* sp) {
                 s.cv_sched.notify_all();
             }
         }
+        unpin_jobs(s.ctx, finished);
         if (!any) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
@@ -355,14 +380,29 @@ int ugsdr_hip_server_submit(ugsdr_hip_server* h, const ugsdr_acq_request* req, u
     }
     j->bytes = j->plan.footprint(false);
     {
+        // the keys were resolved above; a key dropped since then fails here
+        std::lock_guard<std::mutex> lk(s.ctx->mu);
+        for (uint64_t k : j->keys)
+            if (!s.ctx->replicas.count(k)) {
+                set_error("unknown replica key");
+                return UGSDR_ERR_STATE;
+            }
+        pin_replicas(s.ctx, j->keys, +1);
+    }
+    {
         std::lock_guard<std::mutex> lk(s.mu);
+        int rc = UGSDR_OK;
         if (s.stop) {
             set_error("server is shutting down");
-            return UGSDR_ERR_STATE;
-        }
-        if (j->bytes > s.cfg.max_device_bytes) {
+            rc = UGSDR_ERR_STATE;
+        } else if (j->bytes > s.cfg.max_device_bytes) {
             set_error("request exceeds the server's device-memory budget");
-            return UGSDR_ERR_NOMEM;
+            rc = UGSDR_ERR_NOMEM;
+        }
+        if (rc) {
+            std::lock_guard<std::mutex> lc(s.ctx->mu);
+            pin_replicas(s.ctx, j->keys, -1);
+            return rc;
         }
         j->ticket = s.next_ticket++;
         s.jobs[j->ticket] = j;
