@@ -40,6 +40,52 @@ def test_bank_nco_vs_oracle(ctx, oracle):
     assert np.max(np.abs(out2[0] - out[2])) <= 1e-5 * scale[2]
 
 
+def _bank_case(ctx, oracle, w, fmt, taps, channels=None, codes=None):
+    channels = w.channels if channels is None else channels
+    codes = w.codes if codes is None else codes
+    x = iq_to_complex(w.iq)
+    if fmt == U.FMT_CI8:
+        iq = np.clip(np.round(w.iq / 16), -127, 127).astype(np.int8)
+        x = iq_to_complex(iq)
+    elif fmt == U.FMT_CF32:
+        iq = x.view(np.float32)
+    else:
+        iq = w.iq
+    out, _ = ctx.correlate_bank(iq, fmt, spm=w.spm, n_blocks=w.n_blocks, fs=w.fs, channels=channels, codes=codes,
+                                taps_chips=taps, mode=U.BANK_NCO)
+    ref = oracle.bank_nco(x, w.spm, w.n_blocks, w.fs, channels, codes, taps)
+    scale = np.abs(x).reshape(w.n_blocks, w.spm).sum(axis=1)
+    err = np.abs(out - ref).max(axis=(1, 2))
+    assert np.all(err <= 1e-5 * scale), (err, scale)
+
+
+@pytest.mark.parametrize("fmt,taps", [
+    (U.FMT_CI16, [0.25, 0.0, -0.25]),              # quarter grid, descending, three taps
+    (U.FMT_CI8, [-0.5, -0.25, 0.0, 0.25, 0.5]),     # quarter grid, int8 samples
+    (U.FMT_CF32, [-0.25, 0.0, 0.25]),               # quarter grid, float samples
+    (U.FMT_CI16, [-0.3, 0.0, 0.7, 1.1]),            # off the quarter grid: the multi-channel kernel
+    (U.FMT_CI16, [-1.0, -0.5, 0.0, 0.5, 1.0, 0.125, -0.125, 2.0]),   # eight taps
+])
+def test_bank_nco_tap_sets_and_formats(ctx, oracle, fmt, taps):
+    """every NCO kernel shape against the oracle: taps a quarter chip apart in
+    either direction (the quarter-table kernel) and arbitrary taps (the
+    multi-channel kernel), 7 channels (a partial channel group), 3 blocks"""
+    print(f"case bank NCO fmt={fmt} taps={taps}", flush=True)
+    w = W.bank_l5(n_ch=7, n_blocks=3, seed=0xC41)
+    _bank_case(ctx, oracle, w, fmt, np.array(taps))
+
+
+def test_bank_nco_code_shorter_than_block(ctx, oracle):
+    """a code that repeats within a block (1023 chips at 2.046 Mchip/s over
+    1 ms blocks) runs the one-channel kernel"""
+    print("case bank NCO short code", flush=True)
+    w = W.bank_l5(n_ch=3, n_blocks=2, seed=0xC42)
+    rng = np.random.default_rng(5)
+    codes = np.where(rng.random((3, 1023)) < 0.5, -1, 1).astype(np.int8)
+    chans = [(c[0], c[1], 2.046e6, c[3] % 1023) for c in w.channels]
+    _bank_case(ctx, oracle, w, U.FMT_CI16, np.array([-0.5, 0.0, 0.5]), channels=chans, codes=codes)
+
+
 def test_bank_ref_split_vs_oracle(ctx, oracle):
     """Tracker::GetEpl / CorrelateSplit semantics, E/P/L at +-0.25 chip"""
     print("case bank ref_split 4 ch x 3 taps x 3 blocks", flush=True)

@@ -162,6 +162,274 @@ __global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
     bank_reduce_store<kMaxTaps>(acc, a.n_taps, a.out + ((int64_t)b * a.n_ch + c) * a.n_taps, red);
 }
 
+// NCO mode, several channels per workgroup and several blocks per workgroup.
+//
+// Each thread takes samples n = tid + s T of a block (T = 256, s the slot),
+// loads each sample once for the workgroup's CH channels and, per channel:
+//   * mixes it with an FP32 carrier phasor z = e^(-j 2 pi phase(n)), which
+//     advances by one complex multiply with e^(-j 2 pi T df) per slot and is
+//     re-seeded every kReseed slots from the FP64 phase (sincospif of the
+//     reduced phase), so the recurrence error stays below
+//     kReseed (2 + sqrt 2) 2^-24 (about 3e-6 at kReseed = 32);
+//   * keeps the code phase as a 20.44 fixed-point count of chips, started from
+//     the FP64 code phase of the thread's first sample and advanced by the
+//     integer increment round(T dr 2^44) per slot (drift below 2^-45 chip per
+//     slot, the FP64 arithmetic's own order), so a tap's chip index is the
+//     integer part, the tap's integer offset and the carry of the 32-bit
+//     fraction plus the tap's fraction; one conditional subtraction of L
+//     (min(b, b - L) on unsigned values) keeps the index within the code;
+//   * accumulates each tap in FP32.
+// The channels' codes sit in LDS as int8 with `margin` chips of wrap padding
+// on each side; the FP64 phases of each block start are computed per thread.
+constexpr int kBankCH = 4;    // channels per workgroup
+constexpr int kBankBB = 4;    // blocks per workgroup
+constexpr int kReseed = 32;   // slots between carrier re-seeds
+constexpr double kFix = 17592186044416.0;   // 2^44
+
+template <int NT>
+__global__ __launch_bounds__(kBankT) void bank_nco_multi(BankArgs a) {
+    constexpr int CH = kBankCH;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int Li = (int)a.code_len, M = a.tap_margin, LT = Li + 2 * M;
+    const int c0 = blockIdx.x * CH;
+    const int nch = min(CH, a.n_ch - c0);
+    signed char* code = reinterpret_cast<signed char*>(smem);
+    float2* red = reinterpret_cast<float2*>(smem + ((CH * LT + 15) / 16) * 16);
+    for (int i = threadIdx.x; i < CH * LT; i += kBankT) {
+        const int c = i / LT, k = i - c * LT;
+        int j = k - M;
+        j += j < 0 ? Li : 0;
+        j -= j >= Li ? Li : 0;
+        code[i] = c < nch ? a.codes[(int64_t)(c0 + c) * a.code_len + j] : (signed char)1;
+    }
+    // taps: integer part (plus the margin) and 32-bit fraction
+    int ti[NT];
+    uint32_t tf[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const double tp = t < a.n_taps ? a.taps[t] : 0.0, fl = floor(tp);
+        ti[t] = (int)fl + M;
+        tf[t] = (uint32_t)(uint64_t)llrint((tp - fl) * 4294967296.0);
+    }
+    double f[CH], th0[CH], rate[CH], tau0[CH];
+    float2 w[CH];
+    uint64_t inc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        const int cc = c0 + min(c, nch - 1);
+        f[c] = a.ch[4 * cc];
+        th0[c] = a.ch[4 * cc + 1];
+        rate[c] = a.ch[4 * cc + 2];
+        tau0[c] = a.ch[4 * cc + 3];
+        double st = (double)kBankT * (f[c] / a.fs);
+        st -= rint(st);
+        double s, co;
+        sincospi(2.0 * st, &s, &co);
+        w[c] = make_float2((float)co, (float)-s);
+        inc[c] = (uint64_t)llrint((double)kBankT * (rate[c] / a.fs) * kFix);
+    }
+    const double L = (double)a.code_len;
+    const uint32_t Lu = (uint32_t)Li;
+    __syncthreads();
+    const int b_end = min(a.n_blocks, (int)(blockIdx.y + 1) * kBankBB);
+    for (int b = blockIdx.y * kBankBB; b < b_end; ++b) {
+        const double n0 = (double)b * (double)a.spm;
+        double theta[CH];
+        float2 z[CH];
+        uint64_t u[CH];
+        float2 acc[CH][NT];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            double th = fma(f[c], n0 / a.fs, th0[c]);
+            theta[c] = th - floor(th);
+            double tau = fmod(fma(rate[c], n0 / a.fs, tau0[c]), L);
+            tau += tau < 0 ? L : 0.0;
+            u[c] = (uint64_t)((tau + (double)threadIdx.x * (rate[c] / a.fs)) * kFix);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[c][t] = make_float2(0.f, 0.f);
+        }
+        const void* blk = a.fmt == UGSDR_FMT_CI16 ? (const void*)(static_cast<const short2*>(a.iq) + b * a.spm)
+                          : a.fmt == UGSDR_FMT_CI8 ? (const void*)(static_cast<const char2*>(a.iq) + b * a.spm)
+                                                   : (const void*)(static_cast<const float2*>(a.iq) + b * a.spm);
+        int s = 0;
+        for (int64_t n = threadIdx.x; n < a.spm; n += kBankT, ++s) {
+            const float2 x = bank_load(blk, a.fmt, n);
+            if (s % kReseed == 0) {
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    double ph = fma((double)n, f[c] / a.fs, theta[c]);
+                    ph -= rint(ph);
+                    float sn, cs;
+                    sincospif(2.0f * (float)ph, &sn, &cs);
+                    z[c] = make_float2(cs, -sn);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const float2 m = make_float2(fmaf(x.x, z[c].x, -x.y * z[c].y), fmaf(x.x, z[c].y, x.y * z[c].x));
+                z[c] = make_float2(fmaf(z[c].x, w[c].x, -z[c].y * w[c].y), fmaf(z[c].x, w[c].y, z[c].y * w[c].x));
+                const uint32_t bi = (uint32_t)(u[c] >> 44);
+                const uint32_t bw = min(bi, bi - Lu);
+                const uint32_t fr = (uint32_t)(u[c] >> 12);
+                const signed char* cc = code + c * LT + bw;
+                u[c] += inc[c];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const uint32_t sum = fr + tf[t];
+                    const float cv = (float)cc[ti[t] + (sum < fr ? 1 : 0)];
+                    acc[c][t].x = fmaf(m.x, cv, acc[c][t].x);
+                    acc[c][t].y = fmaf(m.y, cv, acc[c][t].y);
+                }
+            }
+        }
+        // workgroup sums of every (channel, tap)
+        const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const float sx = wave_sum(acc[c][t].x), sy = wave_sum(acc[c][t].y);
+                if (l == 0) red[(wv * CH + c) * NT + t] = make_float2(sx, sy);
+            }
+        __syncthreads();
+        if (threadIdx.x < CH * NT) {
+            const int c = threadIdx.x / NT, t = threadIdx.x - c * NT;
+            if (c < nch && t < a.n_taps) {
+                float2 sum = make_float2(0.f, 0.f);
+                for (int i = 0; i < kBankT / 64; ++i) sum = cadd(sum, red[(i * CH + c) * NT + t]);
+                a.out[((int64_t)b * a.n_ch + c0 + c) * a.n_taps + t] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// NCO mode for taps a quarter chip apart (the E / P / L and VE / VL taps,
+// listed in either direction): with the code expanded to one entry per
+// quarter chip, floor(u + k/4) = floor((floor(4u) + k) / 4) for integer k,
+// so tap t reads the entry at the sample's quarter index plus q0 + t D, a
+// compile-time offset from one per-sample address: no per-tap index
+// arithmetic.  One channel per workgroup (41 KB of table at 10230 chips),
+// kBankBB blocks per workgroup; carrier and code NCOs as in bank_nco_multi,
+// the carrier re-seeded at the start of every group of kReseed slots.
+template <class S>
+__device__ __forceinline__ float2 load_iq(const S* p, int64_t i) {
+    const S v = p[i];
+    return make_float2((float)v.x, (float)v.y);
+}
+
+template <int NT, int D, class S>
+__global__ __launch_bounds__(kBankT) void bank_nco_quarter(BankArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int Li = (int)a.code_len, M = a.tap_margin;   // M: quarter entries of padding on each side
+    const int LQ = 4 * Li, LT = LQ + 2 * M;
+    const int c = blockIdx.x;
+    signed char* cq = reinterpret_cast<signed char*>(smem);
+    float2* red = reinterpret_cast<float2*>(smem + (LT + 15) / 16 * 16);
+    for (int i = threadIdx.x; i < LT; i += kBankT) {
+        int q = i - M;
+        q += q < 0 ? LQ : 0;
+        q -= q >= LQ ? LQ : 0;
+        cq[i] = a.codes[(int64_t)c * a.code_len + (q >> 2)];
+    }
+    // tap 0's offset in quarter chips, plus the margin
+    const int q0 = M + (int)llrint(4.0 * a.taps[0]);
+    const double f = a.ch[4 * c], th0 = a.ch[4 * c + 1], rate = a.ch[4 * c + 2], tau0 = a.ch[4 * c + 3];
+    const double df = f / a.fs, dr = rate / a.fs, L = (double)a.code_len;
+    float2 w;
+    {
+        double st = (double)kBankT * df;
+        st -= rint(st);
+        double sn, co;
+        sincospi(2.0 * st, &sn, &co);
+        w = make_float2((float)co, (float)-sn);
+    }
+    const uint64_t inc = (uint64_t)llrint((double)kBankT * dr * kFix);
+    const uint32_t LQu = (uint32_t)LQ;
+    const int tid = threadIdx.x;
+    const int n_slots = a.spm > tid ? (int)((a.spm - tid + kBankT - 1) / kBankT) : 0;
+    __syncthreads();
+    const int b_end = min(a.n_blocks, (int)(blockIdx.y + 1) * kBankBB);
+    for (int b = blockIdx.y * kBankBB; b < b_end; ++b) {
+        const double n0 = (double)b * (double)a.spm;
+        double theta = fma(f, n0 / a.fs, th0);
+        theta -= floor(theta);
+        double tau = fmod(fma(rate, n0 / a.fs, tau0), L);
+        tau += tau < 0 ? L : 0.0;
+        uint64_t u = (uint64_t)((tau + (double)tid * dr) * kFix);
+        float2 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = make_float2(0.f, 0.f);
+        const S* blk = static_cast<const S*>(a.iq) + (int64_t)b * a.spm;
+        for (int s0 = 0; s0 < n_slots; s0 += kReseed) {
+            const int64_t nb = tid + (int64_t)s0 * kBankT;
+            float2 z;
+            {
+                double ph = fma((double)nb, df, theta);
+                ph -= rint(ph);
+                float sn, cs;
+                sincospif(2.0f * (float)ph, &sn, &cs);
+                z = make_float2(cs, -sn);
+            }
+            const int s1 = min(n_slots, s0 + kReseed);
+            const S* xp = blk + nb;
+            auto step = [&](float2 x) {
+                const float2 m = make_float2(fmaf(x.x, z.x, -x.y * z.y), fmaf(x.x, z.y, x.y * z.x));
+                z = make_float2(fmaf(z.x, w.x, -z.y * w.y), fmaf(z.x, w.y, z.y * w.x));
+                const uint32_t q = (uint32_t)(u >> 42);   // quarter-chip index
+                const uint32_t qw = min(q, q - LQu);
+                u += inc;
+                const signed char* p = cq + q0 + qw;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const float cv = (float)p[t * D];
+                    acc[t].x = fmaf(m.x, cv, acc[t].x);
+                    acc[t].y = fmaf(m.y, cv, acc[t].y);
+                }
+            };
+            int s = s0;
+            // four samples' loads in flight before their arithmetic
+            for (; s + 4 <= s1; s += 4, xp += 4 * kBankT) {
+                float2 x[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) x[v] = load_iq(xp, v * kBankT);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) step(x[v]);
+            }
+            for (; s < s1; ++s, xp += kBankT) step(load_iq(xp, 0));
+        }
+        const int wv = tid >> 6, l = tid & 63;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const float sx = wave_sum(acc[t].x), sy = wave_sum(acc[t].y);
+            if (l == 0) red[wv * NT + t] = make_float2(sx, sy);
+        }
+        __syncthreads();
+        if (tid < NT) {
+            float2 sum = make_float2(0.f, 0.f);
+            for (int i = 0; i < kBankT / 64; ++i) sum = cadd(sum, red[i * NT + tid]);
+            a.out[((int64_t)b * a.n_ch + c) * NT + tid] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+template <int NT, int D>
+static void launch_quarter(const BankArgs& a, dim3 g, size_t lds, hipStream_t st) {
+    if (a.fmt == UGSDR_FMT_CI16) hipLaunchKernelGGL((bank_nco_quarter<NT, D, short2>), g, dim3(kBankT), lds, st, a);
+    else if (a.fmt == UGSDR_FMT_CI8) hipLaunchKernelGGL((bank_nco_quarter<NT, D, char2>), g, dim3(kBankT), lds, st, a);
+    else hipLaunchKernelGGL((bank_nco_quarter<NT, D, float2>), g, dim3(kBankT), lds, st, a);
+}
+
+template <int NT, int D>
+static hipError_t quarter_lds(size_t lds) {
+    for (const void* k : {reinterpret_cast<const void*>(bank_nco_quarter<NT, D, short2>),
+                          reinterpret_cast<const void*>(bank_nco_quarter<NT, D, char2>),
+                          reinterpret_cast<const void*>(bank_nco_quarter<NT, D, float2>)})
+        if (hipError_t e = set_lds_limit(k, lds)) return e;
+    return hipSuccess;
+}
+
 __global__ __launch_bounds__(kBankT) void bank_split_kernel(BankArgs a) {
     const int c = blockIdx.x, b = blockIdx.y;  // the channels of one block run together (IQ block reused from L2)
     __shared__ float2 red[(kBankT / 64) * 2 * kMaxTaps];
@@ -287,15 +555,75 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
         double reach = 0;
         for (int t = 0; t < r->n_taps; ++t) reach = std::max(reach, std::fabs(std::floor(r->taps_chips[t])) + 1);
         UGSDR_CHECK_ARG(reach < (double)r->code_len, "taps reach beyond one code period");
-        a.tap_margin = (int)reach;
-        const size_t lds = ((r->code_len + 2 * a.tap_margin + 15) / 16) * 16 + (kBankT / 64) * kMaxTaps * sizeof(float2);
-        if (lds > 160 * 1024) {
-            set_error("code too long for the LDS code table");
-            return UGSDR_ERR_UNSUPPORTED;
+        // The multi-channel kernel keeps a block's code phase within one
+        // period plus the block's own span, so it needs positive code rates,
+        // at most one code period (plus 2 chips) per block and a 20-bit chip
+        // count; other channel sets run the one-channel kernel.
+        double span_max = 0;
+        bool multi = r->code_len < (1 << 19);
+        for (int c = 0; c < r->n_channels; ++c) {
+            multi = multi && ch[c].code_rate > 0;
+            span_max = std::max(span_max, (double)r->spm * ch[c].code_rate / r->fs);
         }
-        UGSDR_HIP_TRY(set_lds_limit(reinterpret_cast<const void*>(bank_nco_kernel), lds));
+        multi = multi && span_max <= (double)r->code_len + 2.0;
+        const int margin_multi = (int)reach + 3;
+        const int nt = r->n_taps <= 3 ? 3 : r->n_taps <= 5 ? 5 : kMaxTaps;
+        const size_t lds_multi = ((size_t)kBankCH * (r->code_len + 2 * margin_multi) + 15) / 16 * 16 +
+                                 (size_t)(kBankT / 64) * kBankCH * nt * sizeof(float2);
+        multi = multi && lds_multi <= 64 * 1024;
+        // taps a quarter chip apart (4 o_t = 4 o_0 + t D, D = +-1; three or
+        // five of them) with codes up to 12000 chips take the quarter-table
+        // kernel (4 L + 2 margin bytes of LDS)
+        int dir = 0;
+        if (r->n_taps == 3 || r->n_taps == 5) {
+            const double d = 4.0 * (r->taps_chips[1] - r->taps_chips[0]);
+            dir = std::fabs(d - 1.0) < 1e-9 ? 1 : std::fabs(d + 1.0) < 1e-9 ? -1 : 0;
+            const double q0 = 4.0 * r->taps_chips[0];
+            if (std::fabs(q0 - std::nearbyint(q0)) > 1e-9) dir = 0;
+            for (int t = 0; t < r->n_taps && dir; ++t)
+                if (std::fabs(4.0 * r->taps_chips[t] - (q0 + t * dir)) > 1e-9) dir = 0;
+        }
+        const bool quarter = multi && r->code_len <= 12000 && dir != 0;
+        double tap_abs = 0;
+        for (int t = 0; t < r->n_taps; ++t) tap_abs = std::max(tap_abs, std::fabs(r->taps_chips[t]));
+        const int margin_q = 4 * (int)std::ceil(tap_abs) + 8;
+        const size_t lds_q = ((size_t)4 * r->code_len + 2 * margin_q + 15) / 16 * 16 +
+                             (size_t)(kBankT / 64) * r->n_taps * sizeof(float2);
         ctx->mark(0);
-        hipLaunchKernelGGL(bank_nco_kernel, grid, dim3(kBankT), lds, ctx->stream, a);
+        if (quarter) {
+            a.tap_margin = margin_q;
+            const dim3 g((unsigned)r->n_channels, (unsigned)((r->n_blocks + kBankBB - 1) / kBankBB));
+            if (r->n_taps == 3) {
+                UGSDR_HIP_TRY((dir > 0 ? quarter_lds<3, 1>(lds_q) : quarter_lds<3, -1>(lds_q)));
+                if (dir > 0) launch_quarter<3, 1>(a, g, lds_q, ctx->stream);
+                else launch_quarter<3, -1>(a, g, lds_q, ctx->stream);
+            } else {
+                UGSDR_HIP_TRY((dir > 0 ? quarter_lds<5, 1>(lds_q) : quarter_lds<5, -1>(lds_q)));
+                if (dir > 0) launch_quarter<5, 1>(a, g, lds_q, ctx->stream);
+                else launch_quarter<5, -1>(a, g, lds_q, ctx->stream);
+            }
+        } else if (multi) {
+            a.tap_margin = margin_multi;
+            const dim3 g((unsigned)((r->n_channels + kBankCH - 1) / kBankCH),
+                         (unsigned)((r->n_blocks + kBankBB - 1) / kBankBB));
+            const void* k = nt == 3 ? reinterpret_cast<const void*>(bank_nco_multi<3>)
+                            : nt == 5 ? reinterpret_cast<const void*>(bank_nco_multi<5>)
+                                      : reinterpret_cast<const void*>(bank_nco_multi<kMaxTaps>);
+            UGSDR_HIP_TRY(set_lds_limit(k, lds_multi));
+            if (nt == 3) hipLaunchKernelGGL(bank_nco_multi<3>, g, dim3(kBankT), lds_multi, ctx->stream, a);
+            else if (nt == 5) hipLaunchKernelGGL(bank_nco_multi<5>, g, dim3(kBankT), lds_multi, ctx->stream, a);
+            else hipLaunchKernelGGL(bank_nco_multi<kMaxTaps>, g, dim3(kBankT), lds_multi, ctx->stream, a);
+        } else {
+            a.tap_margin = (int)reach;
+            const size_t lds =
+                ((r->code_len + 2 * a.tap_margin + 15) / 16) * 16 + (kBankT / 64) * kMaxTaps * sizeof(float2);
+            if (lds > 160 * 1024) {
+                set_error("code too long for the LDS code table");
+                return UGSDR_ERR_UNSUPPORTED;
+            }
+            UGSDR_HIP_TRY(set_lds_limit(reinterpret_cast<const void*>(bank_nco_kernel), lds));
+            hipLaunchKernelGGL(bank_nco_kernel, grid, dim3(kBankT), lds, ctx->stream, a);
+        }
         UGSDR_HIP_TRY(hipGetLastError());
         ctx->mark(1);
     } else {
