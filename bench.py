@@ -18,7 +18,8 @@ L1 C/A, BOC 4 ms in FP32 and in the reduced FP16/BF16 form, Galileo E1 code
 hypotheses), the correlator bank, closed-loop tracking, searches straight from
 a packed BbpDdc recording, the DFE channel step and the anti-jamming front
 end; `serving` and `serving_boc_fp16` are the request server under Poisson
-arrivals.  `--leg bank|tracking|tracking_many|jse|galileo|galileo_c3` runs one of
+arrivals (L1 / L5 requests of T ms coherent), `serving_noncoherent_1ms` the
+same stream with T blocks of 1 ms summed non-coherently.  `--leg bank|tracking|tracking_many|jse|galileo|galileo_c3` runs one of
 those alone.
 """
 from __future__ import annotations
@@ -510,14 +511,17 @@ def recorded_workloads(ctx):
     return out
 
 
-def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0):
-    """Poisson arrivals of the mixed request stream into the continuous-batching server"""
+def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0, coherent: bool = True):
+    """Poisson arrivals of the mixed request stream into the continuous-batching
+    server.  coherent: L1 / L5 requests of T ms coherent (N = 4092 T or
+    20460 T, Doppler step 500 / T Hz); otherwise T blocks of 1 ms summed
+    non-coherently (step 500 Hz)."""
     import numpy as np
 
     import ugsdr_amd as U
     from ugsdr_amd import workloads as W
 
-    mix = W.ServeMix(ctx, seed=0xC5, boc_precision=boc_precision)
+    mix = W.ServeMix(ctx, seed=0xC5, boc_precision=boc_precision, coherent=coherent)
     warm = [mix.request(mix.draw()) for _ in range(64)]
     reqs = mix.stream(n_requests, rate)
     built = [mix.request(r) for r in reqs]
@@ -536,6 +540,9 @@ def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0):
         elapsed = time.perf_counter() - t0
         st = srv.stats()
     cells = sum(r.cells for r in reqs)
+    # transform work of the searches as requested: rows x blocks x n_fft
+    # points (a periodic-fold search runs shorter transforms for the same result)
+    points = sum(len(r.keys) * len(r.doppler) * r.n_blocks * r.n_fft for r in reqs)
     kinds = {k: sum(1 for r in reqs if r.kind == k) for k in ("L1", "L5", "BOC")}
     lat_by = {k: [l for l, r in zip(lat, reqs) if r.kind == k] for k in kinds}
     return {
@@ -546,6 +553,8 @@ def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0):
         "cells_per_s": cells / elapsed, "mix": kinds, "rounds": st["rounds"],
         "max_batch_seen": st["max_batch_seen"], "failed": st["failed"],
         "boc_precision": {0: "fp32", 1: "fp16", 2: "bf16"}[boc_precision],
+        "l1_l5_integration": "T ms coherent" if coherent else "1 ms coherent x T non-coherent",
+        "transform_points_per_s": points / elapsed,
     }
 
 
@@ -716,6 +725,11 @@ def main():
             out["serving_boc_fp16"] = serving(ctx, args.serve_requests, args.serve_rate, 1)
         except Exception as e:
             out["serving_boc_fp16"] = {"error": repr(e)}
+        log("serving the stream with L1 / L5 as 1 ms x T non-coherent blocks")
+        try:
+            out["serving_noncoherent_1ms"] = serving(ctx, args.serve_requests, args.serve_rate, 0, False)
+        except Exception as e:
+            out["serving_noncoherent_1ms"] = {"error": repr(e)}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")

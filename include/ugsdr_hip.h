@@ -436,8 +436,13 @@ typedef struct {
     int64_t max_device_bytes;  /* admission budget of in-flight requests (0: 8 GiB)   */
     int32_t max_inflight;      /* request slots (0: 64)                               */
     int32_t n_streams;         /* streams the rounds rotate over (0: 4)               */
-    int32_t max_batch;         /* requests per correlation launch (0: 32)             */
-    int32_t reserved;
+    int32_t max_batch;         /* requests per correlation launch (0: 16)             */
+    int32_t round_budget_us;   /* batch-shape autotuner: a round of one bucket is cut
+                                * when its estimated time (its row-blocks P x D x K
+                                * times the bucket's measured time per row-block,
+                                * averaged over the bucket's completed rounds) would
+                                * exceed this (0: 2000 us); before a bucket's first
+                                * round completes, its rounds hold at most 4        */
 } ugsdr_server_config;
 
 typedef struct {

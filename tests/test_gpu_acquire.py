@@ -201,15 +201,18 @@ def test_runtime_plan_lengths(ctx, oracle, n_fft):
     (16368, 16368, True), (16368, 4092, True), (16368, 4092, False)])
 def test_corr_variants_match_oracle(ctx, oracle, n_fft, surface_len, coherent):
     """every correlation variant of a C/A length (the three-stage 1023*M
-    kernel and the generic LDS kernels) against the oracle on one input"""
+    kernel and the generic LDS kernels) against the oracle on one input.
+    Folded cases store the replicas as samples: a replica put from chips of
+    several code periods takes the periodic-fold search instead (covered by
+    test_periodic_fold_matches_oracle), which never runs the fold kernels."""
     rng = np.random.default_rng(n_fft + surface_len)
     fs = n_fft * 1000.0 * surface_len / n_fft
     P, dop = 3, U.doppler_bins(2000, 500)
     reps, keys = [], []
     for _ in range(P):
         code = np.where(rng.random(1023) < 0.5, -1, 1).astype(np.int8)
-        keys.append(ctx.replica_put(code, n_fft // surface_len, n_fft))
         reps.append(oracle.repeat_upsample(code, n_fft // surface_len, n_fft))
+        keys.append(ctx.replica_put(code, 1, n_fft) if surface_len == n_fft else ctx.replica_put_samples(reps[-1]))
     x = (rng.normal(0, 20, n_fft) + 1j * rng.normal(0, 20, n_fft)).astype(np.complex64)
     x += (2.0 * np.roll(reps[1], 777) * np.exp(2j * np.pi * 1000 * np.arange(n_fft) / fs)).astype(np.complex64)
     iq = np.ascontiguousarray(x).view(np.float32)
@@ -229,6 +232,41 @@ def test_corr_variants_match_oracle(ctx, oracle, n_fft, surface_len, coherent):
     with pytest.raises(U.UgsdrError):
         ctx.set_threads(n_fft, 333)
     for k in keys:
+        ctx.replica_drop(k)
+
+
+@pytest.mark.parametrize("n_fft,surface_len,n_blocks,coherent", [
+    (16368, 4092, 2, True), (16368, 4092, 1, False), (8184, 4092, 3, True), (40920, 4092, 1, True),
+    (81840, 4092, 1, False), (65472, 16368, 2, True)])
+def test_periodic_fold_matches_oracle(ctx, oracle, n_fft, surface_len, n_blocks, coherent):
+    """replicas put from chips with T = n_fft / surface_len code periods: the
+    search runs at surface_len on the T periods' spectra combined coherently.
+    Against the oracle's n_fft-point search folded to surface_len, on Doppler
+    bins whose d * surface_len / fs fraction is nonzero (the combination's
+    phase factors), and against the same search on sample replicas (no period:
+    the n_fft-point transforms and the fold kernels)."""
+    rng = np.random.default_rng(n_fft + surface_len + n_blocks)
+    fs = surface_len * 1000.0
+    dop = U.doppler_bins(2250, 125)
+    codes, reps, x = planted_case(oracle, rng, n_fft, surface_len, 1023, fs, n_blocks=n_blocks, shift=1234,
+                                  f_dop=875.0)
+    T = n_fft // surface_len
+    say(f"case periodic fold N={n_fft} L={surface_len} T={T} K={n_blocks} coherent={coherent}")
+    keys = [ctx.replica_put(c, T, n_fft) for c in codes]
+    skeys = [ctx.replica_put_samples(r) for r in reps]
+    iq = np.ascontiguousarray(x).view(np.float32)
+    kw = dict(want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft, n_blocks=n_blocks, surface_len=surface_len,
+              doppler=dop, coherent=coherent)
+    gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, keys=keys, **kw)
+    gs, ssurf, _ = ctx.acquire(iq, U.FMT_CF32, keys=skeys, **kw)
+    ref, rsurf, _ = oracle.acquire(x, reps, n_fft=n_fft, n_blocks=n_blocks, surface_len=surface_len, fs_in=fs,
+                                   fs_acq=fs, doppler=dop, coherent=coherent, precision=64, want_surfaces=True)
+    assert_parity(gpu, ref, f"periodic N={n_fft}/L={surface_len}")
+    assert_parity(gs, ref, f"samples N={n_fft}/L={surface_len}")
+    assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
+    assert np.max(np.abs(ssurf - rsurf)) < 2e-5 * rsurf.max()
+    assert gpu[1]["index"] == 1234 % surface_len and gpu[1]["doppler"] == 875.0
+    for k in keys + skeys:
         ctx.replica_drop(k)
 
 
@@ -252,7 +290,9 @@ def planted_case(oracle, rng, n_fft, surface_len, n_chips, fs, n_blocks=2, shift
 @pytest.mark.parametrize("n_fft,surface_len,n_chips,fs", [
     (20460, 20460, 10230, 20.46e6), (40920, 40920, 10230, 20.46e6), (40920, 20460, 10230, 20.46e6),
     (61380, 61380, 10230, 20.46e6), (61380, 20460, 10230, 20.46e6), (81840, 81840, 8184, 20.46e6),
-    (4200, 4200, 1023, 4.2e6), (26600, 26600, 1023, 6.65e6), (26600, 6650, 1023, 6.65e6)])
+    (4200, 4200, 1023, 4.2e6), (26600, 26600, 1023, 6.65e6), (26600, 6650, 1023, 6.65e6),
+    # T ms coherent L1 C/A requests beyond 16 ms-samples (runtime four-step plans)
+    (24552, 4092, 1023, 4.092e6), (28644, 4092, 1023, 4.092e6), (36828, 4092, 1023, 4.092e6)])
 def test_fourstep_and_recording_lengths_every_variant(ctx, oracle, n_fft, surface_len, n_chips, fs):
     rng = np.random.default_rng(n_fft * 7 + surface_len)
     dop = U.doppler_bins(2000, 500)

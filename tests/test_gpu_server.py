@@ -27,11 +27,16 @@ def assert_same(res, ref):
             assert a[k] == pytest.approx(b[k], rel=1e-5, abs=1e-12), (k, a, b)
 
 
-@pytest.mark.parametrize("boc_precision", [U.PREC_FP32, U.PREC_FP16])
-def test_server_results_equal_solo(ctx, boc_precision):
-    print(f"case server isolation: 24 mixed requests, BOC precision {boc_precision}", flush=True)
-    mix = W.ServeMix(ctx, seed=0xC5, pool=2, boc_precision=boc_precision)
-    reqs = mix.stream(24, 2000.0)
+@pytest.mark.parametrize("boc_precision,coherent", [(U.PREC_FP32, False), (U.PREC_FP16, False),
+                                                    (U.PREC_FP32, True)])
+def test_server_results_equal_solo(ctx, boc_precision, coherent):
+    """coherent=True: L1 / L5 requests of T ms coherent (N = 4092 T or
+    20460 T, Doppler step 500 / T Hz), so every coherent length of the stream
+    shares rounds with the others"""
+    print(f"case server isolation: 24 mixed requests, BOC precision {boc_precision}, coherent T {coherent}",
+          flush=True)
+    mix = W.ServeMix(ctx, seed=0xC5, pool=2, boc_precision=boc_precision, coherent=coherent)
+    reqs = mix.stream(48 if coherent else 24, 2000.0)
     solo = []
     for r in reqs:
         req, hold = mix.request(r)
@@ -53,7 +58,10 @@ def test_server_results_equal_solo(ctx, boc_precision):
             assert_same(res, ref)
         st = srv.stats()
     print(f"server stats {st}, latency ms p50 {np.median(lat):.3f} max {max(lat):.3f}", flush=True)
-    assert st["completed"] == 24 and st["failed"] == 0
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    if coherent:
+        assert any(r.n_fft > 20460 and r.kind == "L1" for r in reqs)   # T_coh > 1 present
+        assert any(r.n_fft > 20460 and r.kind == "L5" for r in reqs)
     assert st["max_batch_seen"] > 1   # requests shared correlation launches
     kinds = {r.kind for r in reqs}
     assert kinds == {"L1", "L5", "BOC"}

@@ -149,7 +149,7 @@ def track_state(code_phase: float, carrier_frequency: float, base_code_frequency
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_device_bytes", C.c_int64), ("max_inflight", C.c_int32), ("n_streams", C.c_int32),
-                ("max_batch", C.c_int32), ("reserved", C.c_int32)]
+                ("max_batch", C.c_int32), ("round_budget_us", C.c_int32)]
 
 
 class ServerStats(C.Structure):
@@ -726,9 +726,9 @@ class Server:
     """Continuous-batching acquisition server on a context (ugsdr_hip_server_*)."""
 
     def __init__(self, ctx: Context, max_device_bytes: int = 0, max_inflight: int = 0, n_streams: int = 0,
-                 max_batch: int = 0):
+                 max_batch: int = 0, round_budget_us: int = 0):
         self._p = C.c_void_p()
-        cfg = ServerConfig(max_device_bytes, max_inflight, n_streams, max_batch, 0)
+        cfg = ServerConfig(max_device_bytes, max_inflight, n_streams, max_batch, round_budget_us)
         _check(lib().ugsdr_hip_server_create(ctx.handle, C.byref(cfg), C.byref(self._p)))
         self._n_prn = {}
 

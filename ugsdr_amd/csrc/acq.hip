@@ -665,7 +665,8 @@ static int64_t gcd64(int64_t a, int64_t b) { while (b) { int64_t t = a % b; a = 
 int64_t AcqPlan::footprint(bool surfaces) const {
     int64_t b = in_bytes + need * 8 + K * F * N * 8 + (int64_t)table_bytes + P * D * 32 +
                 P * (int64_t)sizeof(ugsdr_acq_result) + P * L * 4;
-    if (!small) b += K * F * N * 8;
+    if (Tsub > 1) b += K * Tsub * F * N * 8;
+    if (!small) b += K * Tsub * F * N * 8;
     if (surfaces) b += P * D * L * 4;
     return b;
 }
@@ -694,6 +695,7 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
     pl.L = r->surface_len;
     pl.D = r->n_doppler;
     pl.P = r->n_prn;
+    pl.Tsub = 1;
     if (pl.N > (1 << 22) || pl.D * pl.P > (1 << 30) || pl.K > (1 << 20)) {
         set_error("search too large");
         return UGSDR_ERR_UNSUPPORTED;
@@ -714,6 +716,28 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
         pl.specs[p] = static_cast<const float2*>(it->second.spec);
         pl.spec_scale = it->second.scale;
     }
+    // Periodic fold (FP32, folded surface, every replica made of whole code
+    // periods of surface_len samples): the n_fft-point circular correlation of
+    // a replica of T = n_fft / L periods is L-periodic, and its n_fft-point
+    // inverse DFT only sees the replica spectrum's bins k = T k', where it is
+    // T times the one-period spectrum; so the search folded to L equals an
+    // L-point search of the T periods' spectra combined coherently (scaled by
+    // T, the folded copies).  The transform shrinks by T.
+    {
+        bool periodic = r->precision == UGSDR_PREC_FP32 && r->reshape && pl.L < pl.N && pl.N % pl.L == 0 &&
+                        pl.N / pl.L <= kMaxPeriods && pl.K * pl.D * (pl.N / pl.L) * 64 < ((int64_t)1 << 31);
+        for (int64_t p = 0; p < pl.P && periodic; ++p) {
+            const Replica& rep = ctx->replicas.find(r->replica_keys[p])->second;
+            periodic = rep.spec_short && rep.period == pl.L;
+        }
+        if (periodic) {
+            pl.Tsub = pl.N / pl.L;
+            pl.N = pl.L;
+            for (int64_t p = 0; p < pl.P; ++p)
+                pl.specs[p] = static_cast<const float2*>(ctx->replicas.find(r->replica_keys[p])->second.spec_short);
+            pl.fold_mode = 0;   // the surface is no longer folded: |.| as is
+        }
+    }
     if (!plan_transform((int)pl.N, pl.small, pl.plan, pl.fs)) {
         set_error("unsupported transform length " + std::to_string(pl.N));
         return UGSDR_ERR_UNSUPPORTED;
@@ -727,7 +751,7 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
         pl.up = lcm / fs_in;
         pl.down = lcm / fs_acq;
     }
-    pl.need = pl.N * pl.K;
+    pl.need = pl.N * pl.Tsub * pl.K;
     const int64_t avail = (pl.up == 1 && pl.down == 1) ? r->n_samples : (r->n_samples * pl.up + pl.down - 1) / pl.down;
     if (avail < pl.need) {
         set_error("signal shorter than n_blocks * n_fft at fs_acq");
@@ -818,6 +842,35 @@ int launch_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipS
     return launch_front_args(fa, st);
 }
 
+// Periodic fold: combined class spectrum of block k and class f from its T
+// period sub-blocks, out = T sum_j e^(-j 2 pi j phi_f) sub_(kT+j, f).  With
+// d L / fs = m + phi, sub-block j's tone restarts at e^(-j 2 pi d j L / fs) =
+// e^(-j 2 pi j phi): the factor depends on the class only.
+// Grid: chunks workgroups per (k, f), in one dimension.
+__global__ void combine_periods_kernel(const float2* sub, float2* out, int64_t L, int F, int T, int chunks,
+                                       const double* class_frac) {
+    __shared__ float2 w[kMaxPeriods];
+    const int kf = blockIdx.x / chunks, chunk = blockIdx.x - kf * chunks, k = kf / F, f = kf - k * F;
+    for (int j = threadIdx.x; j < T; j += blockDim.x) {
+        double t = (double)j * class_frac[f];
+        t -= floor(t);
+        double sn, cs;
+        sincospi(2.0 * t, &sn, &cs);
+        w[j] = make_float2((float)(T * cs), (float)(-T * sn));
+    }
+    __syncthreads();
+    const float2* base = sub + ((int64_t)k * T * F + f) * L;
+    for (int64_t i = chunk * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)chunks * blockDim.x) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int j = 0; j < T; ++j) {
+            const float2 x = base[(int64_t)j * F * L + i];
+            acc.x = fmaf(w[j].x, x.x, fmaf(-w[j].y, x.y, acc.x));
+            acc.y = fmaf(w[j].x, x.y, fmaf(w[j].y, x.x, acc.y));
+        }
+        out[(int64_t)kf * L + i] = acc;
+    }
+}
+
 int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     UGSDR_HIP_TRY(s.classes.reserve((size_t)pl.K * pl.F * pl.N * sizeof(float2)));
     FwdArgs a{};
@@ -825,9 +878,21 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     a.F = pl.F;
     a.sig = s.sig.as<float2>();
     a.class_frac = reinterpret_cast<const double*>(s.tables.as<char>() + pl.off_frac);
-    a.out = s.classes.as<float2>();
     a.mode = 0;
-    return launch_forward(a, (int)(pl.K * pl.F), st, s.fwd_scratch);
+    if (pl.Tsub == 1) {
+        a.out = s.classes.as<float2>();
+        return launch_forward(a, (int)(pl.K * pl.F), st, s.fwd_scratch);
+    }
+    // every period sub-block's class spectra, then their combination per block
+    UGSDR_HIP_TRY(s.classes_sub.reserve((size_t)pl.K * pl.Tsub * pl.F * pl.N * sizeof(float2)));
+    a.out = s.classes_sub.as<float2>();
+    if (int rc = launch_forward(a, (int)(pl.K * pl.Tsub * pl.F), st, s.fwd_scratch)) return rc;
+    const int chunks = (int)std::min<int64_t>((pl.N + 255) / 256, 64);
+    hipLaunchKernelGGL(combine_periods_kernel, dim3((unsigned)(pl.K * pl.F * chunks)), dim3(256), 0, st,
+                       s.classes_sub.as<float2>(), s.classes.as<float2>(), pl.N, pl.F, (int)pl.Tsub, chunks,
+                       a.class_frac);
+    UGSDR_HIP_TRY(hipGetLastError());
+    return UGSDR_OK;
 }
 
 int default_threads(int64_t N) {

@@ -15,6 +15,10 @@ namespace ugsdr_hip {
 
 struct AcqPlan {
     int64_t N = 0, K = 0, L = 0, D = 0, P = 0;
+    // periodic fold: each of the K blocks is Tsub whole code periods of N
+    // samples (replicas of Tsub periods folded to one), searched as an
+    // N-length transform of the periods' coherently combined spectra
+    int64_t Tsub = 1;
     int F = 0;
     int fold_mode = 0;        // 0: |.| then fold, 1: fold complex then |.|
     int reshape = 1;

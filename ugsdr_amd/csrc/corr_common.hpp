@@ -38,6 +38,8 @@ struct ReqDesc {
 // requests one correlation launch carries (their descriptors travel in the
 // kernel arguments, read through the scalar cache; no upload)
 constexpr int kMaxBatch = 16;
+// periodic-fold searches combine at most this many code periods per block
+constexpr int kMaxPeriods = 64;
 
 struct CorrArgs {
     RadixPlan plan;

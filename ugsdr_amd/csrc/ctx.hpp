@@ -69,22 +69,30 @@ struct Replica {
     int precision = UGSDR_PREC_FP32;
     float scale = 1.0f;    // reduced precision: stored = spectrum * scale
     int64_t users = 0;     // queued or running server requests that read it
+    // a replica of `repeats` whole code periods (replica_put with n_fft a
+    // multiple of repeats, FP32): the period in samples and conj(DFT) of one
+    // period, for the periodic-fold search (otherwise 0 / null)
+    int64_t period = 0;
+    void* spec_short = nullptr;
 };
 
 // device buffers one acquisition request holds while in flight
 struct AcqSlot {
     DevBuf iq, sig, classes, tables, rows, results, surf, fwd_scratch;
+    // periodic-fold searches: the class spectra of every code-period sub-block
+    DevBuf classes_sub;
     // reduced precision: 16-bit class spectra and their per-block scales
     DevBuf classes_r, block_scale;
     std::vector<char> host_tables;   // kept alive until the copies complete
     void release_all() {
-        for (DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch, &classes_r, &block_scale})
+        for (DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch, &classes_r, &block_scale,
+                          &classes_sub})
             d->release();
     }
     int64_t bytes() const {
         int64_t b = 0;
-        for (const DevBuf* d :
-             {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch, &classes_r, &block_scale})
+        for (const DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch, &classes_r,
+                                &block_scale, &classes_sub})
             b += (int64_t)d->bytes;
         return b;
     }

@@ -44,7 +44,8 @@ int64_t ugsdr_hip_ctx::device_bytes() const {
     b += slot.bytes();
     b += (int64_t)track.buf.bytes;
     for (const auto& kv : replicas)
-        b += kv.second.n_fft * (kv.second.precision == UGSDR_PREC_FP32 ? 8 : 4);
+        b += kv.second.n_fft * (kv.second.precision == UGSDR_PREC_FP32 ? 8 : 4) +
+             (kv.second.spec_short ? kv.second.period * 8 : 0);
     return b;
 }
 
@@ -86,8 +87,10 @@ int ugsdr_hip_ctx_destroy(ugsdr_hip_ctx* ctx) {
         std::lock_guard<std::mutex> lk(ctx->mu);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
-        for (auto& kv : ctx->replicas)
+        for (auto& kv : ctx->replicas) {
             if (kv.second.spec) (void)hipFree(kv.second.spec);
+            if (kv.second.spec_short) (void)hipFree(kv.second.spec_short);
+        }
         for (auto& e : ctx->ev)
             if (e) (void)hipEventDestroy(e);
         ctx->replicas.clear();
@@ -288,6 +291,19 @@ static int replica_store(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chip
             (void)hipFree(rep.spec);
             return rc;
         }
+    } else if (chips && repeats > 1 && n_fft % repeats == 0) {
+        // whole code periods: one period's spectrum for the periodic-fold
+        // search (Upsampler::Transform of one period: the long replica's
+        // first n_fft / repeats samples, the same chip indices)
+        const int64_t L = n_fft / repeats;
+        UGSDR_HIP_TRY(hipMalloc(&rep.spec_short, (size_t)L * sizeof(float2)));
+        rc = replica_spectrum(ctx, chips, n_chips, 1, nullptr, L, static_cast<float2*>(rep.spec_short));
+        if (rc) {
+            (void)hipFree(rep.spec);
+            (void)hipFree(rep.spec_short);
+            return rc;
+        }
+        rep.period = L;
     }
     *key = ctx->next_key++;
     ctx->replicas[*key] = rep;
@@ -318,6 +334,7 @@ int ugsdr_hip_replica_drop(ugsdr_hip_ctx* ctx, uint64_t key) {
     UGSDR_HIP_TRY(hipSetDevice(ctx->device));
     UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
     UGSDR_HIP_TRY(hipFree(it->second.spec));
+    if (it->second.spec_short) UGSDR_HIP_TRY(hipFree(it->second.spec_short));
     ctx->replicas.erase(it);
     return UGSDR_OK;
 }

@@ -106,7 +106,8 @@ static int acquire_noalloc(ugsdr_hip_ctx* ctx, char* base, const ugsdr_acq_reque
               cv.take(s.results, (size_t)pl.P * sizeof(ugsdr_acq_result) + (peaks ? (size_t)pl.P * pl.L * 4 : 0));
     if (ok && !r->iq_on_device) ok = cv.take(s.iq, (size_t)pl.in_bytes);
     if (ok && keep_surf) ok = cv.take(s.surf, (size_t)pl.P * pl.D * pl.L * sizeof(float));
-    if (ok && !pl.small) ok = cv.take(s.fwd_scratch, (size_t)pl.K * pl.F * pl.N * sizeof(float2));
+    if (ok && pl.Tsub > 1) ok = cv.take(s.classes_sub, (size_t)pl.K * pl.Tsub * pl.F * pl.N * sizeof(float2));
+    if (ok && !pl.small) ok = cv.take(s.fwd_scratch, (size_t)pl.K * pl.Tsub * pl.F * pl.N * sizeof(float2));
     // correlation workspace: the three-stage kernel's partial surfaces, or one
     // four-step slice per workgroup (fewer workgroups when the array is short:
     // every row is still computed whole by one workgroup)

@@ -74,9 +74,17 @@ struct Job {
 
 struct Round {
     std::vector<std::shared_ptr<Job>> jobs;
-    hipEvent_t done = nullptr;
+    hipEvent_t start = nullptr, done = nullptr;
     int stream = 0;
+    std::pair<int64_t, int64_t> bucket{0, 0};
+    int64_t row_blocks = 0;   // sum of P x D x K over the round's requests
 };
+
+constexpr int kFirstRoundJobs = 4;
+
+// (transform bucket, surface length): the key requests share a correlation launch by
+static std::pair<int64_t, int64_t> bucket_of(const Job& j) { return {j.plan.bucket(), j.plan.L}; }
+static int64_t row_blocks(const Job& j) { return j.plan.P * j.plan.D * j.plan.K; }
 
 struct Server {
     ugsdr_hip_ctx* ctx = nullptr;
@@ -94,6 +102,11 @@ struct Server {
     uint64_t next_ticket = 1;
     int64_t inflight_bytes = 0;
     int next_stream = 0;
+    // the batch-shape autotuner: per bucket, the measured wall time of a
+    // round per row-block (exponential average over the bucket's rounds);
+    // rounds are cut so that their estimate stays within the budget
+    std::map<std::pair<int64_t, int64_t>, double> cost_us;
+    double budget_us = 2000.0;
     bool stop = false;
     bool sched_done = false;
     ugsdr_server_stats st{};
@@ -108,7 +121,7 @@ static hipEvent_t take_event(Server& s) {
         return e;
     }
     hipEvent_t e = nullptr;
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;   // timed: the autotuner reads the rounds' durations
     return e;
 }
 
@@ -138,6 +151,7 @@ static void fail_job(Job& j, int rc) {
 // launches one round (jobs of one bucket) on stream index si; returns rc
 static int launch_round(Server& s, Round& rd) {
     hipStream_t st = s.streams[rd.stream];
+    UGSDR_HIP_TRY(hipEventRecord(rd.start, st));
     std::vector<const AcqPlan*> plans;
     std::vector<AcqSlot*> devs;
     for (auto& jp : rd.jobs) {
@@ -210,24 +224,44 @@ static void scheduler_loop(Server* sp) {
             s.cv_sched.wait_for(lk, std::chrono::microseconds(200));
             continue;
         }
-        // group by bucket, chunk by max_batch
+        // group by bucket; cut each group into rounds of at most max_batch
+        // requests whose estimated time (row-blocks x the bucket's measured
+        // cost) stays within the budget.  A bucket without a measurement yet
+        // runs rounds of at most kFirstRoundJobs requests until its first
+        // round completes.
         std::map<std::pair<int64_t, int64_t>, std::vector<std::shared_ptr<Job>>> groups;
-        for (auto& j : admitted) groups[{j->plan.bucket(), j->plan.L}].push_back(j);
+        for (auto& j : admitted) groups[bucket_of(*j)].push_back(j);
         std::vector<Round> rounds;
-        for (auto& g : groups)
-            for (size_t i = 0; i < g.second.size(); i += (size_t)s.cfg.max_batch) {
-                Round rd;
-                rd.jobs.assign(g.second.begin() + i,
-                               g.second.begin() + std::min(g.second.size(), i + (size_t)s.cfg.max_batch));
+        for (auto& g : groups) {
+            const auto it = s.cost_us.find(g.first);
+            const double cost = it == s.cost_us.end() ? -1.0 : it->second;
+            Round rd;
+            double est = 0.0;
+            auto close = [&]() {
+                rd.bucket = g.first;
+                rd.start = take_event(s);
                 rd.done = take_event(s);
                 rd.stream = s.next_stream;
                 s.next_stream = (s.next_stream + 1) % (int)s.streams.size();
                 rounds.push_back(std::move(rd));
+                rd = Round();
+                est = 0.0;
+            };
+            for (auto& j : g.second) {
+                const double e = cost < 0 ? 0.0 : cost * (double)row_blocks(*j);
+                const int cap = cost < 0 ? std::min(s.cfg.max_batch, kFirstRoundJobs) : s.cfg.max_batch;
+                if (!rd.jobs.empty() && ((int)rd.jobs.size() >= cap || est + e > s.budget_us))
+                    close();
+                rd.jobs.push_back(j);
+                rd.row_blocks += row_blocks(*j);
+                est += e;
             }
+            if (!rd.jobs.empty()) close();
+        }
         lk.unlock();
         std::vector<std::shared_ptr<Job>> failed;
         for (auto& rd : rounds) {
-            int rc = rd.done ? launch_round(s, rd) : UGSDR_ERR_HIP;
+            int rc = rd.done && rd.start ? launch_round(s, rd) : UGSDR_ERR_HIP;
             std::lock_guard<std::mutex> g(s.mu);
             if (rc) {
                 // the round's work may be partly queued: wait for the stream, then fail its jobs
@@ -240,6 +274,7 @@ static void scheduler_loop(Server* sp) {
                     failed.push_back(j);
                 }
                 if (rd.done) s.event_pool.push_back(rd.done);
+                if (rd.start) s.event_pool.push_back(rd.start);
                 s.cv_done.notify_all();
                 continue;
             }
@@ -287,7 +322,16 @@ static void completer_loop(Server* sp) {
                     s.inflight_bytes -= j.bytes;
                     finished.push_back(jp);
                 }
+                if (q == hipSuccess && it->row_blocks > 0) {
+                    float ms = 0.f;
+                    if (hipEventElapsedTime(&ms, it->start, it->done) == hipSuccess && ms > 0.f) {
+                        const double c = 1000.0 * ms / (double)it->row_blocks;
+                        auto& e = s.cost_us[it->bucket];
+                        e = e > 0.0 ? 0.75 * e + 0.25 * c : c;
+                    }
+                }
                 s.event_pool.push_back(it->done);
+                s.event_pool.push_back(it->start);
                 it = s.inflight.erase(it);
                 any = true;
             }
@@ -326,6 +370,7 @@ int ugsdr_hip_server_create(ugsdr_hip_ctx* ctx, const ugsdr_server_config* cfg, 
     if (s.cfg.max_inflight <= 0) s.cfg.max_inflight = 64;
     if (s.cfg.n_streams <= 0) s.cfg.n_streams = 4;
     if (s.cfg.max_batch <= 0 || s.cfg.max_batch > kMaxBatch) s.cfg.max_batch = kMaxBatch;
+    if (s.cfg.round_budget_us > 0) s.budget_us = (double)s.cfg.round_budget_us;
     s.cus = device_cu_count(ctx->device);
     s.streams.resize(s.cfg.n_streams);
     for (auto& st : s.streams) {

@@ -325,17 +325,21 @@ class ServeRequest:
 
 class ServeMix:
     """Mixed acquisition request stream with Poisson arrivals:
-    50 % L1 C/A-like (int8 @ 4.092 MHz, 1-10 ms as 1 ms x K blocks),
-    30 % L5-like (seeded 10230-chip codes, int16 @ 20.46 MHz, 1-4 ms as 1 ms x K),
+    50 % L1 C/A-like (int8 @ 4.092 MHz, T = 1-10 ms),
+    30 % L5-like (seeded 10230-chip codes, int16 @ 20.46 MHz, T = 1-4 ms),
     20 % BOC(1,1) (seeded 4092-chip codes, int16 @ 20.46 MHz, 4 ms coherent);
     each with 1-16 PRNs and a Doppler span of +-2..+-6 kHz, Doppler step
-    1/(2 T_coh).  Signals come from a small pool of pre-synthesised records
-    (4 planted satellites each, C/N0 U[44,48]); replica spectra are put in the
-    context's cache once per (code, length).  boc_precision selects the
-    storage of the BOC requests (FP32 plain, or FP16/BF16: the reduced form
-    with matrix-core DFT stages)."""
+    1/(2 T_coh).  coherent=True takes T as the coherent length: one block of
+    N = 4092 T (L1) or 20460 T (L5) samples folded to the 1 ms code period
+    (GetOneMsPeak<reshape, coherent>), Doppler step 500/T Hz; coherent=False
+    takes T as 1 ms coherent blocks summed non-coherently (K = T, step 500 Hz),
+    about 1/T of the work.  Signals come from a small pool of pre-synthesised
+    records (4 planted satellites each, C/N0 U[44,48]); replica spectra are
+    put in the context's cache once per (code, length).  boc_precision
+    selects the storage of the BOC requests (FP32 plain, or FP16/BF16: the
+    reduced form with matrix-core DFT stages)."""
 
-    def __init__(self, ctx, seed: int = 0xC5, pool: int = 4, boc_precision: int = 0):
+    def __init__(self, ctx, seed: int = 0xC5, pool: int = 4, boc_precision: int = 0, coherent: bool = False):
         from . import gps_l1ca, boc
         self.ctx = ctx
         self.rng = np.random.default_rng(seed)
@@ -345,6 +349,12 @@ class ServeMix:
         self.keys = {"L1": [ctx.replica_put(c, 1, 4092) for c in self.l1_codes],
                      "L5": [ctx.replica_put(c, 1, 20460) for c in self.l5_codes],
                      "BOC": [ctx.replica_put(c, 1, 81840, boc_precision) for c in self.boc_codes]}
+        self.coherent = coherent
+        if coherent:   # replicas of T code periods at N = T x spm
+            for T in range(2, 11):
+                self.keys[("L1", T)] = [ctx.replica_put(c, T, 4092 * T) for c in self.l1_codes]
+            for T in range(2, 5):
+                self.keys[("L5", T)] = [ctx.replica_put(c, T, 20460 * T) for c in self.l5_codes]
         self.boc_precision = boc_precision
         self.pool = {"L1": [], "L5": [], "BOC": []}
         for i in range(pool):
@@ -364,11 +374,19 @@ class ServeMix:
         if u < 0.5:
             T = int(self.rng.integers(1, 11))
             prns = self.rng.choice(32, n_prn, replace=False)
+            if self.coherent:
+                keys = self.keys["L1"] if T == 1 else self.keys[("L1", T)]
+                return ServeRequest("L1", sig("L1")[: 2 * 4092 * T], FMT_CI8, 4.092e6, 4092 * T, 1, 4092, True,
+                                    doppler_bins(span, 500.0 / T), [keys[p] for p in prns])
             return ServeRequest("L1", sig("L1")[: 2 * 4092 * T], FMT_CI8, 4.092e6, 4092, T, 4092, True,
                                 doppler_bins(span, 500.0), [self.keys["L1"][p] for p in prns])
         if u < 0.8:
             T = int(self.rng.integers(1, 5))
             prns = self.rng.choice(32, n_prn, replace=False)
+            if self.coherent:
+                keys = self.keys["L5"] if T == 1 else self.keys[("L5", T)]
+                return ServeRequest("L5", sig("L5")[: 2 * 20460 * T], FMT_CI16, 20.46e6, 20460 * T, 1, 20460, True,
+                                    doppler_bins(span, 500.0 / T), [keys[p] for p in prns])
             return ServeRequest("L5", sig("L5")[: 2 * 20460 * T], FMT_CI16, 20.46e6, 20460, T, 20460, True,
                                 doppler_bins(span, 500.0), [self.keys["L5"][p] for p in prns])
         prns = self.rng.choice(36, n_prn, replace=False)
