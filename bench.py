@@ -11,7 +11,9 @@ Prints one JSON line: whole-search throughput in cells/s, per-search p50/p99
 latency, the correlation kernel's HBM roofline figure (algorithmic 8 bytes per
 cell-block over the kernel's HIP-event duration) and the CPU baseline (the
 oracle's FP32 restatement of the reference's per-PRN/per-Doppler loop, timed on
-this host's cores on the same input).
+this host's CPU share on the same input, median of 5 after a warm-up, with the
+CPU model and counts recorded; `cpu_baselines_other` holds the same for the
+other acquisition configurations and the correlator bank).
 
 `other_workloads` adds the other acquisition configurations (1 ms and 4 ms
 L1 C/A, BOC 4 ms in FP32 and in the reduced FP16/BF16 form, Galileo E1 code
@@ -19,7 +21,8 @@ hypotheses), the correlator bank, closed-loop tracking, searches straight from
 a packed BbpDdc recording, the DFE channel step and the anti-jamming front
 end; `serving` and `serving_boc_fp16` are the request server under Poisson
 arrivals (L1 / L5 requests of T ms coherent), `serving_noncoherent_1ms` the
-same stream with T blocks of 1 ms summed non-coherently.  `--leg bank|tracking|tracking_many|jse|galileo|galileo_c3` runs one of
+same stream with T blocks of 1 ms summed non-coherently, `serving_round_graphs`
+the first stream with the server's rounds run as captured hipGraphs.  `--leg bank|tracking|tracking_many|jse|galileo|galileo_c3` runs one of
 those alone.
 """
 from __future__ import annotations
@@ -87,52 +90,151 @@ def traffic_from_profiles(workload: str):
         return None
 
 
-def cpu_baseline(w, threads: int):
-    """oracle FP32 path (reference structure: per PRN x Doppler x block: mix,
-    DFT, conj-spectrum multiply, inverse DFT/N, |.|, fold, max, mean/sigma),
-    PRN-parallel over `threads` host threads; the whole search, median of 3."""
+def host_cpu() -> dict:
+    """the host's CPU model and counts: hardware_concurrency (os.cpu_count),
+    the CPUs this process may run on, and the host's per-process share when
+    it sets one (OMP_NUM_THREADS)"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return {"model": model, "hardware_concurrency": os.cpu_count(), "affinity": affinity,
+            "omp_num_threads": int(omp) if omp.isdigit() else None}
+
+
+def cpu_pool_threads(requested: int = 0) -> int:
+    """threads for the CPU baseline pool: the CPUs this process may use,
+    capped by the host's stated share"""
+    if requested:
+        return requested
+    h = host_cpu()
+    n = h["affinity"]
+    if h["omp_num_threads"]:
+        n = min(n, h["omp_num_threads"])
+    return max(1, n)
+
+
+def median_runs(fn, runs: int = 5):
+    """one warm-up call, then `runs` timed calls: (median s, per-run s, last result)"""
+    fn()
+    times, out = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), times, out
+
+
+def cpu_search_baseline(w, threads: int, budget_s: float = 1.5, runs: int = 5):
+    """The oracle's FP32 search (the reference's structure: per PRN x Doppler x
+    block: mix, forward DFT, conj-spectrum multiply, inverse DFT / N, |.|,
+    fold, max, mean / sigma), PRN-parallel over `threads` host threads, on the
+    workload's own input.  Bounded sample: the first PRNs and Doppler bins,
+    sized so one run takes about budget_s; cells/s of the sample, median of
+    `runs` after one warm-up."""
     import numpy as np
 
     from oracle import refcpu
 
     x = (w.iq[0::2].astype(np.float32) + 1j * w.iq[1::2].astype(np.float32)).astype(np.complex64)
     reps = w.replicas()
-    kw = dict(n_fft=w.n_fft, n_blocks=w.n_blocks, surface_len=w.surface_len, fs_in=w.fs, fs_acq=w.fs,
-              doppler=w.doppler, reshape=w.reshape, precision=32, threads=threads)
-    # bounded sample: as many PRNs as fit ~10-20 s of work
+    n_prn, n_dop = len(reps), len(w.doppler)
+
+    def search(n_p, n_d):
+        return refcpu.acquire(x, reps[:n_p], n_fft=w.n_fft, n_blocks=w.n_blocks, surface_len=w.surface_len,
+                              fs_in=w.fs, fs_acq=w.fs, doppler=w.doppler[:n_d], reshape=w.reshape, precision=32,
+                              threads=threads)
+
+    # probe: one PRN per thread over every bin (also warms the transform plans)
+    n_p = min(n_prn, threads)
     t0 = time.perf_counter()
-    refcpu.acquire(x, reps[:threads], **kw)
-    per_round = time.perf_counter() - t0
-    n_prn = len(reps)
-    rounds = max(1, min(int(15.0 / max(per_round, 1e-3)), (n_prn + threads - 1) // threads))
-    sample = reps[: min(n_prn, rounds * threads)]
-    times = []
-    for _ in range(3 if per_round * rounds < 5 else 1):
-        t0 = time.perf_counter()
-        res, _, _ = refcpu.acquire(x, sample, **kw)
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
-    cells = len(sample) * len(w.doppler) * w.surface_len
-    # FFT calibration against numpy's pocketfft (FP64) at this length
-    z = np.random.default_rng(0).normal(size=w.n_fft) + 1j * np.random.default_rng(1).normal(size=w.n_fft)
-    t0 = time.perf_counter()
-    for _ in range(20):
-        refcpu.fft(z)
-    t_or = (time.perf_counter() - t0) / 20
-    t0 = time.perf_counter()
-    for _ in range(20):
-        np.fft.fft(z)
-    t_np = (time.perf_counter() - t0) / 20
+    search(n_p, n_dop)
+    probe = time.perf_counter() - t0
+    if probe > budget_s:
+        n_d = max(1, min(n_dop, int(n_dop * budget_s / probe)))
+    else:
+        n_d = n_dop
+        n_p = min(n_prn, n_p * max(1, int(budget_s / max(probe, 1e-4))))
+    t, times, (res, _, _) = median_runs(lambda: search(n_p, n_d), runs)
+    cells = n_p * n_d * w.surface_len
+    whole = n_p == n_prn and n_d == n_dop
     return {
         "value": cells / t,
         "unit": "cells/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{len(sample)} of {n_prn} PRNs x {len(w.doppler)} bins x {w.n_blocks} blocks of the same "
-                  f"input, FP32, {threads} threads, median of {len(times)} ({t:.2f} s)",
-        "fft_time_vs_numpy_pocketfft": t_or / t_np,
-        "results": res,
+        "sample": (f"{'the whole search' if whole else 'sample'}: {n_p} of {n_prn} PRNs x {n_d} of {n_dop} "
+                   f"bins x {w.n_blocks} blocks of N = {w.n_fft}, FP32, {threads} threads, median of {runs} "
+                   f"after 1 warm-up ({t:.3f} s)"),
+        "runs_s": times,
+        "results": res if n_d == n_dop else None,   # per PRN comparable with the GPU search
     }
+
+
+def cpu_bank_baseline(bw, threads: int, budget_s: float = 1.5, runs: int = 5):
+    """The oracle's correlator bank (the reference's Correlator taps per channel
+    with the FP64 carrier and code phases), channel-parallel over `threads`
+    host threads: channel-samples/s over the first blocks of the workload's
+    input, median of `runs` after one warm-up."""
+    import numpy as np
+
+    from oracle import refcpu
+
+    def run(nb):
+        x = (bw.iq[0:2 * nb * bw.spm:2].astype(np.float32) +
+             1j * bw.iq[1:2 * nb * bw.spm:2].astype(np.float32)).astype(np.complex64)
+        return refcpu.bank_nco(x, bw.spm, nb, bw.fs, bw.channels, bw.codes, bw.taps, threads=threads)
+
+    nb = 4
+    t0 = time.perf_counter()
+    run(nb)
+    probe = time.perf_counter() - t0
+    nb = max(1, min(bw.n_blocks, int(nb * budget_s / max(probe, 1e-4))))
+    t, times, _ = median_runs(lambda: run(nb), runs)
+    cs = len(bw.channels) * nb * bw.spm
+    return {
+        "value": cs / t,
+        "unit": "channel-samples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{len(bw.channels)} channels x {len(bw.taps)} taps over {nb} of {bw.n_blocks} blocks of "
+                   f"{bw.spm} samples, {threads} threads, median of {runs} after 1 warm-up ({t:.3f} s)"),
+        "runs_s": times,
+    }
+
+
+def fft_calibration(lengths=(4092, 16368, 81840), reps: int = 20):
+    """the oracle's FP32 transform time over numpy's pocketfft (complex64) per length"""
+    import numpy as np
+
+    from oracle import refcpu
+
+    out = {}
+    for n in lengths:
+        rng = np.random.default_rng(n)
+        z = (rng.normal(size=n) + 1j * rng.normal(size=n)).astype(np.complex64)
+        refcpu.fft(z, precision=32)
+        np.fft.fft(z)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            refcpu.fft(z, precision=32)
+        t_or = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            np.fft.fft(z)
+        t_np = (time.perf_counter() - t0) / reps
+        out[str(n)] = {"oracle_us": 1e6 * t_or, "pocketfft_us": 1e6 * t_np, "ratio": t_or / t_np}
+    return out
 
 
 def time_search(ctx, w, steps: int, warmup: int, precision: int = 0):
@@ -511,23 +613,37 @@ def recorded_workloads(ctx):
     return out
 
 
-def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0, coherent: bool = True):
+def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0, coherent: bool = True,
+            round_graphs: bool = False):
     """Poisson arrivals of the mixed request stream into the continuous-batching
     server.  coherent: L1 / L5 requests of T ms coherent (N = 4092 T or
     20460 T, Doppler step 500 / T Hz); otherwise T blocks of 1 ms summed
-    non-coherently (step 500 Hz)."""
+    non-coherently (step 500 Hz).  round_graphs: the server's rounds run as
+    captured and replayed hipGraphs."""
     import numpy as np
 
     import ugsdr_amd as U
     from ugsdr_amd import workloads as W
 
     mix = W.ServeMix(ctx, seed=0xC5, boc_precision=boc_precision, coherent=coherent)
-    warm = [mix.request(mix.draw()) for _ in range(64)]
+    # warm-up: a shorter stream of the same mix at the same rate into the same
+    # server (its replica cache, slot buffers, cost estimates and round graphs
+    # then hold the steady state); not timed
+    warm_reqs = mix.stream(max(64, n_requests // 4), rate)
+    warm = [mix.request(r) for r in warm_reqs]
     reqs = mix.stream(n_requests, rate)
     built = [mix.request(r) for r in reqs]
-    with U.Server(ctx, n_streams=4, max_batch=16) as srv:
-        for t in [srv.submit(q) for q, _ in warm]:
+    with U.Server(ctx, n_streams=4, max_batch=16, round_graphs=round_graphs) as srv:
+        t0 = time.perf_counter()
+        warm_t = []
+        for (q, _), r in zip(warm, warm_reqs):
+            now = time.perf_counter() - t0
+            if r.arrival_s > now:
+                time.sleep(r.arrival_s - now)
+            warm_t.append(srv.submit(q))
+        for t in warm_t:
             srv.poll(t, -1)
+        st0 = srv.stats()
         t0 = time.perf_counter()
         tickets = []
         for (q, _), r in zip(built, reqs):
@@ -550,10 +666,16 @@ def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0, coherent:
         "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99), "max_ms": max(lat),
         "mean_ms": statistics.mean(lat),
         "p99_ms_by_kind": {k: percentile(v, 0.99) for k, v in lat_by.items() if v},
-        "cells_per_s": cells / elapsed, "mix": kinds, "rounds": st["rounds"],
+        "cells_per_s": cells / elapsed, "mix": kinds, "rounds": st["rounds"] - st0["rounds"],
         "max_batch_seen": st["max_batch_seen"], "failed": st["failed"],
+        "warmup_requests": len(warm_reqs),
+        "graphs": {k: st[k] - st0[k]
+                   for k in ("graph_launches", "graph_instantiations", "graph_updates", "graph_fallbacks")},
+        "launch_host_us_per_round": (st["launch_host_us"] - st0["launch_host_us"]) /
+                                    max(1, st["rounds"] - st0["rounds"]),
         "boc_precision": {0: "fp32", 1: "fp16", 2: "bf16"}[boc_precision],
         "l1_l5_integration": "T ms coherent" if coherent else "1 ms coherent x T non-coherent",
+        "round_graphs": round_graphs,
         "transform_points_per_s": points / elapsed,
     }
 
@@ -730,18 +852,46 @@ def main():
             out["serving_noncoherent_1ms"] = serving(ctx, args.serve_requests, args.serve_rate, 0, False)
         except Exception as e:
             out["serving_noncoherent_1ms"] = {"error": repr(e)}
+        log("serving the first stream again with its rounds as captured, replayed hipGraphs")
+        try:
+            out["serving_round_graphs"] = serving(ctx, args.serve_requests, args.serve_rate, 0, True, True)
+        except Exception as e:
+            out["serving_round_graphs"] = {"error": repr(e)}
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(w, threads)
+        threads = cpu_pool_threads(args.cpu_threads)
+        host = host_cpu()
+        log(f"cpu baseline on {threads} threads ({host['model']}, hardware_concurrency {host['hardware_concurrency']})")
+        cpu = cpu_search_baseline(w, threads)
         cpu_res = cpu.pop("results")
+        cpu["host"] = host
         out["cpu_baseline"] = cpu
         out["speedup_vs_cpu"] = value / cpu["value"]
-        agree = all(cpu_res[i]["detected"] == res[i]["detected"] and
-                    (not res[i]["detected"] or (cpu_res[i]["index"] == res[i]["index"] and
-                                                cpu_res[i]["doppler_bin"] == res[i]["doppler_bin"]))
-                    for i in range(len(cpu_res)))
-        out["cpu_gpu_same_detections"] = agree
+        if cpu_res is not None:
+            agree = all(cpu_res[i]["detected"] == res[i]["detected"] and
+                        (not res[i]["detected"] or (cpu_res[i]["index"] == res[i]["index"] and
+                                                    cpu_res[i]["doppler_bin"] == res[i]["doppler_bin"]))
+                        for i in range(len(cpu_res)))
+            out["cpu_gpu_same_detections"] = agree
+        out["cpu_fft_vs_pocketfft"] = fft_calibration()
+        if not args.no_extra:
+            # the other configurations' CPU figures beside their GPU ones
+            ow = out.get("other_workloads", {})
+            base = {}
+            for name in ("L1CA_1MS", "L1CA_4MS", "BOC_4MS"):
+                if name == args.workload:
+                    continue
+                log(f"cpu baseline {name}")
+                c = cpu_search_baseline(W.CONFIGS[name](), threads)
+                c.pop("results")
+                g = ow.get(name, {}).get("cells_per_s")
+                c["speedup_gpu_vs_cpu"] = g / c["value"] if g else None
+                base[name] = c
+            log("cpu baseline BANK_L5")
+            c = cpu_bank_baseline(W.bank_l5(), threads)
+            g = ow.get("BANK_L5", {}).get("channel_samples_per_s")
+            c["speedup_gpu_vs_cpu"] = g / c["value"] if g else None
+            base["BANK_L5"] = c
+            out["cpu_baselines_other"] = base
     ctx.close()
     print(json.dumps(out))
 

@@ -443,10 +443,21 @@ typedef struct {
                                 * averaged over the bucket's completed rounds) would
                                 * exceed this (0: 2000 us); before a bucket's first
                                 * round completes, its rounds hold at most 4        */
+    int32_t round_graphs;      /* nonzero: each round's device steps are captured
+                                * into a hipGraph and run as one launch, replaying
+                                * (with updated parameters) an executable of the
+                                * same kernel sequence; 0: the steps are launched
+                                * one by one, which costs the scheduler about half
+                                * the host time per round on ROCm 7.2             */
 } ugsdr_server_config;
 
 typedef struct {
     int64_t submitted, completed, failed, inflight, inflight_bytes, rounds, batches, max_batch_seen;
+    /* round graphs: launches, executables instantiated, executables updated and
+     * replayed, rounds that ran without a graph (no idle executable of the
+     * round's kernel sequence yet, or the capture failed) */
+    int64_t graph_launches, graph_instantiations, graph_updates, graph_fallbacks;
+    int64_t launch_host_us;    /* scheduler time spent launching rounds        */
 } ugsdr_server_stats;
 
 int ugsdr_hip_server_create(ugsdr_hip_ctx* ctx, const ugsdr_server_config* cfg, ugsdr_hip_server** out);

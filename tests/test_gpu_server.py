@@ -27,14 +27,16 @@ def assert_same(res, ref):
             assert a[k] == pytest.approx(b[k], rel=1e-5, abs=1e-12), (k, a, b)
 
 
-@pytest.mark.parametrize("boc_precision,coherent", [(U.PREC_FP32, False), (U.PREC_FP16, False),
-                                                    (U.PREC_FP32, True)])
-def test_server_results_equal_solo(ctx, boc_precision, coherent):
+@pytest.mark.parametrize("boc_precision,coherent,graphs", [(U.PREC_FP32, False, False), (U.PREC_FP16, False, False),
+                                                           (U.PREC_FP32, True, False), (U.PREC_FP32, True, True),
+                                                           (U.PREC_FP16, False, True)])
+def test_server_results_equal_solo(ctx, boc_precision, coherent, graphs):
     """coherent=True: L1 / L5 requests of T ms coherent (N = 4092 T or
     20460 T, Doppler step 500 / T Hz), so every coherent length of the stream
-    shares rounds with the others"""
-    print(f"case server isolation: 24 mixed requests, BOC precision {boc_precision}, coherent T {coherent}",
-          flush=True)
+    shares rounds with the others.  graphs: the rounds' device steps captured
+    and replayed as graphs instead of launched one by one."""
+    print(f"case server isolation: mixed requests, BOC precision {boc_precision}, coherent T {coherent}, "
+          f"round graphs {graphs}", flush=True)
     mix = W.ServeMix(ctx, seed=0xC5, pool=2, boc_precision=boc_precision, coherent=coherent)
     reqs = mix.stream(48 if coherent else 24, 2000.0)
     solo = []
@@ -42,7 +44,7 @@ def test_server_results_equal_solo(ctx, boc_precision, coherent):
         req, hold = mix.request(r)
         res, _, _ = ctx.acquire_req(req)
         solo.append(res)
-    with U.Server(ctx, n_streams=4, max_batch=8) as srv:
+    with U.Server(ctx, n_streams=4, max_batch=8, round_graphs=graphs) as srv:
         tickets = []
         holds = []
         for r in reqs:
@@ -65,6 +67,33 @@ def test_server_results_equal_solo(ctx, boc_precision, coherent):
     assert st["max_batch_seen"] > 1   # requests shared correlation launches
     kinds = {r.kind for r in reqs}
     assert kinds == {"L1", "L5", "BOC"}
+    if not graphs:
+        assert st["graph_launches"] == 0 and st["graph_fallbacks"] == 0
+    else:
+        # a round runs as a graph once its kernel sequence has an idle executable
+        assert st["graph_launches"] + st["graph_fallbacks"] == st["rounds"]
+
+
+@pytest.mark.parametrize("kind", ["L1", "BOC"])
+def test_server_graph_replay(ctx, kind):
+    """one request shape submitted again and again on one stream: the first
+    round runs eagerly while its capture is instantiated in the background,
+    the later rounds update that executable and replay it, and every result
+    equals the request run alone"""
+    print(f"case server graph replay {kind}", flush=True)
+    mix = W.ServeMix(ctx, seed=0xC8, pool=1)
+    r = next(x for x in (mix.draw() for _ in range(100)) if x.kind == kind)
+    req, hold = mix.request(r)
+    ref, _, _ = ctx.acquire_req(req)
+    with U.Server(ctx, n_streams=1, max_batch=1, round_graphs=True) as srv:
+        for _ in range(12):
+            res, _ = srv.poll(srv.submit(req), -1)
+            assert_same(res, ref)
+        st = srv.stats()
+    print(f"server stats {st}", flush=True)
+    assert st["rounds"] == 12 and st["graph_launches"] + st["graph_fallbacks"] == 12
+    assert st["graph_launches"] >= 8 and st["graph_updates"] == st["graph_launches"]
+    assert 1 <= st["graph_instantiations"] <= 4
 
 
 def test_server_budget_and_errors(ctx):

@@ -149,12 +149,14 @@ def track_state(code_phase: float, carrier_frequency: float, base_code_frequency
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_device_bytes", C.c_int64), ("max_inflight", C.c_int32), ("n_streams", C.c_int32),
-                ("max_batch", C.c_int32), ("round_budget_us", C.c_int32)]
+                ("max_batch", C.c_int32), ("round_budget_us", C.c_int32), ("round_graphs", C.c_int32)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("submitted", "completed", "failed", "inflight", "inflight_bytes",
-                                          "rounds", "batches", "max_batch_seen")]
+                                          "rounds", "batches", "max_batch_seen", "graph_launches",
+                                          "graph_instantiations", "graph_updates", "graph_fallbacks",
+                                          "launch_host_us")]
 
 
 PENDING = 1
@@ -726,9 +728,9 @@ class Server:
     """Continuous-batching acquisition server on a context (ugsdr_hip_server_*)."""
 
     def __init__(self, ctx: Context, max_device_bytes: int = 0, max_inflight: int = 0, n_streams: int = 0,
-                 max_batch: int = 0, round_budget_us: int = 0):
+                 max_batch: int = 0, round_budget_us: int = 0, round_graphs: bool = False):
         self._p = C.c_void_p()
-        cfg = ServerConfig(max_device_bytes, max_inflight, n_streams, max_batch, round_budget_us)
+        cfg = ServerConfig(max_device_bytes, max_inflight, n_streams, max_batch, round_budget_us, int(round_graphs))
         _check(lib().ugsdr_hip_server_create(ctx.handle, C.byref(cfg), C.byref(self._p)))
         self._n_prn = {}
 

@@ -1059,6 +1059,19 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
     return UGSDR_OK;
 }
 
+int presize_slot(const AcqPlan& pl, AcqSlot& s, bool peaks) {
+    const size_t kfn = (size_t)pl.K * pl.F * pl.N;
+    UGSDR_HIP_TRY(s.classes.reserve(kfn * sizeof(float2)));
+    if (pl.Tsub > 1) UGSDR_HIP_TRY(s.classes_sub.reserve(kfn * pl.Tsub * sizeof(float2)));
+    if (!pl.small) UGSDR_HIP_TRY(s.fwd_scratch.reserve(kfn * pl.Tsub * sizeof(float2)));
+    if (pl.precision != UGSDR_PREC_FP32) {
+        UGSDR_HIP_TRY(s.classes_r.reserve(kfn * 4));
+        UGSDR_HIP_TRY(s.block_scale.reserve((size_t)pl.K * pl.F * 8));
+    }
+    UGSDR_HIP_TRY(s.results.reserve((size_t)pl.P * sizeof(ugsdr_acq_result) + (peaks ? (size_t)pl.P * pl.L * 4 : 0)));
+    return UGSDR_OK;
+}
+
 int enqueue_final(const AcqPlan& pl, AcqSlot& s, hipStream_t st, bool peaks) {
     UGSDR_HIP_TRY(s.results.reserve((size_t)pl.P * sizeof(ugsdr_acq_result) + (peaks ? (size_t)pl.P * pl.L * 4 : 0)));
     FinalArgs fa{};
@@ -1261,7 +1274,9 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
         gc.drop();
         if (int rc = enqueue_front(pl, r, s, st, keep_surf, FrontStep::Upload)) return rc;
         UGSDR_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        g_capturing = true;
         int rc = enqueue_search(ctx, pl, r, s, st, keep_surf, peaks != nullptr, threads, FrontStep::Launch);
+        g_capturing = false;
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(st, &g);
         hipGraphExec_t exec = nullptr;

@@ -68,6 +68,11 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
                  DevBuf& scratch, int cu_count, int threads = 0);
 // scratch the correlation of this bucket may use (upper bound, any variant)
 size_t corr_scratch_bytes(const AcqPlan& p0, int cu_count);
+// Grows the slot buffers the class, reduction and selection steps of a search
+// use to the sizes those steps reserve, so that they allocate nothing (a
+// stream capture must not allocate).  enqueue_front's Upload step sizes the
+// rest.
+int presize_slot(const AcqPlan& pl, AcqSlot& s, bool peaks);
 // workgroup sizes the correlation kernel is built with for a length
 std::vector<int> thread_candidates(int64_t N);
 int default_threads(int64_t N);

@@ -38,11 +38,17 @@ void set_error(const std::string& msg);
 hipError_t set_lds_limit(const void* kernel, size_t bytes);
 
 // grow-only device buffer
+// Set by a thread while it captures a stream into a graph: a DevBuf that
+// would have to grow then fails instead (no free or allocation inside a
+// capture), and the caller runs the steps without a graph.
+inline thread_local bool g_capturing = false;
+
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
     hipError_t reserve(size_t n) {
         if (n <= bytes) return hipSuccess;
+        if (g_capturing) return hipErrorStreamCaptureUnsupported;
         if (ptr) {
             hipError_t e = hipFree(ptr);
             if (e != hipSuccess) return e;

@@ -9,6 +9,17 @@
 //                 copy; each round goes to the next stream of a small pool so
 //                 rounds overlap on the device
 //   completer  -> polls the rounds' events, stamps latencies, frees slots
+//
+// With round graphs on, a round's device steps (every request's front end,
+// class spectra and reduction, the batched correlation, every request's
+// selection) are captured and run as one graph launch; the uploads and result
+// copies around them stay outside the graph on the same stream.  Executable graphs
+// are cached per stream and topology (the captured sequence of kernels): a
+// round whose topology has an idle executable updates that executable's node
+// parameters (hipGraphExecUpdate) and replays it.  Otherwise the round's steps
+// run eagerly and its capture goes to an instantiation thread, which adds an
+// executable for the topology's later rounds (at most kGraphPool per
+// topology, shared by the streams), so no round waits for hipGraphInstantiate.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -86,6 +97,20 @@ constexpr int kFirstRoundJobs = 4;
 static std::pair<int64_t, int64_t> bucket_of(const Job& j) { return {j.plan.bucket(), j.plan.L}; }
 static int64_t row_blocks(const Job& j) { return j.plan.P * j.plan.D * j.plan.K; }
 
+constexpr size_t kGraphPool = 4;
+
+// an executable graph and the event recorded after its last launch
+struct GraphExec {
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+};
+
+// a captured graph handed to the instantiation thread
+struct GraphWork {
+    std::string key;
+    hipGraph_t graph;
+};
+
 struct Server {
     ugsdr_hip_ctx* ctx = nullptr;
     ugsdr_server_config cfg{};
@@ -107,6 +132,25 @@ struct Server {
     // rounds are cut so that their estimate stays within the budget
     std::map<std::pair<int64_t, int64_t>, double> cost_us;
     double budget_us = 2000.0;
+    // round graphs: topology -> executables, shared by the streams (an idle
+    // executable's last launch has completed, whichever stream it ran on).
+    // The scheduler updates and launches them; the instantiation thread adds
+    // new ones, so that hipGraphInstantiate never holds up a round (gmu
+    // guards the containers; a deque keeps the scheduler's references valid)
+    bool graphs = false;
+    // rounds are captured on this stream, never on the streams they run on:
+    // the completer queries the running rounds' events meanwhile, which is
+    // not permitted on an event whose stream is capturing
+    hipStream_t cap_stream = nullptr;
+    std::mutex gmu;
+    std::condition_variable gcv;
+    std::map<std::string, std::deque<GraphExec>> gexec;
+    std::map<std::string, int> gpending;   // graphs queued for instantiation
+    std::deque<GraphWork> gwork;
+    bool gstop = false;
+    std::thread ginst;
+    std::atomic<int64_t> graph_launches{0}, graph_instantiations{0}, graph_updates{0}, graph_fallbacks{0};
+    std::atomic<int64_t> launch_host_ns{0};
     bool stop = false;
     bool sched_done = false;
     ugsdr_server_stats st{};
@@ -148,26 +192,147 @@ static void fail_job(Job& j, int rc) {
     j.t_done = Clock::now();
 }
 
+// Topology key of a captured graph: its nodes' types and kernel functions in
+// capture order (a stream capture is a chain, so the order fixes the edges).
+static bool graph_topology(hipGraph_t g, std::string& key) {
+    size_t n = 0;
+    if (hipGraphGetNodes(g, nullptr, &n) != hipSuccess) return false;
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n && hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess) return false;
+    key.clear();
+    for (size_t i = 0; i < n; ++i) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nodes[i], &t) != hipSuccess) return false;
+        key.append(reinterpret_cast<const char*>(&t), sizeof t);
+        if (t == hipGraphNodeTypeKernel) {
+            hipKernelNodeParams kp{};
+            if (hipGraphKernelNodeGetParams(nodes[i], &kp) != hipSuccess) return false;
+            key.append(reinterpret_cast<const char*>(&kp.func), sizeof kp.func);
+        }
+    }
+    return true;
+}
+
+// Instantiation thread: turns queued captures into executables of their
+// topology.
+static void instantiate_loop(Server* sp) {
+    Server& s = *sp;
+    (void)hipSetDevice(s.ctx->device);
+    std::unique_lock<std::mutex> lk(s.gmu);
+    while (true) {
+        s.gcv.wait(lk, [&] { return s.gstop || !s.gwork.empty(); });
+        if (s.gwork.empty()) break;   // stopping, nothing left
+        GraphWork w = std::move(s.gwork.front());
+        s.gwork.pop_front();
+        lk.unlock();
+        GraphExec e;
+        bool ok = hipEventCreateWithFlags(&e.done, hipEventDisableTiming) == hipSuccess;
+        if (ok && hipGraphInstantiate(&e.exec, w.graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipEventDestroy(e.done);
+            ok = false;
+        }
+        (void)hipGraphDestroy(w.graph);
+        if (!ok) (void)hipGetLastError();
+        lk.lock();
+        --s.gpending[w.key];
+        if (ok) {
+            s.gexec[w.key].push_back(e);
+            ++s.graph_instantiations;
+        }
+    }
+}
+
+// An idle executable of this topology (its last launch done), updated to the
+// captured graph g's parameters; null if there is none.  With none idle and
+// room in the pool, g is handed to the instantiation thread (taking it: *g is
+// cleared) for a later round of this topology.
+static GraphExec* graph_exec_for(Server& s, const std::string& key, hipGraph_t* g) {
+    GraphExec* idle = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s.gmu);
+        auto& pool = s.gexec[key];
+        for (auto& e : pool)
+            if (e.exec && hipEventQuery(e.done) == hipSuccess) {
+                idle = &e;
+                break;
+            }
+        if (!idle) {
+            int& pend = s.gpending[key];
+            if (pool.size() + (size_t)pend < kGraphPool && !s.gstop) {
+                ++pend;
+                s.gwork.push_back(GraphWork{key, *g});
+                *g = nullptr;
+                s.gcv.notify_one();
+            }
+            return nullptr;
+        }
+    }
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult ur = hipGraphExecUpdateError;
+    if (hipGraphExecUpdate(idle->exec, *g, &bad, &ur) == hipSuccess && ur == hipGraphExecUpdateSuccess) {
+        ++s.graph_updates;
+        return idle;
+    }
+    // not updatable after all: retire it (its slot in the pool stays taken)
+    (void)hipGetLastError();
+    (void)hipGraphExecDestroy(idle->exec);
+    idle->exec = nullptr;
+    return nullptr;
+}
+
+// Runs steps(stream) captured into a graph (on the capture stream) and
+// launched on stream si.  False: nothing from steps is on stream si (no
+// executable of the round's topology was ready, or the capture failed); the
+// caller runs them without a graph.
+template <class Steps>
+static bool run_as_graph(Server& s, int si, Steps&& steps) {
+    hipStream_t st = s.streams[si], cs = s.cap_stream;
+    if (!cs || hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    g_capturing = true;
+    const int rc = steps(cs);
+    g_capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &g);
+    std::string key;
+    GraphExec* e = nullptr;
+    if (!rc && ec == hipSuccess && g && graph_topology(g, key)) e = graph_exec_for(s, key, &g);
+    if (g) (void)hipGraphDestroy(g);
+    if (!e || hipGraphLaunch(e->exec, st) != hipSuccess) {
+        (void)hipGetLastError();
+        ++s.graph_fallbacks;
+        return false;
+    }
+    (void)hipEventRecord(e->done, st);
+    ++s.graph_launches;
+    return true;
+}
+
 // launches one round (jobs of one bucket) on stream index si; returns rc
 static int launch_round(Server& s, Round& rd) {
     hipStream_t st = s.streams[rd.stream];
     UGSDR_HIP_TRY(hipEventRecord(rd.start, st));
     std::vector<const AcqPlan*> plans;
     std::vector<AcqSlot*> devs;
-    for (auto& jp : rd.jobs) {
-        Job& j = *jp;
+    std::vector<ugsdr_acq_request> reqs(rd.jobs.size());
+    // host copies and uploads (outside any graph): each request's IQ into its
+    // slot's pinned stage, its tables and IQ to the device, and every slot
+    // buffer grown to what the device steps take
+    for (size_t i = 0; i < rd.jobs.size(); ++i) {
+        Job& j = *rd.jobs[i];
         ServerSlot& sl = s.slots[j.slot];
-        ugsdr_acq_request r = j.req;
+        ugsdr_acq_request& r = reqs[i];
+        r = j.req;
         if (!r.iq_on_device) {
             const size_t bytes = (size_t)j.plan.in_bytes;
             UGSDR_HIP_TRY(sl.iq_stage.reserve(bytes));
             std::memcpy(sl.iq_stage.ptr, j.iq.data(), bytes);
             r.iq = sl.iq_stage.ptr;  // pinned: the async copy does not block this thread
         }
-        if (int rc = enqueue_front(j.plan, &r, sl.dev, st, false)) return rc;
-        if (int rc = enqueue_classes(j.plan, sl.dev, st)) return rc;
-        if (j.plan.precision != UGSDR_PREC_FP32)
-            if (int rc = enqueue_reduce_classes(j.plan, sl.dev, st)) return rc;
+        if (int rc = enqueue_front(j.plan, &r, sl.dev, st, false, FrontStep::Upload)) return rc;
+        if (int rc = presize_slot(j.plan, sl.dev, false)) return rc;
         plans.push_back(&j.plan);
         devs.push_back(&sl.dev);
     }
@@ -181,14 +346,27 @@ static int launch_round(Server& s, Round& rd) {
         UGSDR_HIP_TRY(hipStreamSynchronize(st));
         UGSDR_HIP_TRY(s.scratch[rd.stream].reserve(need_scratch));
     }
-    if (int rc = p0.precision == UGSDR_PREC_FP32
-                     ? enqueue_corr(plans, devs, st, s.scratch[rd.stream], s.cus, p0.threads)
-                     : enqueue_corr_reduced(plans, devs, st, s.scratch[rd.stream], s.cus, p0.threads))
-        return rc;
+    auto device_steps = [&](hipStream_t st) -> int {
+        for (size_t i = 0; i < rd.jobs.size(); ++i) {
+            const AcqPlan& pl = *plans[i];
+            if (int rc = launch_front(pl, &reqs[i], *devs[i], st)) return rc;
+            if (int rc = enqueue_classes(pl, *devs[i], st)) return rc;
+            if (pl.precision != UGSDR_PREC_FP32)
+                if (int rc = enqueue_reduce_classes(pl, *devs[i], st)) return rc;
+        }
+        if (int rc = p0.precision == UGSDR_PREC_FP32
+                         ? enqueue_corr(plans, devs, st, s.scratch[rd.stream], s.cus, p0.threads)
+                         : enqueue_corr_reduced(plans, devs, st, s.scratch[rd.stream], s.cus, p0.threads))
+            return rc;
+        for (size_t i = 0; i < rd.jobs.size(); ++i)
+            if (int rc = enqueue_final(*plans[i], *devs[i], st, false)) return rc;
+        return UGSDR_OK;
+    };
+    if (!(s.graphs && run_as_graph(s, rd.stream, device_steps)))
+        if (int rc = device_steps(st)) return rc;
     for (auto& jp : rd.jobs) {
         Job& j = *jp;
         ServerSlot& sl = s.slots[j.slot];
-        if (int rc = enqueue_final(j.plan, sl.dev, st, false)) return rc;
         const size_t rb = (size_t)j.plan.P * sizeof(ugsdr_acq_result);
         UGSDR_HIP_TRY(sl.results.reserve(rb));
         UGSDR_HIP_TRY(hipMemcpyAsync(sl.results.ptr, sl.dev.results.ptr, rb, hipMemcpyDeviceToHost, st));
@@ -261,7 +439,9 @@ static void scheduler_loop(Server* sp) {
         lk.unlock();
         std::vector<std::shared_ptr<Job>> failed;
         for (auto& rd : rounds) {
+            const auto t_launch = Clock::now();
             int rc = rd.done && rd.start ? launch_round(s, rd) : UGSDR_ERR_HIP;
+            s.launch_host_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t_launch).count();
             std::lock_guard<std::mutex> g(s.mu);
             if (rc) {
                 // the round's work may be partly queued: wait for the stream, then fail its jobs
@@ -371,6 +551,7 @@ int ugsdr_hip_server_create(ugsdr_hip_ctx* ctx, const ugsdr_server_config* cfg, 
     if (s.cfg.n_streams <= 0) s.cfg.n_streams = 4;
     if (s.cfg.max_batch <= 0 || s.cfg.max_batch > kMaxBatch) s.cfg.max_batch = kMaxBatch;
     if (s.cfg.round_budget_us > 0) s.budget_us = (double)s.cfg.round_budget_us;
+    s.graphs = s.cfg.round_graphs != 0;
     s.cus = device_cu_count(ctx->device);
     s.streams.resize(s.cfg.n_streams);
     for (auto& st : s.streams) {
@@ -383,7 +564,14 @@ int ugsdr_hip_server_create(ugsdr_hip_ctx* ctx, const ugsdr_server_config* cfg, 
             return UGSDR_ERR_HIP;
         }
     }
+    // without a capture stream the rounds run eagerly
+    if (s.graphs && hipStreamCreateWithFlags(&s.cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        s.cap_stream = nullptr;
+        s.graphs = false;
+    }
     s.scratch.resize(s.cfg.n_streams);
+    if (s.graphs) s.ginst = std::thread(instantiate_loop, &s);
     s.slots.resize(s.cfg.max_inflight);
     for (int i = s.cfg.max_inflight - 1; i >= 0; --i) s.free_slots.push_back(i);
     s.sched = std::thread(scheduler_loop, &s);
@@ -503,6 +691,11 @@ int ugsdr_hip_server_stats(ugsdr_hip_server* h, ugsdr_server_stats* out) {
     *out = h->s.st;
     out->inflight = (int64_t)(h->s.cfg.max_inflight - (int)h->s.free_slots.size());
     out->inflight_bytes = h->s.inflight_bytes;
+    out->graph_launches = h->s.graph_launches;
+    out->graph_instantiations = h->s.graph_instantiations;
+    out->graph_updates = h->s.graph_updates;
+    out->graph_fallbacks = h->s.graph_fallbacks;
+    out->launch_host_us = h->s.launch_host_ns / 1000;
     return UGSDR_OK;
 }
 
@@ -521,7 +714,19 @@ int ugsdr_hip_server_destroy(ugsdr_hip_server* h) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }
+    {
+        std::lock_guard<std::mutex> lk(s.gmu);
+        s.gstop = true;
+    }
+    s.gcv.notify_all();
+    if (s.ginst.joinable()) s.ginst.join();   // finishes the queued instantiations first
+    if (s.cap_stream) (void)hipStreamDestroy(s.cap_stream);
     for (auto e : s.event_pool) (void)hipEventDestroy(e);
+    for (auto& kv : s.gexec)
+        for (auto& e : kv.second) {
+            if (e.exec) (void)hipGraphExecDestroy(e.exec);
+            if (e.done) (void)hipEventDestroy(e.done);
+        }
     for (auto& sl : s.slots) sl.dev.release_all();
     for (auto& d : s.scratch) d.release();
     delete h;
