@@ -139,7 +139,7 @@ constexpr int kRedGP = 344;      // words per phase-1 output row in global memor
 // scratch words of one workgroup: max_k phase-1 slices of N / 341 rows,
 // then the FP32 surface
 __host__ __device__ constexpr int64_t red_wg_words(int64_t N, int max_k) {
-    return ((int64_t)max_k * (N / kRedN2) * kRedGP + N + 3) / 4 * 4;
+    return ((int64_t)max_k * (N / kRedN2) * kRedGP + 3) / 4 * 4;
 }
 
 // smallest v >= x with v % mod == rem
@@ -195,7 +195,10 @@ struct RedShape {
 // and the four-step twiddle are applied to FP32 stage outputs) into a
 // workgroup slice of global memory; phase 2 runs the row DFTs per batch of
 // rows over every block, the last stage accumulating |.| of its FP32
-// outputs in registers.
+// outputs in registers.  Each finished surface value goes straight into the
+// thread's share of the row statistics (top two with the first index of the
+// maximum, FP64 sum and sum of squares), so the surface itself is written
+// only when the caller asks for it.
 template <class H, int R1a, int R1b, int T, int GPP>
 __global__ __launch_bounds__(T, T / 256) void corr_mfma(CorrArgs a) {
     using L = RedShape<R1a, R1b>;
@@ -237,13 +240,16 @@ __global__ __launch_bounds__(T, T / 256) void corr_mfma(CorrArgs a) {
     }
     __syncthreads();
     // workgroup scratch: the phase-1 output of every block of a row (maxK
-    // slices of N1 rows k1 of GP words, columns n2 < N2 used), then the
-    // row's FP32 surface
+    // slices of N1 rows k1 of GP words, columns n2 < N2 used)
     uint32_t* Gall = reinterpret_cast<uint32_t*>(a.scratch) + (int64_t)blockIdx.x * red_wg_words(N, a.max_k);
-    float* S = reinterpret_cast<float*>(Gall + (int64_t)a.max_k * N1 * GP);
     for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
         const ReqDesc q = a.reqs[find_req(a, row)];
         const int local = row - q.row_start;
+        // the thread's share of the row statistics
+        float v1 = -1.0f, v2 = -1.0f;
+        int i1 = 0x7fffffff;
+        double sum = 0.0, sumsq = 0.0;
+        float* surf = q.surfaces ? q.surfaces + (int64_t)local * N : nullptr;
         const int p = local / q.D, di = local - p * q.D;
         const int cls = q.cls[di], sh = q.shift[di];
         const gptr<uint32_t> C = as_global(reinterpret_cast<const uint32_t*>(q.specs[p]));
@@ -389,19 +395,26 @@ __global__ __launch_bounds__(T, T / 256) void corr_mfma(CorrArgs a) {
                     }
                 }
             __syncthreads();
-            {
-                const gwptr<float> Sg = as_global_w(S + r0);
-                walk_cols<4>(
-                    opaque(tid), T, nb, N2, [&](int t, int k2) { return stage[k2 * L::LS + t + L2::skew(t)]; },
-                    [&](int t, int k2, float v) { Sg[t + N1 * k2] = v; });
-            }
+            walk_cols<4>(
+                opaque(tid), T, nb, N2, [&](int t, int k2) { return stage[k2 * L::LS + t + L2::skew(t)]; },
+                [&](int t, int k2, float v) {
+                    const int m = r0 + t + N1 * k2;
+                    top2_merge(v1, i1, v2, v, m, -1.0f);
+                    sum += (double)v;
+                    sumsq += (double)v * (double)v;
+                    if (surf) surf[m] = v;
+                });
             __syncthreads();
         }
-        row_stats_glob<T>(S, N, red, tid, q.rowstats + local, 1.0f);
-        if (q.surfaces) {
-            float* o = q.surfaces + (int64_t)local * N;
-            for (int j = tid; j < N; j += T) o[j] = S[j];
-        }
+        // sum of squared deviations from the FP64 sums: the surface is
+        // positive with a spread comparable to its mean, so the difference
+        // keeps all but a digit or two of FP64's precision
+        block_top2<T>(v1, i1, v2, red, tid);
+        sum = block_sum<T>(sum, red + 64, tid);
+        sumsq = block_sum<T>(sumsq, red + 64, tid);
+        const double mean = sum / (double)N;
+        const double dev = sumsq - sum * mean;
+        stats_write<T>(v1, i1, v2, mean, dev > 0.0 ? dev : 0.0, N, tid, q.rowstats + local, 1.0f);
         __syncthreads();
     }
 }
