@@ -327,11 +327,26 @@ __global__ __launch_bounds__(T, (T / 256 > 2 ? T / 256 : 2)) void corr_large(Cor
         const int p = local / q.D, di = local - p * q.D;
         const int cls = q.cls[di], sh = q.shift[di];
         const gptr<float2> C = as_global(q.specs[p]);
+        // the last block's surface values go straight into the thread's share
+        // of the row statistics (top two, FP64 sum and sum of squares) and,
+        // when asked for, the caller's surface: the row's running surface S
+        // is neither written for it nor read back
+        float v1 = -1.0f, v2 = -1.0f;
+        int i1 = 0x7fffffff;
+        double sum = 0.0, sumsq = 0.0;
+        float* surf = q.surfaces ? q.surfaces + (int64_t)local * L : nullptr;
+        const float inv_n = a.inv_n;
+        auto finish = [&](int64_t j, float val) {
+            top2_merge(v1, i1, v2, val, (int)j, -1.0f);
+            sum += (double)val;
+            sumsq += (double)val * (double)val;
+            if (surf) surf[j] = val * inv_n;
+        };
         for (int k = 0; k < q.K; ++k) {
             const int tq = opaque(tid);
             const gptr<float2> Xk = as_global(q.X + ((int64_t)k * q.F + cls) * N);
             const int fm = a.fold_mode;
-            const bool first = k == 0;
+            const bool first = k == 0, last = k == q.K - 1;
             fourstep<T, true, P1, P2>(
                 Ld.buf, a.fs, Ld.lo1, Ld.hi1, Ld.lo2, Ld.hi2, Ld.loN, Ld.hiN, G,
                 [&](int64_t n) {
@@ -343,9 +358,15 @@ __global__ __launch_bounds__(T, (T / 256 > 2 ? T / 256 : 2)) void corr_large(Cor
                 [&](int64_t kk, float2 v, float pre) {
                     // unfolded surface: |.| straight into the block sum
                     // (each output index is written by one thread)
-                    if (R == 1) S[kk] = pre + cabs_hw(v);
-                    else if (fm == 0) A[kk] = cabs_hw(v);
-                    else B[kk] = v;
+                    if (R == 1) {
+                        const float val = pre + cabs_hw(v);
+                        if (last) finish(kk, val);
+                        else S[kk] = val;
+                    } else if (fm == 0) {
+                        A[kk] = cabs_hw(v);
+                    } else {
+                        B[kk] = v;
+                    }
                 },
                 tq);
             if (R == 1) {
@@ -362,15 +383,20 @@ __global__ __launch_bounds__(T, (T / 256 > 2 ? T / 256 : 2)) void corr_large(Cor
                     for (int r = 1; r < R; ++r) z = cadd(z, B[j + (int64_t)r * L]);
                     acc = cabs_hw(z);
                 }
-                S[j] = (k == 0) ? acc : S[j] + acc;
+                const float val = first ? acc : S[j] + acc;
+                if (last) finish(j, val);
+                else S[j] = val;
             }
             __syncthreads();
         }
-        row_stats_glob<T>(S, L, Ld.red, tid, q.rowstats + local, a.inv_n);
-        if (q.surfaces) {
-            float* o = q.surfaces + (int64_t)local * L;
-            for (int j = tid; j < L; j += T) o[j] = S[j] * a.inv_n;
-        }
+        // sum of squared deviations from the FP64 sums (a positive surface
+        // whose spread is comparable to its mean: no precision to speak of lost)
+        block_top2<T>(v1, i1, v2, Ld.red, tid);
+        sum = block_sum<T>(sum, Ld.red + 64, tid);
+        sumsq = block_sum<T>(sumsq, Ld.red + 64, tid);
+        const double mean = sum / (double)L;
+        const double dev = sumsq - sum * mean;
+        stats_write<T>(v1, i1, v2, mean, dev > 0.0 ? dev : 0.0, L, tid, q.rowstats + local, a.inv_n);
         __syncthreads();
     }
 }
