@@ -89,6 +89,15 @@ int ugsdr_hip_set_autotune(ugsdr_hip_ctx* ctx, int enable);
  * of the eager path.  graph_replays counts the replays of this context. */
 int ugsdr_hip_set_graphs(ugsdr_hip_ctx* ctx, int enable);
 int64_t ugsdr_hip_graph_replays(ugsdr_hip_ctx* ctx);
+/* The form the last ugsdr_hip_acquire on this context took (no reference
+ * counterpart; the results do not depend on it): n_transform, the points of
+ * each inverse transform's row (n_fft, or the surface length of a
+ * periodic-fold search); periods, the code periods combined per block by the
+ * periodic fold (1: none); samples_per_chip, the upsampled-code search's
+ * factor (0: not taken; the row then runs as samples_per_chip inverse
+ * transforms of n_transform / samples_per_chip points on spectra shared by
+ * the PRNs of a Doppler bin).  Any pointer may be null. */
+int ugsdr_hip_last_search_form(ugsdr_hip_ctx* ctx, int64_t* n_transform, int32_t* periods, int32_t* samples_per_chip);
 /* The correlation variants built for a transform length, each named by its
  * workgroup size (a 1023*M length also has the three-stage kernel, named by
  * its own size: 192 at N = 4092, 512 at N = 8184, 12276 and 16368, where it

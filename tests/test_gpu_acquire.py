@@ -202,9 +202,10 @@ def test_runtime_plan_lengths(ctx, oracle, n_fft):
 def test_corr_variants_match_oracle(ctx, oracle, n_fft, surface_len, coherent):
     """every correlation variant of a C/A length (the three-stage 1023*M
     kernel and the generic LDS kernels) against the oracle on one input.
-    Folded cases store the replicas as samples: a replica put from chips of
-    several code periods takes the periodic-fold search instead (covered by
-    test_periodic_fold_matches_oracle), which never runs the fold kernels."""
+    The replicas are stored as samples: replicas put from chips take the
+    upsampled-code search (one code period) or the periodic fold (several),
+    covered by test_upsampled_code_matches_oracle and
+    test_periodic_fold_matches_oracle, which never run these variants."""
     rng = np.random.default_rng(n_fft + surface_len)
     fs = n_fft * 1000.0 * surface_len / n_fft
     P, dop = 3, U.doppler_bins(2000, 500)
@@ -212,7 +213,7 @@ def test_corr_variants_match_oracle(ctx, oracle, n_fft, surface_len, coherent):
     for _ in range(P):
         code = np.where(rng.random(1023) < 0.5, -1, 1).astype(np.int8)
         reps.append(oracle.repeat_upsample(code, n_fft // surface_len, n_fft))
-        keys.append(ctx.replica_put(code, 1, n_fft) if surface_len == n_fft else ctx.replica_put_samples(reps[-1]))
+        keys.append(ctx.replica_put_samples(reps[-1]))
     x = (rng.normal(0, 20, n_fft) + 1j * rng.normal(0, 20, n_fft)).astype(np.complex64)
     x += (2.0 * np.roll(reps[1], 777) * np.exp(2j * np.pi * 1000 * np.arange(n_fft) / fs)).astype(np.complex64)
     iq = np.ascontiguousarray(x).view(np.float32)
@@ -225,6 +226,7 @@ def test_corr_variants_match_oracle(ctx, oracle, n_fft, surface_len, coherent):
         ctx.set_threads(n_fft, t)
         gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft,
                                    n_blocks=1, surface_len=surface_len, doppler=dop, keys=keys, coherent=coherent)
+        assert ctx.last_search_form() == {"n_transform": n_fft, "periods": 1, "samples_per_chip": 0}
         assert_parity(gpu, ref, f"N={n_fft}/L={surface_len}/T={t}")
         assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
         assert gpu[1]["index"] == 777 % surface_len and gpu[1]["doppler"] == 1000.0
@@ -258,7 +260,10 @@ def test_periodic_fold_matches_oracle(ctx, oracle, n_fft, surface_len, n_blocks,
     kw = dict(want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft, n_blocks=n_blocks, surface_len=surface_len,
               doppler=dop, coherent=coherent)
     gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, keys=keys, **kw)
+    assert ctx.last_search_form() == {"n_transform": surface_len, "periods": T,
+                                      "samples_per_chip": surface_len // 1023}
     gs, ssurf, _ = ctx.acquire(iq, U.FMT_CF32, keys=skeys, **kw)
+    assert ctx.last_search_form() == {"n_transform": n_fft, "periods": 1, "samples_per_chip": 0}
     ref, rsurf, _ = oracle.acquire(x, reps, n_fft=n_fft, n_blocks=n_blocks, surface_len=surface_len, fs_in=fs,
                                    fs_acq=fs, doppler=dop, coherent=coherent, precision=64, want_surfaces=True)
     assert_parity(gpu, ref, f"periodic N={n_fft}/L={surface_len}")
@@ -266,6 +271,41 @@ def test_periodic_fold_matches_oracle(ctx, oracle, n_fft, surface_len, n_blocks,
     assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
     assert np.max(np.abs(ssurf - rsurf)) < 2e-5 * rsurf.max()
     assert gpu[1]["index"] == 1234 % surface_len and gpu[1]["doppler"] == 875.0
+    for k in keys + skeys:
+        ctx.replica_drop(k)
+
+
+@pytest.mark.parametrize("m,n_blocks", [(4, 1), (4, 3), (8, 2), (12, 1), (16, 1), (16, 3)])
+def test_upsampled_code_matches_oracle(ctx, oracle, m, n_blocks):
+    """replicas of one 1023-chip code period held for m samples per chip: the
+    row runs as m inverse 1023-point transforms of conj(C) Z_p, with Z formed
+    once per (Doppler bin, block) from the class spectra and the boxcar's
+    spectrum.  Against the oracle's N-point search and the same search on
+    sample replicas, on bins of every class fraction (250 Hz steps on 1 kHz
+    classes) and an odd Doppler span, surfaces included."""
+    n_fft = 1023 * m
+    fs = n_fft * 1000.0
+    rng = np.random.default_rng(1000 + m * 10 + n_blocks)
+    dop = U.doppler_bins(2750, 250)
+    codes, reps, x = planted_case(oracle, rng, n_fft, n_fft, 1023, fs, n_blocks=n_blocks, shift=517 * m // 4,
+                                  f_dop=750.0, n_prn=5)
+    say(f"case upsampled code N={n_fft} U={m} K={n_blocks}")
+    keys = [ctx.replica_put(c, 1, n_fft) for c in codes]
+    skeys = [ctx.replica_put_samples(r) for r in reps]
+    iq = np.ascontiguousarray(x).view(np.float32)
+    kw = dict(want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft, n_blocks=n_blocks, surface_len=n_fft,
+              doppler=dop)
+    gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, keys=keys, **kw)
+    assert ctx.last_search_form() == {"n_transform": n_fft, "periods": 1, "samples_per_chip": m}
+    gs, ssurf, _ = ctx.acquire(iq, U.FMT_CF32, keys=skeys, **kw)
+    assert ctx.last_search_form()["samples_per_chip"] == 0
+    ref, rsurf, _ = oracle.acquire(x, reps, n_fft=n_fft, n_blocks=n_blocks, surface_len=n_fft, fs_in=fs, fs_acq=fs,
+                                   doppler=dop, precision=64, want_surfaces=True)
+    assert_parity(gpu, ref, f"upsampled N={n_fft}")
+    assert_parity(gs, ref, f"samples N={n_fft}")
+    assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
+    # 1 ms coherent: the planted 750 Hz may peak in a neighbouring 250 Hz bin
+    assert gpu[1]["index"] == (517 * m // 4) % n_fft and abs(gpu[1]["doppler"] - 750.0) <= 250.0
     for k in keys + skeys:
         ctx.replica_drop(k)
 
@@ -317,19 +357,22 @@ def test_fourstep_and_recording_lengths_every_variant(ctx, oracle, n_fft, surfac
         ctx.replica_drop(k)
 
 
+@pytest.mark.parametrize("upsampled", [True, False])
 @pytest.mark.parametrize("n_prn,n_bins,n_blocks", [(3, 9, 3), (4, 70, 4)])
-def test_corr1023_tail_rows_split_over_workgroups(ctx, oracle, n_prn, n_bins, n_blocks):
+def test_corr1023_tail_rows_split_over_workgroups(ctx, oracle, n_prn, n_bins, n_blocks, upsampled):
     """N = 16368 with several blocks per row: whole rounds of rows run one row
     per workgroup, the rest are split block-wise over all workgroups and
-    merged (3 x 9 rows: every row split; 4 x 70 rows: one round + a tail)"""
+    merged (3 x 9 rows: every row split; 4 x 70 rows: one round + a tail).
+    upsampled: replicas put from chips (the upsampled-code search); else as
+    samples (the three-stage transform of the whole row)."""
     n_fft = 16368
     rng = np.random.default_rng(n_prn * 100 + n_bins)
     fs = 16.368e6
     reps, keys = [], []
     for _ in range(n_prn):
         code = np.where(rng.random(1023) < 0.5, -1, 1).astype(np.int8)
-        keys.append(ctx.replica_put(code, 1, n_fft))
         reps.append(oracle.repeat_upsample(code, 1, n_fft))
+        keys.append(ctx.replica_put(code, 1, n_fft) if upsampled else ctx.replica_put_samples(reps[-1]))
     n = n_fft * n_blocks
     x = (rng.normal(0, 20, n) + 1j * rng.normal(0, 20, n)).astype(np.complex64)
     x += (1.0 * np.tile(np.roll(reps[1], 4321), n_blocks) *
@@ -340,9 +383,10 @@ def test_corr1023_tail_rows_split_over_workgroups(ctx, oracle, n_prn, n_bins, n_
                                    fs_acq=fs, doppler=dop, precision=64, want_surfaces=True)
     t1023 = U.corr_variants(n_fft)[0]
     ctx.set_threads(n_fft, t1023)
-    say(f"case corr1023 tail split P={n_prn} D={n_bins} K={n_blocks}")
+    say(f"case corr1023 tail split P={n_prn} D={n_bins} K={n_blocks} upsampled={upsampled}")
     gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft,
                                n_blocks=n_blocks, surface_len=n_fft, doppler=dop, keys=keys)
+    assert ctx.last_search_form()["samples_per_chip"] == (16 if upsampled else 0)
     ctx.set_threads(n_fft, 0)
     assert_parity(gpu, ref, f"tail P={n_prn} D={n_bins} K={n_blocks}")
     assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()

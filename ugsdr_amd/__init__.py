@@ -178,6 +178,7 @@ SIGNATURES = {
     "ugsdr_hip_corr_variants": ([I64, VP, I32], C.c_int),
     "ugsdr_hip_set_graphs": ([VP, C.c_int], C.c_int),
     "ugsdr_hip_graph_replays": ([VP], I64),
+    "ugsdr_hip_last_search_form": ([VP, C.POINTER(I64), C.POINTER(I32), C.POINTER(I32)], C.c_int),
     "ugsdr_hip_set_threads": ([VP, I64, I32], C.c_int),
     "ugsdr_format_bytes": ([C.c_int, I64, D], I64),
     "ugsdr_hip_track": ([VP, VP, VP, VP, VP], C.c_int),
@@ -402,6 +403,13 @@ class Context:
 
     def graph_replays(self) -> int:
         return int(lib().ugsdr_hip_graph_replays(self._p))
+
+    def last_search_form(self) -> dict:
+        """form of the last acquire: per-row transform points, code periods
+        combined by the periodic fold, the upsampled-code factor (0: off)"""
+        n, t, u = C.c_int64(), C.c_int32(), C.c_int32()
+        _check(lib().ugsdr_hip_last_search_form(self._p, C.byref(n), C.byref(t), C.byref(u)))
+        return {"n_transform": n.value, "periods": t.value, "samples_per_chip": u.value}
 
     def set_threads(self, n_fft: int, threads: int):
         """pin the correlation variant for a length (0 unpins)"""

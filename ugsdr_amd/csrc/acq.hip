@@ -693,11 +693,36 @@ int64_t AcqPlan::footprint(bool surfaces) const {
                 P * (int64_t)sizeof(ugsdr_acq_result) + P * L * 4;
     if (Tsub > 1) b += K * Tsub * F * N * 8;
     if (!small) b += K * Tsub * F * N * 8;
+    if (ucode) b += D * K * N * 8;
     if (surfaces) b += P * D * L * 4;
     return b;
 }
 
-int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
+// conj of the U-sample boxcar's N-point spectrum, H[k] = sum_{j<U} e^(-j 2 pi jk/N)
+// = e^(-j pi (U-1) k/N) sin(pi U k/N) / sin(pi k/N), built in FP64 once per (N, U)
+static int boxcar_table(ugsdr_hip_ctx* ctx, int64_t N, int U, const float2** out) {
+    DevBuf& b = ctx->boxcar[N * 64 + U];
+    if (!b.ptr) {
+        std::vector<float2> h((size_t)N);
+        for (int64_t k = 0; k < N; ++k) {
+            double re = (double)U, im = 0.0;
+            if (k) {
+                const double mag = std::sin(M_PI * (double)U * (double)k / (double)N) / std::sin(M_PI * (double)k / (double)N);
+                const double ph = -M_PI * (double)(U - 1) * (double)k / (double)N;
+                re = mag * std::cos(ph);
+                im = mag * std::sin(ph);
+            }
+            h[(size_t)k] = make_float2((float)re, (float)-im);
+        }
+        UGSDR_HIP_TRY(b.reserve((size_t)N * sizeof(float2)));
+        UGSDR_HIP_TRY(hipMemcpyAsync(b.ptr, h.data(), (size_t)N * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
+        UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    *out = b.as<float2>();
+    return UGSDR_OK;
+}
+
+int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl, bool allow_ucode) {
     pl.precision = r ? r->precision : UGSDR_PREC_FP32;
     UGSDR_CHECK_ARG(r, "null request");
     UGSDR_CHECK_ARG(r->iq && r->n_samples > 0, "empty signal");
@@ -768,6 +793,33 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
         set_error("unsupported transform length " + std::to_string(pl.N));
         return UGSDR_ERR_UNSUPPORTED;
     }
+    // Upsampled-code search (FP32, unfolded surface, every replica one code
+    // period of Lc chips held for U = N / Lc samples each): the replica's
+    // spectrum is C[k mod Lc] H[k] with C the chips' Lc-point spectrum and H
+    // the U-sample boxcar's, so with k = k' + Lc u and tau = U q + p the
+    // row's inverse transform is, per phase p, the Lc-point inverse transform
+    // of conj(C) Z_p, where Z_p[k'] = e^(j 2 pi k'p / N) sum_u
+    // X_rot[k' + Lc u] conj(H[k' + Lc u]) e^(j 2 pi up / U) does not depend on
+    // the PRN: it is formed once per (Doppler bin, block) and shared by every
+    // row of that bin.
+    pl.ucode = 0;
+    pl.boxcar = nullptr;
+    if (allow_ucode && pl.precision == UGSDR_PREC_FP32 && pl.L == pl.N) {
+        bool ok = true;
+        int64_t Lc = 0;
+        for (int64_t p = 0; p < pl.P && ok; ++p) {
+            const Replica& rep = ctx->replicas.find(r->replica_keys[p])->second;
+            ok = rep.code_spec && (pl.Tsub > 1 || rep.repeats == 1) && pl.N % rep.code_len == 0 &&
+                 (p == 0 || rep.code_len == Lc);
+            Lc = rep.code_len;
+        }
+        if (ok && upsampled_code_supported(Lc, pl.N / Lc)) {
+            pl.ucode = (int)(pl.N / Lc);
+            for (int64_t p = 0; p < pl.P; ++p)
+                pl.specs[p] = static_cast<const float2*>(ctx->replicas.find(r->replica_keys[p])->second.code_spec);
+            if (int rc = boxcar_table(ctx, pl.N, pl.ucode, &pl.boxcar)) return rc;
+        }
+    }
     // front end: unpack + mix + resample
     const int64_t fs_in = (int64_t)std::llround(r->fs_in), fs_acq = (int64_t)std::llround(r->fs_acq);
     pl.up = pl.down = 1;
@@ -808,7 +860,9 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl) {
     pl.off_dop = pl.off_frac + pl.cls_frac.size() * 8;
     pl.off_cls = pl.off_dop + pl.D * 8;
     pl.off_shift = pl.off_cls + pl.D * 4;
-    pl.off_specs = (pl.off_shift + pl.D * 4 + 7) / 8 * 8;
+    pl.off_ucls = pl.off_shift + pl.D * 4;
+    pl.off_ushift = pl.off_ucls + (pl.ucode ? pl.D * 4 : 0);
+    pl.off_specs = (pl.off_ushift + (pl.ucode ? pl.D * 4 : 0) + 7) / 8 * 8;
     pl.table_bytes = pl.off_specs + pl.P * 8;
     return UGSDR_OK;
 }
@@ -823,6 +877,14 @@ int enqueue_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hip
     memcpy(hp + pl.off_dop, pl.doppler.data(), pl.D * 8);
     memcpy(hp + pl.off_cls, pl.cls.data(), pl.D * 4);
     memcpy(hp + pl.off_shift, pl.shift.data(), pl.D * 4);
+    if (pl.ucode) {
+        int* uc = reinterpret_cast<int*>(hp + pl.off_ucls);
+        int* us = reinterpret_cast<int*>(hp + pl.off_ushift);
+        for (int64_t d = 0; d < pl.D; ++d) {
+            uc[d] = (int)(d * pl.K);
+            us[d] = 0;
+        }
+    }
     memcpy(hp + pl.off_specs, pl.specs.data(), pl.P * 8);
     UGSDR_HIP_TRY(s.tables.reserve(pl.table_bytes));
     UGSDR_HIP_TRY(hipMemcpyAsync(s.tables.ptr, hp, pl.table_bytes, hipMemcpyHostToDevice, st));
@@ -907,7 +969,8 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     a.mode = 0;
     if (pl.Tsub == 1) {
         a.out = s.classes.as<float2>();
-        return launch_forward(a, (int)(pl.K * pl.F), st, s.fwd_scratch);
+        if (int rc = launch_forward(a, (int)(pl.K * pl.F), st, s.fwd_scratch)) return rc;
+        return pl.ucode ? launch_zprep(pl, s, st) : UGSDR_OK;
     }
     // every period sub-block's class spectra, then their combination per block
     UGSDR_HIP_TRY(s.classes_sub.reserve((size_t)pl.K * pl.Tsub * pl.F * pl.N * sizeof(float2)));
@@ -918,7 +981,7 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
                        s.classes_sub.as<float2>(), s.classes.as<float2>(), pl.N, pl.F, (int)pl.Tsub, chunks,
                        a.class_frac);
     UGSDR_HIP_TRY(hipGetLastError());
-    return UGSDR_OK;
+    return pl.ucode ? launch_zprep(pl, s, st) : UGSDR_OK;
 }
 
 int default_threads(int64_t N) {
@@ -977,14 +1040,14 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
         }
         char* dp = s.tables.as<char>();
         ReqDesc& q = reqs[i];
-        q.X = s.classes.as<float2>();
+        q.X = pl.ucode ? s.zbuf.as<float2>() : s.classes.as<float2>();
         q.specs = reinterpret_cast<const float2* const*>(dp + pl.off_specs);
-        q.cls = reinterpret_cast<const int*>(dp + pl.off_cls);
-        q.shift = reinterpret_cast<const int*>(dp + pl.off_shift);
+        q.cls = reinterpret_cast<const int*>(dp + (pl.ucode ? pl.off_ucls : pl.off_cls));
+        q.shift = reinterpret_cast<const int*>(dp + (pl.ucode ? pl.off_ushift : pl.off_shift));
         q.rowstats = s.rows.as<RowStat>();
         q.surfaces = s.surf.ptr ? s.surf.as<float>() : nullptr;
         q.K = (int)pl.K;
-        q.F = pl.F;
+        q.F = pl.ucode ? 1 : pl.F;
         q.D = (int)pl.D;
         q.P = (int)pl.P;
         q.row_start = rows;
@@ -992,7 +1055,14 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
         maxK = std::max(maxK, (int)pl.K);
     }
     const int T = threads > 0 ? threads : default_threads(p0.N);
-    const bool three_stage = p0.small && T == corr1023_threads(p0.N) && corr1023_supports(p0.N, p0.L);
+    // the upsampled-code search runs on the three-stage kernel whatever
+    // variant the length is tuned to
+    const bool three_stage =
+        p0.small && (p0.ucode || T == corr1023_threads(p0.N)) && corr1023_supports(p0.N, p0.L);
+    if (p0.ucode && !three_stage) {
+        set_error("upsampled-code search without a three-stage kernel for its length");
+        return UGSDR_ERR_UNSUPPORTED;
+    }
     // three-stage kernel: whole rounds of rows one per workgroup, the rest
     // (when rows have several blocks) split block-wise as the batch tail
     int full_rows = rows;
@@ -1018,6 +1088,7 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
     c.inv_n = (float)(1.0 / (double)p0.N);
     c.full_rows = full_rows;
     c.units = units;
+    c.ucode = p0.ucode;
     int rc = UGSDR_OK;
     if (three_stage) {
         rc = launch_corr1023(c, cus, scratch, st);
@@ -1090,6 +1161,7 @@ int presize_slot(const AcqPlan& pl, AcqSlot& s, bool peaks) {
     UGSDR_HIP_TRY(s.classes.reserve(kfn * sizeof(float2)));
     if (pl.Tsub > 1) UGSDR_HIP_TRY(s.classes_sub.reserve(kfn * pl.Tsub * sizeof(float2)));
     if (!pl.small) UGSDR_HIP_TRY(s.fwd_scratch.reserve(kfn * pl.Tsub * sizeof(float2)));
+    if (pl.ucode) UGSDR_HIP_TRY(s.zbuf.reserve((size_t)pl.D * pl.K * pl.N * sizeof(float2)));
     if (pl.precision != UGSDR_PREC_FP32) {
         UGSDR_HIP_TRY(s.classes_r.reserve(kfn * 4));
         UGSDR_HIP_TRY(s.block_scale.reserve((size_t)pl.K * pl.F * 8));
@@ -1276,6 +1348,9 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
     if (r && r->precision != UGSDR_PREC_FP32) return acquire_reduced(ctx, r, out, surfaces, peaks);
     AcqPlan pl;
     if (int rc = make_acq_plan(ctx, r, pl)) return rc;
+    ctx->last_n = pl.N;
+    ctx->last_periods = (int)pl.Tsub;
+    ctx->last_ucode = pl.ucode;
     AcqSlot& s = ctx->slot;
     const bool keep_surf = surfaces || peaks;
     hipStream_t st = ctx->stream;
@@ -1430,6 +1505,19 @@ extern "C" int ugsdr_hip_set_graphs(ugsdr_hip_ctx* ctx, int enable) {
 }
 
 extern "C" int64_t ugsdr_hip_graph_replays(ugsdr_hip_ctx* ctx) { return ctx ? ctx->graph.replays : -1; }
+
+extern "C" int ugsdr_hip_last_search_form(ugsdr_hip_ctx* ctx, int64_t* n_transform, int32_t* periods,
+                                          int32_t* samples_per_chip) {
+    if (!ctx) {
+        ugsdr_hip::set_error("null context");
+        return UGSDR_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (n_transform) *n_transform = ctx->last_n;
+    if (periods) *periods = ctx->last_periods;
+    if (samples_per_chip) *samples_per_chip = ctx->last_ucode;
+    return UGSDR_OK;
+}
 
 extern "C" int ugsdr_hip_set_autotune(ugsdr_hip_ctx* ctx, int enable) {
     if (!ctx) {

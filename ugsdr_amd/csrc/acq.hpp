@@ -19,6 +19,12 @@ struct AcqPlan {
     // samples (replicas of Tsub periods folded to one), searched as an
     // N-length transform of the periods' coherently combined spectra
     int64_t Tsub = 1;
+    // upsampled-code search: every replica is one code period of N / ucode
+    // chips held for ucode samples each, so the correlation runs as ucode
+    // inverse transforms of N / ucode points per row on the PRN-independent
+    // spectra Z of each (Doppler bin, block) (0: off)
+    int ucode = 0;
+    const float2* boxcar = nullptr;   // conj of the ucode-sample boxcar's N-point spectrum
     int F = 0;
     int fold_mode = 0;        // 0: |.| then fold, 1: fold complex then |.|
     int reshape = 1;
@@ -42,17 +48,24 @@ struct AcqPlan {
     FourStep fs{};
     // table layout (bytes) in the request's table buffer
     size_t off_frac = 0, off_dop = 0, off_cls = 0, off_shift = 0, off_specs = 0, table_bytes = 0;
+    // upsampled-code search: the correlation's view of Z (bin d's spectra at
+    // block k are "class" d * K with no rotation)
+    size_t off_ucls = 0, off_ushift = 0;
     // device bytes the request holds while in flight (admission)
     int64_t footprint(bool surfaces) const;
     // batch key: requests sharing it run in one correlation launch
-    int64_t bucket() const { return (N << 6) | ((int64_t)precision << 3) | (fold_mode << 1) | (L == N ? 1 : 0); }
+    int64_t bucket() const {
+        return (N << 6) | ((int64_t)precision << 3) | (ucode ? 4 : 0) | (fold_mode << 1) | (L == N ? 1 : 0);
+    }
     // reduced precision: the request's storage type and its replicas' scale
     int precision = UGSDR_PREC_FP32;
     float spec_scale = 1.0f;
 };
 
 // Validates the request, resolves replica keys (caller holds ctx->mu).
-int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl);
+// allow_ucode: the upsampled-code search may be chosen (it allocates the
+// context's boxcar table on first use of a length).
+int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl, bool allow_ucode = true);
 // Uploads tables and IQ and runs the front end (unpack, mix, resample).
 // Upload only queues the copies and sizes the buffers; Launch only runs the
 // kernel (for graph capture, where the copies stay outside the graph).
@@ -94,6 +107,9 @@ bool corr1023_supports(int64_t N, int64_t L);
 struct CorrArgs;
 int corr1023_grid(int64_t N, int cus);
 int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st);
+// upsampled-code search: Z of every (Doppler bin, block) of a request from its
+// class spectra (pl.ucode > 0; s.zbuf sized D K N complex)
+int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
 // Reduced precision (16-bit storage, matrix-core stages): whether a length
 // pair has a plan, the 16-bit packing of the class spectra, and one
 // correlation launch over a batch of requests of one bucket.

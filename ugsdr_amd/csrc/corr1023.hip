@@ -26,6 +26,7 @@
 #include "corr_common.hpp"
 #include "ctx.hpp"
 #include "dev_fft.hpp"
+#include "internal.hpp"
 
 using namespace ugsdr_dev;
 
@@ -126,10 +127,16 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
 
 // Blocks k0 .. k1-1 of one row into the thread's surface samples
 // S[s][u] = surface[tid + s T + u 1023].
-template <int M, int QL>
-__device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT][QL], const C1023Lds<M>& Ld,
-                                                const ReqDesc& q, int cls, int sh, gptr<float2> C, int k0, int k1,
-                                                int fold_mode, int tid) {
+//
+// UP (upsampled-code search): X holds the row's PRN-independent spectra Z
+// (phase p of frequency k' at k' M + p) and C the code's 1023-point conj
+// spectrum, input i is Z[i] conj(C)[i / M], and stages 1 and 2 alone give
+// the M phases' 1023-point transforms: stage-2 output qq of butterfly j is
+// sample tau = M (j % 31 + 31 qq) + j / 31 of the row, accumulated straight
+// into S[qq] (no stage 3, two barriers per block).
+template <int M, int QL, bool UP, class SA>
+__device__ __forceinline__ void corr1023_blocks(SA& S, const C1023Lds<M>& Ld, const ReqDesc& q, int cls, int sh,
+                                                gptr<float2> C, int k0, int k1, int fold_mode, int tid) {
     using S_ = C1023Shape<M>;
     constexpr int N = S_::N, T = S_::T, B1 = S_::B1, B2 = S_::B2, NSLOT = S_::NSLOT;
     constexpr int FOLD = M / QL;
@@ -155,9 +162,9 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
             s1 -= s1 >= (unsigned)N ? N : 0;
             s2 -= s2 >= (unsigned)N ? N : 0;
             x[2 * st] = ld_off(Xb, s1 * 8u);
-            c[2 * st] = ld_off(C, i1 * 8u);
+            c[2 * st] = ld_off(C, (UP ? i1 / (unsigned)M : i1) * 8u);
             x[2 * st + 1] = ld_off(Xb, s2 * 8u);
-            c[2 * st + 1] = ld_off(C, i2 * 8u);
+            c[2 * st + 1] = ld_off(C, (UP ? i2 / (unsigned)M : i2) * 8u);
         }
     };
     // with PREFETCH, group 0 of the next block is gathered during this
@@ -246,14 +253,24 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
                 u[r] = cmul(u[r], tw2[tb + r]);
             }
             Pfa33<true>::prep(u);
-            __syncthreads();
-            if (act) {
-                float2* dst = buf + (j - kk) * 33 + kk;
-                Pfa33<true>::store(u, [&](int qq, float2 x) { dst[qq * 31] = x; });
+            if constexpr (UP) {
+                // the next block's first gathers fly during this block's outputs
+                if constexpr (S_::PREFETCH) {
+                    if (k + 1 < k1) gather(k + 1, 0, xv[0], cv[0]);
+                }
+                if (act) Pfa33<true>::store(u, [&](int qq, float2 x) { S[qq] += cabs_hw(x); });
+                continue;   // the next block's stage 1 writes LDS only after its barrier
+            } else {
+                __syncthreads();
+                if (act) {
+                    float2* dst = buf + (j - kk) * 33 + kk;
+                    Pfa33<true>::store(u, [&](int qq, float2 x) { dst[qq * 31] = x; });
+                }
             }
         }
         __syncthreads();
         // ---- stage 3: radix M, Ns = 1023, accumulate |.| per owned output
+        if constexpr (!UP) {
         if constexpr (S_::PREFETCH) {
             if (k + 1 < k1) gather(k + 1, 0, xv[0], cv[0]);
         }
@@ -292,8 +309,52 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
                 }
             }
         }
+        }
     }
+}
 
+// upsampled-code search: row sample of the thread's stage-2 output qq
+template <int M>
+__device__ __forceinline__ int up_tau(int tid, int qq) {
+    return M * (tid % 31 + 31 * qq) + tid / 31;
+}
+
+// upsampled-code search: the thread's 33 surface samples (stage-2 butterfly
+// tid < 31 M) -> row statistics (and the surface)
+template <int M>
+__device__ __forceinline__ void corr1023u_stats(const float (&S)[33], double* red, const ReqDesc& q, int local,
+                                                float inv_n, int tid) {
+    using S_ = C1023Shape<M>;
+    constexpr int T = S_::T, B2 = S_::B2, L = kC1023 * M;
+    const bool act = tid < B2;
+    float v1 = -1.0f, v2 = -1.0f;
+    int i1 = 0x7fffffff;
+    double sum = 0.0;
+    if (act) {
+#pragma unroll
+        for (int qq = 0; qq < 33; ++qq) {
+            top2_merge(v1, i1, v2, S[qq], up_tau<M>(tid, qq), -1.0f);
+            sum += (double)S[qq];
+        }
+    }
+    block_top2<T>(v1, i1, v2, red, tid);
+    sum = block_sum<T>(sum, red + 64, tid);
+    const double mean = sum / (double)L;
+    double qv = 0.0;
+    if (act) {
+#pragma unroll
+        for (int qq = 0; qq < 33; ++qq) {
+            const double d = (double)S[qq] - mean;
+            qv += d * d;
+        }
+    }
+    qv = block_sum<T>(qv, red + 64, tid);
+    stats_write<T>(v1, i1, v2, mean, qv, L, tid, q.rowstats + local, inv_n);
+    if (q.surfaces && act) {
+        float* o = q.surfaces + (int64_t)local * L;
+#pragma unroll
+        for (int qq = 0; qq < 33; ++qq) o[up_tau<M>(tid, qq)] = S[qq] * inv_n;
+    }
 }
 
 // surface samples of a finished row -> row statistics (and the surface)
@@ -392,11 +453,12 @@ __device__ __forceinline__ void corr1023_merge_row(const CorrArgs& a, int tail, 
 // a.n_merge workgroups instead finish the tail rows from the stream-K
 // partial surfaces (launched before this kernel), so that short work runs
 // beside the first round of rows rather than after the last.
-template <int M, int QL>
+template <int M, int QL, bool UP>
 __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void corr1023(CorrArgs a) {
     using S_ = C1023Shape<M>;
     constexpr int NSLOT = S_::NSLOT;
     static_assert(M % QL == 0, "surface length must divide the transform");
+    static_assert(!UP || QL == M, "the upsampled-code search takes no fold");
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     if ((int)blockIdx.x < a.n_merge) {
@@ -410,13 +472,22 @@ __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void
     const ReqDesc q = a.reqs[find_req(a, row)];
     const int local = row - q.row_start;
     const int p = local / q.D, di = local - p * q.D;
-    float S[NSLOT][QL];
+    if constexpr (UP) {
+        float S[33];
 #pragma unroll
-    for (int s = 0; s < NSLOT; ++s)
+        for (int i = 0; i < 33; ++i) S[i] = 0.f;
+        corr1023_blocks<M, QL, true>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), 0, q.K, 0, tid);
+        corr1023u_stats<M>(S, Ld.red, q, local, a.inv_n, tid);
+    } else {
+        float S[NSLOT][QL];
 #pragma unroll
-        for (int u = 0; u < QL; ++u) S[s][u] = 0.f;
-    corr1023_blocks<M, QL>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), 0, q.K, a.fold_mode, tid);
-    corr1023_stats<M, QL>(S, Ld.red, q, local, a.inv_n, tid);
+        for (int s = 0; s < NSLOT; ++s)
+#pragma unroll
+            for (int u = 0; u < QL; ++u) S[s][u] = 0.f;
+        corr1023_blocks<M, QL, false>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), 0, q.K, a.fold_mode,
+                                      tid);
+        corr1023_stats<M, QL>(S, Ld.red, q, local, a.inv_n, tid);
+    }
 }
 
 // Stream-K over the tail rows (fewer than one round of workgroups): the
@@ -426,7 +497,7 @@ __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void
 // a row cut by the share's start or end goes to share slot 0 or 1 (only the
 // first and the last row of a share can be cut).  corr1023_merge finishes
 // the rows.
-template <int M, int QL>
+template <int M, int QL, bool UP>
 __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void corr1023_sk(CorrArgs a) {
     using S_ = C1023Shape<M>;
     constexpr int T = S_::T, NSLOT = S_::NSLOT, L = kC1023 * QL;
@@ -441,22 +512,34 @@ __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void
         const int local = q.tail_row0 + lu / q.K, k0 = lu % q.K;
         const int k1 = min(q.K, k0 + (u_end - u));
         const int p = local / q.D, di = local - p * q.D;
-        float S[NSLOT][QL];
-#pragma unroll
-        for (int s = 0; s < NSLOT; ++s)
-#pragma unroll
-            for (int v = 0; v < QL; ++v) S[s][v] = 0.f;
-        corr1023_blocks<M, QL>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), k0, k1, a.fold_mode, tid);
         float* part = reinterpret_cast<float*>(a.scratch);
         const bool whole = k0 == 0 && k1 == q.K;
         float* dst = whole ? part + ((int64_t)2 * gridDim.x + (q.row_start + local - a.full_rows)) * L
                            : part + ((int64_t)w * 2 + (u == u_begin ? 0 : 1)) * L;
+        if constexpr (UP) {
+            float S[33];
 #pragma unroll
-        for (int s = 0; s < NSLOT; ++s) {
-            const int j = tid + s * T;
-            if (j < kC1023) {
+            for (int i = 0; i < 33; ++i) S[i] = 0.f;
+            corr1023_blocks<M, QL, true>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), k0, k1, 0, tid);
+            if (tid < S_::B2) {
 #pragma unroll
-                for (int v = 0; v < QL; ++v) dst[j + v * kC1023] = S[s][v];
+                for (int qq = 0; qq < 33; ++qq) dst[up_tau<M>(tid, qq)] = S[qq];
+            }
+        } else {
+            float S[NSLOT][QL];
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s)
+#pragma unroll
+                for (int v = 0; v < QL; ++v) S[s][v] = 0.f;
+            corr1023_blocks<M, QL, false>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), k0, k1,
+                                          a.fold_mode, tid);
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s) {
+                const int j = tid + s * T;
+                if (j < kC1023) {
+#pragma unroll
+                    for (int v = 0; v < QL; ++v) dst[j + v * kC1023] = S[s][v];
+                }
             }
         }
         u += k1 - k0;
@@ -468,17 +551,17 @@ static hipError_t lds_attr(K kernel, size_t bytes) {
     return set_lds_limit(reinterpret_cast<const void*>(kernel), bytes);
 }
 
-template <int M, int QL>
+template <int M, int QL, bool UP = false>
 static int launch_1023(const CorrArgs& c, hipStream_t st) {
     const size_t lds = C1023Shape<M>::lds_bytes;
     constexpr int T = C1023Shape<M>::T;
     if (c.sk_grid > 0) {
-        UGSDR_HIP_TRY(lds_attr(corr1023_sk<M, QL>, lds));
-        hipLaunchKernelGGL((corr1023_sk<M, QL>), dim3(c.sk_grid), dim3(T), lds, st, c);
+        UGSDR_HIP_TRY(lds_attr(corr1023_sk<M, QL, UP>, lds));
+        hipLaunchKernelGGL((corr1023_sk<M, QL, UP>), dim3(c.sk_grid), dim3(T), lds, st, c);
     }
     if (c.full_rows + c.n_merge > 0) {
-        UGSDR_HIP_TRY(lds_attr(corr1023<M, QL>, lds));
-        hipLaunchKernelGGL((corr1023<M, QL>), dim3(c.full_rows + c.n_merge), dim3(T), lds, st, c);
+        UGSDR_HIP_TRY(lds_attr(corr1023<M, QL, UP>, lds));
+        hipLaunchKernelGGL((corr1023<M, QL, UP>), dim3(c.full_rows + c.n_merge), dim3(T), lds, st, c);
     }
     return UGSDR_OK;
 }
@@ -530,6 +613,18 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
         UGSDR_HIP_TRY(scratch.reserve(((size_t)G_sk * 2 + tail_rows) * c.L * 4));
         a.scratch = scratch.as<float2>();
     }
+    if (c.ucode) {
+        if (c.ucode != M || QL != M) {
+            set_error("corr1023: upsampled-code search of another shape");
+            return UGSDR_ERR_UNSUPPORTED;
+        }
+        switch (M) {
+            case 4: return launch_1023<4, 4, true>(a, st);
+            case 8: return launch_1023<8, 8, true>(a, st);
+            case 12: return launch_1023<12, 12, true>(a, st);
+            default: return launch_1023<16, 16, true>(a, st);
+        }
+    }
     switch (M * 100 + QL) {
         case 404: return launch_1023<4, 4>(a, st);
         case 808: return launch_1023<8, 8>(a, st);
@@ -540,6 +635,76 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
         case 1604: return launch_1023<16, 4>(a, st);
         default: set_error("corr1023: unsupported length pair"); return UGSDR_ERR_UNSUPPORTED;
     }
+}
+
+bool upsampled_code_supported(int64_t code_len, int64_t U) {
+    return code_len == kC1023 && (U == 4 || U == 8 || U == 12 || U == 16);
+}
+
+// Upsampled-code search, the PRN-independent part: for Doppler bin d and block
+// k of a request, Z[(d K + k) N + k' U + p] = e^(j 2 pi k'p / N) sum_u
+// X_c[(k' + Lc u + m) mod N] conj(H[k' + Lc u]) e^(j 2 pi up / U), with X_c
+// the block's spectrum of bin d's class c and m its integer rotation.  One
+// thread per k' (the U-point inverse DFT in registers); the twiddle powers
+// from e^(j 2 pi k' / N) by the same product tree as the three-stage
+// kernel's last stage.
+template <int U>
+__global__ __launch_bounds__(256) void zprep_kernel(const float2* X, const float2* conjH, const int* cls,
+                                                   const int* shift, int N, int K, int F, int chunks, float2* Z) {
+    constexpr int Lc = kC1023;
+    const int dk = blockIdx.x / chunks, chunk = blockIdx.x - dk * chunks;
+    const int d = dk / K, k = dk - d * K;
+    const int kp = chunk * 256 + threadIdx.x;
+    if (kp >= Lc) return;
+    const float2* Xb = X + ((int64_t)k * F + cls[d]) * N;
+    const int m = shift[d];
+    float2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int n = kp + Lc * u;
+        int src = n + m;
+        src -= src >= N ? N : 0;
+        v[u] = cmul(Xb[src], conjH[n]);
+    }
+    Dft<U, true>::run(v);
+    double sn, cs;
+    sincospi(2.0 * (double)kp / (double)N, &sn, &cs);
+    float2 tw[U];
+    twiddle_powers<U>(make_float2((float)cs, (float)sn), tw);
+    float2* z = Z + (int64_t)dk * N + (int64_t)kp * U;
+#pragma unroll
+    for (int p = 0; p < U; ++p) z[p] = cmul(v[p], tw[p]);
+}
+
+int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
+    UGSDR_HIP_TRY(s.zbuf.reserve((size_t)pl.D * pl.K * pl.N * sizeof(float2)));
+    const char* tp = s.tables.as<char>();
+    const int* cls = reinterpret_cast<const int*>(tp + pl.off_cls);
+    const int* shift = reinterpret_cast<const int*>(tp + pl.off_shift);
+    const int chunks = (kC1023 + 255) / 256;
+    const dim3 grid((unsigned)(pl.D * pl.K * chunks));
+    const int N = (int)pl.N, K = (int)pl.K;
+    switch (pl.ucode) {
+        case 4:
+            hipLaunchKernelGGL(zprep_kernel<4>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
+                               N, K, pl.F, chunks, s.zbuf.as<float2>());
+            break;
+        case 8:
+            hipLaunchKernelGGL(zprep_kernel<8>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
+                               N, K, pl.F, chunks, s.zbuf.as<float2>());
+            break;
+        case 12:
+            hipLaunchKernelGGL(zprep_kernel<12>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls,
+                               shift, N, K, pl.F, chunks, s.zbuf.as<float2>());
+            break;
+        case 16:
+            hipLaunchKernelGGL(zprep_kernel<16>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls,
+                               shift, N, K, pl.F, chunks, s.zbuf.as<float2>());
+            break;
+        default: set_error("upsampled-code search of an unsupported factor"); return UGSDR_ERR_UNSUPPORTED;
+    }
+    UGSDR_HIP_TRY(hipGetLastError());
+    return UGSDR_OK;
 }
 
 }  // namespace ugsdr_hip

@@ -57,6 +57,7 @@ struct CorrArgs {
     int max_k;              // reduced path: most blocks of any request
     int sk_grid;            // three-stage kernel: workgroups of the tail split
     int n_merge;            // three-stage kernel: tail rows to merge
+    int ucode;              // upsampled-code search: samples per chip (0: off)
 };
 
 // the request owning a batch row (uniform per workgroup: kept in scalar registers)

@@ -80,6 +80,13 @@ struct Replica {
     // period, for the periodic-fold search (otherwise 0 / null)
     int64_t period = 0;
     void* spec_short = nullptr;
+    // the replica is one code period of code_len chips upsampled by an
+    // integer factor (repeats == 1, or one period of a periodic replica):
+    // conj(DFT) of the code_len chips themselves, for the upsampled-code
+    // search (otherwise null)
+    int64_t repeats = 1;
+    void* code_spec = nullptr;
+    int64_t code_len = 0;
 };
 
 // device buffers one acquisition request holds while in flight
@@ -87,18 +94,20 @@ struct AcqSlot {
     DevBuf iq, sig, classes, tables, rows, results, surf, fwd_scratch;
     // periodic-fold searches: the class spectra of every code-period sub-block
     DevBuf classes_sub;
+    // upsampled-code searches: the PRN-independent spectra Z per (Doppler bin, block)
+    DevBuf zbuf;
     // reduced precision: 16-bit class spectra and their per-block scales
     DevBuf classes_r, block_scale;
     std::vector<char> host_tables;   // kept alive until the copies complete
     void release_all() {
         for (DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch, &classes_r, &block_scale,
-                          &classes_sub})
+                          &classes_sub, &zbuf})
             d->release();
     }
     int64_t bytes() const {
         int64_t b = 0;
         for (const DevBuf* d : {&iq, &sig, &classes, &tables, &rows, &results, &surf, &fwd_scratch, &classes_r,
-                                &block_scale, &classes_sub})
+                                &block_scale, &classes_sub, &zbuf})
             b += (int64_t)d->bytes;
         return b;
     }
@@ -136,6 +145,12 @@ struct ugsdr_hip_ctx {
         int64_t spm = 0, period = 0, len3p = 0;
         int n_ch = 0;
     } track;
+    // upsampled-code searches: conj of the U-sample boxcar's N-point
+    // spectrum, per (N, U) (key N * 64 + U)
+    std::map<int64_t, ugsdr_hip::DevBuf> boxcar;
+    // the form of the last ugsdr_hip_acquire (ugsdr_hip_last_search_form)
+    int64_t last_n = 0;
+    int last_periods = 0, last_ucode = 0;
     // autotuner: transform length -> correlation workgroup size
     std::map<int64_t, int> tuned;
     bool autotune = true;

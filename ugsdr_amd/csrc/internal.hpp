@@ -15,6 +15,9 @@ int dft_device(ugsdr_hip_ctx* ctx, float2* d, float2* tmp, int64_t n, int invers
 // conj(DFT(replica)) of length n_fft into out (device), from chips or samples
 int replica_spectrum(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chips, int64_t repeats, const float* samples,
                      int64_t n_fft, float2* out);
+// whether the upsampled-code search handles a code of code_len chips held
+// for U samples each (corr1023.hip)
+bool upsampled_code_supported(int64_t code_len, int64_t U);
 // resolves the no-alloc workspace's device address on `device` (current),
 // once per device (noalloc.hip)
 int noalloc_scratch_init(int device);

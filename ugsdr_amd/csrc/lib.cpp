@@ -45,7 +45,8 @@ int64_t ugsdr_hip_ctx::device_bytes() const {
     b += (int64_t)track.buf.bytes;
     for (const auto& kv : replicas)
         b += kv.second.n_fft * (kv.second.precision == UGSDR_PREC_FP32 ? 8 : 4) +
-             (kv.second.spec_short ? kv.second.period * 8 : 0);
+             (kv.second.spec_short ? kv.second.period * 8 : 0) + kv.second.code_len * 8;
+    for (const auto& kv : boxcar) b += (int64_t)kv.second.bytes;
     return b;
 }
 
@@ -90,7 +91,9 @@ int ugsdr_hip_ctx_destroy(ugsdr_hip_ctx* ctx) {
         for (auto& kv : ctx->replicas) {
             if (kv.second.spec) (void)hipFree(kv.second.spec);
             if (kv.second.spec_short) (void)hipFree(kv.second.spec_short);
+            if (kv.second.code_spec) (void)hipFree(kv.second.code_spec);
         }
+        for (auto& kv : ctx->boxcar) kv.second.release();
         for (auto& e : ctx->ev)
             if (e) (void)hipEventDestroy(e);
         ctx->replicas.clear();
@@ -279,6 +282,7 @@ static int replica_store(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chip
     Replica rep;
     rep.n_fft = n_fft;
     rep.precision = UGSDR_PREC_FP32;
+    rep.repeats = chips ? repeats : 0;
     UGSDR_HIP_TRY(hipMalloc(&rep.spec, (size_t)n_fft * sizeof(float2)));
     int rc = replica_spectrum(ctx, chips, n_chips, repeats, samples, n_fft, static_cast<float2*>(rep.spec));
     if (rc) {
@@ -304,6 +308,21 @@ static int replica_store(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chip
             return rc;
         }
         rep.period = L;
+    }
+    if (precision == UGSDR_PREC_FP32 && chips && n_fft % repeats == 0 && (n_fft / repeats) % n_chips == 0 &&
+        upsampled_code_supported(n_chips, n_fft / repeats / n_chips)) {
+        // one period upsampled U times: the chips' own spectrum for the
+        // upsampled-code search
+        rc = hipMalloc(&rep.code_spec, (size_t)n_chips * sizeof(float2)) == hipSuccess
+                 ? replica_spectrum(ctx, chips, n_chips, 1, nullptr, n_chips, static_cast<float2*>(rep.code_spec))
+                 : (set_error("hipMalloc failed"), UGSDR_ERR_HIP);
+        if (rc) {
+            (void)hipFree(rep.spec);
+            if (rep.spec_short) (void)hipFree(rep.spec_short);
+            if (rep.code_spec) (void)hipFree(rep.code_spec);
+            return rc;
+        }
+        rep.code_len = n_chips;
     }
     *key = ctx->next_key++;
     ctx->replicas[*key] = rep;
@@ -335,6 +354,7 @@ int ugsdr_hip_replica_drop(ugsdr_hip_ctx* ctx, uint64_t key) {
     UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
     UGSDR_HIP_TRY(hipFree(it->second.spec));
     if (it->second.spec_short) UGSDR_HIP_TRY(hipFree(it->second.spec_short));
+    if (it->second.code_spec) UGSDR_HIP_TRY(hipFree(it->second.code_spec));
     ctx->replicas.erase(it);
     return UGSDR_OK;
 }

@@ -84,7 +84,9 @@ static int acquire_noalloc(ugsdr_hip_ctx* ctx, char* base, const ugsdr_acq_reque
     const int H = r->hypotheses;
     UGSDR_CHECK_ARG(H <= 1 || (r->n_prn > 0 && r->n_prn % H == 0), "n_prn must be a multiple of the hypothesis count");
     AcqPlan pl;
-    if (int rc = make_acq_plan(ctx, r, pl)) return rc;
+    // the upsampled-code search's boxcar table is allocated on first use:
+    // this entry point runs the other search forms only
+    if (int rc = make_acq_plan(ctx, r, pl, false)) return rc;
     const bool keep_surf = surfaces || peaks;
     const int cus = device_cu_count(ctx->device);
     // the buffers ugsdr_hip_acquire's slot would hold, in the sizes its
