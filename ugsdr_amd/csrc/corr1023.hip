@@ -442,9 +442,10 @@ __device__ __forceinline__ void corr1023_merge_row(const CorrArgs& a, int tail, 
         }
     }
     __syncthreads();
-    row_stats_glob<T>(S, L, red, tid, q.rowstats + local, a.inv_n);
+    const int slot = row_ref(q, local, a.ucode != 0).slot;
+    row_stats_glob<T>(S, L, red, tid, q.rowstats + slot, a.inv_n);
     if (q.surfaces) {
-        float* o = q.surfaces + (int64_t)local * L;
+        float* o = q.surfaces + (int64_t)slot * L;
         for (int j = tid; j < L; j += T) o[j] = S[j] * a.inv_n;
     }
 }
@@ -468,17 +469,21 @@ __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void
         return;
     }
     const C1023Lds<M> Ld(smem, tid);
-    const int row = blockIdx.x - a.n_merge;
-    const ReqDesc q = a.reqs[find_req(a, row)];
-    const int local = row - q.row_start;
-    const int p = local / q.D, di = local - p * q.D;
     if constexpr (UP) {
+        const int row = up_row_of(blockIdx.x - a.n_merge, a.full_rows, a.row_group);
+        const ReqDesc q = a.reqs[find_req(a, row)];
+        const RowRef rr = row_ref(q, row - q.row_start, true);
         float S[33];
 #pragma unroll
         for (int i = 0; i < 33; ++i) S[i] = 0.f;
-        corr1023_blocks<M, QL, true>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), 0, q.K, 0, tid);
-        corr1023u_stats<M>(S, Ld.red, q, local, a.inv_n, tid);
+        corr1023_blocks<M, QL, true>(S, Ld, q, q.cls[rr.di], q.shift[rr.di], as_global(q.specs[rr.p]), 0, q.K, 0,
+                                     tid);
+        corr1023u_stats<M>(S, Ld.red, q, rr.slot, a.inv_n, tid);
     } else {
+        const int row = blockIdx.x - a.n_merge;
+        const ReqDesc q = a.reqs[find_req(a, row)];
+        const int local = row - q.row_start;
+        const int p = local / q.D, di = local - p * q.D;
         float S[NSLOT][QL];
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s)
@@ -511,7 +516,8 @@ __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void
         const int lu = u - q.tail_unit_start;
         const int local = q.tail_row0 + lu / q.K, k0 = lu % q.K;
         const int k1 = min(q.K, k0 + (u_end - u));
-        const int p = local / q.D, di = local - p * q.D;
+        const RowRef rr = row_ref(q, local, UP);
+        const int p = rr.p, di = rr.di;
         float* part = reinterpret_cast<float*>(a.scratch);
         const bool whole = k0 == 0 && k1 == q.K;
         float* dst = whole ? part + ((int64_t)2 * gridDim.x + (q.row_start + local - a.full_rows)) * L
@@ -609,6 +615,8 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
     CorrArgs a = c;
     a.sk_grid = G_sk;
     a.n_merge = G_sk > 0 ? tail_rows : 0;
+    const int G = corr1023_grid(c.N, cus);
+    a.row_group = c.ucode && G % 8 == 0 ? G : 0;
     if (G_sk > 0) {
         UGSDR_HIP_TRY(scratch.reserve(((size_t)G_sk * 2 + tail_rows) * c.L * 4));
         a.scratch = scratch.as<float2>();

@@ -58,7 +58,38 @@ struct CorrArgs {
     int sk_grid;            // three-stage kernel: workgroups of the tail split
     int n_merge;            // three-stage kernel: tail rows to merge
     int ucode;              // upsampled-code search: samples per chip (0: off)
+    // upsampled-code search: rows are ordered Doppler-bin-major, and the
+    // workgroups the device holds at once (row_group, a multiple of the 8
+    // XCDs) take them so that each XCD's share is consecutive rows, which
+    // read the same spectra Z through that XCD's L2 (0: no reordering)
+    int row_group;
 };
+
+// upsampled-code search: the row a main-kernel workgroup of index b takes
+// (workgroups are dispatched round-robin over the 8 XCDs)
+__device__ __forceinline__ int up_row_of(int b, int full_rows, int G) {
+    if (G <= 0 || b >= full_rows / G * G) return b;
+    const int g = b / G, w = b - g * G;
+    return g * G + (w & 7) * (G >> 3) + (w >> 3);
+}
+
+// a request-local row's (PRN, Doppler bin) and its result slot (PRN-major):
+// Doppler-bin-major rows for the upsampled-code search, PRN-major otherwise
+struct RowRef {
+    int p, di, slot;
+};
+__device__ __forceinline__ RowRef row_ref(const ReqDesc& q, int local, bool bin_major) {
+    RowRef r;
+    if (bin_major) {
+        r.di = local / q.P;
+        r.p = local - r.di * q.P;
+    } else {
+        r.p = local / q.D;
+        r.di = local - r.p * q.D;
+    }
+    r.slot = r.p * q.D + r.di;
+    return r;
+}
 
 // the request owning a batch row (uniform per workgroup: kept in scalar registers)
 __device__ __forceinline__ int find_req(const CorrArgs& a, int row) {
