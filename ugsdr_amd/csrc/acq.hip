@@ -1017,7 +1017,7 @@ std::vector<int> thread_candidates(int64_t N) {
 size_t corr_scratch_bytes(const AcqPlan& p0, int cus) {
     if (p0.precision != UGSDR_PREC_FP32) return reduced_scratch_bytes(p0.N, (int)p0.K, cus);
     if (!p0.small) return (2 * (size_t)p0.N + (p0.L + 1) / 2) * sizeof(float2) * (size_t)cus;
-    if (corr1023_supports(p0.N, p0.L)) return (size_t)3 * corr1023_grid(p0.N, cus) * p0.L * 4;
+    if (corr1023_supports(p0.N, p0.L)) return (size_t)3 * corr1023_grid(p0.N, cus, p0.ucode != 0) * p0.L * 4;
     return 0;
 }
 
@@ -1067,7 +1067,7 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
     // (when rows have several blocks) split block-wise as the batch tail
     int full_rows = rows;
     if (three_stage && maxK > 1) {
-        const int G = corr1023_grid(p0.N, cus);
+        const int G = corr1023_grid(p0.N, cus, p0.ucode != 0);
         full_rows = rows / G * G;
     }
     for (size_t i = 0; i < plans.size(); ++i) {

@@ -105,7 +105,7 @@ bool corr1023_supports(int64_t N, int64_t L);
 // ReqDesc tail fields) and merged by a second kernel.  corr1023_grid: the
 // workgroups the device holds at once for length N.
 struct CorrArgs;
-int corr1023_grid(int64_t N, int cus);
+int corr1023_grid(int64_t N, int cus, bool upsampled_code = false);
 int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st);
 // upsampled-code search: Z of every (Doppler bin, block) of a request from its
 // class spectra (pl.ucode > 0; s.zbuf sized D K N complex)

@@ -127,16 +127,10 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
 
 // Blocks k0 .. k1-1 of one row into the thread's surface samples
 // S[s][u] = surface[tid + s T + u 1023].
-//
-// UP (upsampled-code search): X holds the row's PRN-independent spectra Z
-// (phase p of frequency k' at k' M + p) and C the code's 1023-point conj
-// spectrum, input i is Z[i] conj(C)[i / M], and stages 1 and 2 alone give
-// the M phases' 1023-point transforms: stage-2 output qq of butterfly j is
-// sample tau = M (j % 31 + 31 qq) + j / 31 of the row, accumulated straight
-// into S[qq] (no stage 3, two barriers per block).
-template <int M, int QL, bool UP, class SA>
-__device__ __forceinline__ void corr1023_blocks(SA& S, const C1023Lds<M>& Ld, const ReqDesc& q, int cls, int sh,
-                                                gptr<float2> C, int k0, int k1, int fold_mode, int tid) {
+template <int M, int QL>
+__device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT][QL], const C1023Lds<M>& Ld,
+                                                const ReqDesc& q, int cls, int sh, gptr<float2> C, int k0, int k1,
+                                                int fold_mode, int tid) {
     using S_ = C1023Shape<M>;
     constexpr int N = S_::N, T = S_::T, B1 = S_::B1, B2 = S_::B2, NSLOT = S_::NSLOT;
     constexpr int FOLD = M / QL;
@@ -162,9 +156,9 @@ __device__ __forceinline__ void corr1023_blocks(SA& S, const C1023Lds<M>& Ld, co
             s1 -= s1 >= (unsigned)N ? N : 0;
             s2 -= s2 >= (unsigned)N ? N : 0;
             x[2 * st] = ld_off(Xb, s1 * 8u);
-            c[2 * st] = ld_off(C, (UP ? i1 / (unsigned)M : i1) * 8u);
+            c[2 * st] = ld_off(C, i1 * 8u);
             x[2 * st + 1] = ld_off(Xb, s2 * 8u);
-            c[2 * st + 1] = ld_off(C, (UP ? i2 / (unsigned)M : i2) * 8u);
+            c[2 * st + 1] = ld_off(C, i2 * 8u);
         }
     };
     // with PREFETCH, group 0 of the next block is gathered during this
@@ -253,24 +247,14 @@ __device__ __forceinline__ void corr1023_blocks(SA& S, const C1023Lds<M>& Ld, co
                 u[r] = cmul(u[r], tw2[tb + r]);
             }
             Pfa33<true>::prep(u);
-            if constexpr (UP) {
-                // the next block's first gathers fly during this block's outputs
-                if constexpr (S_::PREFETCH) {
-                    if (k + 1 < k1) gather(k + 1, 0, xv[0], cv[0]);
-                }
-                if (act) Pfa33<true>::store(u, [&](int qq, float2 x) { S[qq] += cabs_hw(x); });
-                continue;   // the next block's stage 1 writes LDS only after its barrier
-            } else {
-                __syncthreads();
-                if (act) {
-                    float2* dst = buf + (j - kk) * 33 + kk;
-                    Pfa33<true>::store(u, [&](int qq, float2 x) { dst[qq * 31] = x; });
-                }
+            __syncthreads();
+            if (act) {
+                float2* dst = buf + (j - kk) * 33 + kk;
+                Pfa33<true>::store(u, [&](int qq, float2 x) { dst[qq * 31] = x; });
             }
         }
         __syncthreads();
         // ---- stage 3: radix M, Ns = 1023, accumulate |.| per owned output
-        if constexpr (!UP) {
         if constexpr (S_::PREFETCH) {
             if (k + 1 < k1) gather(k + 1, 0, xv[0], cv[0]);
         }
@@ -309,51 +293,6 @@ __device__ __forceinline__ void corr1023_blocks(SA& S, const C1023Lds<M>& Ld, co
                 }
             }
         }
-        }
-    }
-}
-
-// upsampled-code search: row sample of the thread's stage-2 output qq
-template <int M>
-__device__ __forceinline__ int up_tau(int tid, int qq) {
-    return M * (tid % 31 + 31 * qq) + tid / 31;
-}
-
-// upsampled-code search: the thread's 33 surface samples (stage-2 butterfly
-// tid < 31 M) -> row statistics (and the surface)
-template <int M>
-__device__ __forceinline__ void corr1023u_stats(const float (&S)[33], double* red, const ReqDesc& q, int local,
-                                                float inv_n, int tid) {
-    using S_ = C1023Shape<M>;
-    constexpr int T = S_::T, B2 = S_::B2, L = kC1023 * M;
-    const bool act = tid < B2;
-    float v1 = -1.0f, v2 = -1.0f;
-    int i1 = 0x7fffffff;
-    double sum = 0.0;
-    if (act) {
-#pragma unroll
-        for (int qq = 0; qq < 33; ++qq) {
-            top2_merge(v1, i1, v2, S[qq], up_tau<M>(tid, qq), -1.0f);
-            sum += (double)S[qq];
-        }
-    }
-    block_top2<T>(v1, i1, v2, red, tid);
-    sum = block_sum<T>(sum, red + 64, tid);
-    const double mean = sum / (double)L;
-    double qv = 0.0;
-    if (act) {
-#pragma unroll
-        for (int qq = 0; qq < 33; ++qq) {
-            const double d = (double)S[qq] - mean;
-            qv += d * d;
-        }
-    }
-    qv = block_sum<T>(qv, red + 64, tid);
-    stats_write<T>(v1, i1, v2, mean, qv, L, tid, q.rowstats + local, inv_n);
-    if (q.surfaces && act) {
-        float* o = q.surfaces + (int64_t)local * L;
-#pragma unroll
-        for (int qq = 0; qq < 33; ++qq) o[up_tau<M>(tid, qq)] = S[qq] * inv_n;
     }
 }
 
@@ -454,12 +393,11 @@ __device__ __forceinline__ void corr1023_merge_row(const CorrArgs& a, int tail, 
 // a.n_merge workgroups instead finish the tail rows from the stream-K
 // partial surfaces (launched before this kernel), so that short work runs
 // beside the first round of rows rather than after the last.
-template <int M, int QL, bool UP>
+template <int M, int QL>
 __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void corr1023(CorrArgs a) {
     using S_ = C1023Shape<M>;
     constexpr int NSLOT = S_::NSLOT;
     static_assert(M % QL == 0, "surface length must divide the transform");
-    static_assert(!UP || QL == M, "the upsampled-code search takes no fold");
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     if ((int)blockIdx.x < a.n_merge) {
@@ -469,30 +407,17 @@ __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void
         return;
     }
     const C1023Lds<M> Ld(smem, tid);
-    if constexpr (UP) {
-        const int row = up_row_of(blockIdx.x - a.n_merge, a.full_rows, a.row_group);
-        const ReqDesc q = a.reqs[find_req(a, row)];
-        const RowRef rr = row_ref(q, row - q.row_start, true);
-        float S[33];
+    const int row = blockIdx.x - a.n_merge;
+    const ReqDesc q = a.reqs[find_req(a, row)];
+    const int local = row - q.row_start;
+    const int p = local / q.D, di = local - p * q.D;
+    float S[NSLOT][QL];
 #pragma unroll
-        for (int i = 0; i < 33; ++i) S[i] = 0.f;
-        corr1023_blocks<M, QL, true>(S, Ld, q, q.cls[rr.di], q.shift[rr.di], as_global(q.specs[rr.p]), 0, q.K, 0,
-                                     tid);
-        corr1023u_stats<M>(S, Ld.red, q, rr.slot, a.inv_n, tid);
-    } else {
-        const int row = blockIdx.x - a.n_merge;
-        const ReqDesc q = a.reqs[find_req(a, row)];
-        const int local = row - q.row_start;
-        const int p = local / q.D, di = local - p * q.D;
-        float S[NSLOT][QL];
+    for (int s = 0; s < NSLOT; ++s)
 #pragma unroll
-        for (int s = 0; s < NSLOT; ++s)
-#pragma unroll
-            for (int u = 0; u < QL; ++u) S[s][u] = 0.f;
-        corr1023_blocks<M, QL, false>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), 0, q.K, a.fold_mode,
-                                      tid);
-        corr1023_stats<M, QL>(S, Ld.red, q, local, a.inv_n, tid);
-    }
+        for (int u = 0; u < QL; ++u) S[s][u] = 0.f;
+    corr1023_blocks<M, QL>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), 0, q.K, a.fold_mode, tid);
+    corr1023_stats<M, QL>(S, Ld.red, q, local, a.inv_n, tid);
 }
 
 // Stream-K over the tail rows (fewer than one round of workgroups): the
@@ -502,7 +427,7 @@ __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void
 // a row cut by the share's start or end goes to share slot 0 or 1 (only the
 // first and the last row of a share can be cut).  corr1023_merge finishes
 // the rows.
-template <int M, int QL, bool UP>
+template <int M, int QL>
 __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void corr1023_sk(CorrArgs a) {
     using S_ = C1023Shape<M>;
     constexpr int T = S_::T, NSLOT = S_::NSLOT, L = kC1023 * QL;
@@ -516,36 +441,23 @@ __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void
         const int lu = u - q.tail_unit_start;
         const int local = q.tail_row0 + lu / q.K, k0 = lu % q.K;
         const int k1 = min(q.K, k0 + (u_end - u));
-        const RowRef rr = row_ref(q, local, UP);
-        const int p = rr.p, di = rr.di;
+        const int p = local / q.D, di = local - p * q.D;
+        float S[NSLOT][QL];
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s)
+#pragma unroll
+            for (int v = 0; v < QL; ++v) S[s][v] = 0.f;
+        corr1023_blocks<M, QL>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), k0, k1, a.fold_mode, tid);
         float* part = reinterpret_cast<float*>(a.scratch);
         const bool whole = k0 == 0 && k1 == q.K;
         float* dst = whole ? part + ((int64_t)2 * gridDim.x + (q.row_start + local - a.full_rows)) * L
                            : part + ((int64_t)w * 2 + (u == u_begin ? 0 : 1)) * L;
-        if constexpr (UP) {
-            float S[33];
 #pragma unroll
-            for (int i = 0; i < 33; ++i) S[i] = 0.f;
-            corr1023_blocks<M, QL, true>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), k0, k1, 0, tid);
-            if (tid < S_::B2) {
+        for (int s = 0; s < NSLOT; ++s) {
+            const int j = tid + s * T;
+            if (j < kC1023) {
 #pragma unroll
-                for (int qq = 0; qq < 33; ++qq) dst[up_tau<M>(tid, qq)] = S[qq];
-            }
-        } else {
-            float S[NSLOT][QL];
-#pragma unroll
-            for (int s = 0; s < NSLOT; ++s)
-#pragma unroll
-                for (int v = 0; v < QL; ++v) S[s][v] = 0.f;
-            corr1023_blocks<M, QL, false>(S, Ld, q, q.cls[di], q.shift[di], as_global(q.specs[p]), k0, k1,
-                                          a.fold_mode, tid);
-#pragma unroll
-            for (int s = 0; s < NSLOT; ++s) {
-                const int j = tid + s * T;
-                if (j < kC1023) {
-#pragma unroll
-                    for (int v = 0; v < QL; ++v) dst[j + v * kC1023] = S[s][v];
-                }
+                for (int v = 0; v < QL; ++v) dst[j + v * kC1023] = S[s][v];
             }
         }
         u += k1 - k0;
@@ -557,17 +469,279 @@ static hipError_t lds_attr(K kernel, size_t bytes) {
     return set_lds_limit(reinterpret_cast<const void*>(kernel), bytes);
 }
 
-template <int M, int QL, bool UP = false>
+template <int M, int QL>
 static int launch_1023(const CorrArgs& c, hipStream_t st) {
     const size_t lds = C1023Shape<M>::lds_bytes;
     constexpr int T = C1023Shape<M>::T;
     if (c.sk_grid > 0) {
-        UGSDR_HIP_TRY(lds_attr(corr1023_sk<M, QL, UP>, lds));
-        hipLaunchKernelGGL((corr1023_sk<M, QL, UP>), dim3(c.sk_grid), dim3(T), lds, st, c);
+        UGSDR_HIP_TRY(lds_attr(corr1023_sk<M, QL>, lds));
+        hipLaunchKernelGGL((corr1023_sk<M, QL>), dim3(c.sk_grid), dim3(T), lds, st, c);
     }
     if (c.full_rows + c.n_merge > 0) {
-        UGSDR_HIP_TRY(lds_attr(corr1023<M, QL, UP>, lds));
-        hipLaunchKernelGGL((corr1023<M, QL, UP>), dim3(c.full_rows + c.n_merge), dim3(T), lds, st, c);
+        UGSDR_HIP_TRY(lds_attr(corr1023<M, QL>, lds));
+        hipLaunchKernelGGL((corr1023<M, QL>), dim3(c.full_rows + c.n_merge), dim3(T), lds, st, c);
+    }
+    return UGSDR_OK;
+}
+
+// ---------------------------------------------------------- upsampled code
+// Kernels of the upsampled-code search.  A row is M independent 1023-point
+// inverse transforms, one per phase p (row sample tau = M q + p): phase p's
+// input is Z_p[k'] conj(C)[k'], Z stored phase-major per (Doppler bin,
+// block), C the code's 1023-point conj spectrum.  Each wave owns two phases
+// end to end: their 66 radix-31 stage-1 butterflies (on the matrix cores in
+// five groups of 16, as in the three-stage kernel), the phases' LDS region,
+// and their 62 radix-33 stage-2 butterflies, whose 33 outputs each are summed
+// as |.| in registers over the blocks.  A wave reads back only what it wrote,
+// so the block loop has no workgroup barrier: waves of one CU drift freely
+// and hide each other's latency.
+template <int M>
+struct UpShape {
+    static constexpr int W = M / 2;   // waves, two phases each
+    static constexpr int T = 64 * W;
+    static constexpr int G = 5;       // stage-1 groups of 16 butterflies per wave (66 of 80 used)
+    // LDS: the phases' data, the stage-2 table [31][33], reduction scratch,
+    // the stage-1 operand table [4][64]
+    static constexpr size_t lds_bytes =
+        (size_t)(M * kC1023 + kC1023) * sizeof(float2) + 128 * sizeof(double) + 256 * sizeof(float2);
+};
+
+template <int M>
+struct UpLds {
+    float2 *buf, *tw2, *tw1;
+    double* red;
+    __device__ UpLds(unsigned char* smem, int tid) {
+        constexpr int T = UpShape<M>::T;
+        buf = reinterpret_cast<float2*>(smem);
+        tw2 = buf + M * kC1023;   // [k * 33 + r] = exp(+2 pi i r k / 1023)
+        red = reinterpret_cast<double*>(tw2 + kC1023);
+        tw1 = reinterpret_cast<float2*>(red + 128);
+        for (int i = tid; i < kC1023; i += T) {
+            const int k = i / 33, r = i - k * 33;
+            double s, c;
+            sincospi(2.0 * (double)((r * k) % kC1023) / (double)kC1023, &s, &c);
+            tw2[i] = make_float2((float)c, (float)s);
+        }
+        for (int i = tid; i < 256; i += T) {
+            const int l = i & 63, k = l & 15, p = 4 * (i >> 6) + (l >> 4);
+            double s, c;
+            sincospi(2.0 * (double)((k * p) % 31) / 31.0, &s, &c);
+            tw1[i] = make_float2((float)c, (float)s);
+        }
+        __syncthreads();
+    }
+};
+
+// a wave's LDS writes before its later reads of other lanes' data: a wave's
+// LDS instructions complete in order, so this only keeps the compiler from
+// moving the accesses across each other
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// row sample of stage-2 output qq of the wave-lane (wave wv, lane l < 62)
+template <int M>
+__device__ __forceinline__ int upw_tau(int wv, int l, int qq) {
+    const int pp = l >= 31 ? 1 : 0, kk = l - 31 * pp;
+    return M * (kk + 31 * qq) + 2 * wv + pp;
+}
+
+// Blocks k0 .. k1-1 of the row of Doppler bin di with code spectrum C into
+// the lane's 33 surface samples (lanes < 62 of every wave).
+template <int M>
+__device__ __forceinline__ void upw_blocks(float (&S)[33], const UpLds<M>& Ld, const ReqDesc& q, int di,
+                                           gptr<float2> C, int k0, int k1, int tid) {
+    constexpr int N = kC1023 * M, G = UpShape<M>::G;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    float2* region = Ld.buf + 2 * wv * kC1023;   // the wave's two phases, [33][31] each
+    // stage-1 gathers of group g of block kb: butterfly b = 16 g + col of the
+    // wave (phase 2 wv + b / 33, residue b % 33), 8 inputs per lane.  The
+    // lane index is laundered per block (opaque), so the per-group offsets
+    // are recomputed in the loop instead of being hoisted into registers.
+    auto gather = [&](int t, int kb, int g, float2 (&x)[8], float2 (&c)[8]) {
+        const int col = t & 15, kq = (t >> 4) & 3;
+        const gptr<float2> Zb = as_global(q.X + ((int64_t)di * q.K + kb) * N + 2 * wv * kC1023);
+        const int b = min(g * 16 + col, 65);
+        const int pp = b >= 33 ? 1 : 0, jj = b - 33 * pp;
+        const unsigned zo = (unsigned)(pp * kC1023);
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            const int m = 4 * st + kq;
+            const unsigned i1 = jj + 33 * m, i2 = jj + 33 * (m == 0 ? 0 : 31 - m);
+            x[2 * st] = ld_off(Zb, (zo + i1) * 8u);
+            c[2 * st] = ld_off(C, i1 * 8u);
+            x[2 * st + 1] = ld_off(Zb, (zo + i2) * 8u);
+            c[2 * st + 1] = ld_off(C, i2 * 8u);
+        }
+    };
+    float2 xv[2][8], cv[2][8];
+    for (int k = k0; k < k1; ++k) {
+        const int t = opaque(tid);
+        const int lane = t & 63, col = lane & 15, kq = lane >> 4;
+        // ---- stage 1: radix 31 on the matrix cores (see corr1023_blocks);
+        // each group's outputs are stored at once: the region's previous
+        // contents were read by this wave's previous stage 2, which is done.
+        // Group 0's gathers are not prefetched during stage 2 (no registers
+        // left there): the other waves of the SIMD cover their latency.
+        gather(t, k, 0, xv[0], cv[0]);
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi) {
+            if (gi + 1 < G) gather(t, k, gi + 1, xv[(gi + 1) & 1], cv[(gi + 1) & 1]);
+            f32x4 ar = {0.f, 0.f, 0.f, 0.f}, ai = ar, br = ar, bi = ar;
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const float2 x1 = cmul(xv[gi & 1][2 * st], cv[gi & 1][2 * st]);
+                float2 x2 = cmul(xv[gi & 1][2 * st + 1], cv[gi & 1][2 * st + 1]);
+                if (st == 0 && kq == 0) x2 = make_float2(0.f, 0.f);
+                const float2 sp = cadd(x1, x2), dp = csub(x1, x2);
+                const float2 cs = Ld.tw1[st * 64 + lane];
+                ar = mfma4(cs.x, sp.x, ar);
+                ai = mfma4(cs.x, sp.y, ai);
+                br = mfma4(cs.y, dp.x, br);
+                bi = mfma4(cs.y, dp.y, bi);
+            }
+            const int b = gi * 16 + col;
+            if (b < 66) {
+                const int pp = b >= 33 ? 1 : 0, jj = b - 33 * pp;
+                float2* dst = region + pp * kC1023 + jj * 31;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int kk = 4 * kq + r;
+                    dst[kk] = make_float2(ar[r] - bi[r], ai[r] + br[r]);
+                    if (kk > 0) dst[31 - kk] = make_float2(ar[r] + bi[r], ai[r] - br[r]);
+                }
+            }
+            // one group's work at a time: keeps the scheduler from hoisting
+            // every group's gathers (and their registers) to the top
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        wave_lds_sync();
+        // ---- stage 2: radix 33 (3 x 11) per (phase, residue kk), |.| of its outputs
+        {
+            const int pp = lane >= 31 ? 1 : 0, kk = min(lane - 31 * pp, 30);
+            const float2* src = region + pp * kC1023 + kk;
+            float2 u[33];
+#pragma unroll
+            for (int r = 0; r < 33; ++r) u[r] = src[r * 31];
+            int tb = kk * 33;
+#pragma unroll
+            for (int r = 1; r < 33; ++r) {
+                if (r > 1 && (r - 1) % 8 == 0) tb = after(tb, u[r - 1].x);
+                u[r] = cmul(u[r], Ld.tw2[tb + r]);
+            }
+            Pfa33<true>::prep(u);
+            if (lane < 62) Pfa33<true>::store(u, [&](int qq, float2 x) { S[qq] += cabs_hw(x); });
+        }
+        wave_lds_sync();
+    }
+}
+
+// the lanes' 33 surface samples -> row statistics (and the surface)
+template <int M>
+__device__ __forceinline__ void upw_stats(const float (&S)[33], double* red, const ReqDesc& q, int slot, float inv_n,
+                                          int tid) {
+    constexpr int T = UpShape<M>::T, L = kC1023 * M;
+    const int lane = tid & 63, wv = tid >> 6;
+    const bool act = lane < 62;
+    float v1 = -1.0f, v2 = -1.0f;
+    int i1 = 0x7fffffff;
+    double sum = 0.0;
+    if (act) {
+#pragma unroll
+        for (int qq = 0; qq < 33; ++qq) {
+            top2_merge(v1, i1, v2, S[qq], upw_tau<M>(wv, lane, qq), -1.0f);
+            sum += (double)S[qq];
+        }
+    }
+    block_top2<T>(v1, i1, v2, red, tid);
+    sum = block_sum<T>(sum, red + 64, tid);
+    const double mean = sum / (double)L;
+    double qv = 0.0;
+    if (act) {
+#pragma unroll
+        for (int qq = 0; qq < 33; ++qq) {
+            const double d = (double)S[qq] - mean;
+            qv += d * d;
+        }
+    }
+    qv = block_sum<T>(qv, red + 64, tid);
+    stats_write<T>(v1, i1, v2, mean, qv, L, tid, q.rowstats + slot, inv_n);
+    if (q.surfaces && act) {
+        float* o = q.surfaces + (int64_t)slot * L;
+#pragma unroll
+        for (int qq = 0; qq < 33; ++qq) o[upw_tau<M>(wv, lane, qq)] = S[qq] * inv_n;
+    }
+}
+
+// One row per workgroup (Doppler-bin-major rows, XCD-grouped); the first
+// a.n_merge workgroups finish the stream-K tail rows instead.
+template <int M>
+__global__ __launch_bounds__(UpShape<M>::T, 2) void corr1023w(CorrArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    if ((int)blockIdx.x < a.n_merge) {
+        float* S = reinterpret_cast<float*>(smem);
+        double* red = reinterpret_cast<double*>(S + (a.L + 1) / 2 * 2);
+        corr1023_merge_row<UpShape<M>::T>(a, blockIdx.x, a.sk_grid, S, red, tid);
+        return;
+    }
+    const UpLds<M> Ld(smem, tid);
+    const int row = up_row_of(blockIdx.x - a.n_merge, a.full_rows, a.row_group);
+    const ReqDesc q = a.reqs[find_req(a, row)];
+    const RowRef rr = row_ref(q, row - q.row_start, true);
+    float S[33];
+#pragma unroll
+    for (int i = 0; i < 33; ++i) S[i] = 0.f;
+    upw_blocks<M>(S, Ld, q, rr.di, as_global(q.specs[rr.p]), 0, q.K, tid);
+    upw_stats<M>(S, Ld.red, q, rr.slot, a.inv_n, tid);
+}
+
+// Stream-K over the tail rows, as corr1023_sk: partial surfaces in natural
+// order for corr1023_merge_row.
+template <int M>
+__global__ __launch_bounds__(UpShape<M>::T, 2) void corr1023w_sk(CorrArgs a) {
+    constexpr int L = kC1023 * M;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const UpLds<M> Ld(smem, tid);
+    const int w = blockIdx.x;
+    const int u_begin = sk_begin(a.units, gridDim.x, w), u_end = sk_begin(a.units, gridDim.x, w + 1);
+    for (int u = u_begin; u < u_end;) {
+        const ReqDesc q = a.reqs[find_req_unit(a, u)];
+        const int lu = u - q.tail_unit_start;
+        const int local = q.tail_row0 + lu / q.K, k0 = lu % q.K;
+        const int k1 = min(q.K, k0 + (u_end - u));
+        const RowRef rr = row_ref(q, local, true);
+        float S[33];
+#pragma unroll
+        for (int i = 0; i < 33; ++i) S[i] = 0.f;
+        upw_blocks<M>(S, Ld, q, rr.di, as_global(q.specs[rr.p]), k0, k1, tid);
+        float* part = reinterpret_cast<float*>(a.scratch);
+        const bool whole = k0 == 0 && k1 == q.K;
+        float* dst = whole ? part + ((int64_t)2 * gridDim.x + (q.row_start + local - a.full_rows)) * L
+                           : part + ((int64_t)w * 2 + (u == u_begin ? 0 : 1)) * L;
+        if (lane < 62) {
+#pragma unroll
+            for (int qq = 0; qq < 33; ++qq) dst[upw_tau<M>(wv, lane, qq)] = S[qq];
+        }
+        u += k1 - k0;
+    }
+}
+
+template <int M>
+static int launch_1023w(const CorrArgs& c, hipStream_t st) {
+    const size_t lds = UpShape<M>::lds_bytes;
+    constexpr int T = UpShape<M>::T;
+    if (c.sk_grid > 0) {
+        UGSDR_HIP_TRY(lds_attr(corr1023w_sk<M>, lds));
+        hipLaunchKernelGGL((corr1023w_sk<M>), dim3(c.sk_grid), dim3(T), lds, st, c);
+    }
+    if (c.full_rows + c.n_merge > 0) {
+        UGSDR_HIP_TRY(lds_attr(corr1023w<M>, lds));
+        hipLaunchKernelGGL((corr1023w<M>), dim3(c.full_rows + c.n_merge), dim3(T), lds, st, c);
     }
     return UGSDR_OK;
 }
@@ -590,19 +764,19 @@ bool corr1023_supports(int64_t N, int64_t L) {
     return QL == M || QL == 4;
 }
 
-// workgroups of the three-stage kernel one CU holds (LDS bound)
-static int per_cu(int M) {
+// workgroups of the three-stage (or the upsampled-code) kernel one CU holds (LDS bound)
+static int per_cu(int M, bool up) {
     size_t lds = 0;
     switch (M) {
-        case 4: lds = C1023Shape<4>::lds_bytes; break;
-        case 8: lds = C1023Shape<8>::lds_bytes; break;
-        case 12: lds = C1023Shape<12>::lds_bytes; break;
-        default: lds = C1023Shape<16>::lds_bytes; break;
+        case 4: lds = up ? UpShape<4>::lds_bytes : C1023Shape<4>::lds_bytes; break;
+        case 8: lds = up ? UpShape<8>::lds_bytes : C1023Shape<8>::lds_bytes; break;
+        case 12: lds = up ? UpShape<12>::lds_bytes : C1023Shape<12>::lds_bytes; break;
+        default: lds = up ? UpShape<16>::lds_bytes : C1023Shape<16>::lds_bytes; break;
     }
     return std::max(1, (int)(160 * 1024 / lds));
 }
 
-int corr1023_grid(int64_t N, int cus) { return cus * per_cu((int)(N / kC1023)); }
+int corr1023_grid(int64_t N, int cus, bool up) { return cus * per_cu((int)(N / kC1023), up); }
 
 int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st) {
     const int M = c.N / kC1023, QL = c.L / kC1023;
@@ -611,11 +785,11 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
         return UGSDR_ERR_UNSUPPORTED;
     }
     const int tail_rows = c.rows - c.full_rows;
-    const int G_sk = tail_rows > 0 ? std::min(corr1023_grid(c.N, cus), c.units) : 0;
+    const int G_sk = tail_rows > 0 ? std::min(corr1023_grid(c.N, cus, c.ucode != 0), c.units) : 0;
     CorrArgs a = c;
     a.sk_grid = G_sk;
     a.n_merge = G_sk > 0 ? tail_rows : 0;
-    const int G = corr1023_grid(c.N, cus);
+    const int G = corr1023_grid(c.N, cus, c.ucode != 0);
     a.row_group = c.ucode && G % 8 == 0 ? G : 0;
     if (G_sk > 0) {
         UGSDR_HIP_TRY(scratch.reserve(((size_t)G_sk * 2 + tail_rows) * c.L * 4));
@@ -627,10 +801,10 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
             return UGSDR_ERR_UNSUPPORTED;
         }
         switch (M) {
-            case 4: return launch_1023<4, 4, true>(a, st);
-            case 8: return launch_1023<8, 8, true>(a, st);
-            case 12: return launch_1023<12, 12, true>(a, st);
-            default: return launch_1023<16, 16, true>(a, st);
+            case 4: return launch_1023w<4>(a, st);
+            case 8: return launch_1023w<8>(a, st);
+            case 12: return launch_1023w<12>(a, st);
+            default: return launch_1023w<16>(a, st);
         }
     }
     switch (M * 100 + QL) {
@@ -650,7 +824,7 @@ bool upsampled_code_supported(int64_t code_len, int64_t U) {
 }
 
 // Upsampled-code search, the PRN-independent part: for Doppler bin d and block
-// k of a request, Z[(d K + k) N + k' U + p] = e^(j 2 pi k'p / N) sum_u
+// k of a request, Z[(d K + k) N + p Lc + k'] = e^(j 2 pi k'p / N) sum_u
 // X_c[(k' + Lc u + m) mod N] conj(H[k' + Lc u]) e^(j 2 pi up / U), with X_c
 // the block's spectrum of bin d's class c and m its integer rotation.  One
 // thread per k' (the U-point inverse DFT in registers); the twiddle powers
@@ -679,9 +853,9 @@ __global__ __launch_bounds__(256) void zprep_kernel(const float2* X, const float
     sincospi(2.0 * (double)kp / (double)N, &sn, &cs);
     float2 tw[U];
     twiddle_powers<U>(make_float2((float)cs, (float)sn), tw);
-    float2* z = Z + (int64_t)dk * N + (int64_t)kp * U;
+    float2* z = Z + (int64_t)dk * N + kp;
 #pragma unroll
-    for (int p = 0; p < U; ++p) z[p] = cmul(v[p], tw[p]);
+    for (int p = 0; p < U; ++p) z[p * Lc] = cmul(v[p], tw[p]);
 }
 
 int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
