@@ -680,16 +680,28 @@ def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0, coherent:
     }
 
 
-def corr_kernel_name(n_fft: int, surface_len: int, threads: int) -> str:
-    """the correlation kernel a (length, variant) pair runs: the first variant
-    of a 1023*M length (M = 4, 8, 12, 16) is the three-stage kernel"""
+def corr_kernel_name(form: dict, surface_len: int, threads: int) -> str:
+    """the correlation kernel a search ran, from the form the library reports
+    (transform length, periods folded, samples per chip of the upsampled-code
+    search) and the tuned variant: the upsampled-code search runs the
+    wave-autonomous kernel for 1023-chip codes and the phase transforms for
+    longer ones; otherwise the first variant of a 1023*M length (M = 4, 8,
+    12, 16) is the three-stage kernel"""
     import ugsdr_amd as U
 
-    if n_fft % 1023 == 0 and n_fft // 1023 in (4, 8, 12, 16) and threads == U.corr_variants(n_fft)[0]:
-        return f"corr1023<{n_fft // 1023},{surface_len // 1023}> (31 x 33 x {n_fft // 1023}, {threads} threads)"
-    if n_fft <= 16384:
-        return f"corr_small<{threads},...> (N={n_fft})"
-    return f"corr_large<{threads},...> (N={n_fft})"
+    n = form["n_transform"]
+    upc = form["samples_per_chip"]
+    if upc:
+        lc = n // upc
+        if lc == 1023:
+            return f"corr1023w<{upc}> (upsampled code: {upc} phases of 1023 points, wave-autonomous)"
+        return f"corr1023p<{lc // 1023}> (upsampled code: {upc} phase transforms of {lc} = 31 x 33 x {lc // 1023})"
+    fold = f", {form['periods']} periods folded" if form["periods"] > 1 else ""
+    if n % 1023 == 0 and n // 1023 in (4, 8, 12, 16) and threads == U.corr_variants(n)[0]:
+        return f"corr1023<{n // 1023},{surface_len // 1023}> (31 x 33 x {n // 1023}, {threads} threads{fold})"
+    if n <= 16384:
+        return f"corr_small<{threads},...> (N={n}{fold})"
+    return f"corr_large<{threads},...> (N={n}{fold})"
 
 
 def main():
@@ -777,6 +789,7 @@ def main():
         for k, v in ctx.last_times().items():
             stage[k].append(v)
     ctx.set_profiling(False)
+    form = ctx.last_search_form()
     corr_ms = statistics.mean(stage["corr"])
     alg_bytes = (B_CB if prec == U.PREC_FP32 else 4) * w.cells * w.n_blocks
     achieved = alg_bytes / (corr_ms * 1e-3) / 1e9
@@ -819,11 +832,13 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": corr_kernel_name(w.n_fft, w.surface_len, chosen) if prec == U.PREC_FP32
+            "kernel": corr_kernel_name(form, w.surface_len, chosen) if prec == U.PREC_FP32
             else f"corr_mfma<{args.precision},{chosen or 1024}> (N={w.n_fft})",
             "algorithmic_bytes_per_launch": alg_bytes,
         },
+        "search_form": form,
         "detected_prns": [i for i, r in enumerate(res) if r["detected"]],
+        # thread variants of the transform of the whole row (sample replicas)
         "autotune": {"variants": variants, "ms_per_launch": tune_ms, "chosen": chosen},
         "graph_replays_in_timed_steps": replays,
     }

@@ -120,7 +120,9 @@ def test_l1ca_wide_10ms(ctx, oracle):
 
 
 def test_boc_4ms_subset(ctx, oracle):
-    """81840-point four-step path; 8 planted PRNs + 4 absent ones, all 97 bins."""
+    """N = 81840 from the expanded codes' chips (the upsampled-code search: ten
+    8184-point phase transforms per row); 8 planted PRNs + 4 absent ones, all
+    97 bins."""
     w = W.boc_4ms()
     planted = sorted(s.prn for s in w.sats)
     absent = [p for p in range(36) if p not in planted][:4]
@@ -128,6 +130,8 @@ def test_boc_4ms_subset(ctx, oracle):
     say(f"case BOC_4MS PRNs {prns} x 97 bins x 5 blocks, N=81840")
     keys = [ctx.replica_put(w.codes[p], 1, w.n_fft) for p in prns]
     gpu, _, _ = gpu_run(ctx, w, keys)
+    # the expanded code's 8184 half-chips held for 10 samples each
+    assert ctx.last_search_form() == {"n_transform": 81840, "periods": 1, "samples_per_chip": 10}
     ref, _, _ = oracle_run(oracle, w, prns=prns)
     assert_parity(gpu, ref, "BOC_4MS")
     for i, p in enumerate(planted):
@@ -310,6 +314,41 @@ def test_upsampled_code_matches_oracle(ctx, oracle, m, n_blocks):
         ctx.replica_drop(k)
 
 
+@pytest.mark.parametrize("n_chips,m,n_blocks", [(4092, 2, 2), (4092, 5, 1), (8184, 2, 1), (8184, 10, 2),
+                                                 (12276, 3, 1), (16368, 2, 1)])
+def test_upsampled_long_code_matches_oracle(ctx, oracle, n_chips, m, n_blocks):
+    """codes of 1023 M chips (M = 4, 8, 12, 16; 8184 is the BOC(1,1)-expanded
+    4092-chip code) held for m samples per chip: the row runs as m inverse
+    transforms of 1023 M points, one per phase, each over every block.
+    Against the oracle's N-point search and the same search on sample
+    replicas, surfaces included."""
+    n_fft = n_chips * m
+    fs = n_fft * 1000.0
+    rng = np.random.default_rng(7000 + n_chips + m * 10 + n_blocks)
+    dop = U.doppler_bins(2750, 250)
+    shift = 517 * m + 3
+    codes, reps, x = planted_case(oracle, rng, n_fft, n_fft, n_chips, fs, n_blocks=n_blocks, shift=shift,
+                                  f_dop=750.0, n_prn=4)
+    say(f"case upsampled long code N={n_fft} chips={n_chips} U={m} K={n_blocks}")
+    keys = [ctx.replica_put(c, 1, n_fft) for c in codes]
+    skeys = [ctx.replica_put_samples(r) for r in reps]
+    iq = np.ascontiguousarray(x).view(np.float32)
+    kw = dict(want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft, n_blocks=n_blocks, surface_len=n_fft,
+              doppler=dop)
+    gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, keys=keys, **kw)
+    assert ctx.last_search_form() == {"n_transform": n_fft, "periods": 1, "samples_per_chip": m}
+    gs, ssurf, _ = ctx.acquire(iq, U.FMT_CF32, keys=skeys, **kw)
+    assert ctx.last_search_form()["samples_per_chip"] == 0
+    ref, rsurf, _ = oracle.acquire(x, reps, n_fft=n_fft, n_blocks=n_blocks, surface_len=n_fft, fs_in=fs, fs_acq=fs,
+                                   doppler=dop, precision=64, want_surfaces=True)
+    assert_parity(gpu, ref, f"upsampled N={n_fft}")
+    assert_parity(gs, ref, f"samples N={n_fft}")
+    assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
+    assert gpu[1]["index"] == shift % n_fft and abs(gpu[1]["doppler"] - 750.0) <= 250.0
+    for k in keys + skeys:
+        ctx.replica_drop(k)
+
+
 def planted_case(oracle, rng, n_fft, surface_len, n_chips, fs, n_blocks=2, shift=777, f_dop=1000.0, n_prn=3):
     """n_prn random +-1 codes of n_chips per surface_len samples, the second
     one planted at code phase `shift` and Doppler f_dop in complex AWGN"""
@@ -338,22 +377,28 @@ def test_fourstep_and_recording_lengths_every_variant(ctx, oracle, n_fft, surfac
     dop = U.doppler_bins(2000, 500)
     codes, reps, x = planted_case(oracle, rng, n_fft, surface_len, n_chips, fs)
     repeats = n_fft // surface_len
+    # replicas put as samples too: the 8184-chip code (BOC) would otherwise
+    # run the upsampled-code search, not the transform of the whole row
     keys = [ctx.replica_put(c, repeats, n_fft) for c in codes]
+    skeys = [ctx.replica_put_samples(r) for r in reps] if n_chips == 8184 else []
     iq = np.ascontiguousarray(x).view(np.float32)
     ref, rsurf, _ = oracle.acquire(x, reps, n_fft=n_fft, n_blocks=2, surface_len=surface_len, fs_in=fs, fs_acq=fs,
                                    doppler=dop, precision=64, want_surfaces=True)
     variants = U.corr_variants(n_fft)
     assert variants
     for t in variants:
-        say(f"case length N={n_fft} L={surface_len} K=2 threads={t}")
-        ctx.set_threads(n_fft, t)
-        gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft,
-                                   n_blocks=2, surface_len=surface_len, doppler=dop, keys=keys)
-        assert_parity(gpu, ref, f"N={n_fft}/L={surface_len}/T={t}")
-        assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
-        assert gpu[1]["index"] == 777 % surface_len and gpu[1]["doppler"] == 1000.0
+        for kk in (keys, skeys):
+            if not kk:
+                continue
+            say(f"case length N={n_fft} L={surface_len} K=2 threads={t} samples={kk is skeys}")
+            ctx.set_threads(n_fft, t)
+            gpu, surf, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, fs_in=fs, fs_acq=fs, n_fft=n_fft,
+                                       n_blocks=2, surface_len=surface_len, doppler=dop, keys=kk)
+            assert_parity(gpu, ref, f"N={n_fft}/L={surface_len}/T={t}")
+            assert np.max(np.abs(surf - rsurf)) < 2e-5 * rsurf.max()
+            assert gpu[1]["index"] == 777 % surface_len and gpu[1]["doppler"] == 1000.0
     ctx.set_threads(n_fft, 0)
-    for k in keys:
+    for k in keys + skeys:
         ctx.replica_drop(k)
 
 

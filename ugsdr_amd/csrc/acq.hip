@@ -1016,6 +1016,7 @@ std::vector<int> thread_candidates(int64_t N) {
 
 size_t corr_scratch_bytes(const AcqPlan& p0, int cus) {
     if (p0.precision != UGSDR_PREC_FP32) return reduced_scratch_bytes(p0.N, (int)p0.K, cus);
+    if (p0.ucode && p0.N / p0.ucode != 1023) return 0;   // phase transforms: no scratch
     if (!p0.small) return (2 * (size_t)p0.N + (p0.L + 1) / 2) * sizeof(float2) * (size_t)cus;
     if (corr1023_supports(p0.N, p0.L)) return (size_t)3 * corr1023_grid(p0.N, cus, p0.ucode != 0) * p0.L * 4;
     return 0;
@@ -1055,11 +1056,13 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
         maxK = std::max(maxK, (int)pl.K);
     }
     const int T = threads > 0 ? threads : default_threads(p0.N);
-    // the upsampled-code search runs on the three-stage kernel whatever
-    // variant the length is tuned to
+    // the upsampled-code search runs on its own kernels whatever variant the
+    // length is tuned to: the wave-autonomous ones for 1023-chip codes (the
+    // three-stage launch below), the phase transforms for longer codes
+    const bool phase_tf = p0.ucode && p0.N / p0.ucode != 1023;
     const bool three_stage =
-        p0.small && (p0.ucode || T == corr1023_threads(p0.N)) && corr1023_supports(p0.N, p0.L);
-    if (p0.ucode && !three_stage) {
+        !phase_tf && p0.small && (p0.ucode || T == corr1023_threads(p0.N)) && corr1023_supports(p0.N, p0.L);
+    if (p0.ucode && !phase_tf && !three_stage) {
         set_error("upsampled-code search without a three-stage kernel for its length");
         return UGSDR_ERR_UNSUPPORTED;
     }
@@ -1090,7 +1093,9 @@ int enqueue_corr(const std::vector<const AcqPlan*>& plans, const std::vector<Acq
     c.units = units;
     c.ucode = p0.ucode;
     int rc = UGSDR_OK;
-    if (three_stage) {
+    if (phase_tf) {
+        rc = launch_corr1023p(c, cus, st);
+    } else if (three_stage) {
         rc = launch_corr1023(c, cus, scratch, st);
     } else if (p0.small) {
         const int grid = rows;

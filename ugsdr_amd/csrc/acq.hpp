@@ -110,6 +110,9 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
 // upsampled-code search: Z of every (Doppler bin, block) of a request from its
 // class spectra (pl.ucode > 0; s.zbuf sized D K N complex)
 int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
+// upsampled-code search with codes of 1023 M chips: U phase transforms of
+// 1023 M points per row on the three-stage engine (c.ucode = U, c.N = U 1023 M)
+int launch_corr1023p(const CorrArgs& c, int cus, hipStream_t st);
 // Reduced precision (16-bit storage, matrix-core stages): whether a length
 // pair has a plan, the 16-bit packing of the class spectra, and one
 // correlation launch over a batch of requests of one bucket.

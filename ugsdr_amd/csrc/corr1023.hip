@@ -578,15 +578,14 @@ __device__ __forceinline__ void upw_blocks(float (&S)[33], const UpLds<M>& Ld, c
         }
     };
     float2 xv[2][8], cv[2][8];
+    if (k0 < k1) gather(opaque(tid), k0, 0, xv[0], cv[0]);
     for (int k = k0; k < k1; ++k) {
         const int t = opaque(tid);
         const int lane = t & 63, col = lane & 15, kq = lane >> 4;
         // ---- stage 1: radix 31 on the matrix cores (see corr1023_blocks);
         // each group's outputs are stored at once: the region's previous
         // contents were read by this wave's previous stage 2, which is done.
-        // Group 0's gathers are not prefetched during stage 2 (no registers
-        // left there): the other waves of the SIMD cover their latency.
-        gather(t, k, 0, xv[0], cv[0]);
+        // Group 0's gathers were issued during the previous block's stage 2.
 #pragma unroll
         for (int gi = 0; gi < G; ++gi) {
             if (gi + 1 < G) gather(t, k, gi + 1, xv[(gi + 1) & 1], cv[(gi + 1) & 1]);
@@ -633,6 +632,8 @@ __device__ __forceinline__ void upw_blocks(float (&S)[33], const UpLds<M>& Ld, c
                 u[r] = cmul(u[r], Ld.tw2[tb + r]);
             }
             Pfa33<true>::prep(u);
+            // the next block's first gathers fly during this block's outputs
+            if (k + 1 < k1) gather(t, k + 1, 0, xv[0], cv[0]);
             if (lane < 62) Pfa33<true>::store(u, [&](int qq, float2 x) { S[qq] += cabs_hw(x); });
         }
         wave_lds_sync();
@@ -819,8 +820,105 @@ int launch_corr1023(const CorrArgs& c, int cus, DevBuf& scratch, hipStream_t st)
     }
 }
 
+// Upsampled-code search: codes of 1023 chips held for U = 4, 8, 12 or 16
+// samples run on the wave-autonomous kernels (U phases of 1023 points);
+// codes of 1023 M chips (M = 4, 8, 12, 16) held for U = 2 ... 16 samples run
+// U phase transforms of 1023 M points on the three-stage engine.
 bool upsampled_code_supported(int64_t code_len, int64_t U) {
-    return code_len == kC1023 && (U == 4 || U == 8 || U == 12 || U == 16);
+    if (code_len == kC1023) return U == 4 || U == 8 || U == 12 || U == 16;
+    if (code_len % kC1023 || !corr1023_threads(code_len)) return false;
+    return U == 2 || U == 3 || U == 4 || U == 5 || U == 8 || U == 10 || U == 12 || U == 16;
+}
+
+// Upsampled-code search for codes of 1023 M chips held for U samples each
+// (the BOC(1,1) 4 ms code: 8184 half-chips of 10 samples at 20.46 MHz): per
+// row, U phase transforms of 1023 M points (phase p's input Z_p conj(C),
+// Z_p of bin d and block k at ((d K + k) U + p) 1023 M), each over every
+// block on the three-stage engine, its |.| sums (row samples
+// tau = U (j + 1023 u) + p) folded into the thread's share of the row
+// statistics before the next phase.  Doppler-bin-major rows, XCD-grouped.
+template <int M>
+__global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void corr1023p(CorrArgs a) {
+    using S_ = C1023Shape<M>;
+    constexpr int NSLOT = S_::NSLOT, T = S_::T;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    const C1023Lds<M> Ld(smem, tid);
+    const int U = a.ucode;
+    const int row = up_row_of(blockIdx.x, a.rows, a.row_group);
+    const ReqDesc q0 = a.reqs[find_req(a, row)];
+    const RowRef rr = row_ref(q0, row - q0.row_start, true);
+    ReqDesc q = q0;
+    q.F = U;
+    float v1 = -1.0f, v2 = -1.0f;
+    int i1 = 0x7fffffff;
+    double sum = 0.0, sumsq = 0.0;
+    float* surf = q0.surfaces ? q0.surfaces + (int64_t)rr.slot * a.L : nullptr;
+    for (int p = 0; p < U; ++p) {
+        float S[NSLOT][M];
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s)
+#pragma unroll
+            for (int u = 0; u < M; ++u) S[s][u] = 0.f;
+        corr1023_blocks<M, M>(S, Ld, q, rr.di * q0.K * U + p, 0, as_global(q0.specs[rr.p]), 0, q0.K, 0, tid);
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s) {
+            const int j = tid + s * T;
+            if (j < kC1023) {
+#pragma unroll
+                for (int u = 0; u < M; ++u) {
+                    const int tau = U * (j + kC1023 * u) + p;
+                    const float v = S[s][u];
+                    top2_merge(v1, i1, v2, v, tau, -1.0f);
+                    sum += (double)v;
+                    sumsq += (double)v * (double)v;
+                    if (surf) surf[tau] = v * a.inv_n;
+                }
+            }
+        }
+    }
+    // sum of squared deviations from the FP64 sums (a positive surface whose
+    // spread is comparable to its mean)
+    block_top2<T>(v1, i1, v2, Ld.red, tid);
+    sum = block_sum<T>(sum, Ld.red + 64, tid);
+    sumsq = block_sum<T>(sumsq, Ld.red + 64, tid);
+    const double mean = sum / (double)a.L;
+    const double dev = sumsq - sum * mean;
+    stats_write<T>(v1, i1, v2, mean, dev > 0.0 ? dev : 0.0, a.L, tid, q0.rowstats + rr.slot, a.inv_n);
+}
+
+int launch_corr1023p(const CorrArgs& c, int cus, hipStream_t st) {
+    const int M = c.ucode > 0 ? c.N / c.ucode / kC1023 : 0;
+    CorrArgs a = c;
+    const int G = cus * per_cu(M, false);
+    a.row_group = G % 8 == 0 ? G : 0;
+    a.full_rows = a.rows;
+    a.units = 0;
+    a.sk_grid = 0;
+    a.n_merge = 0;
+    switch (M) {
+        case 4:
+            UGSDR_HIP_TRY(lds_attr(corr1023p<4>, C1023Shape<4>::lds_bytes));
+            hipLaunchKernelGGL(corr1023p<4>, dim3(a.rows), dim3(C1023Shape<4>::T), C1023Shape<4>::lds_bytes, st, a);
+            break;
+        case 8:
+            UGSDR_HIP_TRY(lds_attr(corr1023p<8>, C1023Shape<8>::lds_bytes));
+            hipLaunchKernelGGL(corr1023p<8>, dim3(a.rows), dim3(C1023Shape<8>::T), C1023Shape<8>::lds_bytes, st, a);
+            break;
+        case 12:
+            UGSDR_HIP_TRY(lds_attr(corr1023p<12>, C1023Shape<12>::lds_bytes));
+            hipLaunchKernelGGL(corr1023p<12>, dim3(a.rows), dim3(C1023Shape<12>::T), C1023Shape<12>::lds_bytes, st,
+                               a);
+            break;
+        case 16:
+            UGSDR_HIP_TRY(lds_attr(corr1023p<16>, C1023Shape<16>::lds_bytes));
+            hipLaunchKernelGGL(corr1023p<16>, dim3(a.rows), dim3(C1023Shape<16>::T), C1023Shape<16>::lds_bytes, st,
+                               a);
+            break;
+        default: set_error("upsampled-code search: unsupported phase transform"); return UGSDR_ERR_UNSUPPORTED;
+    }
+    UGSDR_HIP_TRY(hipGetLastError());
+    return UGSDR_OK;
 }
 
 // Upsampled-code search, the PRN-independent part: for Doppler bin d and block
@@ -833,7 +931,7 @@ bool upsampled_code_supported(int64_t code_len, int64_t U) {
 template <int U>
 __global__ __launch_bounds__(256) void zprep_kernel(const float2* X, const float2* conjH, const int* cls,
                                                    const int* shift, int N, int K, int F, int chunks, float2* Z) {
-    constexpr int Lc = kC1023;
+    const int Lc = N / U;
     const int dk = blockIdx.x / chunks, chunk = blockIdx.x - dk * chunks;
     const int d = dk / K, k = dk - d * K;
     const int kp = chunk * 256 + threadIdx.x;
@@ -863,10 +961,26 @@ int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     const char* tp = s.tables.as<char>();
     const int* cls = reinterpret_cast<const int*>(tp + pl.off_cls);
     const int* shift = reinterpret_cast<const int*>(tp + pl.off_shift);
-    const int chunks = (kC1023 + 255) / 256;
+    const int chunks = (int)((pl.N / pl.ucode + 255) / 256);
     const dim3 grid((unsigned)(pl.D * pl.K * chunks));
     const int N = (int)pl.N, K = (int)pl.K;
     switch (pl.ucode) {
+        case 2:
+            hipLaunchKernelGGL(zprep_kernel<2>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
+                               N, K, pl.F, chunks, s.zbuf.as<float2>());
+            break;
+        case 3:
+            hipLaunchKernelGGL(zprep_kernel<3>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
+                               N, K, pl.F, chunks, s.zbuf.as<float2>());
+            break;
+        case 5:
+            hipLaunchKernelGGL(zprep_kernel<5>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
+                               N, K, pl.F, chunks, s.zbuf.as<float2>());
+            break;
+        case 10:
+            hipLaunchKernelGGL(zprep_kernel<10>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls,
+                               shift, N, K, pl.F, chunks, s.zbuf.as<float2>());
+            break;
         case 4:
             hipLaunchKernelGGL(zprep_kernel<4>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
                                N, K, pl.F, chunks, s.zbuf.as<float2>());

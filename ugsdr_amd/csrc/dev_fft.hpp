@@ -316,6 +316,29 @@ struct Dft<12, INV> {
     }
 };
 
+// 10-point DFT as a 2 x 5 prime-factor transform: input n = (5 n1 + 2 n2) mod 10
+// feeds DFT-2 number n2 at position n1; output (k1, k2) is X[(5 k1 + 6 k2) mod 10]
+// (5 = 1 mod 2 = 0 mod 5, 6 = 0 mod 2 = 1 mod 5).
+template <bool INV>
+struct Dft<10, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 y[2][5];
+#pragma unroll
+        for (int n2 = 0; n2 < 5; ++n2) {
+            float2 t[2] = {v[(2 * n2) % 10], v[(5 + 2 * n2) % 10]};
+            Dft<2, INV>::run(t);
+            y[0][n2] = t[0];
+            y[1][n2] = t[1];
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 2; ++k1) {
+            Dft<5, INV>::run(y[k1]);
+#pragma unroll
+            for (int k2 = 0; k2 < 5; ++k2) v[(5 * k1 + 6 * k2) % 10] = y[k1][k2];
+        }
+    }
+};
+
 // ------------------------------------------------------------ twiddle table
 // lo[i] = exp(-2 pi i i / n), i < 128;  hi[i] = exp(-2 pi i (128 i) / n)
 __device__ __forceinline__ void tw_build(float2* lo, float2* hi, int n, int tid, int nthreads) {
