@@ -495,33 +495,34 @@ static int launch_1023(const CorrArgs& c, hipStream_t st) {
 // as |.| in registers over the blocks.  A wave reads back only what it wrote,
 // so the block loop has no workgroup barrier: waves of one CU drift freely
 // and hide each other's latency.
+//
+// 1023 = 31 x 33 with 31 and 33 coprime runs as a prime-factor (Good-Thomas)
+// transform, with no twiddles between the stages: stage-1 butterfly jj takes
+// inputs k' = (33 m + 31 jj) mod 1023, m = 0..30, into a 31-point transform
+// (output k1), stage 2 a 33-point transform over jj (output k2), and output
+// (k1, k2) is sample q = (528 k1 + 496 k2) mod 1023 (528 = 1 mod 31 = 0 mod
+// 33, 496 = 0 mod 31 = 1 mod 33), since k' q = 33 m k1 + 31 jj k2 mod 1023.
+// Z and the code spectrum are stored in that input order (element k' at
+// pfa1023_pos(k') = 33 m + jj), so a stage-1 group's 16 butterflies read
+// 16 consecutive elements.
 template <int M>
 struct UpShape {
     static constexpr int W = M / 2;   // waves, two phases each
     static constexpr int T = 64 * W;
     static constexpr int G = 5;       // stage-1 groups of 16 butterflies per wave (66 of 80 used)
-    // LDS: the phases' data, the stage-2 table [31][33], reduction scratch,
-    // the stage-1 operand table [4][64]
-    static constexpr size_t lds_bytes =
-        (size_t)(M * kC1023 + kC1023) * sizeof(float2) + 128 * sizeof(double) + 256 * sizeof(float2);
+    // LDS: the phases' data, reduction scratch, the stage-1 operand table [4][64]
+    static constexpr size_t lds_bytes = (size_t)M * kC1023 * sizeof(float2) + 128 * sizeof(double) + 256 * sizeof(float2);
 };
 
 template <int M>
 struct UpLds {
-    float2 *buf, *tw2, *tw1;
+    float2 *buf, *tw1;
     double* red;
     __device__ UpLds(unsigned char* smem, int tid) {
         constexpr int T = UpShape<M>::T;
         buf = reinterpret_cast<float2*>(smem);
-        tw2 = buf + M * kC1023;   // [k * 33 + r] = exp(+2 pi i r k / 1023)
-        red = reinterpret_cast<double*>(tw2 + kC1023);
+        red = reinterpret_cast<double*>(buf + M * kC1023);
         tw1 = reinterpret_cast<float2*>(red + 128);
-        for (int i = tid; i < kC1023; i += T) {
-            const int k = i / 33, r = i - k * 33;
-            double s, c;
-            sincospi(2.0 * (double)((r * k) % kC1023) / (double)kC1023, &s, &c);
-            tw2[i] = make_float2((float)c, (float)s);
-        }
         for (int i = tid; i < 256; i += T) {
             const int l = i & 63, k = l & 15, p = 4 * (i >> 6) + (l >> 4);
             double s, c;
@@ -542,11 +543,16 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_sched_barrier(0);
 }
 
-// row sample of stage-2 output qq of the wave-lane (wave wv, lane l < 62)
+// storage position of element k' (< 1023) in prime-factor order: 33 m + jj
+// with m = 16 k' mod 31, jj = 16 k' mod 33 (k' = 33 m + 31 jj mod 1023)
+__host__ __device__ __forceinline__ int pfa1023_pos(int k) { return (16 * k % 31) * 33 + 16 * k % 33; }
+
+// row sample of stage-2 output qq of the wave-lane (wave wv, lane l < 62):
+// phase 2 wv + l / 31, stage-1 output k1 = l % 31, stage-2 output qq
 template <int M>
 __device__ __forceinline__ int upw_tau(int wv, int l, int qq) {
     const int pp = l >= 31 ? 1 : 0, kk = l - 31 * pp;
-    return M * (kk + 31 * qq) + 2 * wv + pp;
+    return M * ((528 * kk + 496 * qq) % kC1023) + 2 * wv + pp;
 }
 
 // Blocks k0 .. k1-1 of the row of Doppler bin di with code spectrum C into
@@ -569,6 +575,7 @@ __device__ __forceinline__ void upw_blocks(float (&S)[33], const UpLds<M>& Ld, c
         const unsigned zo = (unsigned)(pp * kC1023);
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
+            // inputs m and 31 - m of butterfly jj, stored in prime-factor order
             const int m = 4 * st + kq;
             const unsigned i1 = jj + 33 * m, i2 = jj + 33 * (m == 0 ? 0 : 31 - m);
             x[2 * st] = ld_off(Zb, (zo + i1) * 8u);
@@ -618,19 +625,14 @@ __device__ __forceinline__ void upw_blocks(float (&S)[33], const UpLds<M>& Ld, c
             __builtin_amdgcn_sched_barrier(0);
         }
         wave_lds_sync();
-        // ---- stage 2: radix 33 (3 x 11) per (phase, residue kk), |.| of its outputs
+        // ---- stage 2: radix 33 (3 x 11) per (phase, stage-1 output kk), no
+        // twiddles (prime-factor map), |.| of its outputs
         {
             const int pp = lane >= 31 ? 1 : 0, kk = min(lane - 31 * pp, 30);
             const float2* src = region + pp * kC1023 + kk;
             float2 u[33];
 #pragma unroll
             for (int r = 0; r < 33; ++r) u[r] = src[r * 31];
-            int tb = kk * 33;
-#pragma unroll
-            for (int r = 1; r < 33; ++r) {
-                if (r > 1 && (r - 1) % 8 == 0) tb = after(tb, u[r - 1].x);
-                u[r] = cmul(u[r], Ld.tw2[tb + r]);
-            }
             Pfa33<true>::prep(u);
             // the next block's first gathers fly during this block's outputs
             if (k + 1 < k1) gather(t, k + 1, 0, xv[0], cv[0]);
@@ -927,33 +929,86 @@ int launch_corr1023p(const CorrArgs& c, int cus, hipStream_t st) {
 // the block's spectrum of bin d's class c and m its integer rotation.  One
 // thread per k' (the U-point inverse DFT in registers); the twiddle powers
 // from e^(j 2 pi k' / N) by the same product tree as the three-stage
-// kernel's last stage.
-template <int U>
-__global__ __launch_bounds__(256) void zprep_kernel(const float2* X, const float2* conjH, const int* cls,
-                                                   const int* shift, int N, int K, int F, int chunks, float2* Z) {
+// kernel's last stage.  PFA (Lc = 1023, one workgroup of 1024 threads per
+// (bin, block)): each phase is stored in prime-factor order, permuted
+// through LDS (two buffers of four phases, one barrier per four phases) so
+// the stores stay coalesced.
+template <int U, bool PFA>
+__global__ __launch_bounds__(PFA ? 1024 : 256) void zprep_kernel(const float2* X, const float2* conjH,
+                                                                 const int* cls, const int* shift, int N, int K,
+                                                                 int F, int chunks, float2* Z) {
+    // PFA: two buffers of PG phases (one barrier per PG phases)
+    constexpr int PG = U % 4 == 0 ? 4 : 1;
+    __shared__ float2 stage[PFA ? 2 * PG * kC1023 : 1];
     const int Lc = N / U;
     const int dk = blockIdx.x / chunks, chunk = blockIdx.x - dk * chunks;
     const int d = dk / K, k = dk - d * K;
-    const int kp = chunk * 256 + threadIdx.x;
-    if (kp >= Lc) return;
+    const int kp = chunk * (int)blockDim.x + threadIdx.x;
+    const bool act = kp < Lc;
+    if (!PFA && !act) return;
     const float2* Xb = X + ((int64_t)k * F + cls[d]) * N;
     const int m = shift[d];
     float2 v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int n = kp + Lc * u;
-        int src = n + m;
-        src -= src >= N ? N : 0;
-        v[u] = cmul(Xb[src], conjH[n]);
-    }
-    Dft<U, true>::run(v);
-    double sn, cs;
-    sincospi(2.0 * (double)kp / (double)N, &sn, &cs);
     float2 tw[U];
-    twiddle_powers<U>(make_float2((float)cs, (float)sn), tw);
-    float2* z = Z + (int64_t)dk * N + kp;
+    if (act) {
 #pragma unroll
-    for (int p = 0; p < U; ++p) z[p * Lc] = cmul(v[p], tw[p]);
+        for (int u = 0; u < U; ++u) {
+            const int n = kp + Lc * u;
+            int src = n + m;
+            src -= src >= N ? N : 0;
+            v[u] = cmul(Xb[src], conjH[n]);
+        }
+        Dft<U, true>::run(v);
+        double sn, cs;
+        sincospi(2.0 * (double)kp / (double)N, &sn, &cs);
+        twiddle_powers<U>(make_float2((float)cs, (float)sn), tw);
+    }
+    float2* z = Z + (int64_t)dk * N + kp;
+    if constexpr (PFA) {
+        const int pos = act ? pfa1023_pos(kp) : 0;
+#pragma unroll
+        for (int g = 0; g < U / PG; ++g) {
+            float2* b = stage + (g & 1) * PG * kC1023;
+            if (act) {
+#pragma unroll
+                for (int i = 0; i < PG; ++i) b[i * kC1023 + pos] = cmul(v[g * PG + i], tw[g * PG + i]);
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int i = 0; i < PG; ++i) z[(g * PG + i) * kC1023] = b[i * kC1023 + kp];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < U; ++p) z[p * Lc] = cmul(v[p], tw[p]);
+    }
+}
+
+__global__ void pfa1023_permute_kernel(const float2* in, float2* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < kC1023) out[pfa1023_pos(k)] = in[k];
+}
+
+int code_spectrum_layout(ugsdr_hip_ctx* ctx, int64_t code_len, float2* spec) {
+    if (code_len != kC1023) return UGSDR_OK;
+    UGSDR_HIP_TRY(ctx->aux.reserve((size_t)kC1023 * sizeof(float2)));
+    UGSDR_HIP_TRY(hipMemcpyAsync(ctx->aux.ptr, spec, (size_t)kC1023 * sizeof(float2), hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+    hipLaunchKernelGGL(pfa1023_permute_kernel, dim3((kC1023 + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->aux.as<float2>(), spec);
+    UGSDR_HIP_TRY(hipGetLastError());
+    UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return UGSDR_OK;
+}
+
+template <int U, bool PFA>
+static void zprep_launch(const AcqPlan& pl, AcqSlot& s, const int* cls, const int* shift, hipStream_t st) {
+    constexpr int T = PFA ? 1024 : 256;
+    const int chunks = (int)((pl.N / U + T - 1) / T);
+    hipLaunchKernelGGL((zprep_kernel<U, PFA>), dim3((unsigned)(pl.D * pl.K * chunks)), dim3(T), 0, st,
+                       s.classes.as<float2>(), pl.boxcar, cls, shift, (int)pl.N, (int)pl.K, pl.F, chunks,
+                       s.zbuf.as<float2>());
 }
 
 int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
@@ -961,41 +1016,20 @@ int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     const char* tp = s.tables.as<char>();
     const int* cls = reinterpret_cast<const int*>(tp + pl.off_cls);
     const int* shift = reinterpret_cast<const int*>(tp + pl.off_shift);
-    const int chunks = (int)((pl.N / pl.ucode + 255) / 256);
-    const dim3 grid((unsigned)(pl.D * pl.K * chunks));
-    const int N = (int)pl.N, K = (int)pl.K;
+    // 1023-chip codes run on the wave-autonomous kernels (prime-factor order)
+    const bool pfa = pl.N / pl.ucode == kC1023;
     switch (pl.ucode) {
-        case 2:
-            hipLaunchKernelGGL(zprep_kernel<2>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
-                               N, K, pl.F, chunks, s.zbuf.as<float2>());
-            break;
-        case 3:
-            hipLaunchKernelGGL(zprep_kernel<3>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
-                               N, K, pl.F, chunks, s.zbuf.as<float2>());
-            break;
-        case 5:
-            hipLaunchKernelGGL(zprep_kernel<5>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
-                               N, K, pl.F, chunks, s.zbuf.as<float2>());
-            break;
-        case 10:
-            hipLaunchKernelGGL(zprep_kernel<10>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls,
-                               shift, N, K, pl.F, chunks, s.zbuf.as<float2>());
-            break;
-        case 4:
-            hipLaunchKernelGGL(zprep_kernel<4>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
-                               N, K, pl.F, chunks, s.zbuf.as<float2>());
-            break;
-        case 8:
-            hipLaunchKernelGGL(zprep_kernel<8>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls, shift,
-                               N, K, pl.F, chunks, s.zbuf.as<float2>());
-            break;
+        case 2: zprep_launch<2, false>(pl, s, cls, shift, st); break;
+        case 3: zprep_launch<3, false>(pl, s, cls, shift, st); break;
+        case 5: zprep_launch<5, false>(pl, s, cls, shift, st); break;
+        case 10: zprep_launch<10, false>(pl, s, cls, shift, st); break;
+        case 4: pfa ? zprep_launch<4, true>(pl, s, cls, shift, st) : zprep_launch<4, false>(pl, s, cls, shift, st); break;
+        case 8: pfa ? zprep_launch<8, true>(pl, s, cls, shift, st) : zprep_launch<8, false>(pl, s, cls, shift, st); break;
         case 12:
-            hipLaunchKernelGGL(zprep_kernel<12>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls,
-                               shift, N, K, pl.F, chunks, s.zbuf.as<float2>());
+            pfa ? zprep_launch<12, true>(pl, s, cls, shift, st) : zprep_launch<12, false>(pl, s, cls, shift, st);
             break;
         case 16:
-            hipLaunchKernelGGL(zprep_kernel<16>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.boxcar, cls,
-                               shift, N, K, pl.F, chunks, s.zbuf.as<float2>());
+            pfa ? zprep_launch<16, true>(pl, s, cls, shift, st) : zprep_launch<16, false>(pl, s, cls, shift, st);
             break;
         default: set_error("upsampled-code search of an unsupported factor"); return UGSDR_ERR_UNSUPPORTED;
     }

@@ -18,6 +18,10 @@ int replica_spectrum(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chips, i
 // whether the upsampled-code search handles a code of code_len chips held
 // for U samples each (corr1023.hip)
 bool upsampled_code_supported(int64_t code_len, int64_t U);
+// the upsampled-code search's storage order of a code's conj spectrum (in
+// place, on ctx's stream): 1023-chip codes in the prime-factor input order of
+// the wave-autonomous kernels, longer codes unchanged (corr1023.hip)
+int code_spectrum_layout(ugsdr_hip_ctx* ctx, int64_t code_len, float2* spec);
 // resolves the no-alloc workspace's device address on `device` (current),
 // once per device (noalloc.hip)
 int noalloc_scratch_init(int device);
