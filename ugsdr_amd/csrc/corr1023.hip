@@ -932,7 +932,7 @@ int launch_corr1023p(const CorrArgs& c, int cus, hipStream_t st) {
 // kernel's last stage.  PFA (Lc = 1023, one workgroup of 1024 threads per
 // (bin, block)): each phase is stored in prime-factor order, permuted
 // through LDS (two buffers of four phases, one barrier per four phases) so
-// the stores stay coalesced.
+// the global stores stay coalesced.
 template <int U, bool PFA>
 __global__ __launch_bounds__(PFA ? 1024 : 256) void zprep_kernel(const float2* X, const float2* conjH,
                                                                  const int* cls, const int* shift, int N, int K,
@@ -965,18 +965,23 @@ __global__ __launch_bounds__(PFA ? 1024 : 256) void zprep_kernel(const float2* X
     }
     float2* z = Z + (int64_t)dk * N + kp;
     if constexpr (PFA) {
-        const int pos = act ? pfa1023_pos(kp) : 0;
+        // written in natural order, read back at storage position kp: the
+        // element stored there is k' = (33 m + 31 jj) mod 1023 (kp = 33 m +
+        // jj), 31 elements apart across a row of lanes, so both LDS passes
+        // are free of bank conflicts
+        const int sm = kp / 33, sj = kp - 33 * sm;
+        const int src = act ? (33 * sm + 31 * sj) % kC1023 : 0;
 #pragma unroll
         for (int g = 0; g < U / PG; ++g) {
             float2* b = stage + (g & 1) * PG * kC1023;
             if (act) {
 #pragma unroll
-                for (int i = 0; i < PG; ++i) b[i * kC1023 + pos] = cmul(v[g * PG + i], tw[g * PG + i]);
+                for (int i = 0; i < PG; ++i) b[i * kC1023 + kp] = cmul(v[g * PG + i], tw[g * PG + i]);
             }
             __syncthreads();
             if (act) {
 #pragma unroll
-                for (int i = 0; i < PG; ++i) z[(g * PG + i) * kC1023] = b[i * kC1023 + kp];
+                for (int i = 0; i < PG; ++i) z[(g * PG + i) * kC1023] = b[i * kC1023 + src];
             }
         }
     } else {
