@@ -335,6 +335,7 @@ def extra_workloads(ctx):
     planted = sorted(s.prn for s in w.sats)
     for tag, prec in (("BOC_4MS_FP16", U.PREC_FP16), ("BOC_4MS_BF16", U.PREC_BF16)):
         dt, stages, res = time_search(ctx, w, 5, 2, prec)
+        form = ctx.last_search_form()
         d_db = [20 * np.log10(res[p]["snr"] / res32[p]["snr"]) for p in planted]
         same = all(res[p]["index"] == res32[p]["index"] and res[p]["doppler_bin"] == res32[p]["doppler_bin"]
                    for p in planted)
@@ -343,7 +344,9 @@ def extra_workloads(ctx):
                     "cell_blocks": w.cells * w.n_blocks, "stage_ms": stages,
                     "roofline": {"bound": "hbm", "achieved": alg / (stages["corr"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                                  "unit": "GB/s", "frac": alg / (stages["corr"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                 "bytes_per_cell_block": 4},
+                                 "bytes_per_cell_block": 4,
+                                 "kernel": corr_kernel_name(form, w.surface_len, 0, "fp16" if prec == U.PREC_FP16
+                                                            else "bf16")},
                     "pnr_delta_db_vs_fp32_planted": {"max_abs": float(np.max(np.abs(d_db))),
                                                      "values": [float(v) for v in d_db]},
                     "planted_index_and_bin_equal_fp32": same,
@@ -680,17 +683,26 @@ def serving(ctx, n_requests: int, rate: float, boc_precision: int = 0, coherent:
     }
 
 
-def corr_kernel_name(form: dict, surface_len: int, threads: int) -> str:
+def corr_kernel_name(form: dict, surface_len: int, threads: int, precision: str = "fp32") -> str:
     """the correlation kernel a search ran, from the form the library reports
     (transform length, periods folded, samples per chip of the upsampled-code
     search) and the tuned variant: the upsampled-code search runs the
     wave-autonomous kernel for 1023-chip codes and the phase transforms for
-    longer ones; otherwise the first variant of a 1023*M length (M = 4, 8,
-    12, 16) is the three-stage kernel"""
+    longer ones (the reduced form: matrix-core phase transforms of N1 x 341
+    points); otherwise the first variant of a 1023*M length (M = 4, 8, 12,
+    16) is the three-stage kernel, and the reduced form the four-step
+    matrix-core kernel"""
     import ugsdr_amd as U
 
     n = form["n_transform"]
     upc = form["samples_per_chip"]
+    if precision != "fp32":
+        t = threads or 1024
+        if upc:
+            lc = n // upc
+            return (f"corr_upm<{precision},{lc // 341},{t}> (upsampled code: {upc} phase transforms of "
+                    f"{lc} = {lc // 341} x 31 x 11 on the matrix cores)")
+        return f"corr_mfma<{precision},{t}> (N={n})"
     if upc:
         lc = n // upc
         if lc == 1023:
@@ -832,8 +844,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": corr_kernel_name(form, w.surface_len, chosen) if prec == U.PREC_FP32
-            else f"corr_mfma<{args.precision},{chosen or 1024}> (N={w.n_fft})",
+            "kernel": corr_kernel_name(form, w.surface_len, chosen, args.precision),
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "search_form": form,

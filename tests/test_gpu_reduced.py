@@ -131,6 +131,57 @@ def test_reduced_fourstep_lengths_vs_fp64(ctx, oracle, prec, n_fft):
         ctx.replica_drop(k)
 
 
+@pytest.mark.parametrize("prec", [U.PREC_FP16, U.PREC_BF16])
+@pytest.mark.parametrize("n1,samples_per_chip,threads", [(24, 10, 512), (24, 10, 1024), (12, 5, 512),
+                                                         (12, 4, 1024)])
+def test_reduced_upsampled_code_vs_fp64(ctx, oracle, prec, n1, samples_per_chip, threads):
+    """the upsampled-code form of the reduced search (codes of N1 * 341
+    chips held for U samples, U phase transforms of N1 * 341 points per row
+    on the matrix cores): 3 blocks, 7 bins, 3 PRNs with one planted, against
+    the FP64 oracle's n_fft-point search and the FP32 GPU search, in both
+    workgroup shapes"""
+    n_chips = n1 * 341
+    n_fft = n_chips * samples_per_chip
+    fs = n_fft * 250.0   # 4 ms blocks
+    rng = np.random.default_rng(n_fft + prec + threads)
+    dop = U.doppler_bins(750, 250)
+    codes = [np.where(rng.random(n_chips) < 0.5, -1, 1).astype(np.int8) for _ in range(3)]
+    reps = np.stack([oracle.repeat_upsample(c, 1, n_fft) for c in codes])
+    nb, delay = 3, 12345 % n_fft
+    t = np.arange(nb * n_fft)
+    x = (rng.normal(0, 20, nb * n_fft) + 1j * rng.normal(0, 20, nb * n_fft)).astype(np.complex64)
+    x += (1.5 * np.roll(np.tile(reps[2], nb), delay) * np.exp(2j * np.pi * 500 * t / fs)).astype(np.complex64)
+    iq = np.ascontiguousarray(x).view(np.float32)
+    say(f"case reduced upsampled code N1={n1} U={samples_per_chip} N={n_fft} {NAME[prec]} threads={threads}")
+    kw = dict(fs_in=fs, fs_acq=fs, n_fft=n_fft, n_blocks=nb, surface_len=n_fft, doppler=dop)
+    k32 = [ctx.replica_put(c, 1, n_fft) for c in codes]
+    kr = [ctx.replica_put(c, 1, n_fft, prec) for c in codes]
+    ctx.set_threads(n_fft, threads)
+    try:
+        g32, s32, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, keys=k32, **kw)
+        gr, sr, _ = ctx.acquire(iq, U.FMT_CF32, want_surfaces=True, keys=kr, precision=prec, **kw)
+        form = ctx.last_search_form()
+    finally:
+        ctx.set_threads(n_fft, 0)
+    assert form["samples_per_chip"] == samples_per_chip and form["n_transform"] == n_fft, form
+    ref, s64, _ = oracle.acquire(x, reps, n_fft=n_fft, n_blocks=nb, surface_len=n_fft, fs_in=fs, fs_acq=fs,
+                                 doppler=dop, precision=64, want_surfaces=True)
+    err_r = np.linalg.norm(sr - s64) / np.linalg.norm(s64)
+    say(f"  surface rel err vs FP64: {err_r:.2e}; FP32 {np.linalg.norm(s32 - s64) / np.linalg.norm(s64):.2e}")
+    assert err_r <= SURF_TOL[prec]
+    for i in range(3):
+        d64 = 20 * np.log10(gr[i]["snr"] / ref[i]["snr"])
+        say(f"  PRN {i}: idx {gr[i]['index']}/{ref[i]['index']} bin {gr[i]['doppler_bin']}/{ref[i]['doppler_bin']} "
+            f"PNR vs FP64 {d64:+.4f} dB")
+    d_db = 20 * np.log10(gr[2]["snr"] / g32[2]["snr"])
+    assert abs(d_db) <= PNR_DB, d_db
+    assert gr[2]["index"] == g32[2]["index"] == ref[2]["index"] == delay
+    assert gr[2]["doppler_bin"] == g32[2]["doppler_bin"] == ref[2]["doppler_bin"]
+    assert gr[2]["detected"] and ref[2]["detected"]
+    for k in k32 + kr:
+        ctx.replica_drop(k)
+
+
 def test_reduced_rejects_unsupported(ctx):
     """lengths without a matrix-core plan and folded surfaces report UNSUPPORTED"""
     w = W.l1ca_1ms()

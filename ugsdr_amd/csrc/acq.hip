@@ -693,17 +693,19 @@ int64_t AcqPlan::footprint(bool surfaces) const {
                 P * (int64_t)sizeof(ugsdr_acq_result) + P * L * 4;
     if (Tsub > 1) b += K * Tsub * F * N * 8;
     if (!small) b += K * Tsub * F * N * 8;
-    if (ucode) b += D * K * N * 8;
+    if (ucode) b += D * K * N * (precision == UGSDR_PREC_FP32 ? 8 : 4);
     if (surfaces) b += P * D * L * 4;
     return b;
 }
 
 // conj of the U-sample boxcar's N-point spectrum, H[k] = sum_{j<U} e^(-j 2 pi jk/N)
-// = e^(-j pi (U-1) k/N) sin(pi U k/N) / sin(pi k/N), built in FP64 once per (N, U)
-static int boxcar_table(ugsdr_hip_ctx* ctx, int64_t N, int U, const float2** out) {
+// = e^(-j pi (U-1) k/N) sin(pi U k/N) / sin(pi k/N), built in FP64 once per (N, U),
+// and the bound max_{k' < N/U} sum_u |H[k' + u N/U]| on |Z| / max |X|
+static int boxcar_table(ugsdr_hip_ctx* ctx, int64_t N, int U, const float2** out, float* bound) {
     DevBuf& b = ctx->boxcar[N * 64 + U];
     if (!b.ptr) {
         std::vector<float2> h((size_t)N);
+        std::vector<double> habs((size_t)N);
         for (int64_t k = 0; k < N; ++k) {
             double re = (double)U, im = 0.0;
             if (k) {
@@ -713,12 +715,23 @@ static int boxcar_table(ugsdr_hip_ctx* ctx, int64_t N, int U, const float2** out
                 im = mag * std::sin(ph);
             }
             h[(size_t)k] = make_float2((float)re, (float)-im);
+            habs[(size_t)k] = std::hypot(re, im);
         }
+        const int64_t Lc = N / U;
+        double mx = 0.0;
+        for (int64_t k = 0; k < Lc; ++k) {
+            double s = 0.0;
+            for (int u = 0; u < U; ++u) s += habs[(size_t)(k + Lc * u)];
+            mx = std::max(mx, s);
+        }
+        // the float bound rounds up: a scale derived from it never overflows
+        ctx->boxcar_bound[N * 64 + U] = (float)(mx * (1.0 + 1e-6));
         UGSDR_HIP_TRY(b.reserve((size_t)N * sizeof(float2)));
         UGSDR_HIP_TRY(hipMemcpyAsync(b.ptr, h.data(), (size_t)N * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
         UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     *out = b.as<float2>();
+    if (bound) *bound = ctx->boxcar_bound[N * 64 + U];
     return UGSDR_OK;
 }
 
@@ -801,10 +814,12 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl, b
     // of conj(C) Z_p, where Z_p[k'] = e^(j 2 pi k'p / N) sum_u
     // X_rot[k' + Lc u] conj(H[k' + Lc u]) e^(j 2 pi up / U) does not depend on
     // the PRN: it is formed once per (Doppler bin, block) and shared by every
-    // row of that bin.
+    // row of that bin.  The reduced form takes it for codes of N1 * 341
+    // chips (its matrix-core phase transforms), with 16-bit code spectra.
     pl.ucode = 0;
     pl.boxcar = nullptr;
-    if (allow_ucode && pl.precision == UGSDR_PREC_FP32 && pl.L == pl.N) {
+    if (allow_ucode && pl.L == pl.N) {
+        const bool fp32 = pl.precision == UGSDR_PREC_FP32;
         bool ok = true;
         int64_t Lc = 0;
         for (int64_t p = 0; p < pl.P && ok; ++p) {
@@ -813,11 +828,14 @@ int make_acq_plan(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, AcqPlan& pl, b
                  (p == 0 || rep.code_len == Lc);
             Lc = rep.code_len;
         }
-        if (ok && upsampled_code_supported(Lc, pl.N / Lc)) {
+        if (ok && (fp32 ? upsampled_code_supported(Lc, pl.N / Lc) : reduced_ucode_supported(Lc, pl.N / Lc))) {
             pl.ucode = (int)(pl.N / Lc);
-            for (int64_t p = 0; p < pl.P; ++p)
-                pl.specs[p] = static_cast<const float2*>(ctx->replicas.find(r->replica_keys[p])->second.code_spec);
-            if (int rc = boxcar_table(ctx, pl.N, pl.ucode, &pl.boxcar)) return rc;
+            for (int64_t p = 0; p < pl.P; ++p) {
+                const Replica& rep = ctx->replicas.find(r->replica_keys[p])->second;
+                pl.specs[p] = static_cast<const float2*>(rep.code_spec);
+                if (!fp32) pl.spec_scale = rep.code_scale;
+            }
+            if (int rc = boxcar_table(ctx, pl.N, pl.ucode, &pl.boxcar, &pl.zbound)) return rc;
         }
     }
     // front end: unpack + mix + resample
@@ -967,10 +985,17 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     a.sig = s.sig.as<float2>();
     a.class_frac = reinterpret_cast<const double*>(s.tables.as<char>() + pl.off_frac);
     a.mode = 0;
+    // the reduced form's Z is scaled per (bin, block) from the class blocks' max |X|
+    auto zprep = [&]() -> int {
+        if (!pl.ucode) return UGSDR_OK;
+        if (pl.precision != UGSDR_PREC_FP32)
+            if (int rc = enqueue_block_max(pl, s, st)) return rc;
+        return launch_zprep(pl, s, st);
+    };
     if (pl.Tsub == 1) {
         a.out = s.classes.as<float2>();
         if (int rc = launch_forward(a, (int)(pl.K * pl.F), st, s.fwd_scratch)) return rc;
-        return pl.ucode ? launch_zprep(pl, s, st) : UGSDR_OK;
+        return zprep();
     }
     // every period sub-block's class spectra, then their combination per block
     UGSDR_HIP_TRY(s.classes_sub.reserve((size_t)pl.K * pl.Tsub * pl.F * pl.N * sizeof(float2)));
@@ -981,7 +1006,7 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
                        s.classes_sub.as<float2>(), s.classes.as<float2>(), pl.N, pl.F, (int)pl.Tsub, chunks,
                        a.class_frac);
     UGSDR_HIP_TRY(hipGetLastError());
-    return pl.ucode ? launch_zprep(pl, s, st) : UGSDR_OK;
+    return zprep();
 }
 
 int default_threads(int64_t N) {
@@ -1015,7 +1040,7 @@ std::vector<int> thread_candidates(int64_t N) {
 }
 
 size_t corr_scratch_bytes(const AcqPlan& p0, int cus) {
-    if (p0.precision != UGSDR_PREC_FP32) return reduced_scratch_bytes(p0.N, (int)p0.K, cus);
+    if (p0.precision != UGSDR_PREC_FP32) return p0.ucode ? 0 : reduced_scratch_bytes(p0.N, (int)p0.K, cus);
     if (p0.ucode && p0.N / p0.ucode != 1023) return 0;   // phase transforms: no scratch
     if (!p0.small) return (2 * (size_t)p0.N + (p0.L + 1) / 2) * sizeof(float2) * (size_t)cus;
     if (corr1023_supports(p0.N, p0.L)) return (size_t)3 * corr1023_grid(p0.N, cus, p0.ucode != 0) * p0.L * 4;
@@ -1166,10 +1191,11 @@ int presize_slot(const AcqPlan& pl, AcqSlot& s, bool peaks) {
     UGSDR_HIP_TRY(s.classes.reserve(kfn * sizeof(float2)));
     if (pl.Tsub > 1) UGSDR_HIP_TRY(s.classes_sub.reserve(kfn * pl.Tsub * sizeof(float2)));
     if (!pl.small) UGSDR_HIP_TRY(s.fwd_scratch.reserve(kfn * pl.Tsub * sizeof(float2)));
-    if (pl.ucode) UGSDR_HIP_TRY(s.zbuf.reserve((size_t)pl.D * pl.K * pl.N * sizeof(float2)));
-    if (pl.precision != UGSDR_PREC_FP32) {
-        UGSDR_HIP_TRY(s.classes_r.reserve(kfn * 4));
-        UGSDR_HIP_TRY(s.block_scale.reserve((size_t)pl.K * pl.F * 8));
+    const bool fp32 = pl.precision == UGSDR_PREC_FP32;
+    if (pl.ucode) UGSDR_HIP_TRY(s.zbuf.reserve((size_t)pl.D * pl.K * pl.N * (fp32 ? 8 : 4)));
+    if (!fp32) {
+        if (!pl.ucode) UGSDR_HIP_TRY(s.classes_r.reserve(kfn * 4));
+        UGSDR_HIP_TRY(s.block_scale.reserve((size_t)pl.K * pl.F * 8 + (pl.ucode ? (size_t)pl.D * pl.K * 4 : 0)));
     }
     UGSDR_HIP_TRY(s.results.reserve((size_t)pl.P * sizeof(ugsdr_acq_result) + (peaks ? (size_t)pl.P * pl.L * 4 : 0)));
     return UGSDR_OK;
@@ -1359,7 +1385,9 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
     AcqSlot& s = ctx->slot;
     const bool keep_surf = surfaces || peaks;
     hipStream_t st = ctx->stream;
-    const int threads = tuned_threads(ctx, pl.N);
+    // the upsampled-code search runs its own kernels whatever the length's
+    // variant: nothing to tune
+    const int threads = pl.ucode ? 0 : tuned_threads(ctx, pl.N);
     // A request shape seen twice in a row is captured into a hipGraph on the
     // second call and replayed afterwards: the kernels and their arguments
     // (device buffers included) are the same, only the host IQ upload and

@@ -25,6 +25,7 @@ struct AcqPlan {
     // spectra Z of each (Doppler bin, block) (0: off)
     int ucode = 0;
     const float2* boxcar = nullptr;   // conj of the ucode-sample boxcar's N-point spectrum
+    float zbound = 0.f;               // max over k' of sum_u |H[k' + u N / ucode]|
     int F = 0;
     int fold_mode = 0;        // 0: |.| then fold, 1: fold complex then |.|
     int reshape = 1;
@@ -117,6 +118,8 @@ int launch_corr1023p(const CorrArgs& c, int cus, hipStream_t st);
 // pair has a plan, the 16-bit packing of the class spectra, and one
 // correlation launch over a batch of requests of one bucket.
 bool reduced_supported(int64_t N, int64_t L);
+// a reduced-precision plan this build runs (upsampled-code or four-step form)
+bool reduced_plan_ok(const AcqPlan& pl);
 // ProcessGalileo's choice among a satellite's code hypotheses
 // (fse.hpp:382-388): per run of h consecutive results, the first of largest
 // level, detected when its SNR clears the threshold.  in: n results, out:
@@ -124,6 +127,9 @@ bool reduced_supported(int64_t N, int64_t L);
 void reduce_hypotheses(const ugsdr_acq_result* in, int64_t n, int h, double threshold, ugsdr_acq_result* out);
 size_t reduced_scratch_bytes(int64_t N, int max_k, int grid);
 int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
+// per (block, class) max |X| of the FP32 class spectra into s.block_scale
+// (the reduced form's block scales)
+int enqueue_block_max(const AcqPlan& pl, AcqSlot& s, hipStream_t st);
 int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::vector<AcqSlot*>& slots,
                          hipStream_t st, DevBuf& scratch, int cu_count, int threads);
 int enqueue_final(const AcqPlan& pl, AcqSlot& s, hipStream_t st, bool peaks);

@@ -20,12 +20,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "acq.hpp"
 #include "corr_common.hpp"
 #include "ctx.hpp"
 #include "dev_fft.hpp"
+#include "dev_mfma.hpp"
 #include "internal.hpp"
 
 using namespace ugsdr_dev;
@@ -933,10 +935,15 @@ int launch_corr1023p(const CorrArgs& c, int cus, hipStream_t st) {
 // (bin, block)): each phase is stored in prime-factor order, permuted
 // through LDS (two buffers of four phases, one barrier per four phases) so
 // the global stores stay coalesced.
-template <int U, bool PFA>
+// H (reduced precision, not PFA): Z is stored as 16-bit pairs scaled by
+// 2^-e per (bin, block), with 2^e >= the class block's max |X| times bound
+// (>= max |Z|), and zinv[(d K + k)] = 2^e.
+template <int U, bool PFA, class H = void>
 __global__ __launch_bounds__(PFA ? 1024 : 256) void zprep_kernel(const float2* X, const float2* conjH,
                                                                  const int* cls, const int* shift, int N, int K,
-                                                                 int F, int chunks, float2* Z) {
+                                                                 int F, int chunks, void* Zv, const unsigned* bmax,
+                                                                 float bound, float* zinv) {
+    static_assert(std::is_void_v<H> || !PFA, "16-bit Z is stored in natural order");
     // PFA: two buffers of PG phases (one barrier per PG phases)
     constexpr int PG = U % 4 == 0 ? 4 : 1;
     __shared__ float2 stage[PFA ? 2 * PG * kC1023 : 1];
@@ -963,7 +970,23 @@ __global__ __launch_bounds__(PFA ? 1024 : 256) void zprep_kernel(const float2* X
         sincospi(2.0 * (double)kp / (double)N, &sn, &cs);
         twiddle_powers<U>(make_float2((float)cs, (float)sn), tw);
     }
-    float2* z = Z + (int64_t)dk * N + kp;
+    if constexpr (!std::is_void_v<H>) {
+        const float mx = __uint_as_float(bmax[k * F + cls[d]]) * bound;
+        int e = 0;
+        if (mx > 0.f) frexpf(mx, &e);
+        const float sc = ldexpf(1.f, -e);
+        if (chunk == 0 && threadIdx.x == 0) zinv[dk] = ldexpf(1.f, e);
+        if (act) {
+            uint32_t* z = static_cast<uint32_t*>(Zv) + (int64_t)dk * N + kp;
+#pragma unroll
+            for (int p = 0; p < U; ++p) {
+                const float2 w = cmul(v[p], tw[p]);
+                z[p * Lc] = pack16<H>(make_float2(w.x * sc, w.y * sc));
+            }
+        }
+        return;
+    }
+    float2* z = static_cast<float2*>(Zv) + (int64_t)dk * N + kp;
     if constexpr (PFA) {
         // written in natural order, read back at storage position kp: the
         // element stored there is k' = (33 m + 31 jj) mod 1023 (kp = 33 m +
@@ -1011,16 +1034,37 @@ template <int U, bool PFA>
 static void zprep_launch(const AcqPlan& pl, AcqSlot& s, const int* cls, const int* shift, hipStream_t st) {
     constexpr int T = PFA ? 1024 : 256;
     const int chunks = (int)((pl.N / U + T - 1) / T);
-    hipLaunchKernelGGL((zprep_kernel<U, PFA>), dim3((unsigned)(pl.D * pl.K * chunks)), dim3(T), 0, st,
-                       s.classes.as<float2>(), pl.boxcar, cls, shift, (int)pl.N, (int)pl.K, pl.F, chunks,
-                       s.zbuf.as<float2>());
+    const dim3 grid((unsigned)(pl.D * pl.K * chunks));
+    if (pl.precision == UGSDR_PREC_FP32) {
+        hipLaunchKernelGGL((zprep_kernel<U, PFA>), grid, dim3(T), 0, st, s.classes.as<float2>(), pl.boxcar, cls,
+                           shift, (int)pl.N, (int)pl.K, pl.F, chunks, s.zbuf.ptr, nullptr, 0.f, nullptr);
+        return;
+    }
+    if constexpr (!PFA) {
+        // block_scale: [K F] class-block max bits, [K F] their inverses, then zinv [D K]
+        const unsigned* bmax = s.block_scale.as<unsigned>();
+        float* zinv = reinterpret_cast<float*>(s.block_scale.as<unsigned>() + 2 * pl.K * pl.F);
+        if (pl.precision == UGSDR_PREC_FP16)
+            hipLaunchKernelGGL((zprep_kernel<U, false, Fp16Ops>), grid, dim3(T), 0, st, s.classes.as<float2>(),
+                               pl.boxcar, cls, shift, (int)pl.N, (int)pl.K, pl.F, chunks, s.zbuf.ptr, bmax,
+                               pl.zbound, zinv);
+        else
+            hipLaunchKernelGGL((zprep_kernel<U, false, Bf16Ops>), grid, dim3(T), 0, st, s.classes.as<float2>(),
+                               pl.boxcar, cls, shift, (int)pl.N, (int)pl.K, pl.F, chunks, s.zbuf.ptr, bmax,
+                               pl.zbound, zinv);
+    }
 }
 
 int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
-    UGSDR_HIP_TRY(s.zbuf.reserve((size_t)pl.D * pl.K * pl.N * sizeof(float2)));
+    const bool fp32 = pl.precision == UGSDR_PREC_FP32;
+    UGSDR_HIP_TRY(s.zbuf.reserve((size_t)pl.D * pl.K * pl.N * (fp32 ? 8 : 4)));
     const char* tp = s.tables.as<char>();
     const int* cls = reinterpret_cast<const int*>(tp + pl.off_cls);
     const int* shift = reinterpret_cast<const int*>(tp + pl.off_shift);
+    if (!fp32 && pl.N / pl.ucode == kC1023) {
+        set_error("reduced-precision upsampled-code search of a 1023-chip code");
+        return UGSDR_ERR_UNSUPPORTED;
+    }
     // 1023-chip codes run on the wave-autonomous kernels (prime-factor order)
     const bool pfa = pl.N / pl.ucode == kC1023;
     switch (pl.ucode) {

@@ -85,8 +85,9 @@ struct Replica {
     // conj(DFT) of the code_len chips themselves, for the upsampled-code
     // search (otherwise null)
     int64_t repeats = 1;
-    void* code_spec = nullptr;
+    void* code_spec = nullptr;   // reduced precision: 16-bit pairs, times code_scale
     int64_t code_len = 0;
+    float code_scale = 1.0f;
 };
 
 // device buffers one acquisition request holds while in flight
@@ -148,6 +149,9 @@ struct ugsdr_hip_ctx {
     // upsampled-code searches: conj of the U-sample boxcar's N-point
     // spectrum, per (N, U) (key N * 64 + U)
     std::map<int64_t, ugsdr_hip::DevBuf> boxcar;
+    // and max over k' < N / U of sum_u |H[k' + u N / U]| (bounds |Z| by the
+    // class blocks' max |X|; the reduced form scales Z with it)
+    std::map<int64_t, float> boxcar_bound;
     // the form of the last ugsdr_hip_acquire (ugsdr_hip_last_search_form)
     int64_t last_n = 0;
     int last_periods = 0, last_ucode = 0;

@@ -32,6 +32,12 @@ namespace ugsdr_hip {
 // converts an FP32 spectrum in rep.spec to the reduced storage type in place
 // (new allocation), recording the scale
 int replica_reduce(ugsdr_hip_ctx* ctx, Replica& rep, int precision);
+// reduced precision on the upsampled-code form: codes of N1 * 341 chips
+// (N1 in 12, 24) held for U samples (U a Z-prep factor)
+bool reduced_ucode_supported(int64_t code_len, int64_t U);
+// the replica's code spectrum to 16-bit pairs scaled by 1/sqrt(code_len)
+// (reduced-precision replicas)
+int code_spec_reduce(ugsdr_hip_ctx* ctx, Replica& rep, int precision);
 // copies the spectrum back as FP32 complex (undoing the storage scale)
 int replica_read(ugsdr_hip_ctx* ctx, const Replica& rep, float* out);
 }  // namespace ugsdr_hip

@@ -309,14 +309,18 @@ static int replica_store(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chip
         }
         rep.period = L;
     }
-    if (precision == UGSDR_PREC_FP32 && chips && n_fft % repeats == 0 && (n_fft / repeats) % n_chips == 0 &&
-        upsampled_code_supported(n_chips, n_fft / repeats / n_chips)) {
+    const bool fp32 = precision == UGSDR_PREC_FP32;
+    if (chips && n_fft % repeats == 0 && (n_fft / repeats) % n_chips == 0 &&
+        (fp32 ? upsampled_code_supported(n_chips, n_fft / repeats / n_chips)
+              : repeats == 1 && reduced_ucode_supported(n_chips, n_fft / n_chips))) {
         // one period upsampled U times: the chips' own spectrum for the
-        // upsampled-code search
+        // upsampled-code search (16-bit for a reduced-precision replica)
         rc = hipMalloc(&rep.code_spec, (size_t)n_chips * sizeof(float2)) == hipSuccess
                  ? replica_spectrum(ctx, chips, n_chips, 1, nullptr, n_chips, static_cast<float2*>(rep.code_spec))
                  : (set_error("hipMalloc failed"), UGSDR_ERR_HIP);
         if (!rc) rc = code_spectrum_layout(ctx, n_chips, static_cast<float2*>(rep.code_spec));
+        rep.code_len = n_chips;
+        if (!rc && !fp32) rc = code_spec_reduce(ctx, rep, precision);
         if (rc) {
             (void)hipFree(rep.spec);
             if (rep.spec_short) (void)hipFree(rep.spec_short);

@@ -419,6 +419,245 @@ __global__ __launch_bounds__(T, T / 256) void corr_mfma(CorrArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ upsampled code
+// Reduced-precision upsampled-code search: every replica is one period of
+// Lc = N1 * 341 chips held for U samples each (the BOC(1,1) 4 ms code: 8184
+// half-chips at 10 samples), so a row (PRN x Doppler bin) is, per phase
+// p < U, the Lc-point inverse transform of Z_p conj(C) over every block,
+// with Z formed once per (bin, block) by the Z-prep kernel as 16-bit pairs
+// with a power-of-two scale per (bin, block).  One workgroup per row runs
+// each phase transform as a four-step product Lc = N1 x 341 held in LDS:
+//   columns: 341 columns of N1 points; the 16-bit products Z_p conj(C) go
+//            from global memory straight into the matrix-core A operands,
+//            one radix-N1 matrix-core stage, the four-step twiddle
+//            w_Lc^(n2 k1) on its FP32 outputs, packed into LDS as N1 rows;
+//   rows:    341 = 31 x 11, two matrix-core stages (as in corr_mfma's
+//            second phase); |.| of the last stage's FP32 outputs with the
+//            scales undone is summed over the blocks in registers.
+// Output m = k1 + N1 k2 of phase p is row sample tau = U m + p; a phase's
+// sums go into the thread's share of the row statistics before the next.
+constexpr int kUpmLD = pad_to(kRedN2 + SeqLayout<1>::SKEW, 32, 1);
+
+template <int N1>
+struct UpmShape {
+    static constexpr int N2 = kRedN2, Lc = N1 * N2;
+    using Lay = SeqLayout<kUpmLD>;
+    // words of one buffer of N1 rows (base(N1 - 1) + N2, rounded up to 4)
+    static constexpr int BUF = ((N1 - 1) * kUpmLD + Lay::SKEW + N2 + 3) / 4 * 4;
+    static constexpr int CT = (N2 + 15) / 16;   // column tiles of 16 columns
+    static constexpr size_t lds_bytes =
+        (size_t)2 * BUF * 4 +
+        (size_t)(MfmaShape<N1>::ENTRIES + MfmaShape<31>::ENTRIES + MfmaShape<11>::ENTRIES) * 16 +
+        (size_t)(N1 + N2 + kTwLo + (Lc + kTwLo - 1) / kTwLo) * 8 + 128 * 8;
+};
+
+template <class H, int N1, int T>
+__global__ __launch_bounds__(T, T / 256) void corr_upm(CorrArgs a) {
+    using S = UpmShape<N1>;
+    using Lay = typename S::Lay;
+    constexpr int NW = T / 64, N2 = S::N2, Lc = S::Lc;
+    constexpr int GPP = T >= 1024 ? 1 : 2;
+    constexpr int SB = (N1 + 15) / 16;   // sequence tiles of the row stages
+    constexpr int P2 = (SB * 31 + GPP * NW - 1) / (GPP * NW);
+    constexpr int KS1 = MfmaShape<N1>::KS, QT1 = MfmaShape<N1>::QT;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* bufR = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* bufM = bufR + S::BUF;
+    uint4* bt1 = reinterpret_cast<uint4*>(bufM + S::BUF);
+    uint4* bt2a = bt1 + MfmaShape<N1>::ENTRIES;
+    uint4* bt2b = bt2a + MfmaShape<31>::ENTRIES;
+    float2* twk = reinterpret_cast<float2*>(bt2b + MfmaShape<11>::ENTRIES);  // exp(+2 pi i k1 / Lc)
+    float2* tw2 = twk + N1;                                                  // exp(+2 pi i j / 341)
+    float2* twlo = tw2 + N2;
+    float2* twhi = twlo + kTwLo;
+    double* red = reinterpret_cast<double*>(twhi + (Lc + kTwLo - 1) / kTwLo);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    mfma_btab_build<N1, H>(bt1, true, 1.f / N1, tid, T);
+    mfma_btab_build<31, H>(bt2a, true, 1.f / 31, tid, T);
+    mfma_btab_build<11, H>(bt2b, true, 1.f / 11, tid, T);
+    tw_table_build(tw2, N2, true, tid, T);
+    tw_build(twlo, twhi, Lc, tid, T);
+    for (int j = tid; j < N1; j += T) {
+        double sn, cs;
+        sincospi(2.0 * (double)j / (double)Lc, &sn, &cs);
+        twk[j] = make_float2((float)cs, (float)sn);
+    }
+    __syncthreads();
+    const int U = a.ucode;
+    const int row = up_row_of(blockIdx.x, a.rows, a.row_group);
+    const ReqDesc q = a.reqs[find_req(a, row)];
+    const RowRef rr = row_ref(q, row - q.row_start, true);
+    const gptr<uint32_t> C = as_global(reinterpret_cast<const uint32_t*>(q.specs[rr.p]));
+    float* surf = q.surfaces ? q.surfaces + (int64_t)rr.slot * a.L : nullptr;
+    float v1 = -1.0f, v2 = -1.0f;
+    int i1 = 0x7fffffff;
+    double sum = 0.0, sumsq = 0.0;
+    for (int p = 0; p < U; ++p) {
+        float acc[P2][GPP][4];
+#pragma unroll
+        for (int i0 = 0; i0 < P2; ++i0)
+#pragma unroll
+            for (int i2 = 0; i2 < GPP; ++i2)
+#pragma unroll
+                for (int i3 = 0; i3 < 4; ++i3) acc[i0][i2][i3] = 0.f;
+        for (int k = 0; k < q.K; ++k) {
+            const int dk = rr.di * q.K + k;
+            const gptr<uint32_t> Z = as_global(reinterpret_cast<const uint32_t*>(q.X) + ((int64_t)dk * U + p) * Lc);
+            // ---- columns: radix N1 from global memory into LDS rows.  The
+            // previous block's first row stage, the last reader of bufR,
+            // ended before the barrier that preceded its second stage.
+            {
+                for (int t = wave; t < S::CT; t += NW) {
+                    const int c = t * 16 + li;
+                    const int cc = c < N2 ? c : 0;
+                    typename H::v8 A[KS1];
+#pragma unroll
+                    for (int ks = 0; ks < KS1; ++ks) {
+                        uint32_t zv[4], cv[4];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int m = ks * 16 + 4 * lg + e;
+                            const int n = cc + N2 * (m < N1 ? m : 0);
+                            zv[e] = Z[n];
+                            cv[e] = C[n];
+                        }
+                        const int m0 = ks * 16 + 4 * lg;
+                        uint4 av;
+                        av.x = m0 + 0 < N1 ? cmul16<H>(zv[0], cv[0]) : 0u;
+                        av.y = m0 + 1 < N1 ? cmul16<H>(zv[1], cv[1]) : 0u;
+                        av.z = m0 + 2 < N1 ? cmul16<H>(zv[2], cv[2]) : 0u;
+                        av.w = m0 + 3 < N1 ? cmul16<H>(zv[3], cv[3]) : 0u;
+                        A[ks] = __builtin_bit_cast(typename H::v8, av);
+                    }
+                    // B operands from LDS per output tile (fewer live registers)
+                    f32x4 Dre[QT1], Dim[QT1];
+#pragma unroll
+                    for (int qt = 0; qt < QT1; ++qt) {
+                        Dre[qt] = f32x4{0.f, 0.f, 0.f, 0.f};
+                        Dim[qt] = Dre[qt];
+#pragma unroll
+                        for (int ks = 0; ks < KS1; ++ks) {
+                            const auto bre = __builtin_bit_cast(typename H::v8, bt1[((ks * QT1 + qt) * 2 + 0) * 64 + lane]);
+                            const auto bim = __builtin_bit_cast(typename H::v8, bt1[((ks * QT1 + qt) * 2 + 1) * 64 + lane]);
+                            Dre[qt] = H::mfma(A[ks], bre, Dre[qt]);
+                            Dim[qt] = H::mfma(A[ks], bim, Dim[qt]);
+                        }
+                    }
+                    // lane (li, lg): output k1 = qt 16 + li of columns s0 .. s0 + 3
+                    const int s0 = t * 16 + 4 * lg;
+#pragma unroll
+                    for (int qt = 0; qt < QT1; ++qt) {
+                        const int k1 = qt * 16 + li;
+                        if (k1 < N1 && s0 < N2) {
+                            float2 w = tw_get<true>(twlo, twhi, s0 * k1);
+                            const float2 st = twk[k1];
+                            uint32_t* o = bufR + Lay::base(k1) + s0;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                if (s0 + r < N2) o[r] = pack16<H>(cmul(make_float2(Dre[qt][r], Dim[qt][r]), w));
+                                w = cmul(w, st);
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // ---- rows, radix 31 (11 butterflies a row); output (jj, q)
+            // times w_341^(jj q) is the next stage's input
+            mfma_tiles<31, H, NW, GPP, 0, 11, 11, Lay>(
+                bufR, N1, bt2a, lane, wave, [&](int, int, int sb, int jj, const f32x4* Dre, const f32x4* Dim) {
+                    const int s0 = sb * 16 + 4 * lg;
+                    if (s0 < N1) {
+#pragma unroll
+                        for (int qt = 0; qt < 2; ++qt) {
+                            const int qq = qt * 16 + li;
+                            if (qq < 31) {
+                                const float2 w = tw2[jj * qq];
+                                uint32_t* o = bufM + Lay::base(s0) + jj * 31 + qq;
+#pragma unroll
+                                for (int r = 0; r < 4; ++r)
+                                    if (s0 + r < N1)
+                                        o[r * kUpmLD] = pack16<H>(cmul(make_float2(Dre[qt][r], Dim[qt][r]), w));
+                            }
+                        }
+                    }
+                });
+            __syncthreads();
+            // ---- rows, radix 11 (31 butterflies a row): output k2 = jj + 31 q,
+            // |.| with the Z, replica and 1/U scales undone.  The next block's
+            // second row stage, which overwrites bufM, comes after two more
+            // barriers.
+            const float sc = q.block_inv[dk] * q.spec_inv;
+            mfma_tiles<11, H, NW, GPP, P2, 31, 31, Lay>(
+                bufM, N1, bt2b, lane, wave, [&](int pass, int gi, int, int, const f32x4* Dre, const f32x4* Dim) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        acc[pass][gi][r] += cabs_hw(make_float2(Dre[0][r], Dim[0][r])) * sc;
+                });
+        }
+        // phase p's samples into the thread's share of the row statistics
+#pragma unroll
+        for (int pass = 0; pass < P2; ++pass)
+#pragma unroll
+            for (int gi = 0; gi < GPP; ++gi) {
+                const int t = mfma_slot_tile<NW, GPP, 31>(pass, gi, N1, wave);
+                if (t >= 0 && li < 11) {
+                    const int sb = t / 31, jj = t - sb * 31;
+                    const int k2 = jj + 31 * li;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int k1 = sb * 16 + 4 * lg + r;
+                        if (k1 < N1) {
+                            const int tau = U * (k1 + N1 * k2) + p;
+                            const float v = acc[pass][gi][r];
+                            top2_merge(v1, i1, v2, v, tau, -1.0f);
+                            sum += (double)v;
+                            sumsq += (double)v * (double)v;
+                            if (surf) surf[tau] = v;
+                        }
+                    }
+                }
+            }
+    }
+    // sum of squared deviations from the FP64 sums (as corr_mfma)
+    block_top2<T>(v1, i1, v2, red, tid);
+    sum = block_sum<T>(sum, red + 64, tid);
+    sumsq = block_sum<T>(sumsq, red + 64, tid);
+    const double mean = sum / (double)a.L;
+    const double dev = sumsq - sum * mean;
+    stats_write<T>(v1, i1, v2, mean, dev > 0.0 ? dev : 0.0, a.L, tid, q.rowstats + rr.slot, 1.0f);
+}
+
+bool reduced_ucode_supported(int64_t code_len, int64_t U) {
+    if (code_len <= 0 || code_len % kRedN2) return false;
+    const int64_t n1 = code_len / kRedN2;
+    if (n1 != 12 && n1 != 24) return false;
+    return U == 2 || U == 3 || U == 4 || U == 5 || U == 8 || U == 10 || U == 12 || U == 16;
+}
+
+template <class H, int N1, int T>
+static int launch_corr_upm(const CorrArgs& c, hipStream_t st) {
+    constexpr size_t lds = UpmShape<N1>::lds_bytes;
+    UGSDR_HIP_TRY(set_lds_limit(reinterpret_cast<const void*>(corr_upm<H, N1, T>), lds));
+    hipLaunchKernelGGL((corr_upm<H, N1, T>), dim3(c.rows), dim3(T), lds, st, c);
+    return UGSDR_OK;
+}
+
+template <class H, int N1>
+static int launch_corr_upm_t(const CorrArgs& c, hipStream_t st, int threads) {
+    return threads == 512 ? launch_corr_upm<H, N1, 512>(c, st) : launch_corr_upm<H, N1, 1024>(c, st);
+}
+
+template <class H>
+static int launch_corr_upm_n(const CorrArgs& c, hipStream_t st, int threads) {
+    switch (c.N / c.ucode / kRedN2) {
+        case 12: return launch_corr_upm_t<H, 12>(c, st, threads);
+        case 24: return launch_corr_upm_t<H, 24>(c, st, threads);
+        default: set_error("reduced upsampled-code search: unsupported code length"); return UGSDR_ERR_UNSUPPORTED;
+    }
+}
+
 // lengths N1 * 341 with a two-stage matrix-core plan for N1
 static bool red_plan(int64_t N, int& ra, int& rb_) {
     if (N <= 0 || N % kRedN2) return false;
@@ -459,7 +698,19 @@ static int launch_corr_mfma_n(const CorrArgs& c, int grid, hipStream_t st, int t
     }
 }
 
+int enqueue_block_max(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
+    const int nblk = (int)(pl.K * pl.F);
+    UGSDR_HIP_TRY(s.block_scale.reserve((size_t)nblk * 8 + (pl.ucode ? (size_t)pl.D * pl.K * 4 : 0)));
+    unsigned* bmax = s.block_scale.as<unsigned>();
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, st, bmax, nblk);
+    const dim3 grid((unsigned)std::min<int64_t>((pl.N + 1023) / 1024, 64), (unsigned)nblk);
+    hipLaunchKernelGGL(block_absmax_kernel, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.N, bmax);
+    UGSDR_HIP_TRY(hipGetLastError());
+    return UGSDR_OK;
+}
+
 int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
+    if (pl.ucode) return UGSDR_OK;   // the upsampled-code form packs Z instead (enqueue_classes)
     const int nblk = (int)(pl.K * pl.F);
     UGSDR_HIP_TRY(s.classes_r.reserve((size_t)nblk * pl.N * 4));
     UGSDR_HIP_TRY(s.block_scale.reserve((size_t)nblk * 8));
@@ -496,7 +747,6 @@ int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::ve
         }
         char* dp = s.tables.as<char>();
         ReqDesc& q = c.reqs[i];
-        q.X = reinterpret_cast<const float2*>(s.classes_r.ptr);
         q.specs = reinterpret_cast<const float2* const*>(dp + pl.off_specs);
         q.cls = reinterpret_cast<const int*>(dp + pl.off_cls);
         q.shift = reinterpret_cast<const int*>(dp + pl.off_shift);
@@ -507,8 +757,18 @@ int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::ve
         q.D = (int)pl.D;
         q.P = (int)pl.P;
         q.row_start = rows;
-        q.block_inv = reinterpret_cast<const float*>(s.block_scale.as<unsigned>() + pl.K * pl.F);
-        q.spec_inv = 1.0f / pl.spec_scale;
+        if (pl.ucode) {
+            // Z and its per-(bin, block) scales; the 1/U of the phase
+            // transforms' normalisation (1/Lc against the search's 1/N)
+            // folds into the replica scale
+            q.X = reinterpret_cast<const float2*>(s.zbuf.ptr);
+            q.block_inv = reinterpret_cast<const float*>(s.block_scale.as<unsigned>() + 2 * pl.K * pl.F);
+            q.spec_inv = (float)(1.0 / ((double)pl.spec_scale * (double)pl.ucode));
+        } else {
+            q.X = reinterpret_cast<const float2*>(s.classes_r.ptr);
+            q.block_inv = reinterpret_cast<const float*>(s.block_scale.as<unsigned>() + pl.K * pl.F);
+            q.spec_inv = 1.0f / pl.spec_scale;
+        }
         rows += (int)(pl.P * pl.D);
     }
     c.N = (int)p0.N;
@@ -517,6 +777,17 @@ int enqueue_corr_reduced(const std::vector<const AcqPlan*>& plans, const std::ve
     c.rows = rows;
     c.n_req = (int)plans.size();
     c.inv_n = 1.0f;
+    if (p0.ucode) {
+        // one workgroup per row, Doppler-bin-major, each XCD's resident
+        // workgroups (one per CU) on consecutive rows
+        c.ucode = p0.ucode;
+        c.row_group = cus % 8 == 0 ? cus : 0;
+        const int rc = p0.precision == UGSDR_PREC_FP16 ? launch_corr_upm_n<Fp16Ops>(c, st, threads)
+                                                       : launch_corr_upm_n<Bf16Ops>(c, st, threads);
+        if (rc) return rc;
+        UGSDR_HIP_TRY(hipGetLastError());
+        return UGSDR_OK;
+    }
     int max_k = 1;
     for (const AcqPlan* pl : plans) max_k = std::max(max_k, (int)pl->K);
     c.max_k = max_k;
@@ -539,19 +810,50 @@ bool reduced_supported(int64_t N, int64_t L) {
     return red_plan(N, ra, rb_) && L == N;
 }
 
+bool reduced_plan_ok(const AcqPlan& pl) { return pl.ucode ? pl.L == pl.N : reduced_supported(pl.N, pl.L); }
+
+int code_spec_reduce(ugsdr_hip_ctx* ctx, Replica& rep, int precision) {
+    // Parseval for +-1 chips: the RMS bin magnitude is sqrt(code_len)
+    const float scale = (float)(1.0 / std::sqrt((double)rep.code_len));
+    void* dst = nullptr;
+    UGSDR_HIP_TRY(hipMalloc(&dst, (size_t)rep.code_len * 4));
+    const int blocks = (int)std::min<int64_t>((rep.code_len + 255) / 256, 4096);
+    if (precision == UGSDR_PREC_FP16)
+        hipLaunchKernelGGL(to_half2_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                           static_cast<const float2*>(rep.code_spec), static_cast<__half2*>(dst), rep.code_len, scale);
+    else
+        hipLaunchKernelGGL(to_bf162_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                           static_cast<const float2*>(rep.code_spec), static_cast<__hip_bfloat162*>(dst),
+                           rep.code_len, scale);
+    const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(ctx->stream);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        (void)hipFree(dst);
+        set_error(std::string("code spectrum packing: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        return UGSDR_ERR_HIP;
+    }
+    UGSDR_HIP_TRY(hipFree(rep.code_spec));
+    rep.code_spec = dst;
+    rep.code_scale = scale;
+    return UGSDR_OK;
+}
+
 int acquire_reduced(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_result* out, float* surfaces,
                     float* peaks) {
     UGSDR_CHECK_ARG(r->precision == UGSDR_PREC_FP16 || r->precision == UGSDR_PREC_BF16, "unknown precision");
     AcqPlan pl;
     if (int rc = make_acq_plan(ctx, r, pl)) return rc;
-    if (!reduced_supported(pl.N, pl.L)) {
-        set_error("reduced precision runs the four-step lengths N1 * 341 (N1 in 60, 112, 120, 180, 240) with "
-                  "surface_len == n_fft; got n_fft " + std::to_string(pl.N) + ", surface_len " +
+    if (!reduced_plan_ok(pl)) {
+        set_error("reduced precision runs the four-step lengths N1 * 341 (N1 in 60, 112, 120, 180, 240), or "
+                  "replicas of one code period of N1 * 341 chips (N1 in 12, 24) upsampled by an integer factor, "
+                  "with surface_len == n_fft; got n_fft " + std::to_string(pl.N) + ", surface_len " +
                   std::to_string(pl.L));
         return UGSDR_ERR_UNSUPPORTED;
     }
     ctx->graph.drop();   // this call may move buffers a cached graph refers to
     ctx->graph.pending.clear();
+    ctx->last_n = pl.N;
+    ctx->last_periods = (int)pl.Tsub;
+    ctx->last_ucode = pl.ucode;
     AcqSlot& s = ctx->slot;
     const bool keep_surf = surfaces || peaks;
     if (!keep_surf) s.surf.release();

@@ -591,12 +591,17 @@ int ugsdr_hip_server_submit(ugsdr_hip_server* h, const ugsdr_acq_request* req, u
     {
         std::lock_guard<std::mutex> lk(s.ctx->mu);
         if (int rc = make_acq_plan(s.ctx, req, j->plan)) return rc;
-        if (req->precision != UGSDR_PREC_FP32 && !reduced_supported(j->plan.N, j->plan.L)) {
+        if (req->precision != UGSDR_PREC_FP32 && !reduced_plan_ok(j->plan)) {
             set_error("no reduced-precision plan for n_fft " + std::to_string(j->plan.N) + ", surface_len " +
                       std::to_string(j->plan.L));
             return UGSDR_ERR_UNSUPPORTED;
         }
-        j->plan.threads = tuned_threads(s.ctx, j->plan.N);
+        // the FP32 upsampled-code search has nothing to tune; the reduced
+        // form takes a pinned variant or its default, as ugsdr_hip_acquire
+        const auto pin = s.ctx->tuned.find(j->plan.N);
+        j->plan.threads = req->precision != UGSDR_PREC_FP32 ? (pin != s.ctx->tuned.end() ? pin->second : 1024)
+                          : j->plan.ucode                   ? 0
+                                                            : tuned_threads(s.ctx, j->plan.N);
     }
     j->req = *req;
     j->doppler.assign(req->doppler, req->doppler + req->n_doppler);
