@@ -226,6 +226,63 @@ __global__ __launch_bounds__(T, 2) void fwd_large(FwdArgs a) {
         tid);
 }
 
+// The four-step forward transform split over workgroups (both factors with
+// compile-time plans): PH 0 runs one batch of `batch` columns of one
+// transform (N1-point DFTs and the twiddle w_N^(n2 k1), into rows k1 of the
+// transform's scratch G), PH 1 one batch of `batch` rows (N2-point DFTs,
+// outputs in natural order).  The two launches carry blocks x batches small
+// workgroups each, so a handful of transforms (a search's K x F class
+// spectra) spreads over the device instead of running one transform per
+// workgroup.
+constexpr int kSplitCap = 4096;   // complex values of a workgroup's batch
+
+template <int T, int PH, class P1, class P2>
+__global__ __launch_bounds__(T) void fwd_split(FwdArgs a, int nbatch, int batch) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int N1 = P1::N, N2 = P2::N;
+    float2* buf = reinterpret_cast<float2*>(smem);
+    float2* lo = buf + kSplitCap;
+    float2* hi = lo + kTwLo;
+    const int tid = threadIdx.x;
+    const int blk = blockIdx.x / nbatch, bat = blockIdx.x - blk * nbatch;
+    float2* G = a.scratch + (int64_t)blk * a.N;
+    if constexpr (PH == 0) {
+        constexpr int ld1 = N1 | 1;
+        float2* loN = lo + tw_len(N1);
+        float2* hiN = loN + kTwLo;
+        tw_build(lo, hi, N1, tid, T);
+        tw_build(loN, hiN, a.N, tid, T);
+        const int c0 = bat * batch, nb = min(batch, N2 - c0);
+        walk_cols<8>(
+            tid, T, nb, N1, [&](int t, int n1) { return fwd_source(a, blk, (int64_t)N2 * n1 + c0 + t); },
+            [&](int t, int n1, float2 v) { buf[t * ld1 + n1] = v; });
+        __syncthreads();
+        Fft<T, kSplitCap, false, P1>::run(buf, a.fs.p1, nb, lo, hi, tid, ld1);
+        walk_cols<4>(
+            tid, T, nb, N1, [&](int t, int k1) { return buf[t * ld1 + k1]; },
+            [&](int t, int k1, float2 v) {
+                const int n2 = c0 + t;
+                G[(int64_t)k1 * N2 + n2] = cmul(v, tw_get<false>(loN, hiN, n2 * k1));
+            });
+    } else {
+        constexpr int ld2 = N2 | 1;
+        tw_build(lo, hi, N2, tid, T);
+        const int r0 = bat * batch, nb = min(batch, N1 - r0);
+        walk_cols<8>(
+            tid, T, N2, nb, [&](int n2, int t) { return G[(int64_t)(r0 + t) * N2 + n2]; },
+            [&](int n2, int t, float2 v) { buf[t * ld2 + n2] = v; });
+        __syncthreads();
+        Fft<T, kSplitCap, false, P2>::run(buf, a.fs.p2, nb, lo, hi, tid, ld2);
+        float2* o = a.out + (int64_t)blk * a.N;
+        const bool conj_out = a.mode != 0;
+        walk_cols<8>(
+            tid, T, nb, N2, [&](int t, int k2) { return buf[t * ld2 + k2]; },
+            [&](int t, int k2, float2 v) {
+                if (conj_out) v.y = -v.y;
+                o[(int64_t)(r0 + t) + (int64_t)N1 * k2] = v;
+            });
+    }
+}
 
 template <int T, int CAP, class P>
 __global__ __launch_bounds__(T, waves_for_t(T, CAP)) void corr_small(CorrArgs a) {
@@ -537,6 +594,21 @@ static int launch_fwd_large(FwdArgs& a, int blocks, hipStream_t st) {
     return UGSDR_OK;
 }
 
+template <class P1, class P2>
+static int launch_fwd_split(FwdArgs& a, int blocks, hipStream_t st) {
+    constexpr int T = 256, N1 = P1::N, N2 = P2::N;
+    static_assert((N1 | 1) <= kSplitCap && (N2 | 1) <= kSplitCap, "a sequence exceeds a split batch");
+    const int cb = std::min(kSplitCap / (N1 | 1), N2), rb = std::min(kSplitCap / (N2 | 1), N1);
+    const int nb1 = (N2 + cb - 1) / cb, nb2 = (N1 + rb - 1) / rb;
+    const size_t lds0 = (size_t)(kSplitCap + tw_len(N1) + tw_len(a.N)) * sizeof(float2);
+    const size_t lds1 = (size_t)(kSplitCap + tw_len(N2)) * sizeof(float2);
+    UGSDR_HIP_TRY(set_lds(fwd_split<T, 0, P1, P2>, lds0));
+    UGSDR_HIP_TRY(set_lds(fwd_split<T, 1, P1, P2>, lds1));
+    hipLaunchKernelGGL((fwd_split<T, 0, P1, P2>), dim3(blocks * nb1), dim3(T), lds0, st, a, nb1, cb);
+    hipLaunchKernelGGL((fwd_split<T, 1, P1, P2>), dim3(blocks * nb2), dim3(T), lds1, st, a, nb2, rb);
+    return UGSDR_OK;
+}
+
 // four-step split with a 341-point second factor when N/341 has a compile-time plan
 static bool fourstep_341(int N, FourStep& f) {
     if (N % 341) return false;
@@ -602,18 +674,20 @@ static int launch_forward(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratc
     } else {
         UGSDR_HIP_TRY(scratch.reserve((size_t)blocks * a.N * sizeof(float2)));
         a.scratch = scratch.as<float2>();
+        // compile-time splits: columns and rows over many small workgroups;
+        // runtime splits: one workgroup per transform
         FourStep probe;
         const bool ct = fourstep_ct(a.N, probe);
         if (!ct) rc = launch_fwd_large<PlanRT, PlanRT>(a, blocks, st);
-        else if (a.fs.n2 == 175) rc = launch_fwd_large<P152, P175>(a, blocks, st);
-        else if (a.fs.n2 == 256) rc = a.fs.n1 == 128 ? launch_fwd_large<P128, P256>(a, blocks, st)
-                                                       : launch_fwd_large<P256, P256>(a, blocks, st);
+        else if (a.fs.n2 == 175) rc = launch_fwd_split<P152, P175>(a, blocks, st);
+        else if (a.fs.n2 == 256) rc = a.fs.n1 == 128 ? launch_fwd_split<P128, P256>(a, blocks, st)
+                                                       : launch_fwd_split<P256, P256>(a, blocks, st);
         else switch (a.fs.n1) {
-            case 60: rc = launch_fwd_large<P60, P341>(a, blocks, st); break;
-            case 112: rc = launch_fwd_large<P112, P341>(a, blocks, st); break;
-            case 120: rc = launch_fwd_large<P120, P341>(a, blocks, st); break;
-            case 180: rc = launch_fwd_large<P180, P341>(a, blocks, st); break;
-            default: rc = launch_fwd_large<P240, P341>(a, blocks, st); break;
+            case 60: rc = launch_fwd_split<P60, P341>(a, blocks, st); break;
+            case 112: rc = launch_fwd_split<P112, P341>(a, blocks, st); break;
+            case 120: rc = launch_fwd_split<P120, P341>(a, blocks, st); break;
+            case 180: rc = launch_fwd_split<P180, P341>(a, blocks, st); break;
+            default: rc = launch_fwd_split<P240, P341>(a, blocks, st); break;
         }
     }
     if (rc) return rc;
