@@ -1083,6 +1083,8 @@ int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
         default: set_error("upsampled-code search of an unsupported factor"); return UGSDR_ERR_UNSUPPORTED;
     }
     UGSDR_HIP_TRY(hipGetLastError());
+    // the reduced form's phase transforms read Z in prime-factor order
+    if (!fp32) return upm_permute(s.zbuf.as<uint32_t>(), pl.D * pl.K * pl.ucode, pl.N / pl.ucode, st);
     return UGSDR_OK;
 }
 

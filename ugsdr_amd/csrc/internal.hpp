@@ -38,6 +38,9 @@ bool reduced_ucode_supported(int64_t code_len, int64_t U);
 // the replica's code spectrum to 16-bit pairs scaled by 1/sqrt(code_len)
 // (reduced-precision replicas)
 int code_spec_reduce(ugsdr_hip_ctx* ctx, Replica& rep, int precision);
+// in place, on st: nblocks blocks of code_len 16-bit words from natural order
+// into the reduced upsampled-code search's prime-factor storage order
+int upm_permute(uint32_t* blocks, int64_t nblocks, int64_t code_len, hipStream_t st);
 // copies the spectrum back as FP32 complex (undoing the storage scale)
 int replica_read(ugsdr_hip_ctx* ctx, const Replica& rep, float* out);
 }  // namespace ugsdr_hip

@@ -99,15 +99,20 @@ __global__ void zero_u32_kernel(unsigned* p, int n) {
 }
 
 // max |X| of block blockIdx.y (magnitudes are non-negative, so their bit
-// patterns order like unsigned integers)
-__global__ void block_absmax_kernel(const float2* X, int64_t N, unsigned* bmax) {
+// patterns order like unsigned integers): one atomic per workgroup, so the
+// workgroups of a block do not queue on its word
+__global__ __launch_bounds__(256) void block_absmax_kernel(const float2* X, int64_t N, unsigned* bmax) {
+    __shared__ float wm[4];
     const float2* x = X + (int64_t)blockIdx.y * N;
     float m = 0.f;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
         m = fmaxf(m, sqrtf(fmaf(x[i].x, x[i].x, x[i].y * x[i].y)));
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(bmax + blockIdx.y, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(bmax + blockIdx.y, __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]))));
 }
 
 // 2^-e with max |X| = f 2^e, f in [0.5, 1): the block lands in [0.5, 1)
@@ -425,18 +430,79 @@ __global__ __launch_bounds__(T, T / 256) void corr_mfma(CorrArgs a) {
 // half-chips at 10 samples), so a row (PRN x Doppler bin) is, per phase
 // p < U, the Lc-point inverse transform of Z_p conj(C) over every block,
 // with Z formed once per (bin, block) by the Z-prep kernel as 16-bit pairs
-// with a power-of-two scale per (bin, block).  One workgroup per row runs
-// each phase transform as a four-step product Lc = N1 x 341 held in LDS:
-//   columns: 341 columns of N1 points; the 16-bit products Z_p conj(C) go
-//            from global memory straight into the matrix-core A operands,
-//            one radix-N1 matrix-core stage, the four-step twiddle
-//            w_Lc^(n2 k1) on its FP32 outputs, packed into LDS as N1 rows;
-//   rows:    341 = 31 x 11, two matrix-core stages (as in corr_mfma's
-//            second phase); |.| of the last stage's FP32 outputs with the
-//            scales undone is summed over the blocks in registers.
-// Output m = k1 + N1 k2 of phase p is row sample tau = U m + p; a phase's
-// sums go into the thread's share of the row statistics before the next.
+// with a power-of-two scale per (bin, block).  N1, 31 and 11 are pairwise
+// coprime, so each phase transform runs as a prime-factor (Good-Thomas)
+// product Lc = N1 x 31 x 11 with no twiddle between stages, held in LDS by
+// one workgroup per row:
+//   columns: element k' = (341 n1 + N1 n2) mod Lc sits in column n2, row
+//            n1; Z and the code spectrum are stored in that order (storage
+//            word n1 x 341 + c, c = b + 11 a for n2 = (11 a + 31 b) mod 341),
+//            so a tile's 16 columns are 16 consecutive words.  The 16-bit
+//            products Z_p conj(C) go from global memory straight into the
+//            matrix-core A operands, one radix-N1 stage, packed into LDS as
+//            N1 rows of 341 (row k1, word c);
+//   rows:    radix 31 over a (11 butterflies b), radix 11 over b (31
+//            butterflies k_a), both on the matrix cores; |.| of the last
+//            stage's FP32 outputs with the scales undone is summed over the
+//            blocks in registers.
+// Output (k1, k_a, k_b) is transform sample m = (k1 E1 + k2 E2) mod Lc with
+// k2 = (187 k_a + 155 k_b) mod 341 (the CRT idempotents), row sample
+// tau = U m + p; a phase's sums go into the thread's share of the row
+// statistics before the next.
 constexpr int kUpmLD = pad_to(kRedN2 + SeqLayout<1>::SKEW, 32, 1);
+
+// modular inverse of a mod m (gcd 1), for the prime-factor index maps
+__host__ __device__ constexpr int mod_inverse(int a, int m) {
+    for (int x = 1; x < m; ++x)
+        if ((a % m) * x % m == 1) return x;
+    return 0;
+}
+
+// storage word of element k' (< N1 x 341) and the natural index of storage word s
+template <int N1>
+struct UpmPfa {
+    static constexpr int N2 = kRedN2, Lc = N1 * N2;
+    static constexpr int I1 = mod_inverse(N2 % N1, N1), I2 = mod_inverse(N1, N2);
+    // CRT idempotents of Lc = N1 x 341: E1 = 1 mod N1, 0 mod 341; E2 the reverse
+    static constexpr int E1 = N2 * I1 % Lc, E2 = N1 * I2 % Lc;
+    static_assert(I1 > 0 && I2 > 0, "N1 and 341 must be coprime");
+    __host__ __device__ static int pos(int k) {
+        const int n1 = (k % N1) * I1 % N1, n2 = (k % N2) * I2 % N2;
+        const int a = (n2 % 31) * 17 % 31, b = (n2 % 11) * 5 % 11;
+        return n1 * N2 + b + 11 * a;
+    }
+    __host__ __device__ static int natural(int s) {
+        const int n1 = s / N2, c = s - n1 * N2, a = c / 11, b = c - 11 * a;
+        const int n2 = (11 * a + 31 * b) % N2;
+        return (N2 * n1 + N1 * n2) % Lc;
+    }
+};
+static_assert(UpmPfa<24>::E1 == 1705 && UpmPfa<24>::E2 == 6480, "CRT idempotents of 8184");
+static_assert(17 * 11 % 31 == 1 && 5 * 31 % 11 == 1 && 187 % 31 == 1 && 187 % 11 == 0 && 155 % 31 == 0 &&
+              155 % 11 == 1, "prime-factor maps of 341 = 31 x 11");
+
+// in place: each block of Lc 16-bit words from natural order into storage
+// order, through LDS (coalesced reads and writes)
+template <int N1>
+__global__ __launch_bounds__(1024) void upm_permute_kernel(uint32_t* z) {
+    using P = UpmPfa<N1>;
+    __shared__ uint32_t buf[P::Lc];
+    uint32_t* b = z + (int64_t)blockIdx.x * P::Lc;
+    for (int i = threadIdx.x; i < P::Lc; i += blockDim.x) buf[i] = b[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < P::Lc; i += blockDim.x) b[i] = buf[P::natural(i)];
+}
+
+int upm_permute(uint32_t* blocks, int64_t nblocks, int64_t code_len, hipStream_t st) {
+    if (nblocks <= 0) return UGSDR_OK;
+    switch (code_len / kRedN2) {
+        case 12: hipLaunchKernelGGL(upm_permute_kernel<12>, dim3((unsigned)nblocks), dim3(1024), 0, st, blocks); break;
+        case 24: hipLaunchKernelGGL(upm_permute_kernel<24>, dim3((unsigned)nblocks), dim3(1024), 0, st, blocks); break;
+        default: set_error("prime-factor storage order: unsupported code length"); return UGSDR_ERR_UNSUPPORTED;
+    }
+    UGSDR_HIP_TRY(hipGetLastError());
+    return UGSDR_OK;
+}
 
 template <int N1>
 struct UpmShape {
@@ -447,8 +513,7 @@ struct UpmShape {
     static constexpr int CT = (N2 + 15) / 16;   // column tiles of 16 columns
     static constexpr size_t lds_bytes =
         (size_t)2 * BUF * 4 +
-        (size_t)(MfmaShape<N1>::ENTRIES + MfmaShape<31>::ENTRIES + MfmaShape<11>::ENTRIES) * 16 +
-        (size_t)(N1 + N2 + kTwLo + (Lc + kTwLo - 1) / kTwLo) * 8 + 128 * 8;
+        (size_t)(MfmaShape<N1>::ENTRIES + MfmaShape<31>::ENTRIES + MfmaShape<11>::ENTRIES) * 16 + 128 * 8;
 };
 
 template <class H, int N1, int T>
@@ -466,23 +531,12 @@ __global__ __launch_bounds__(T, T / 256) void corr_upm(CorrArgs a) {
     uint4* bt1 = reinterpret_cast<uint4*>(bufM + S::BUF);
     uint4* bt2a = bt1 + MfmaShape<N1>::ENTRIES;
     uint4* bt2b = bt2a + MfmaShape<31>::ENTRIES;
-    float2* twk = reinterpret_cast<float2*>(bt2b + MfmaShape<11>::ENTRIES);  // exp(+2 pi i k1 / Lc)
-    float2* tw2 = twk + N1;                                                  // exp(+2 pi i j / 341)
-    float2* twlo = tw2 + N2;
-    float2* twhi = twlo + kTwLo;
-    double* red = reinterpret_cast<double*>(twhi + (Lc + kTwLo - 1) / kTwLo);
+    double* red = reinterpret_cast<double*>(bt2b + MfmaShape<11>::ENTRIES);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lg = lane >> 4;
     mfma_btab_build<N1, H>(bt1, true, 1.f / N1, tid, T);
     mfma_btab_build<31, H>(bt2a, true, 1.f / 31, tid, T);
     mfma_btab_build<11, H>(bt2b, true, 1.f / 11, tid, T);
-    tw_table_build(tw2, N2, true, tid, T);
-    tw_build(twlo, twhi, Lc, tid, T);
-    for (int j = tid; j < N1; j += T) {
-        double sn, cs;
-        sincospi(2.0 * (double)j / (double)Lc, &sn, &cs);
-        twk[j] = make_float2((float)cs, (float)sn);
-    }
     __syncthreads();
     const int U = a.ucode;
     const int row = up_row_of(blockIdx.x, a.rows, a.row_group);
@@ -544,27 +598,23 @@ __global__ __launch_bounds__(T, T / 256) void corr_upm(CorrArgs a) {
                             Dim[qt] = H::mfma(A[ks], bim, Dim[qt]);
                         }
                     }
-                    // lane (li, lg): output k1 = qt 16 + li of columns s0 .. s0 + 3
+                    // lane (li, lg): output k1 = qt 16 + li of storage columns s0 .. s0 + 3
                     const int s0 = t * 16 + 4 * lg;
 #pragma unroll
                     for (int qt = 0; qt < QT1; ++qt) {
                         const int k1 = qt * 16 + li;
                         if (k1 < N1 && s0 < N2) {
-                            float2 w = tw_get<true>(twlo, twhi, s0 * k1);
-                            const float2 st = twk[k1];
                             uint32_t* o = bufR + Lay::base(k1) + s0;
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                if (s0 + r < N2) o[r] = pack16<H>(cmul(make_float2(Dre[qt][r], Dim[qt][r]), w));
-                                w = cmul(w, st);
-                            }
+                            for (int r = 0; r < 4; ++r)
+                                if (s0 + r < N2) o[r] = pack16<H>(make_float2(Dre[qt][r], Dim[qt][r]));
                         }
                     }
                 }
             }
             __syncthreads();
-            // ---- rows, radix 31 (11 butterflies a row); output (jj, q)
-            // times w_341^(jj q) is the next stage's input
+            // ---- rows, radix 31 over a (11 butterflies b a row): output
+            // (b, k_a) at word 31 b + k_a, no twiddle
             mfma_tiles<31, H, NW, GPP, 0, 11, 11, Lay>(
                 bufR, N1, bt2a, lane, wave, [&](int, int, int sb, int jj, const f32x4* Dre, const f32x4* Dim) {
                     const int s0 = sb * 16 + 4 * lg;
@@ -573,21 +623,19 @@ __global__ __launch_bounds__(T, T / 256) void corr_upm(CorrArgs a) {
                         for (int qt = 0; qt < 2; ++qt) {
                             const int qq = qt * 16 + li;
                             if (qq < 31) {
-                                const float2 w = tw2[jj * qq];
                                 uint32_t* o = bufM + Lay::base(s0) + jj * 31 + qq;
 #pragma unroll
                                 for (int r = 0; r < 4; ++r)
-                                    if (s0 + r < N1)
-                                        o[r * kUpmLD] = pack16<H>(cmul(make_float2(Dre[qt][r], Dim[qt][r]), w));
+                                    if (s0 + r < N1) o[r * kUpmLD] = pack16<H>(make_float2(Dre[qt][r], Dim[qt][r]));
                             }
                         }
                     }
                 });
             __syncthreads();
-            // ---- rows, radix 11 (31 butterflies a row): output k2 = jj + 31 q,
-            // |.| with the Z, replica and 1/U scales undone.  The next block's
-            // second row stage, which overwrites bufM, comes after two more
-            // barriers.
+            // ---- rows, radix 11 over b (31 butterflies k_a a row): output
+            // k_b, |.| with the Z, replica and 1/U scales undone.  The next
+            // block's first row stage, which overwrites bufM, comes after the
+            // barrier that ends its column stage.
             const float sc = q.block_inv[dk] * q.spec_inv;
             mfma_tiles<11, H, NW, GPP, P2, 31, 31, Lay>(
                 bufM, N1, bt2b, lane, wave, [&](int pass, int gi, int, int, const f32x4* Dre, const f32x4* Dim) {
@@ -604,12 +652,13 @@ __global__ __launch_bounds__(T, T / 256) void corr_upm(CorrArgs a) {
                 const int t = mfma_slot_tile<NW, GPP, 31>(pass, gi, N1, wave);
                 if (t >= 0 && li < 11) {
                     const int sb = t / 31, jj = t - sb * 31;
-                    const int k2 = jj + 31 * li;
+                    const int k2 = (187 * jj + 155 * li) % N2;   // CRT of (k_a, k_b) = (jj, li)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int k1 = sb * 16 + 4 * lg + r;
                         if (k1 < N1) {
-                            const int tau = U * (k1 + N1 * k2) + p;
+                            const int m = (k1 * UpmPfa<N1>::E1 + k2 * UpmPfa<N1>::E2) % Lc;
+                            const int tau = U * m + p;
                             const float v = acc[pass][gi][r];
                             top2_merge(v1, i1, v2, v, tau, -1.0f);
                             sum += (double)v;
@@ -698,13 +747,18 @@ static int launch_corr_mfma_n(const CorrArgs& c, int grid, hipStream_t st, int t
     }
 }
 
+// at most 16 workgroups of 256 threads per block (16 atomics on its word)
+static dim3 absmax_grid(int64_t N, int nblk) {
+    return dim3((unsigned)std::min<int64_t>((N + 4095) / 4096, 16), (unsigned)nblk);
+}
+
 int enqueue_block_max(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     const int nblk = (int)(pl.K * pl.F);
     UGSDR_HIP_TRY(s.block_scale.reserve((size_t)nblk * 8 + (pl.ucode ? (size_t)pl.D * pl.K * 4 : 0)));
     unsigned* bmax = s.block_scale.as<unsigned>();
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, st, bmax, nblk);
-    const dim3 grid((unsigned)std::min<int64_t>((pl.N + 1023) / 1024, 64), (unsigned)nblk);
-    hipLaunchKernelGGL(block_absmax_kernel, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.N, bmax);
+    hipLaunchKernelGGL(block_absmax_kernel, absmax_grid(pl.N, nblk), dim3(256), 0, st, s.classes.as<float2>(), pl.N,
+                       bmax);
     UGSDR_HIP_TRY(hipGetLastError());
     return UGSDR_OK;
 }
@@ -717,8 +771,9 @@ int enqueue_reduce_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     unsigned* bmax = s.block_scale.as<unsigned>();
     float* inv = reinterpret_cast<float*>(bmax + nblk);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, st, bmax, nblk);
+    hipLaunchKernelGGL(block_absmax_kernel, absmax_grid(pl.N, nblk), dim3(256), 0, st, s.classes.as<float2>(), pl.N,
+                       bmax);
     const dim3 grid((unsigned)std::min<int64_t>((pl.N + 1023) / 1024, 64), (unsigned)nblk);
-    hipLaunchKernelGGL(block_absmax_kernel, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.N, bmax);
     if (pl.precision == UGSDR_PREC_FP16)
         hipLaunchKernelGGL(block_pack_kernel<Fp16Ops>, grid, dim3(256), 0, st, s.classes.as<float2>(), pl.N, bmax,
                            s.classes_r.as<uint32_t>(), inv);
@@ -829,6 +884,16 @@ int code_spec_reduce(ugsdr_hip_ctx* ctx, Replica& rep, int precision) {
     if (e1 != hipSuccess || e2 != hipSuccess) {
         (void)hipFree(dst);
         set_error(std::string("code spectrum packing: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        return UGSDR_ERR_HIP;
+    }
+    // the phase transforms read it in the prime-factor storage order
+    if (int rc = upm_permute(static_cast<uint32_t*>(dst), 1, rep.code_len, ctx->stream)) {
+        (void)hipFree(dst);
+        return rc;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        (void)hipFree(dst);
+        set_error("code spectrum permutation failed");
         return UGSDR_ERR_HIP;
     }
     UGSDR_HIP_TRY(hipFree(rep.code_spec));
