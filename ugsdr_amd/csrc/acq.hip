@@ -574,6 +574,8 @@ using P240 = PlanCT<16, 5, 3>;
 using P128 = PlanCT<16, 8>;
 using P256 = PlanCT<16, 16>;
 using P4200 = PlanCT<8, 7, 5, 5, 3>;  // 1 ms at 4.2 MHz (the BbpDdc recordings' rate rule)
+// first factor of the forward four-step split 16368 = 48 x 341
+using P48 = PlanCT<16, 3>;
 using P152 = PlanCT<19, 8>;
 using P175 = PlanCT<7, 5, 5>;
 
@@ -594,11 +596,16 @@ static int launch_fwd_large(FwdArgs& a, int blocks, hipStream_t st) {
     return UGSDR_OK;
 }
 
+// batches of at most 16 columns and N1 / 4 rows, so that even one or two
+// transforms spread over tens of workgroups
 template <class P1, class P2>
-static int launch_fwd_split(FwdArgs& a, int blocks, hipStream_t st) {
+static int launch_fwd_split(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratch) {
     constexpr int T = 256, N1 = P1::N, N2 = P2::N;
+    UGSDR_HIP_TRY(scratch.reserve((size_t)blocks * a.N * sizeof(float2)));
+    a.scratch = scratch.as<float2>();
     static_assert((N1 | 1) <= kSplitCap && (N2 | 1) <= kSplitCap, "a sequence exceeds a split batch");
-    const int cb = std::min(kSplitCap / (N1 | 1), N2), rb = std::min(kSplitCap / (N2 | 1), N1);
+    const int cb = std::min({kSplitCap / (N1 | 1), N2, 16});
+    const int rb = std::min(kSplitCap / (N2 | 1), std::max(1, N1 / 4));
     const int nb1 = (N2 + cb - 1) / cb, nb2 = (N1 + rb - 1) / rb;
     const size_t lds0 = (size_t)(kSplitCap + tw_len(N1) + tw_len(a.N)) * sizeof(float2);
     const size_t lds1 = (size_t)(kSplitCap + tw_len(N2)) * sizeof(float2);
@@ -660,12 +667,21 @@ static int launch_forward(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratc
         return UGSDR_ERR_UNSUPPORTED;
     }
     if (small) {
+        // 16368 = 48 x 341 (the wide L1 C/A search, one workgroup's LDS per
+        // transform): columns and rows over many small workgroups, as the
+        // four-step lengths (measured 18.3 against 24.2 us for 40 transforms;
+        // at 4092 with two transforms the two launches cost more than they save)
+        if (a.N == 16368) {
+            a.fs.n = a.N;
+            a.fs.n1 = 48;
+            a.fs.n2 = 341;
+        }
         switch (a.N) {
             case 4092: rc = launch_fwd_small<512, 4096, P4092>(a, blocks, st); break;
             case 8184: rc = launch_fwd_small<512, 8192, P8184>(a, blocks, st); break;
             case 4200: rc = launch_fwd_small<512, 8192, P4200>(a, blocks, st); break;
             case 12276: rc = launch_fwd_small<1024, kLdsCap, P12276>(a, blocks, st); break;
-            case 16368: rc = launch_fwd_small<1024, kLdsCap, P16368>(a, blocks, st); break;
+            case 16368: rc = launch_fwd_split<P48, P341>(a, blocks, st, scratch); break;
             default:
                 if (a.N <= 4096) rc = launch_fwd_small<256, 4096, PlanRT>(a, blocks, st);
                 else if (a.N <= 8192) rc = launch_fwd_small<256, 8192, PlanRT>(a, blocks, st);
@@ -679,15 +695,15 @@ static int launch_forward(FwdArgs& a, int blocks, hipStream_t st, DevBuf& scratc
         FourStep probe;
         const bool ct = fourstep_ct(a.N, probe);
         if (!ct) rc = launch_fwd_large<PlanRT, PlanRT>(a, blocks, st);
-        else if (a.fs.n2 == 175) rc = launch_fwd_split<P152, P175>(a, blocks, st);
-        else if (a.fs.n2 == 256) rc = a.fs.n1 == 128 ? launch_fwd_split<P128, P256>(a, blocks, st)
-                                                       : launch_fwd_split<P256, P256>(a, blocks, st);
+        else if (a.fs.n2 == 175) rc = launch_fwd_split<P152, P175>(a, blocks, st, scratch);
+        else if (a.fs.n2 == 256) rc = a.fs.n1 == 128 ? launch_fwd_split<P128, P256>(a, blocks, st, scratch)
+                                                       : launch_fwd_split<P256, P256>(a, blocks, st, scratch);
         else switch (a.fs.n1) {
-            case 60: rc = launch_fwd_split<P60, P341>(a, blocks, st); break;
-            case 112: rc = launch_fwd_split<P112, P341>(a, blocks, st); break;
-            case 120: rc = launch_fwd_split<P120, P341>(a, blocks, st); break;
-            case 180: rc = launch_fwd_split<P180, P341>(a, blocks, st); break;
-            default: rc = launch_fwd_split<P240, P341>(a, blocks, st); break;
+            case 60: rc = launch_fwd_split<P60, P341>(a, blocks, st, scratch); break;
+            case 112: rc = launch_fwd_split<P112, P341>(a, blocks, st, scratch); break;
+            case 120: rc = launch_fwd_split<P120, P341>(a, blocks, st, scratch); break;
+            case 180: rc = launch_fwd_split<P180, P341>(a, blocks, st, scratch); break;
+            default: rc = launch_fwd_split<P240, P341>(a, blocks, st, scratch); break;
         }
     }
     if (rc) return rc;
@@ -766,7 +782,7 @@ int64_t AcqPlan::footprint(bool surfaces) const {
     int64_t b = in_bytes + need * 8 + K * F * N * 8 + (int64_t)table_bytes + P * D * 32 +
                 P * (int64_t)sizeof(ugsdr_acq_result) + P * L * 4;
     if (Tsub > 1) b += K * Tsub * F * N * 8;
-    if (!small) b += K * Tsub * F * N * 8;
+    if (fwd_scratch_needed()) b += K * Tsub * F * N * 8;
     if (ucode) b += D * K * N * (precision == UGSDR_PREC_FP32 ? 8 : 4);
     if (surfaces) b += P * D * L * 4;
     return b;
@@ -1264,7 +1280,7 @@ int presize_slot(const AcqPlan& pl, AcqSlot& s, bool peaks) {
     const size_t kfn = (size_t)pl.K * pl.F * pl.N;
     UGSDR_HIP_TRY(s.classes.reserve(kfn * sizeof(float2)));
     if (pl.Tsub > 1) UGSDR_HIP_TRY(s.classes_sub.reserve(kfn * pl.Tsub * sizeof(float2)));
-    if (!pl.small) UGSDR_HIP_TRY(s.fwd_scratch.reserve(kfn * pl.Tsub * sizeof(float2)));
+    if (pl.fwd_scratch_needed()) UGSDR_HIP_TRY(s.fwd_scratch.reserve(kfn * pl.Tsub * sizeof(float2)));
     const bool fp32 = pl.precision == UGSDR_PREC_FP32;
     if (pl.ucode) UGSDR_HIP_TRY(s.zbuf.reserve((size_t)pl.D * pl.K * pl.N * (fp32 ? 8 : 4)));
     if (!fp32) {

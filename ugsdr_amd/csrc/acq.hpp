@@ -54,6 +54,10 @@ struct AcqPlan {
     size_t off_ucls = 0, off_ushift = 0;
     // device bytes the request holds while in flight (admission)
     int64_t footprint(bool surfaces) const;
+    // the forward transforms of the class spectra run as split four-step
+    // launches through a scratch of K x Tsub x F transforms: the four-step
+    // lengths, and 16368 = 48 x 341
+    bool fwd_scratch_needed() const { return !small || N == 16368; }
     // batch key: requests sharing it run in one correlation launch
     int64_t bucket() const {
         return (N << 6) | ((int64_t)precision << 3) | (ucode ? 4 : 0) | (fold_mode << 1) | (L == N ? 1 : 0);

@@ -109,7 +109,7 @@ static int acquire_noalloc(ugsdr_hip_ctx* ctx, char* base, const ugsdr_acq_reque
     if (ok && !r->iq_on_device) ok = cv.take(s.iq, (size_t)pl.in_bytes);
     if (ok && keep_surf) ok = cv.take(s.surf, (size_t)pl.P * pl.D * pl.L * sizeof(float));
     if (ok && pl.Tsub > 1) ok = cv.take(s.classes_sub, (size_t)pl.K * pl.Tsub * pl.F * pl.N * sizeof(float2));
-    if (ok && !pl.small) ok = cv.take(s.fwd_scratch, (size_t)pl.K * pl.Tsub * pl.F * pl.N * sizeof(float2));
+    if (ok && pl.fwd_scratch_needed()) ok = cv.take(s.fwd_scratch, (size_t)pl.K * pl.Tsub * pl.F * pl.N * sizeof(float2));
     // correlation workspace: the three-stage kernel's partial surfaces, or one
     // four-step slice per workgroup (fewer workgroups when the array is short:
     // every row is still computed whole by one workgroup)
