@@ -310,8 +310,15 @@ __global__ __launch_bounds__(kBankT) void bank_nco_multi(BankArgs a) {
 // so tap t reads the entry at the sample's quarter index plus q0 + t D, a
 // compile-time offset from one per-sample address: no per-tap index
 // arithmetic.  One channel per workgroup (41 KB of table at 10230 chips),
-// kBankBB blocks per workgroup; carrier and code NCOs as in bank_nco_multi,
-// the carrier re-seeded at the start of every group of kReseed slots.
+// kBankBB blocks per workgroup; code NCO as in bank_nco_multi.
+//
+// Carrier: in each group of kReseed slots starting at s0 a thread's samples
+// are mixed with W[s - s0] = e^(-j 2 pi T df (s - s0)), a per-channel table
+// built in FP64 (one LDS read per slot, the same address across the wave),
+// and the group's tap sums are rotated once by z0 = e^(-j 2 pi phase(n0)) of
+// its first sample (FP64 phase) when the group ends:
+//   sum_s c x_s e^(-j 2 pi phase(n_s)) = z0 sum_s c x_s W[s - s0],
+// so no per-sample phasor recurrence remains.
 template <class S>
 __device__ __forceinline__ float2 load_iq(const S* p, int64_t i) {
     const S v = p[i];
@@ -326,6 +333,7 @@ __global__ __launch_bounds__(kBankT) void bank_nco_quarter(BankArgs a) {
     const int c = blockIdx.x;
     signed char* cq = reinterpret_cast<signed char*>(smem);
     float2* red = reinterpret_cast<float2*>(smem + (LT + 15) / 16 * 16);
+    float2* W = red + (kBankT / 64) * NT;
     for (int i = threadIdx.x; i < LT; i += kBankT) {
         int q = i - M;
         q += q < 0 ? LQ : 0;
@@ -336,13 +344,12 @@ __global__ __launch_bounds__(kBankT) void bank_nco_quarter(BankArgs a) {
     const int q0 = M + (int)llrint(4.0 * a.taps[0]);
     const double f = a.ch[4 * c], th0 = a.ch[4 * c + 1], rate = a.ch[4 * c + 2], tau0 = a.ch[4 * c + 3];
     const double df = f / a.fs, dr = rate / a.fs, L = (double)a.code_len;
-    float2 w;
-    {
-        double st = (double)kBankT * df;
+    for (int s = threadIdx.x; s < kReseed; s += kBankT) {
+        double st = (double)kBankT * df * (double)s;
         st -= rint(st);
         double sn, co;
         sincospi(2.0 * st, &sn, &co);
-        w = make_float2((float)co, (float)-sn);
+        W[s] = make_float2((float)co, (float)-sn);
     }
     const uint64_t inc = (uint64_t)llrint((double)kBankT * dr * kFix);
     const uint32_t LQu = (uint32_t)LQ;
@@ -363,19 +370,22 @@ __global__ __launch_bounds__(kBankT) void bank_nco_quarter(BankArgs a) {
         const S* blk = static_cast<const S*>(a.iq) + (int64_t)b * a.spm;
         for (int s0 = 0; s0 < n_slots; s0 += kReseed) {
             const int64_t nb = tid + (int64_t)s0 * kBankT;
-            float2 z;
+            float2 z0;
             {
                 double ph = fma((double)nb, df, theta);
                 ph -= rint(ph);
                 float sn, cs;
                 sincospif(2.0f * (float)ph, &sn, &cs);
-                z = make_float2(cs, -sn);
+                z0 = make_float2(cs, -sn);
             }
+            float2 g[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) g[t] = make_float2(0.f, 0.f);
             const int s1 = min(n_slots, s0 + kReseed);
             const S* xp = blk + nb;
-            auto step = [&](float2 x) {
-                const float2 m = make_float2(fmaf(x.x, z.x, -x.y * z.y), fmaf(x.x, z.y, x.y * z.x));
-                z = make_float2(fmaf(z.x, w.x, -z.y * w.y), fmaf(z.x, w.y, z.y * w.x));
+            auto step = [&](float2 x, int ds) {
+                const float2 ws = W[ds];
+                const float2 m = make_float2(fmaf(x.x, ws.x, -x.y * ws.y), fmaf(x.x, ws.y, x.y * ws.x));
                 const uint32_t q = (uint32_t)(u >> 42);   // quarter-chip index
                 const uint32_t qw = min(q, q - LQu);
                 u += inc;
@@ -383,8 +393,8 @@ __global__ __launch_bounds__(kBankT) void bank_nco_quarter(BankArgs a) {
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
                     const float cv = (float)p[t * D];
-                    acc[t].x = fmaf(m.x, cv, acc[t].x);
-                    acc[t].y = fmaf(m.y, cv, acc[t].y);
+                    g[t].x = fmaf(m.x, cv, g[t].x);
+                    g[t].y = fmaf(m.y, cv, g[t].y);
                 }
             };
             int s = s0;
@@ -394,9 +404,11 @@ __global__ __launch_bounds__(kBankT) void bank_nco_quarter(BankArgs a) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) x[v] = load_iq(xp, v * kBankT);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) step(x[v]);
+                for (int v = 0; v < 4; ++v) step(x[v], s - s0 + v);
             }
-            for (; s < s1; ++s, xp += kBankT) step(load_iq(xp, 0));
+            for (; s < s1; ++s, xp += kBankT) step(load_iq(xp, 0), s - s0);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = cadd(acc[t], cmul(z0, g[t]));
         }
         const int wv = tid >> 6, l = tid & 63;
 #pragma unroll
@@ -588,7 +600,7 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
         for (int t = 0; t < r->n_taps; ++t) tap_abs = std::max(tap_abs, std::fabs(r->taps_chips[t]));
         const int margin_q = 4 * (int)std::ceil(tap_abs) + 8;
         const size_t lds_q = ((size_t)4 * r->code_len + 2 * margin_q + 15) / 16 * 16 +
-                             (size_t)(kBankT / 64) * r->n_taps * sizeof(float2);
+                             (size_t)(kBankT / 64) * r->n_taps * sizeof(float2) + (size_t)kReseed * sizeof(float2);
         ctx->mark(0);
         if (quarter) {
             a.tap_margin = margin_q;
