@@ -123,6 +123,19 @@ __device__ __forceinline__ float2 ld_off(gptr<float2> base, uint32_t byte_off) {
     return *reinterpret_cast<gptr<float2>>(reinterpret_cast<gptr<char>>(base) + byte_off);
 }
 
+// A raw buffer resource over `bytes` bytes at p (no stride, no swizzle):
+// loads through it take one 32-bit lane offset plus an immediate of up to
+// 4095 bytes, with no 64-bit address arithmetic, and read zero at offsets
+// past `bytes` instead of touching memory there.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 buf_ld2(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_off, 0, 0);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+
 __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
@@ -566,24 +579,31 @@ __device__ __forceinline__ void upw_blocks(float (&S)[33], const UpLds<M>& Ld, c
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     float2* region = Ld.buf + 2 * wv * kC1023;   // the wave's two phases, [33][31] each
     // stage-1 gathers of group g of block kb: butterfly b = 16 g + col of the
-    // wave (phase 2 wv + b / 33, residue b % 33), 8 inputs per lane.  The
-    // lane index is laundered per block (opaque), so the per-group offsets
-    // are recomputed in the loop instead of being hoisted into registers.
+    // wave (phase 2 wv + b / 33, residue b % 33), 8 inputs per lane, through
+    // buffer resources over the wave's two phases of Z and over the code
+    // spectrum.  The lane index is laundered per block (opaque), so the
+    // per-group offsets are recomputed in the loop instead of being hoisted
+    // into registers.
+    const __amdgpu_buffer_rsrc_t Cr = buf_rsrc((const void*)C, kC1023 * 8u);
     auto gather = [&](int t, int kb, int g, float2 (&x)[8], float2 (&c)[8]) {
         const int col = t & 15, kq = (t >> 4) & 3;
-        const gptr<float2> Zb = as_global(q.X + ((int64_t)di * q.K + kb) * N + 2 * wv * kC1023);
+        const __amdgpu_buffer_rsrc_t Zr =
+            buf_rsrc(q.X + ((int64_t)di * q.K + kb) * N + 2 * wv * kC1023, 2 * kC1023 * 8u);
         const int b = min(g * 16 + col, 65);
         const int pp = b >= 33 ? 1 : 0, jj = b - 33 * pp;
-        const unsigned zo = (unsigned)(pp * kC1023);
+        const uint32_t zo = (uint32_t)pp * (kC1023 * 8u);
+        // inputs m = 4 st + kq and 31 - m = (19 - kq) + 4 (3 - st) of
+        // butterfly jj, stored in prime-factor order (element jj + 33 m);
+        // steps of 4 inputs are 1056-byte immediates.  Input 31 - m of m = 0
+        // is element jj + 1023, past the code spectrum (read as zero), and
+        // that product is zeroed by the stage below anyway.
+        const uint32_t o1 = (uint32_t)(jj + 33 * kq) * 8u, o2 = (uint32_t)(jj + 33 * (19 - kq)) * 8u;
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
-            // inputs m and 31 - m of butterfly jj, stored in prime-factor order
-            const int m = 4 * st + kq;
-            const unsigned i1 = jj + 33 * m, i2 = jj + 33 * (m == 0 ? 0 : 31 - m);
-            x[2 * st] = ld_off(Zb, (zo + i1) * 8u);
-            c[2 * st] = ld_off(C, i1 * 8u);
-            x[2 * st + 1] = ld_off(Zb, (zo + i2) * 8u);
-            c[2 * st + 1] = ld_off(C, i2 * 8u);
+            x[2 * st] = buf_ld2(Zr, zo + o1 + 1056u * st);
+            c[2 * st] = buf_ld2(Cr, o1 + 1056u * st);
+            x[2 * st + 1] = buf_ld2(Zr, zo + o2 + 1056u * (3 - st));
+            c[2 * st + 1] = buf_ld2(Cr, o2 + 1056u * (3 - st));
         }
     };
     float2 xv[2][8], cv[2][8];
