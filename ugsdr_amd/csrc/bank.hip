@@ -43,6 +43,8 @@ struct BankArgs {
     int64_t code_len;
     double taps[kMaxTaps];       // chips
     int tap_margin;              // nco mode: chips of wrap padding on each side of the LDS code table
+    int q_entries;               // quarter-table kernel: table entries (a multiple of 16)
+    const struct QSeed* q_seed;  // quarter-table kernel: n_ch x n_blocks per-block seeds
     // ref_split mode
     const float* code3;          // n_ch x code3_len
     int64_t code3_len;
@@ -184,6 +186,9 @@ __global__ __launch_bounds__(kBankT) void bank_nco_kernel(BankArgs a) {
 constexpr int kBankCH = 4;    // channels per workgroup
 constexpr int kBankBB = 4;    // blocks per workgroup
 constexpr int kReseed = 32;   // slots between carrier re-seeds
+constexpr int kQSlots = 256;  // quarter-table kernel: most slots per block (spm <= kQSlots T)
+constexpr int kQWrap = 8;     // quarter-table kernel: slots between code-index wrap checks
+constexpr int kQSub = 1;      // quarter-table kernel: sub-groups of kBankT threads per workgroup
 constexpr double kFix = 17592186044416.0;   // 2^44
 
 template <int NT>
@@ -307,18 +312,73 @@ __global__ __launch_bounds__(kBankT) void bank_nco_multi(BankArgs a) {
 // NCO mode for taps a quarter chip apart (the E / P / L and VE / VL taps,
 // listed in either direction): with the code expanded to one entry per
 // quarter chip, floor(u + k/4) = floor((floor(4u) + k) / 4) for integer k,
-// so tap t reads the entry at the sample's quarter index plus q0 + t D, a
-// compile-time offset from one per-sample address: no per-tap index
-// arithmetic.  One channel per workgroup (41 KB of table at 10230 chips),
-// kBankBB blocks per workgroup; code NCO as in bank_nco_multi.
+// so tap t reads the entry at the sample's quarter index plus a
+// compile-time offset: no per-tap index arithmetic.  One channel per
+// workgroup of kQSub sub-groups of kBankT threads sharing the channel's
+// table; each sub-group correlates its own block at a time, kBankBB blocks
+// each.  A thread takes the block's samples n = tid + s T (slot s < 256).
 //
-// Carrier: in each group of kReseed slots starting at s0 a thread's samples
-// are mixed with W[s - s0] = e^(-j 2 pi T df (s - s0)), a per-channel table
-// built in FP64 (one LDS read per slot, the same address across the wave),
-// and the group's tap sums are rotated once by z0 = e^(-j 2 pi phase(n0)) of
-// its first sample (FP64 phase) when the group ends:
-//   sum_s c x_s e^(-j 2 pi phase(n_s)) = z0 sum_s c x_s W[s - s0],
-// so no per-sample phasor recurrence remains.
+// Table: int8 entries, entry i holding quarter index (i - M) mod 4L (M, a
+// multiple of 4, pads the start for the taps below the sample's index), and
+// a.q_entries - M - 4L entries past the period that let the index run up to
+// kQWrap slots beyond it between wrap checks.  Each aligned 32-bit word
+// holds one chip's four entries, so the table is written a chip per store.
+//
+// Code NCO: a 32.32 fixed-point entry index per thread (the integer half is
+// the table position of the thread's lowest tap, so the taps' reads are
+// immediates off it), seeded per block from the FP64 code phase and
+// advanced by round(T 4 dr 2^32) per slot: one 64-bit add per sample, drift
+// below 2^-32 entry per slot.  Every kQWrap slots an index past the period
+// steps back by 4L.
+//
+// Carrier: a thread's samples are mixed with W[s] = e^(-j 2 pi T df s), a
+// per-channel table built in FP64 (one LDS read per slot, the same address
+// across the wave), and the block's tap sums are rotated once by
+// z0 = e^(-j 2 pi phase(b spm + tid)) = E_b e^(-j 2 pi tid df):
+//   sum_s c x_s e^(-j 2 pi phase(n_s)) = z0 sum_s c x_s W[s].
+// The per-block seeds (E_b and the block's code index) come from FP64
+// phases computed once per (channel, block) by bank_quarter_seed.
+struct QSeed {
+    float2 e;     // e^(-j 2 pi phase) of the block's first sample
+    uint64_t u;   // 32.32 table position of its lowest tap
+};
+
+// constant-memory view of the per-block seeds (read through the scalar
+// cache, the same entry across a sub-group) and LDS view of the table, with
+// an LDS pointer's 32-bit address (device pass only)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(4))) const QSeed cqseed;
+typedef __attribute__((address_space(3))) const signed char lds_i8;
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const void*)p;
+}
+#else
+typedef const QSeed cqseed;
+typedef const signed char lds_i8;
+__host__ __device__ inline uint32_t lds_addr(const void*) { return 0; }
+#endif
+
+// the lowest tap's offset in quarter chips (taps listed in direction D)
+__host__ __device__ inline int quarter_lo(const BankArgs& a, int nt, int d) {
+    return (int)llrint(4.0 * a.taps[d > 0 ? 0 : nt - 1]);
+}
+
+__global__ __launch_bounds__(256) void bank_quarter_seed(BankArgs a, int nt, int d, QSeed* seeds) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n_ch * a.n_blocks) return;
+    const int c = i / a.n_blocks, b = i - c * a.n_blocks;
+    const double f = a.ch[4 * c], th0 = a.ch[4 * c + 1], rate = a.ch[4 * c + 2], tau0 = a.ch[4 * c + 3];
+    const double L = (double)a.code_len, tb = (double)b * ((double)a.spm / a.fs);
+    double theta = fma(f, tb, th0);
+    theta -= rint(theta);
+    double sn, co;
+    sincospi(2.0 * theta, &sn, &co);
+    double tau = fmod(fma(rate, tb, tau0), L);
+    tau += tau < 0 ? L : 0.0;
+    const double q = 4.0 * tau + (double)(a.tap_margin + quarter_lo(a, nt, d));
+    seeds[i] = QSeed{make_float2((float)co, (float)-sn), (uint64_t)llrint(q * 4294967296.0)};
+}
+
 template <class S>
 __device__ __forceinline__ float2 load_iq(const S* p, int64_t i) {
     const S v = p[i];
@@ -326,100 +386,115 @@ __device__ __forceinline__ float2 load_iq(const S* p, int64_t i) {
 }
 
 template <int NT, int D, class S>
-__global__ __launch_bounds__(kBankT) void bank_nco_quarter(BankArgs a) {
+__global__ __launch_bounds__(kQSub * kBankT) void bank_nco_quarter(BankArgs a) {
+    constexpr int TQ = kQSub * kBankT;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int Li = (int)a.code_len, M = a.tap_margin;   // M: quarter entries of padding on each side
-    const int LQ = 4 * Li, LT = LQ + 2 * M;
+    const int Li = (int)a.code_len, M = a.tap_margin;
+    const int LQ = 4 * Li, LT = a.q_entries;
     const int c = blockIdx.x;
+    const int sub = threadIdx.x / kBankT, tid = threadIdx.x - sub * kBankT;
     signed char* cq = reinterpret_cast<signed char*>(smem);
-    float2* red = reinterpret_cast<float2*>(smem + (LT + 15) / 16 * 16);
-    float2* W = red + (kBankT / 64) * NT;
-    for (int i = threadIdx.x; i < LT; i += kBankT) {
-        int q = i - M;
-        q += q < 0 ? LQ : 0;
-        q -= q >= LQ ? LQ : 0;
-        cq[i] = a.codes[(int64_t)c * a.code_len + (q >> 2)];
+    float2* red = reinterpret_cast<float2*>(smem + LT);
+    float2* W = red + kQSub * (kBankT / 64) * NT;
+    {
+        // word w: the four entries of chip (w - M/4) mod L
+        const signed char* code = a.codes + (int64_t)c * a.code_len;
+        const int step = TQ % Li;
+        int chip = ((int)threadIdx.x - M / 4) % Li;
+        chip += chip < 0 ? Li : 0;
+        for (int w = threadIdx.x; w < LT / 4; w += TQ) {
+            reinterpret_cast<uint32_t*>(cq)[w] = (uint32_t)(uint8_t)code[chip] * 0x01010101u;
+            chip += step;
+            chip -= chip >= Li ? Li : 0;
+        }
     }
-    // tap 0's offset in quarter chips, plus the margin
-    const int q0 = M + (int)llrint(4.0 * a.taps[0]);
-    const double f = a.ch[4 * c], th0 = a.ch[4 * c + 1], rate = a.ch[4 * c + 2], tau0 = a.ch[4 * c + 3];
-    const double df = f / a.fs, dr = rate / a.fs, L = (double)a.code_len;
-    for (int s = threadIdx.x; s < kReseed; s += kBankT) {
+    const double df = a.ch[4 * c] / a.fs, dr = a.ch[4 * c + 2] / a.fs;
+    const int n_w = (int)((a.spm + kBankT - 1) / kBankT);
+    for (int s = threadIdx.x; s < n_w; s += TQ) {
         double st = (double)kBankT * df * (double)s;
         st -= rint(st);
         double sn, co;
         sincospi(2.0 * st, &sn, &co);
         W[s] = make_float2((float)co, (float)-sn);
     }
-    const uint64_t inc = (uint64_t)llrint((double)kBankT * dr * kFix);
-    const uint32_t LQu = (uint32_t)LQ;
-    const int tid = threadIdx.x;
+    // the thread's offsets from the block's first sample: carrier phasor and
+    // code index
+    float2 zt;
+    {
+        double ph = (double)tid * df;
+        ph -= rint(ph);
+        double sn, co;
+        sincospi(2.0 * ph, &sn, &co);
+        zt = make_float2((float)co, (float)-sn);
+    }
+    // (the integer half also carries the table's LDS address, so a sample's
+    // tap reads take it as their address as it is)
+    const uint64_t ut = (uint64_t)llrint(4.0 * (double)tid * dr * 4294967296.0) +
+                        ((uint64_t)lds_addr(cq) << 32);
+    const uint64_t inc = (uint64_t)llrint((double)kBankT * 4.0 * dr * 4294967296.0);
+    const uint32_t wrap_at = lds_addr(cq) + (uint32_t)(M + quarter_lo(a, NT, D) + LQ);
+    const uint64_t wrap_step = (uint64_t)LQ << 32;
     const int n_slots = a.spm > tid ? (int)((a.spm - tid + kBankT - 1) / kBankT) : 0;
+    const cqseed* seeds = (const cqseed*)(a.q_seed + (int64_t)c * a.n_blocks);
+    const int wv = tid >> 6, l = tid & 63;
     __syncthreads();
-    const int b_end = min(a.n_blocks, (int)(blockIdx.y + 1) * kBankBB);
-    for (int b = blockIdx.y * kBankBB; b < b_end; ++b) {
-        const double n0 = (double)b * (double)a.spm;
-        double theta = fma(f, n0 / a.fs, th0);
-        theta -= floor(theta);
-        double tau = fmod(fma(rate, n0 / a.fs, tau0), L);
-        tau += tau < 0 ? L : 0.0;
-        uint64_t u = (uint64_t)((tau + (double)tid * dr) * kFix);
+    const int b0 = blockIdx.y * (kQSub * kBankBB);
+    for (int i = 0; i < kBankBB; ++i) {
+        if (b0 + i * kQSub >= a.n_blocks) break;   // the same for every sub-group
+        const int b = b0 + i * kQSub + sub;
         float2 acc[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[t] = make_float2(0.f, 0.f);
-        const S* blk = static_cast<const S*>(a.iq) + (int64_t)b * a.spm;
-        for (int s0 = 0; s0 < n_slots; s0 += kReseed) {
-            const int64_t nb = tid + (int64_t)s0 * kBankT;
-            float2 z0;
-            {
-                double ph = fma((double)nb, df, theta);
-                ph -= rint(ph);
-                float sn, cs;
-                sincospif(2.0f * (float)ph, &sn, &cs);
-                z0 = make_float2(cs, -sn);
-            }
+        if (b < a.n_blocks) {
+            const QSeed sd = seeds[b];
+            const float2 z0 = cmul(sd.e, zt);
+            uint64_t u = sd.u + ut;
             float2 g[NT];
 #pragma unroll
             for (int t = 0; t < NT; ++t) g[t] = make_float2(0.f, 0.f);
-            const int s1 = min(n_slots, s0 + kReseed);
-            const S* xp = blk + nb;
-            auto step = [&](float2 x, int ds) {
-                const float2 ws = W[ds];
+            const S* xp = static_cast<const S*>(a.iq) + (int64_t)b * a.spm + tid;
+            auto step = [&](float2 x, int s) {
+                const float2 ws = W[s];
                 const float2 m = make_float2(fmaf(x.x, ws.x, -x.y * ws.y), fmaf(x.x, ws.y, x.y * ws.x));
-                const uint32_t q = (uint32_t)(u >> 42);   // quarter-chip index
-                const uint32_t qw = min(q, q - LQu);
+                const lds_i8* p = (const lds_i8*)(uintptr_t)(uint32_t)(u >> 32);
                 u += inc;
-                const signed char* p = cq + q0 + qw;
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
-                    const float cv = (float)p[t * D];
+                    const float cv = (float)p[D > 0 ? t : NT - 1 - t];
                     g[t].x = fmaf(m.x, cv, g[t].x);
                     g[t].y = fmaf(m.y, cv, g[t].y);
                 }
             };
-            int s = s0;
-            // four samples' loads in flight before their arithmetic
-            for (; s + 4 <= s1; s += 4, xp += 4 * kBankT) {
-                float2 x[4];
+            for (int s = 0; s < n_slots; s += kQWrap) {
+                if ((uint32_t)(u >> 32) >= wrap_at) u -= wrap_step;
+                if (s + kQWrap <= n_slots) {
+                    // four samples' loads in flight before their arithmetic
 #pragma unroll
-                for (int v = 0; v < 4; ++v) x[v] = load_iq(xp, v * kBankT);
+                    for (int h = 0; h < kQWrap; h += 4) {
+                        float2 x[4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) step(x[v], s - s0 + v);
+                        for (int v = 0; v < 4; ++v) x[v] = load_iq(xp, (h + v) * kBankT);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) step(x[v], s + h + v);
+                    }
+                    xp += kQWrap * kBankT;
+                } else {
+                    for (int r = s; r < n_slots; ++r, xp += kBankT) step(load_iq(xp, 0), r);
+                }
             }
-            for (; s < s1; ++s, xp += kBankT) step(load_iq(xp, 0), s - s0);
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[t] = cadd(acc[t], cmul(z0, g[t]));
+            for (int t = 0; t < NT; ++t) acc[t] = cmul(z0, g[t]);
         }
-        const int wv = tid >> 6, l = tid & 63;
+        float2* rs = red + sub * (kBankT / 64) * NT;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const float sx = wave_sum(acc[t].x), sy = wave_sum(acc[t].y);
-            if (l == 0) red[wv * NT + t] = make_float2(sx, sy);
+            if (l == 0) rs[wv * NT + t] = make_float2(sx, sy);
         }
         __syncthreads();
-        if (tid < NT) {
+        if (tid < NT && b < a.n_blocks) {
             float2 sum = make_float2(0.f, 0.f);
-            for (int i = 0; i < kBankT / 64; ++i) sum = cadd(sum, red[i * NT + tid]);
+            for (int k = 0; k < kBankT / 64; ++k) sum = cadd(sum, rs[k * NT + tid]);
             a.out[((int64_t)b * a.n_ch + c) * NT + tid] = sum;
         }
         __syncthreads();
@@ -428,9 +503,10 @@ __global__ __launch_bounds__(kBankT) void bank_nco_quarter(BankArgs a) {
 
 template <int NT, int D>
 static void launch_quarter(const BankArgs& a, dim3 g, size_t lds, hipStream_t st) {
-    if (a.fmt == UGSDR_FMT_CI16) hipLaunchKernelGGL((bank_nco_quarter<NT, D, short2>), g, dim3(kBankT), lds, st, a);
-    else if (a.fmt == UGSDR_FMT_CI8) hipLaunchKernelGGL((bank_nco_quarter<NT, D, char2>), g, dim3(kBankT), lds, st, a);
-    else hipLaunchKernelGGL((bank_nco_quarter<NT, D, float2>), g, dim3(kBankT), lds, st, a);
+    const dim3 T(kQSub * kBankT);
+    if (a.fmt == UGSDR_FMT_CI16) hipLaunchKernelGGL((bank_nco_quarter<NT, D, short2>), g, T, lds, st, a);
+    else if (a.fmt == UGSDR_FMT_CI8) hipLaunchKernelGGL((bank_nco_quarter<NT, D, char2>), g, T, lds, st, a);
+    else hipLaunchKernelGGL((bank_nco_quarter<NT, D, float2>), g, T, lds, st, a);
 }
 
 template <int NT, int D>
@@ -556,7 +632,9 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
             chs[4 * c + 3] = ch[c].code_phase;
         }
         const size_t code_bytes = (size_t)r->n_channels * r->code_len;
-        UGSDR_HIP_TRY(ctx->tables.reserve(chs.size() * 8 + code_bytes));
+        // the quarter-table kernel's per-block seeds follow the codes
+        const size_t seed_off = (chs.size() * 8 + code_bytes + 15) / 16 * 16;
+        UGSDR_HIP_TRY(ctx->tables.reserve(seed_off + (size_t)r->n_channels * r->n_blocks * sizeof(QSeed)));
         UGSDR_HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, chs.data(), chs.size() * 8, hipMemcpyHostToDevice, ctx->stream));
         UGSDR_HIP_TRY(hipMemcpyAsync(ctx->tables.as<char>() + chs.size() * 8, r->codes, code_bytes,
                                      hipMemcpyHostToDevice, ctx->stream));
@@ -584,8 +662,9 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
                                  (size_t)(kBankT / 64) * kBankCH * nt * sizeof(float2);
         multi = multi && lds_multi <= 64 * 1024;
         // taps a quarter chip apart (4 o_t = 4 o_0 + t D, D = +-1; three or
-        // five of them) with codes up to 12000 chips take the quarter-table
-        // kernel (4 L + 2 margin bytes of LDS)
+        // five of them) take the quarter-table kernel where its table (about
+        // 4 L bytes) leaves room for three workgroups per CU and a block is
+        // at most kQSlots slots
         int dir = 0;
         if (r->n_taps == 3 || r->n_taps == 5) {
             const double d = 4.0 * (r->taps_chips[1] - r->taps_chips[0]);
@@ -595,16 +674,32 @@ int bank_impl(ugsdr_hip_ctx* ctx, const ugsdr_bank_request* r, ugsdr_channel_sta
             for (int t = 0; t < r->n_taps && dir; ++t)
                 if (std::fabs(4.0 * r->taps_chips[t] - (q0 + t * dir)) > 1e-9) dir = 0;
         }
-        const bool quarter = multi && r->code_len <= 12000 && dir != 0;
         double tap_abs = 0;
         for (int t = 0; t < r->n_taps; ++t) tap_abs = std::max(tap_abs, std::fabs(r->taps_chips[t]));
         const int margin_q = 4 * (int)std::ceil(tap_abs) + 8;
-        const size_t lds_q = ((size_t)4 * r->code_len + 2 * margin_q + 15) / 16 * 16 +
-                             (size_t)(kBankT / 64) * r->n_taps * sizeof(float2) + (size_t)kReseed * sizeof(float2);
+        // quarter table: the period, the start margin, and past the period
+        // the margin plus kQWrap slots of index advance (the fastest
+        // channel's whole entries per slot, plus its carry) and the taps
+        double dr_max = 0;
+        for (int c = 0; c < r->n_channels; ++c) dr_max = std::max(dr_max, ch[c].code_rate / r->fs);
+        const int64_t q_entries =
+            (4 * r->code_len + 2 * margin_q + kQWrap * ((int64_t)std::floor(kBankT * 4.0 * dr_max) + 1) + r->n_taps +
+             15) / 16 * 16;
+        const int64_t n_w = (r->spm + kBankT - 1) / kBankT;
+        const size_t lds_q =
+            (size_t)q_entries + (size_t)(kQSub * (kBankT / 64) * r->n_taps + n_w) * sizeof(float2);
+        const bool quarter = multi && dir != 0 && n_w <= kQSlots && lds_q <= (kQSub > 1 ? 156 : 52) * 1024;
         ctx->mark(0);
         if (quarter) {
             a.tap_margin = margin_q;
-            const dim3 g((unsigned)r->n_channels, (unsigned)((r->n_blocks + kBankBB - 1) / kBankBB));
+            a.q_entries = (int)q_entries;
+            QSeed* seeds = reinterpret_cast<QSeed*>(ctx->tables.as<char>() + seed_off);
+            const int n_seed = r->n_channels * (int)r->n_blocks;
+            hipLaunchKernelGGL(bank_quarter_seed, dim3((unsigned)((n_seed + 255) / 256)), dim3(256), 0, ctx->stream, a,
+                               (int)r->n_taps, dir, seeds);
+            a.q_seed = seeds;
+            const dim3 g((unsigned)r->n_channels,
+                         (unsigned)((r->n_blocks + kQSub * kBankBB - 1) / (kQSub * kBankBB)));
             if (r->n_taps == 3) {
                 UGSDR_HIP_TRY((dir > 0 ? quarter_lds<3, 1>(lds_q) : quarter_lds<3, -1>(lds_q)));
                 if (dir > 0) launch_quarter<3, 1>(a, g, lds_q, ctx->stream);
