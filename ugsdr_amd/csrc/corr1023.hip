@@ -142,7 +142,16 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
 
 // Blocks k0 .. k1-1 of one row into the thread's surface samples
 // S[s][u] = surface[tid + s T + u 1023].
-template <int M, int QL>
+//
+// PFA (M = 4, 8, 16, coprime to 1023 = 31 x 33; unshifted rows, no fold):
+// the transform runs as a three-factor prime-factor (Good-Thomas) transform
+// N = 31 x 33 x M with no twiddle multiplies.  The spectra are stored in
+// its input order (pfam_elem), so stage-1 butterfly j = n_b M + n_c reads
+// input n_a at j + n_a 33M as in the Stockham order; stage 2 and stage 3
+// are plain 33- and M-point DFTs over the same LDS layout, and the thread's
+// output (j, u) -- j = k_b 31 + k_a -- is sample k of the CRT map
+// k = k_a mod 31, k_b mod 33, u mod M (pfa_sample).
+template <int M, int QL, bool PFA = false>
 __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT][QL], const C1023Lds<M>& Ld,
                                                 const ReqDesc& q, int cls, int sh, gptr<float2> C, int k0, int k1,
                                                 int fold_mode, int tid) {
@@ -254,12 +263,14 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
             float2 u[33];
 #pragma unroll
             for (int r = 0; r < 33; ++r) u[r] = buf[j + r * B2];
-            // twiddles in groups of 8 reads, each group after the last one's products
-            int tb = kk * 33;
+            if constexpr (!PFA) {
+                // twiddles in groups of 8 reads, each group after the last one's products
+                int tb = kk * 33;
 #pragma unroll
-            for (int r = 1; r < 33; ++r) {
-                if (r > 1 && (r - 1) % 8 == 0) tb = after(tb, u[r - 1].x);
-                u[r] = cmul(u[r], tw2[tb + r]);
+                for (int r = 1; r < 33; ++r) {
+                    if (r > 1 && (r - 1) % 8 == 0) tb = after(tb, u[r - 1].x);
+                    u[r] = cmul(u[r], tw2[tb + r]);
+                }
             }
             Pfa33<true>::prep(u);
             __syncthreads();
@@ -279,12 +290,15 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
             if (s > 0) order_after(S[s - 1][0]);
             const int j = opaque(t) + s * T;
             if (j < kC1023) {
-                float2 w[M], tw[M];
+                float2 w[M];
 #pragma unroll
                 for (int r = 0; r < M; ++r) w[r] = buf[j + r * kC1023];
-                twiddle_powers<M>(tw3[j], tw);
+                if constexpr (!PFA) {
+                    float2 tw[M];
+                    twiddle_powers<M>(tw3[j], tw);
 #pragma unroll
-                for (int r = 1; r < M; ++r) w[r] = cmul(w[r], tw[r]);
+                    for (int r = 1; r < M; ++r) w[r] = cmul(w[r], tw[r]);
+                }
                 Dft<M, true>::run(w);
                 if (fold_mode == 0) {
 #pragma unroll
@@ -854,17 +868,40 @@ bool upsampled_code_supported(int64_t code_len, int64_t U) {
     return U == 2 || U == 3 || U == 4 || U == 5 || U == 8 || U == 10 || U == 12 || U == 16;
 }
 
+// the prime-factor engine's sample for stage-3 output u of thread slot j
+// (j = k_b 31 + k_a): k = k_a mod 31, k_b mod 33, u mod M, through
+// k' = (528 k_a + 496 k_b) mod 1023 (528 = 1 mod 31 = 0 mod 33, 496 = 0 mod
+// 31 = 1 mod 33) and k = (1024 k' + E u) mod 1023 M with 1024 = 1 mod 1023 =
+// 0 mod M and E = 0 mod 1023 = 1 mod M
+template <int M>
+__host__ __device__ constexpr int pfa_e() {
+    int e = 0;
+    while ((kC1023 * e) % M != 1) ++e;
+    return kC1023 * e;
+}
+template <int M>
+__device__ __forceinline__ int pfa_sample(int j, int u) {
+    static_assert(1024 % M == 0 && kC1023 % M != 0, "M must divide 1024");
+    const int ka = j % 31, kb = j / 31;
+    const int k1 = (528 * ka + 496 * kb) % kC1023;
+    return (1024 * k1 + pfa_e<M>() * u) % (kC1023 * M);
+}
+
 // Upsampled-code search for codes of 1023 M chips held for U samples each
 // (the BOC(1,1) 4 ms code: 8184 half-chips of 10 samples at 20.46 MHz): per
 // row, U phase transforms of 1023 M points (phase p's input Z_p conj(C),
 // Z_p of bin d and block k at ((d K + k) U + p) 1023 M), each over every
-// block on the three-stage engine, its |.| sums (row samples
-// tau = U (j + 1023 u) + p) folded into the thread's share of the row
-// statistics before the next phase.  Doppler-bin-major rows, XCD-grouped.
+// block on the three-stage engine (its twiddle-free prime-factor form for
+// M = 4, 8, 16), its |.| sums (row samples tau = U k + p, k the output's
+// sample: j + 1023 u, or pfa_sample) folded into the thread's share of the
+// row statistics before the next phase.  Doppler-bin-major rows,
+// XCD-grouped.
 template <int M>
 __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void corr1023p(CorrArgs a) {
     using S_ = C1023Shape<M>;
     constexpr int NSLOT = S_::NSLOT, T = S_::T;
+    // M coprime to 1023: the twiddle-free prime-factor engine
+    constexpr bool PFA = M == 4 || M == 8 || M == 16;
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     const C1023Lds<M> Ld(smem, tid);
@@ -884,14 +921,16 @@ __global__ __launch_bounds__(C1023Shape<M>::T, C1023Shape<M>::WAVES_PER_EU) void
         for (int s = 0; s < NSLOT; ++s)
 #pragma unroll
             for (int u = 0; u < M; ++u) S[s][u] = 0.f;
-        corr1023_blocks<M, M>(S, Ld, q, rr.di * q0.K * U + p, 0, as_global(q0.specs[rr.p]), 0, q0.K, 0, tid);
+        corr1023_blocks<M, M, PFA>(S, Ld, q, rr.di * q0.K * U + p, 0, as_global(q0.specs[rr.p]), 0, q0.K, 0, tid);
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s) {
             const int j = tid + s * T;
             if (j < kC1023) {
 #pragma unroll
                 for (int u = 0; u < M; ++u) {
-                    const int tau = U * (j + kC1023 * u) + p;
+                    int k = j + kC1023 * u;
+                    if constexpr (PFA) k = pfa_sample<M>(j, u);
+                    const int tau = U * k + p;
                     const float v = S[s][u];
                     top2_merge(v1, i1, v2, v, tau, -1.0f);
                     sum += (double)v;
@@ -1033,13 +1072,61 @@ __global__ __launch_bounds__(PFA ? 1024 : 256) void zprep_kernel(const float2* X
     }
 }
 
+// Prime-factor storage order of one 1023 M-point phase (M = 4, 8, 16): the
+// element at position n_a 33M + n_b M + n_c is n = (n_a 33M + n_b 31M +
+// n_c 1023) mod 1023 M (n_a < 31, n_b < 33, n_c < M).
+template <int M>
+__host__ __device__ inline int pfam_elem(int pos) {
+    constexpr int B1 = 33 * M, N = kC1023 * M;
+    const int na = pos / B1, j = pos - na * B1, nb = j / M, nc = j - nb * M;
+    return (na * B1 + nb * 31 * M + nc * kC1023) % N;
+}
+
+// one phase per workgroup, in place: the phase into LDS, then each position
+// from its element (coalesced both ways; the LDS gathers stride 1023
+// elements, conflict-free)
+template <int M>
+__global__ __launch_bounds__(1024) void pfam_permute_kernel(float2* data) {
+    constexpr int N = kC1023 * M;
+    extern __shared__ __align__(16) unsigned char smem[];
+    float2* ph = reinterpret_cast<float2*>(smem);
+    float2* d = data + (int64_t)blockIdx.x * N;
+    for (int i = threadIdx.x; i < N; i += 1024) ph[i] = d[i];
+    __syncthreads();
+    for (int pos = threadIdx.x; pos < N; pos += 1024) d[pos] = ph[pfam_elem<M>(pos)];
+}
+
+template <int M>
+static int launch_pfam_permute(float2* data, int64_t phases, hipStream_t st) {
+    const size_t lds = (size_t)kC1023 * M * sizeof(float2);
+    UGSDR_HIP_TRY(lds_attr(pfam_permute_kernel<M>, lds));
+    hipLaunchKernelGGL(pfam_permute_kernel<M>, dim3((unsigned)phases), dim3(1024), lds, st, data);
+    return UGSDR_OK;
+}
+
+// the prime-factor order for phases of 1023 M points where M is 4, 8 or 16
+// (other lengths stay in natural order)
+static int pfam_permute(float2* data, int64_t len, int64_t phases, hipStream_t st) {
+    switch (len) {
+        case 4 * kC1023: return launch_pfam_permute<4>(data, phases, st);
+        case 8 * kC1023: return launch_pfam_permute<8>(data, phases, st);
+        case 16 * kC1023: return launch_pfam_permute<16>(data, phases, st);
+        default: return UGSDR_OK;
+    }
+}
+
 __global__ void pfa1023_permute_kernel(const float2* in, float2* out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < kC1023) out[pfa1023_pos(k)] = in[k];
 }
 
 int code_spectrum_layout(ugsdr_hip_ctx* ctx, int64_t code_len, float2* spec) {
-    if (code_len != kC1023) return UGSDR_OK;
+    if (code_len != kC1023) {
+        if (int rc = pfam_permute(spec, code_len, 1, ctx->stream)) return rc;
+        UGSDR_HIP_TRY(hipGetLastError());
+        UGSDR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return UGSDR_OK;
+    }
     UGSDR_HIP_TRY(ctx->aux.reserve((size_t)kC1023 * sizeof(float2)));
     UGSDR_HIP_TRY(hipMemcpyAsync(ctx->aux.ptr, spec, (size_t)kC1023 * sizeof(float2), hipMemcpyDeviceToDevice,
                                  ctx->stream));
@@ -1103,8 +1190,10 @@ int launch_zprep(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
         default: set_error("upsampled-code search of an unsupported factor"); return UGSDR_ERR_UNSUPPORTED;
     }
     UGSDR_HIP_TRY(hipGetLastError());
-    // the reduced form's phase transforms read Z in prime-factor order
+    // the reduced form's phase transforms read Z in prime-factor order, and
+    // so do the FP32 ones of 1023 M points with M = 4, 8, 16
     if (!fp32) return upm_permute(s.zbuf.as<uint32_t>(), pl.D * pl.K * pl.ucode, pl.N / pl.ucode, st);
+    if (!pfa) return pfam_permute(s.zbuf.as<float2>(), pl.N / pl.ucode, pl.D * pl.K * pl.ucode, st);
     return UGSDR_OK;
 }
 

@@ -18,9 +18,11 @@ int replica_spectrum(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chips, i
 // whether the upsampled-code search handles a code of code_len chips held
 // for U samples each (corr1023.hip)
 bool upsampled_code_supported(int64_t code_len, int64_t U);
-// the upsampled-code search's storage order of a code's conj spectrum (in
-// place, on ctx's stream): 1023-chip codes in the prime-factor input order of
-// the wave-autonomous kernels, longer codes unchanged (corr1023.hip)
+// the FP32 upsampled-code search's storage order of a code's conj spectrum
+// (in place, on ctx's stream): 1023-chip codes in the prime-factor input
+// order of the wave-autonomous kernels, 1023 M-chip codes with M = 4, 8, 16
+// in that of the prime-factor phase transforms, others unchanged
+// (corr1023.hip)
 int code_spectrum_layout(ugsdr_hip_ctx* ctx, int64_t code_len, float2* spec);
 // resolves the no-alloc workspace's device address on `device` (current),
 // once per device (noalloc.hip)

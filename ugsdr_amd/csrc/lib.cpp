@@ -318,7 +318,7 @@ static int replica_store(ugsdr_hip_ctx* ctx, const int8_t* chips, int64_t n_chip
         rc = hipMalloc(&rep.code_spec, (size_t)n_chips * sizeof(float2)) == hipSuccess
                  ? replica_spectrum(ctx, chips, n_chips, 1, nullptr, n_chips, static_cast<float2*>(rep.code_spec))
                  : (set_error("hipMalloc failed"), UGSDR_ERR_HIP);
-        if (!rc) rc = code_spectrum_layout(ctx, n_chips, static_cast<float2*>(rep.code_spec));
+        if (!rc && fp32) rc = code_spectrum_layout(ctx, n_chips, static_cast<float2*>(rep.code_spec));
         rep.code_len = n_chips;
         if (!rc && !fp32) rc = code_spec_reduce(ctx, rep, precision);
         if (rc) {
