@@ -325,9 +325,18 @@ def extra_workloads(ctx):
     for name, steps in (("L1CA_1MS", 50), ("L1CA_4MS", 30), ("BOC_4MS", 3)):
         w = W.CONFIGS[name]()
         dt, stages, res = time_search(ctx, w, steps, 2)
+        form = ctx.last_search_form()
+        alg = 8 * w.cells * w.n_blocks
         out[name] = {"cells_per_s": w.cells / dt, "ms_per_search": 1e3 * dt, "cells": w.cells,
                      "cell_blocks": w.cells * w.n_blocks, "stage_ms": stages,
                      "detected": [i for i, r in enumerate(res) if r["detected"]]}
+        if form["samples_per_chip"]:
+            # the upsampled-code search's correlation kernel against 8 B per cell-block
+            out[name]["roofline"] = {"bound": "hbm", "achieved": alg / (stages["corr"] * 1e-3) / 1e9,
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": alg / (stages["corr"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                     "bytes_per_cell_block": 8,
+                                     "kernel": corr_kernel_name(form, w.surface_len, 0)}
         if name == "BOC_4MS":
             boc32 = (w, res)
     # the BOC search in the reduced form: 16-bit storage, matrix-core DFT stages

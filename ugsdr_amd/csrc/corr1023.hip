@@ -36,6 +36,42 @@ namespace ugsdr_hip {
 
 constexpr int kC1023 = 1023;
 
+// (cos, sin)(2 pi m / 31), m < 31, the values of the FP64 functions rounded
+// once to FP32: the matrix-core A operands of the radix-31 stage
+__constant__ float2 kCis31[31] = {
+    {1.0000000000000000e+00f, 0.0000000000000000e+00f},
+    {9.7952994125249448e-01f, 2.0129852008866006e-01f},
+    {9.1895781162023060e-01f, 3.9435585511331855e-01f},
+    {8.2076344120727629e-01f, 5.7126821509479231e-01f},
+    {6.8896691907568663e-01f, 7.2479278722911988e-01f},
+    {5.2896401032696239e-01f, 8.4864425749475092e-01f},
+    {3.4730525284482028e-01f, 9.3775213214708042e-01f},
+    {1.5142777750457670e-01f, 9.8846832432811138e-01f},
+    {-5.0649168838712642e-02f, 9.9871650717105276e-01f},
+    {-2.5065253225872042e-01f, 9.6807711886620429e-01f},
+    {-4.4039415155763439e-01f, 8.9780453957074158e-01f},
+    {-6.1210598254766257e-01f, 7.9077573693769887e-01f},
+    {-7.5875812269279086e-01f, 6.5137248272222226e-01f},
+    {-8.7434661614458209e-01f, 4.8530196253108104e-01f},
+    {-9.5413925640004882e-01f, 2.9936312297335804e-01f},
+    {-9.9486932339189504e-01f, 1.0116832198743272e-01f},
+    {-9.9486932339189516e-01f, -1.0116832198743204e-01f},
+    {-9.5413925640004882e-01f, -2.9936312297335821e-01f},
+    {-8.7434661614458220e-01f, -4.8530196253108082e-01f},
+    {-7.5875812269279108e-01f, -6.5137248272222203e-01f},
+    {-6.1210598254766269e-01f, -7.9077573693769865e-01f},
+    {-4.4039415155763423e-01f, -8.9780453957074169e-01f},
+    {-2.5065253225872131e-01f, -9.6807711886620407e-01f},
+    {-5.0649168838713551e-02f, -9.9871650717105276e-01f},
+    {1.5142777750457667e-01f, -9.8846832432811138e-01f},
+    {3.4730525284482028e-01f, -9.3775213214708042e-01f},
+    {5.2896401032696239e-01f, -8.4864425749475103e-01f},
+    {6.8896691907568652e-01f, -7.2479278722911999e-01f},
+    {8.2076344120727629e-01f, -5.7126821509479242e-01f},
+    {9.1895781162023060e-01f, -3.9435585511331872e-01f},
+    {9.7952994125249426e-01f, -2.0129852008866114e-01f},
+};
+
 template <int M>
 struct C1023Shape {
     static constexpr int N = kC1023 * M;
@@ -107,9 +143,7 @@ struct C1023Lds {
         // for output k = l % 16 and pair p = 4 s + l / 16
         for (int i = tid; i < 256; i += T) {
             const int l = i & 63, k = l & 15, p = 4 * (i >> 6) + (l >> 4);
-            double s, c;
-            sincospi(2.0 * (double)((k * p) % 31) / 31.0, &s, &c);
-            tw1[i] = make_float2((float)c, (float)s);
+            tw1[i] = kCis31[(k * p) % 31];
         }
         __syncthreads();
     }
@@ -554,9 +588,7 @@ struct UpLds {
         tw1 = reinterpret_cast<float2*>(red + 128);
         for (int i = tid; i < 256; i += T) {
             const int l = i & 63, k = l & 15, p = 4 * (i >> 6) + (l >> 4);
-            double s, c;
-            sincospi(2.0 * (double)((k * p) % 31) / 31.0, &s, &c);
-            tw1[i] = make_float2((float)c, (float)s);
+            tw1[i] = kCis31[(k * p) % 31];
         }
         __syncthreads();
     }
@@ -621,9 +653,9 @@ __device__ __forceinline__ void upw_blocks(float (&S)[33], const UpLds<M>& Ld, c
         }
     };
     float2 xv[2][8], cv[2][8];
-    if (k0 < k1) gather(opaque(tid), k0, 0, xv[0], cv[0]);
+    if (k0 < k1) gather(tid, k0, 0, xv[0], cv[0]);
     for (int k = k0; k < k1; ++k) {
-        const int t = opaque(tid);
+        const int t = tid;
         const int lane = t & 63, col = lane & 15, kq = lane >> 4;
         // ---- stage 1: radix 31 on the matrix cores (see corr1023_blocks);
         // each group's outputs are stored at once: the region's previous
