@@ -210,9 +210,14 @@ __device__ __forceinline__ void corr1023_blocks(float (&S)[C1023Shape<M>::NSLOT]
             const int pp = 4 * st + kq;
             const unsigned i1 = j + pp * B1;
             const unsigned i2 = j + (pp == 0 ? 0 : 31 - pp) * B1;
-            unsigned s1 = i1 + sh, s2 = i2 + sh;
-            s1 -= s1 >= (unsigned)N ? N : 0;
-            s2 -= s2 >= (unsigned)N ? N : 0;
+            unsigned s1 = i1, s2 = i2;
+            if constexpr (!PFA) {
+                // the row's circular shift (the prime-factor rows have none)
+                s1 += sh;
+                s2 += sh;
+                s1 -= s1 >= (unsigned)N ? N : 0;
+                s2 -= s2 >= (unsigned)N ? N : 0;
+            }
             x[2 * st] = ld_off(Xb, s1 * 8u);
             c[2 * st] = ld_off(C, i1 * 8u);
             x[2 * st + 1] = ld_off(Xb, s2 * 8u);
