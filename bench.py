@@ -270,7 +270,9 @@ def galileo_workload(ctx, c3: bool = False):
     """Galileo E1 code hypotheses: every satellite's four sign permutations in
     one fused search, one result per satellite (ProcessGalileo's selection).
     c3: the BOC acquisition configuration's shape (36 satellites, 4 ms x 5
-    blocks at 20.46 MHz, +-6 kHz / 125 Hz) in the reduced FP16 form"""
+    blocks at 20.46 MHz, +-6 kHz / 125 Hz) in the reduced FP16 form, its
+    hypotheses put as signed chips (81840 = 10 x 8184: the upsampled-code
+    search)"""
     import torch
 
     import ugsdr_amd as U
@@ -283,7 +285,12 @@ def galileo_workload(ctx, c3: bool = False):
     else:
         gw = W.galileo_e1()
         prec = U.PREC_FP32
-    gkeys = [ctx.replica_put_samples(r, prec) for c in gw.codes for r in U.e1_sign_hypotheses(c, gw.n_fft)]
+    # signed chip replicas where the 4 ms code divides the transform (the
+    # upsampled-code search), else the upsampled samples
+    if all(gw.n_fft % len(c) == 0 and len(c) % 4 == 0 for c in gw.codes):
+        gkeys = [ctx.replica_put(h, 1, gw.n_fft, prec) for c in gw.codes for h in U.e1_sign_chips(c)]
+    else:
+        gkeys = [ctx.replica_put_samples(r, prec) for c in gw.codes for r in U.e1_sign_hypotheses(c, gw.n_fft)]
     dev = torch.from_numpy(gw.iq).to("cuda:0")
     req, hold = ctx.make_request(None, U.FMT_CI16, fs_in=gw.fs, fs_acq=gw.fs, n_fft=gw.n_fft, n_blocks=gw.n_blocks,
                                  surface_len=gw.n_fft, doppler=gw.doppler, keys=gkeys, reshape=False,

@@ -138,3 +138,28 @@ def test_hypotheses_reduced_precision_match_fp32(ctx):
         assert abs(20 * np.log10(a["snr"] / b["snr"])) < 0.1
     for s, h, tau, f in gw.planted:
         assert out[U.PREC_FP16][s]["detected"] and out[U.PREC_FP16][s]["index"] == tau
+
+
+def test_hypotheses_as_chip_replicas_match_samples(ctx, oracle):
+    """the four hypotheses put as signed chip sequences (e1_sign_chips, n_fft
+    a multiple of the code length) run the upsampled-code search; per
+    satellite the same selection as the sample replicas and the oracle"""
+    chips, sats, raw, dec = scene()
+    say("case Galileo hypotheses as chip replicas (upsampled-code search) vs sample replicas and the oracle")
+    skeys = put_hypotheses(ctx, chips)
+    ckeys = [ctx.replica_put(h, 1, 4 * SPM) for c in chips for h in U.e1_sign_chips(c)]
+    try:
+        req, hold = request(ctx, raw, ckeys, 4)
+        got, _, _ = ctx.acquire_req(req)
+        assert ctx.last_search_form()["samples_per_chip"] == 4 * SPM // len(chips[0])
+        req_s, hold_s = request(ctx, raw, skeys, 4)
+        ref_s, _, _ = ctx.acquire_req(req_s)
+    finally:
+        for k in skeys + ckeys:
+            ctx.replica_drop(k)
+    ref = R.process_galileo(dec, chips, spm=SPM, fs_in=FS, fs_acq=FS, doppler=DOP)
+    for a, b, c in zip(got, ref_s, ref):
+        assert a["detected"] == b["detected"] == c["detected"]
+        assert a["index"] == b["index"] == c["index"] and a["doppler_bin"] == b["doppler_bin"] == c["doppler_bin"]
+        assert a["level"] == pytest.approx(b["level"], rel=1e-5)
+        assert a["level"] == pytest.approx(c["level"], rel=1e-3)

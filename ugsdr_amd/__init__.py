@@ -304,6 +304,19 @@ def e1_sign_hypotheses(chips: np.ndarray, n_fft: int) -> np.ndarray:
     return np.ascontiguousarray(rep[None, :] * signs)
 
 
+def e1_sign_chips(chips: np.ndarray) -> np.ndarray:
+    """the same four hypotheses as chip sequences (a code of a multiple of 4
+    chips: millisecond i is chips [i L/4, (i + 1) L/4)), for replica_put with
+    repeats=1 when n_fft is a multiple of the code length: upsampled by
+    floor index they equal e1_sign_hypotheses sample for sample, and they take
+    the upsampled-code search"""
+    chips = np.asarray(chips, np.int8)
+    if len(chips) % 4:
+        raise ValueError("the 4 ms code needs a multiple of 4 chips")
+    signs = np.repeat(E1_SIGN_PERMUTATIONS, len(chips) // 4, axis=1)
+    return np.ascontiguousarray(chips[None, :] * signs)
+
+
 def adjust_sampling_rate(fs: float, target: float) -> float:
     return lib().ugsdr_adjust_sampling_rate(fs, target)
 

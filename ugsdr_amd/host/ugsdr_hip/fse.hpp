@@ -211,6 +211,15 @@ public:
         for (std::size_t i = 0; i < svs.size(); ++i) {
             const auto& c = codes[i];
             for (int h = 0; h < 4; ++h) {
+                if (n_fft % c.size() == 0 && c.size() % 4 == 0) {
+                    // whole chips per sample and per ms: the signed chips,
+                    // upsampled by the library (the upsampled-code search)
+                    std::vector<int8_t> signed_chips(c.size());
+                    for (std::size_t k = 0; k < c.size(); ++k)
+                        signed_chips[k] = (int8_t)(c[k] * kPermutations[h][k / (c.size() / 4)]);
+                    replica_keys.push_back(ChipKey(signed_chips, h, n_fft));
+                    continue;
+                }
                 // Upsampler::Transform (floor index), then ms j times its sign
                 for (std::size_t k = 0; k < n_fft; ++k)
                     rep[k] = (float)(c[(k * c.size()) / n_fft] * kPermutations[h][k / spm]);
@@ -345,6 +354,19 @@ private:
         uint64_t k = 0;
         ugsdr_hip::check(ugsdr_hip_replica_put_samples(context->get(), samples.data(), (int64_t)samples.size(),
                                                        options.precision, &k));
+        keys.emplace_back(std::move(id), k);
+        return k;
+    }
+
+    // signed chips of hypothesis h, cached by (hypothesis, length, precision, chips)
+    uint64_t ChipKey(const std::vector<int8_t>& chips, int hypothesis, std::size_t n_fft) {
+        std::vector<int64_t> id{-5 - hypothesis, (int64_t)n_fft, options.precision};
+        id.insert(id.end(), chips.begin(), chips.end());
+        for (auto& kv : keys)
+            if (kv.first == id) return kv.second;
+        uint64_t k = 0;
+        ugsdr_hip::check(ugsdr_hip_replica_put(context->get(), chips.data(), (int64_t)chips.size(), 1,
+                                               (int64_t)n_fft, options.precision, &k));
         keys.emplace_back(std::move(id), k);
         return k;
     }
