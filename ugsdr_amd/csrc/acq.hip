@@ -78,6 +78,7 @@ struct FwdArgs {
     int N;
     int F;
     const float2* sig;           // mode 0
+    const short2* sig16;         // mode 0: int16 I/Q read in place of sig when set
     const double* class_frac;    // mode 0: phi per class
     const signed char* chips;    // mode 1
     int64_t n_chips;
@@ -91,7 +92,14 @@ struct FwdArgs {
 __device__ __forceinline__ float2 fwd_source(const FwdArgs& a, int blk, int64_t n) {
     if (a.mode == 0) {
         const int k = blk / a.F, c = blk % a.F;
-        const float2 x = a.sig[(int64_t)k * a.N + n];
+        const int64_t i = (int64_t)k * a.N + n;
+        float2 x;
+        if (a.sig16) {
+            const short2 v = a.sig16[i];
+            x = make_float2((float)v.x, (float)v.y);
+        } else {
+            x = a.sig[i];
+        }
         const double phi = a.class_frac[c];
         if (phi == 0.0) return x;
         // the reciprocal is loop-invariant in the callers' sample loops
@@ -1024,6 +1032,15 @@ int launch_front_args(const FrontArgs& fa, hipStream_t st) {
 int launch_front(const AcqPlan& pl, const ugsdr_acq_request* r, AcqSlot& s, hipStream_t st) {
     FrontArgs fa{};
     fa.src.p = r->iq_on_device ? r->iq : s.iq.ptr;
+    // decode only: the class transforms read the samples where they lie
+    // (the plan holds n_in_used >= need samples)
+    s.front_src = nullptr;
+    if (pl.up == 1 && pl.down == 1 && !pl.apply_tone && (pl.fmt == UGSDR_FMT_CI16 || pl.fmt == UGSDR_FMT_CF32) &&
+        pl.n_in_used >= pl.need) {
+        s.front_src = fa.src.p;
+        s.front_fmt = pl.fmt;
+        return UGSDR_OK;
+    }
     fa.src.fmt = pl.fmt;
     fa.src.spm = pl.spm_in;
     fa.src.epoch_bytes = pl.epoch_bytes;
@@ -1073,6 +1090,12 @@ int enqueue_classes(const AcqPlan& pl, AcqSlot& s, hipStream_t st) {
     a.N = (int)pl.N;
     a.F = pl.F;
     a.sig = s.sig.as<float2>();
+    if (s.front_src) {
+        if (s.front_fmt == UGSDR_FMT_CI16)
+            a.sig16 = static_cast<const short2*>(s.front_src);
+        else
+            a.sig = static_cast<const float2*>(s.front_src);
+    }
     a.class_frac = reinterpret_cast<const double*>(s.tables.as<char>() + pl.off_frac);
     a.mode = 0;
     // the reduced form's Z is scaled per (bin, block) from the class blocks' max |X|

@@ -93,6 +93,11 @@ struct Replica {
 // device buffers one acquisition request holds while in flight
 struct AcqSlot {
     DevBuf iq, sig, classes, tables, rows, results, surf, fwd_scratch;
+    // the recorded samples the class transforms read directly when the front
+    // end only decodes them (int16 or float I/Q at the acquisition rate, no
+    // tone): set by launch_front, null when it filled sig
+    const void* front_src = nullptr;
+    int front_fmt = 0;
     // periodic-fold searches: the class spectra of every code-period sub-block
     DevBuf classes_sub;
     // upsampled-code searches: the PRN-independent spectra Z per (Doppler bin, block)
