@@ -1549,13 +1549,18 @@ int acquire_impl(ugsdr_hip_ctx* ctx, const ugsdr_acq_request* r, ugsdr_acq_resul
             return rc;
         if (graphs) gc.pending = key;
     }
-    UGSDR_HIP_TRY(hipMemcpyAsync(out, s.results.ptr, (size_t)pl.P * sizeof(ugsdr_acq_result), hipMemcpyDeviceToHost, st));
-    if (peaks)
-        UGSDR_HIP_TRY(hipMemcpyAsync(peaks, s.results.as<char>() + (size_t)pl.P * sizeof(ugsdr_acq_result),
-                                     (size_t)pl.P * pl.L * 4, hipMemcpyDeviceToHost, st));
+    // results and peaks lie together in s.results: one copy to page-locked
+    // staging, then to the caller's memory
+    const size_t res_bytes = (size_t)pl.P * sizeof(ugsdr_acq_result);
+    const size_t peak_bytes = peaks ? (size_t)pl.P * pl.L * 4 : 0;
+    UGSDR_HIP_TRY(ctx->results_host.reserve(res_bytes + peak_bytes));
+    UGSDR_HIP_TRY(hipMemcpyAsync(ctx->results_host.ptr, s.results.ptr, res_bytes + peak_bytes, hipMemcpyDeviceToHost,
+                                 st));
     if (surfaces)
         UGSDR_HIP_TRY(hipMemcpyAsync(surfaces, s.surf.ptr, (size_t)pl.P * pl.D * pl.L * 4, hipMemcpyDeviceToHost, st));
     UGSDR_HIP_TRY(hipStreamSynchronize(st));
+    memcpy(out, ctx->results_host.ptr, res_bytes);
+    if (peaks) memcpy(peaks, static_cast<const char*>(ctx->results_host.ptr) + res_bytes, peak_bytes);
     if (ctx->profile)
         for (int i = 0; i < 4; ++i) UGSDR_HIP_TRY(hipEventElapsedTime(&ctx->last_ms[i], ctx->ev[i], ctx->ev[i + 1]));
     return UGSDR_OK;

@@ -69,6 +69,31 @@ struct DevBuf {
     }
 };
 
+// page-locked host memory (hipHostMalloc), grown on demand
+struct PinnedBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipHostMalloc(&ptr, n, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    PinnedBuf(PinnedBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    ~PinnedBuf() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+};
+
 struct Replica {
     void* spec = nullptr;  // n_fft complex (float2, or half2/bf16x2 for reduced)
     int64_t n_fft = 0;
@@ -130,6 +155,9 @@ struct ugsdr_hip_ctx {
     // scratch
     ugsdr_hip::DevBuf iq, sig, classes, scratch, rows, results, surf, tables, aux, aux2;
     ugsdr_hip::AcqSlot slot;  // buffers of the synchronous acquire path
+    // the synchronous acquire path's results (and peaks) land here first: a
+    // copy to page-locked memory returns sooner than one to pageable memory
+    ugsdr_hip::PinnedBuf results_host;
     // hipGraph of the last repeated request shape (ugsdr_hip_acquire): the
     // shape is captured on its second consecutive call and replayed after
     struct GraphCache {

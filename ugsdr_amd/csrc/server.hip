@@ -45,23 +45,6 @@ namespace ugsdr_hip {
 
 using Clock = std::chrono::steady_clock;
 
-struct PinnedBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    hipError_t reserve(size_t n) {
-        if (n <= bytes) return hipSuccess;
-        if (ptr) (void)hipHostFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-        hipError_t e = hipHostMalloc(&ptr, n, hipHostMallocDefault);
-        if (e == hipSuccess) bytes = n;
-        return e;
-    }
-    ~PinnedBuf() {
-        if (ptr) (void)hipHostFree(ptr);
-    }
-};
-
 struct ServerSlot {
     AcqSlot dev;
     PinnedBuf iq_stage, results;
