@@ -9,7 +9,9 @@
   tests/parity.py, and PRNs 9 and 23 detected;
 * acquisition from packed Nt1065 bytes equals acquisition from the same
   samples handed over as complex float (the front kernel sees identical
-  values, so every result field matches exactly).
+  values, so every result field matches exactly);
+* int16 / float I/Q read in place by the class transforms equal int8 I/Q
+  decoded by the front kernel: results, peaks and surfaces bit for bit.
 """
 import os
 
@@ -122,3 +124,35 @@ def test_acquire_nt1065_packed_equals_decoded(ctx, oracle):
         for k in keys:
             ctx.replica_drop(k)
     assert a == b
+
+
+@pytest.mark.parametrize("make", [W.l1ca_1ms, W.l1ca_4ms], ids=["L1CA_1MS", "L1CA_4MS_fold"])
+def test_in_place_reads_equal_front_kernel(ctx, make):
+    """int16 and float I/Q at the acquisition rate with no tone are read in
+    place by the class transforms (no front kernel); int8 I/Q goes through the
+    front kernel.  The same int8 values in the three formats decode to the same
+    floats, so results, peaks and surfaces match exactly; the entry point that
+    allocates nothing matches them to the parity rules."""
+    w = make()
+    say(f"case in-place reads vs front kernel, {w.name} n_fft={w.n_fft}")
+    x8 = w.iq.astype(np.int8)
+    x16 = x8.astype(np.int16)
+    xf = x8.astype(np.float32)
+    keys = [ctx.replica_put(c, w.repeats, w.n_fft) for c in w.codes]
+    kw = dict(fs_in=w.fs, fs_acq=w.fs, n_fft=w.n_fft, n_blocks=w.n_blocks, surface_len=w.surface_len,
+              doppler=w.doppler, keys=keys, reshape=w.reshape)
+    try:
+        ref = ctx.acquire(x8, U.FMT_CI8, want_surfaces=True, want_peaks=True, **kw)
+        for x, fmt in ((x16, U.FMT_CI16), (xf, U.FMT_CF32)):
+            got = ctx.acquire(x, fmt, want_surfaces=True, want_peaks=True, **kw)
+            assert got[0] == ref[0], fmt
+            for g, r in zip(got[1:], ref[1:]):
+                np.testing.assert_array_equal(g, r)
+        # the entry point without allocation runs the length's default variant
+        # (no autotuned choice), so its sums may round differently: parity rules
+        a, _, _ = ctx.acquire_noalloc(x16, U.FMT_CI16, **kw)
+        fails, _ = compare(a, ref[0])
+        assert not fails, fails
+    finally:
+        for k in keys:
+            ctx.replica_drop(k)
